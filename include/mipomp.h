@@ -1,0 +1,157 @@
+/*
+ * mipomp.h -- C ABI of the MI355X particle-filter engine (libmipomp.so).
+ *
+ * Plain C: pointers, sizes and plain structs; no C++ or torch types.  Every
+ * pointer argument is DEVICE memory unless its name starts with `h_`.  Every
+ * compute entry point takes a HIP stream (`void*` = hipStream_t, NULL = the
+ * null stream), enqueues work and returns without synchronising.  The caller
+ * owns all memory; scratch comes from a caller-provided workspace whose size
+ * the matching *_workspace_size() query returns.  Return value: MP_OK or an
+ * error code; mp_last_error() gives the message of the calling thread's last
+ * failure.  No C++ exception crosses this boundary.
+ *
+ * Each entry point replaces a reference function of pypomp 0.4.10.2; the
+ * reference binds none of them itself (it is pure Python on JAX).  The Python
+ * host layer of this package (pypomp_amd/_native.py, ctypes) is the binding.
+ */
+#ifndef MIPOMP_H
+#define MIPOMP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MP_OK 0
+#define MP_ERR_ARG 1       /* invalid argument (message in mp_last_error) */
+#define MP_ERR_HIP 2       /* a HIP runtime call failed */
+#define MP_ERR_WORKSPACE 3 /* workspace smaller than the size query returned */
+
+/* models (device functors compiled into the library) */
+#define MP_MODEL_LG 1   /* pypomp/models/linear_gaussian.py:139-191 */
+#define MP_MODEL_SEIR 2 /* Euler-binomial SEIR, NegBin reports */
+#define MP_MODEL_SV 3   /* stochastic volatility with leverage */
+
+/* algorithms / forms */
+#define MP_ALGO_PF 0
+#define MP_ALGO_MOP 1
+#define MP_ALGO_PER 2
+#define MP_FORM_BATCHED 0 /* one persistent launch, one workgroup per filter */
+#define MP_FORM_PLAIN 1   /* one filter at a time over the whole GPU, launches per step */
+
+/* output flags of mp_pfilter (pypomp/core/algorithms/pfilter.py:146-156) */
+#define MP_CLL 1
+#define MP_ESS 2
+#define MP_FILTER_MEAN 4
+#define MP_PREDICTION_MEAN 8
+#define MP_SHOULD_TRANS 16 /* theta on the estimation scale */
+
+typedef struct mp_model_desc {
+  int32_t kind;        /* MP_MODEL_* */
+  int32_t S;           /* state words per particle */
+  int32_t d;           /* parameters */
+  int32_t dy;          /* observation words */
+  int32_t dx;          /* LG state dimension (<= 4) */
+  int32_t lg_sep_init; /* LG: initial Cholesky factor P follows X0 in theta */
+  uint32_t accum_mask; /* bit s: state word s is reset to 0 at each observation
+                          interval (reference: accumvars, model_mechanics.py:921) */
+} mp_model_desc;
+
+/* Observation series of U units (U = 1 for a single Pomp).  Unit u owns rows
+ * [obs_off[u], obs_off[u+1]) of ys/times/nstep/tidx0; its Euler step sizes
+ * start at dt_ext + dt_off[u]; tidx0 = unit-local index of the first Euler
+ * step of each observation interval (reference: _calc_ys_covars,
+ * core/algorithms/helpers.py:279-299). */
+typedef struct mp_series_desc {
+  const float* ys;        /* (sum T_u, dy) */
+  const float* times;     /* (sum T_u) */
+  const int32_t* nstep;   /* (sum T_u) */
+  const int32_t* tidx0;   /* (sum T_u) */
+  const float* dt_ext;    /* (sum_u sum_t nstep) */
+  const int64_t* obs_off; /* (U + 1) */
+  const int64_t* dt_off;  /* (U) */
+  float t0;
+  int32_t U;
+  int32_t Tmax;           /* max_u T_u */
+  int64_t max_euler;      /* max_u sum_t nstep */
+} mp_series_desc;
+
+const char* mp_last_error(void);
+int32_t mp_abi_version(void);
+
+/* ---- host-side key utilities (CPU; no device needed) ---------------------
+ * Threefry-2x32-20 and jax.random.split in partitionable mode, used by the host
+ * layer for key derivation (reference: Pomp._update_fresh_key,
+ * core/pomp.py:391-408; rep keys, core/estimation_mixin.py:188-190). */
+int32_t mp_host_threefry2x32(const uint32_t* h_keys, const uint32_t* h_ctr, uint32_t* h_out, int64_t n);
+int32_t mp_host_split(const uint32_t* h_key, int64_t n, uint32_t* h_out);
+
+/* ---- device primitives (parity tests and building blocks) ----------------
+ * keys: (nkeys, 2) uint32.  For each key k and flat index i < n:
+ *   bits[k, i]    = random_bits(k, 32, (n,))[i]     (jax.random.bits)
+ *   uniform[k, i] = jax.random.uniform(k, (n,))[i]  (helpers.py:28 call site)
+ *   normal[k, i]  = jax.random.normal(k, (n,))[i]   (linear_gaussian.py:147,162)
+ *   split[k, i]   = jax.random.split(k, n)[i]       (pfilter.py:23,105,160) */
+int32_t mp_random_bits(const uint32_t* keys, int64_t nkeys, int64_t n, uint32_t* out, void* stream);
+int32_t mp_uniform(const uint32_t* keys, int64_t nkeys, int64_t n, float* out, void* stream);
+int32_t mp_normal(const uint32_t* keys, int64_t nkeys, int64_t n, float* out, void* stream);
+int32_t mp_split(const uint32_t* keys, int64_t nkeys, int64_t n, uint32_t* out, void* stream);
+
+/* Systematic resampling of R independent rows of J log-weights (row r uses
+ * key r: u = uniform(key), comb (u + j) / J), replacing _resample
+ * (core/algorithms/helpers.py:14-31) with its order-independent fixed-point
+ * definition; ancestors (R, J) int32, already clamped to J - 1 like the
+ * reference's gather (helpers.py:87). */
+size_t mp_resample_workspace_size(int64_t R, int64_t J);
+int32_t mp_resample_systematic(const float* logw, const uint32_t* keys, int64_t R, int64_t J,
+                               int32_t* ancestors, void* workspace, size_t ws_bytes, void* stream);
+
+/* ---- filters ---------------------------------------------------------------
+ * R filters; filter f uses key keys[f], unit unit[f] (NULL: unit 0) and
+ * parameters theta + f * theta_stride (theta_stride 0: shared).  loglik (R)
+ * float64 sums over t of the per-observation float32-evaluated terms. */
+size_t mp_filter_workspace_size(int32_t algo, const mp_model_desc* h_model, const mp_series_desc* h_series,
+                                int32_t R, int32_t J, int32_t form);
+
+/* Bootstrap particle filter: _pfilter_internal (core/algorithms/pfilter.py:15-182),
+ * batched over replicates/parameter sets like _vmapped_pfilter_internal2
+ * (pfilter.py:185-207).  cll/ess (R, Tmax), fmean/pmean (R, Tmax, S) may be
+ * NULL unless the matching flag is set. */
+int32_t mp_pfilter(const mp_model_desc* h_model, const mp_series_desc* h_series, const float* theta,
+                   int64_t theta_stride, const uint32_t* keys, const int32_t* unit, int32_t R, int32_t J,
+                   float thresh, int32_t flags, int32_t form, double* loglik, float* cll, float* ess,
+                   float* fmean, float* pmean, void* workspace, size_t ws_bytes, void* stream);
+
+/* MOP(alpha) log-likelihood value: _mop_internal (core/algorithms/mop.py:19-135);
+ * theta on the estimation scale.  Returns +loglik (the reference's F.mop
+ * returns its negative, functional/mop.py:58-62). */
+int32_t mp_mop(const mp_model_desc* h_model, const mp_series_desc* h_series, const float* theta,
+               int64_t theta_stride, const uint32_t* keys, const int32_t* unit, int32_t R, int32_t J,
+               float alpha, int32_t form, double* loglik, void* workspace, size_t ws_bytes, void* stream);
+
+/* One IF2 iteration m: _perfilter_internal (core/algorithms/mif.py:88-258) for R
+ * swarms.  thetas_in/thetas_out (R, J, d) on the estimation scale; sigmas and
+ * sigmas_init (d) as RWSigma.sigmas_array / sigmas_init_array
+ * (core/rw_sigma.py:184-191); geometric cooling factor cool = a^(1/50)
+ * (rw_sigma.py:210-213).  theta_mean (R, d): mean of thetas_out over J. */
+int32_t mp_perfilter(const mp_model_desc* h_model, const mp_series_desc* h_series, const float* thetas_in,
+                     const uint32_t* keys, const int32_t* unit, int32_t R, int32_t J, float thresh,
+                     const float* sigmas, const float* sigmas_init, float cool, int32_t m, float* thetas_out,
+                     float* theta_mean, double* loglik, void* workspace, size_t ws_bytes, void* stream);
+
+/* ---- kernel timing ----------------------------------------------------------
+ * No counterpart in the reference (it has no profiler hooks).  While enabled,
+ * every kernel launch of the entry points above is bracketed by HIP events on
+ * its stream; mp_profile_read synchronises on them and returns, per kind
+ * (0 plain propagate, 1 plain scan, 2 batched filter, 3 key schedule), the
+ * summed milliseconds and launch counts, then clears the records. */
+int32_t mp_profile_enable(int32_t on);
+int32_t mp_profile_read(double* h_ms, int64_t* h_count, int32_t nkinds);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MIPOMP_H */

@@ -1,0 +1,322 @@
+"""Filtering algorithms restated in NumPy -- TEST ORACLE ONLY.
+
+Each function follows a reference function step for step (file:line cited in
+its docstring), including the key tree, the float32 arithmetic and the special
+cases.  ``resampler`` selects the systematic resampler: ``"fixed"`` (the
+build's order-independent definition, which the HIP kernels implement) or
+``"reference"`` (the literal float32-cumsum algorithm).
+
+A model object provides ``S`` (state words), ``accumvars`` (indices reset to 0
+at the start of each observation interval), ``rinit(theta, keys, t0)``,
+``rproc_step(X, theta, keys, t, dt)``, ``dmeas(y, X, theta, t)`` and
+``rmeas(X, theta, keys, t)``; theta is (d,) or per particle (J, d), on the
+natural scale.  ``from_est``/``to_est`` map parameter arrays between scales.
+"""
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import threefry as tf
+from .resample import resample_fixed_fast, resample_reference
+
+F32 = np.float32
+
+
+@dataclass
+class Series:
+    """Observation series and Euler grid (reference: core/algorithms/helpers.py:191-299)."""
+    ys: np.ndarray          # (T, dy) float32
+    times: np.ndarray       # (T,) float32
+    t0: float
+    nstep: np.ndarray       # (T,) int
+    dt_ext: np.ndarray      # (sum nstep,) float32
+
+    @property
+    def T(self):
+        return self.ys.shape[0]
+
+
+def make_series(ys, times, t0, nstep=None, dt=None):
+    """Fixed ``nstep`` or fixed ``dt`` Euler grid, as ``_calc_steps`` (helpers.py:191-215)."""
+    times = np.asarray(times, dtype=np.float64)
+    times0 = np.concatenate([[float(t0)], times])
+    ns, dts = [], []
+    tol = np.sqrt(np.finfo(float).eps)
+    for i in range(len(times)):
+        t1, t2 = float(times0[i]), float(times0[i + 1])
+        if nstep is not None:
+            ns.append(int(nstep)); dts.append((t2 - t1) / int(nstep))
+        else:
+            if t1 >= t2:
+                ns.append(0); dts.append(0.0)
+            elif t1 + dt >= t2:
+                ns.append(1); dts.append(t2 - t1)
+            else:
+                n = int(np.ceil((t2 - t1) / dt / (1 + tol)))
+                ns.append(n); dts.append((t2 - t1) / n)
+    ns = np.array(ns, dtype=np.int64)
+    dt_ext = np.repeat(np.array(dts), ns).astype(F32)
+    return Series(ys=np.asarray(ys, dtype=F32).reshape(len(times), -1),
+                  times=times.astype(F32), t0=float(t0), nstep=ns, dt_ext=dt_ext)
+
+
+def rprocess_interp(model, X, theta, key, series, t, t_idx, nstep):
+    """Euler sub-step loop (core/model_mechanics.py:908-947 and step_pf :584-588):
+    reset accumvars; per sub-step ``(next, sk) = split(key)``, particle keys
+    ``split(sk, J)``, one model step, ``t += dt``."""
+    X = X.copy()
+    for a in model.accumvars:
+        X[:, a] = 0
+    J = X.shape[0]
+    t = F32(t)
+    for _ in range(int(nstep)):
+        k2 = tf.split(key)
+        nxt, sk = k2[0], k2[1]
+        pkeys = tf.split(sk, J)
+        dt = series.dt_ext[t_idx]
+        X = model.rproc_step(X, theta, pkeys, t, dt)
+        t = F32(t + dt)
+        t_idx += 1
+        key = nxt
+    return X, t_idx
+
+
+def normalize_weights(w):
+    """``_normalize_weights`` (core/algorithms/helpers.py:34-57), float32."""
+    w = np.asarray(w, dtype=F32)
+    J = w.shape[-1]
+    mw = np.max(w)
+    if mw == -np.inf:
+        return np.full(J, -F32(np.log(F32(J))), dtype=F32), F32(-np.inf)
+    with np.errstate(invalid="ignore", over="ignore"):
+        ll = F32(mw + np.log(np.nansum(np.exp((w - mw).astype(F32)), dtype=F32)))
+        nw = (w - ll).astype(F32)
+    return nw, ll
+
+
+def _resample(resampler, nw, w_unnorm, key):
+    u = tf.uniform(key)
+    if resampler == "reference":
+        return resample_reference(nw, u)
+    return resample_fixed_fast(w_unnorm, u)
+
+
+def pfilter(model, theta, series, key, J, thresh=0.0, resampler="fixed",
+            diagnostics=False, should_trans=False, return_trace=False):
+    """One bootstrap particle filter: ``_pfilter_internal`` + ``_pfilter_step``
+    (core/algorithms/pfilter.py:15-182).  Returns the log-likelihood (float32
+    sum over t, as the reference) and, with ``diagnostics``, CLL/ESS/filter and
+    prediction means.  ``return_trace`` adds per-step ancestors and weights."""
+    theta = np.asarray(theta, dtype=F32)
+    th = model.from_est(theta) if should_trans else theta
+    K = tf.split(key, J + 1)
+    key, pkeys = K[0], K[1:]
+    X = model.rinit(th, pkeys, series.t0)
+    nw = np.log(np.ones(J, dtype=F32) / F32(J)).astype(F32)
+    loglik = F32(0.0)
+    t, t_idx = F32(series.t0), 0
+    out = {"CLL": [], "ESS": [], "filter_mean": [], "prediction_mean": [],
+           "ancestors": [], "weights": [], "states": []}
+    logthresh = np.log(F32(thresh)) if thresh > 0 else -np.inf
+    for i in range(series.T):
+        k2 = tf.split(key)
+        key, sub = k2[0], k2[1]
+        X, t_idx = rprocess_interp(model, X, th, sub, series, t, t_idx, series.nstep[i])
+        t = series.times[i]
+        meas = model.dmeas(series.ys[i], X, th, t).astype(F32)
+        w = (nw + meas).astype(F32)
+        nw, ll_t = normalize_weights(w)
+        loglik = F32(loglik + ll_t)
+        if diagnostics:
+            out["CLL"].append(ll_t)
+            out["ESS"].append(F32(1.0) / np.sum(np.exp(F32(2.0) * nw), dtype=F32))
+            out["filter_mean"].append((X * np.exp(nw)[:, None]).sum(0, dtype=F32))
+            out["prediction_mean"].append(X.mean(0, dtype=F32))
+        if return_trace:
+            out["weights"].append(w.copy())
+            out["states"].append(X.copy())
+        with np.errstate(invalid="ignore"):
+            do_res = (np.max(nw) - np.min(nw)) > logthresh
+        k2 = tf.split(key)
+        key, sub = k2[0], k2[1]
+        if do_res:
+            a = _resample(resampler, nw, w, sub)
+            X = X[a]
+            nw = np.full(J, F32(0.0) - F32(np.log(F32(J))), dtype=F32)
+        else:
+            a = np.arange(J, dtype=np.int32)
+        if return_trace:
+            out["ancestors"].append(a)
+    out["logLik"] = loglik
+    return out
+
+
+def mop(model, theta_est, series, key, J, alpha, resampler="fixed", return_trace=False):
+    """``_mop_internal`` + ``_mop_step`` (core/algorithms/mop.py:19-135): theta on the
+    estimation scale; returns the MOP log-likelihood (the reference returns its
+    negative).  Value-wise the weights stay constant across particles."""
+    th = model.from_est(np.asarray(theta_est, dtype=F32))
+    K = tf.split(key, J + 1)
+    key, pkeys = K[0], K[1:]
+    X = model.rinit(th, pkeys, series.t0)
+    wF = np.log(np.ones(J, dtype=F32) / F32(J)).astype(F32)
+    loglik = F32(0.0)
+    t, t_idx = F32(series.t0), 0
+    trace = {"ancestors": [], "meas": [], "states": []}
+    for i in range(series.T):
+        wP = (F32(alpha) * wF).astype(F32)
+        k2 = tf.split(key)
+        key, sub = k2[0], k2[1]
+        X, t_idx = rprocess_interp(model, X, th, sub, series, t, t_idx, series.nstep[i])
+        t = series.times[i]
+        meas = model.dmeas(series.ys[i], X, th, t).astype(F32)
+        loglik = F32(loglik + lse(wP + meas) - lse(wP))
+        nw, _ = normalize_weights(meas)
+        k2 = tf.split(key)
+        key, sub = k2[0], k2[1]
+        a = _resample(resampler, nw, meas, sub)
+        if return_trace:
+            trace["ancestors"].append(a)
+            trace["meas"].append(meas)
+            trace["states"].append(X.copy())
+        X = X[a]
+        wF = wP[a]
+    if return_trace:
+        return loglik, trace
+    return loglik
+
+
+def lse(a):
+    """``jax.scipy.special.logsumexp`` in float32."""
+    a = np.asarray(a, dtype=F32)
+    amax = np.max(a)
+    if not np.isfinite(amax):
+        amax = F32(0.0)
+    with np.errstate(divide="ignore"):
+        return F32(np.log(np.sum(np.exp((a - amax).astype(F32)), dtype=F32)) + amax)
+
+
+@dataclass
+class RWSigma:
+    """Random-walk sigmas and geometric cooling (core/rw_sigma.py:184-225)."""
+    sigmas: np.ndarray      # (d,) canonical order
+    init_mask: np.ndarray   # (d,) 1.0 for initial-value parameters
+    a: float = 0.5
+
+    @property
+    def sigmas_array(self):
+        return (self.sigmas.astype(F32) * (F32(1.0) - self.init_mask.astype(F32))).astype(F32)
+
+    @property
+    def sigmas_init_array(self):
+        return (self.sigmas.astype(F32) * self.init_mask.astype(F32)).astype(F32)
+
+    def cooling_factor(self, nt, m, ntimes):
+        frac = F32(F32(nt) / F32(ntimes) + F32(m))
+        factor = F32(self.a ** (1.0 / 50.0))
+        return F32(np.power(factor, frac, dtype=F32))
+
+
+def perfilter(model, m, thetas, series, key, J, rw, thresh=0.0, resampler="fixed"):
+    """``_perfilter_internal`` + ``_perfilter_scan_body`` (core/algorithms/mif.py:88-258).
+    ``thetas`` (J, d) on the estimation scale.  Returns (thetas, loglik)."""
+    thetas = np.asarray(thetas, dtype=F32).copy()
+    d = thetas.shape[1]
+    T = series.T
+    nw = np.full(J, -F32(np.log(F32(J))), dtype=F32)
+    loglik = F32(0.0)
+    cf = [rw.cooling_factor(i, m, T) for i in range(T)]
+    k2 = tf.split(key)
+    key, sub = k2[0], k2[1]
+    sic = (rw.cooling_factor(0, m, T) * rw.sigmas_init_array).astype(F32)
+    thetas = (thetas + sic * tf.normal(sub, J * d).reshape(J, d)).astype(F32)
+    K = tf.split(key, J + 1)
+    key, pkeys = K[0], K[1:]
+    X = model.rinit(model.from_est(thetas), pkeys, series.t0)
+    step_keys = tf.split(key, T + 1)[1:]
+    t, t_idx = F32(series.t0), 0
+    logthresh = np.log(F32(thresh)) if thresh > 0 else -np.inf
+    nan_sub = F32(np.log(F32(1e-18)))
+    sig = rw.sigmas_array
+    for i in range(T):
+        k3 = tf.split(step_keys[i], 3)
+        kpert, kproc, kres = k3[0], k3[1], k3[2]
+        thetas = (thetas + (cf[i] * sig).astype(F32) * tf.normal(kpert, J * d).reshape(J, d)).astype(F32)
+        th_nat = model.from_est(thetas)
+        X, t_idx = rprocess_interp(model, X, th_nat, kproc, series, t, t_idx, series.nstep[i])
+        t = series.times[i]
+        meas = model.dmeas(series.ys[i], X, th_nat, t).astype(F32)
+        meas = np.nan_to_num(meas, nan=nan_sub).astype(F32)
+        w = (nw + meas).astype(F32)
+        nw, ll_t = normalize_weights(w)
+        loglik = F32(loglik + ll_t)
+        with np.errstate(invalid="ignore"):
+            do_res = (np.max(nw) - np.min(nw)) > logthresh
+        if do_res:
+            a = _resample(resampler, nw, w, kres)
+            X = X[a]
+            thetas = thetas[a]
+            nw = np.full(J, F32(0.0) - F32(np.log(F32(J))), dtype=F32)
+    return thetas, loglik
+
+
+def mif(model, thetas_nat, series, key, J, M, rw, thresh=0.0, n_monitors=0,
+        resampler="fixed"):
+    """``_mif_internal`` / ``_mif_scan_body`` (core/algorithms/mif.py:16-85) plus the
+    scale handling of ``F.mif`` (functional/mif.py:85-122).  ``thetas_nat`` (J, d)
+    natural scale.  Returns (logliks (M,), traces (M+1, d) natural, final swarm)."""
+    theta = model.to_est(np.asarray(thetas_nat, dtype=F32))
+    mkeys = tf.split(key, M + 1)[1:]
+    traces = [theta.mean(0, dtype=F32)]
+    lls = []
+    for m in range(M):
+        new, ll_per = perfilter(model, m, theta, series, mkeys[m], J, rw, thresh, resampler)
+        if n_monitors >= 1:
+            mean = theta.mean(0, dtype=F32)
+            mon = tf.split(mkeys[m], n_monitors)
+            vals = [pfilter(model, mean, series, mon[k], J, thresh, resampler,
+                            should_trans=True)["logLik"] for k in range(n_monitors)]
+            ll = F32(np.mean(np.array(vals, dtype=F32), dtype=F32))
+        else:
+            ll = ll_per
+        lls.append(ll)
+        traces.append(new.mean(0, dtype=F32))
+        theta = new
+    traces = np.stack(traces).astype(F32)
+    return (np.array(lls, dtype=F32), model.from_est(traces), model.from_est(theta))
+
+
+def simulate(model, theta, series, key, nsim=1):
+    """``_simulate_internal`` / ``_simulate_helper`` (core/algorithms/simulate.py:15-142).
+    Returns X (T+1, nsim, S) and Y (T, nsim, dy)."""
+    theta = np.asarray(theta, dtype=F32)
+    K = tf.split(key, nsim + 1)
+    key, skeys = K[0], K[1:]
+    X = model.rinit(theta, skeys, series.t0)
+    Xs, Ys = [X.copy()], []
+    t, t_idx = F32(series.t0), 0
+    times0 = np.concatenate([[series.t0], series.times]).astype(F32)
+    for i in range(series.T):
+        k2 = tf.split(key)
+        key, sub = k2[0], k2[1]
+        X, t_idx = rprocess_interp(model, X, theta, sub, series, t, t_idx, series.nstep[i])
+        t = times0[i]
+        K = tf.split(key, nsim + 1)
+        key, mkeys = K[0], K[1:]
+        Y = model.rmeas(X, theta, mkeys, t)
+        Xs.append(X.copy())
+        Ys.append(Y)
+    return np.stack(Xs), np.stack(Ys)
+
+
+def pomp_new_key(key):
+    """``Pomp._update_fresh_key`` (core/pomp.py:391-408): returns the key used by the call."""
+    return tf.split(key)[1]
+
+
+def logmeanexp(x):
+    """``logmeanexp`` over replicates (pypomp/maths.py:24-84), float64."""
+    x = np.asarray(x, dtype=np.float64)
+    mx = np.max(x)
+    return float(mx + np.log(np.mean(np.exp(x - mx))))

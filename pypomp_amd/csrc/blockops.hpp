@@ -1,0 +1,99 @@
+// Wavefront (64-lane) and workgroup reductions / scans for gfx950.
+#pragma once
+
+#include "common.hpp"
+
+namespace mp {
+
+constexpr int WAVE = 64;
+
+// max that propagates NaN, as jnp.max does.
+__device__ __forceinline__ float nanmax(float a, float b) { return (a > b || a != a) ? a : b; }
+__device__ __forceinline__ float nanmin(float a, float b) { return (a < b || a != a) ? a : b; }
+
+__device__ __forceinline__ float wave_nanmax(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = nanmax(v, __shfl_xor(v, o, WAVE));
+  return v;
+}
+__device__ __forceinline__ float wave_nanmin(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = nanmin(v, __shfl_xor(v, o, WAVE));
+  return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+  return v;
+}
+__device__ __forceinline__ float wave_sumf(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+  return v;
+}
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+  return v;
+}
+
+// Inclusive prefix sum across the 64 lanes of a wave.
+__device__ __forceinline__ uint64_t wave_incl_scan_u64(uint64_t v) {
+  const int lane = threadIdx.x & (WAVE - 1);
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    uint64_t t = __shfl_up(v, o, WAVE);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+// Workgroup reductions.  `red` is LDS scratch of at least 2*nwaves doubles.
+// All threads of the block must call; result is returned to every thread.
+__device__ __forceinline__ void block_maxmin(float& mx, float& mn, double* red) {
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  const int nw = (blockDim.x + WAVE - 1) / WAVE;
+  mx = wave_nanmax(mx);
+  mn = wave_nanmin(mn);
+  __syncthreads();
+  float* r = reinterpret_cast<float*>(red);
+  if (lane == 0) { r[w] = mx; r[nw + w] = mn; }
+  __syncthreads();
+  mx = r[0];
+  mn = r[nw];
+  for (int i = 1; i < nw; ++i) { mx = nanmax(mx, r[i]); mn = nanmin(mn, r[nw + i]); }
+}
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  const int nw = (blockDim.x + WAVE - 1) / WAVE;
+  v = wave_sum(v);
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int i = 0; i < nw; ++i) s += red[i];  // fixed order: deterministic
+  return s;
+}
+
+// Sum of `n` doubles per thread-group (n <= 8) in one pass.
+template <int N>
+__device__ __forceinline__ void block_sum_n(double (&v)[N], double* red) {
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  const int nw = (blockDim.x + WAVE - 1) / WAVE;
+#pragma unroll
+  for (int k = 0; k < N; ++k) v[k] = wave_sum(v[k]);
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < N; ++k) red[k * nw + w] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    double s = 0.0;
+    for (int i = 0; i < nw; ++i) s += red[k * nw + i];
+    v[k] = s;
+  }
+}
+
+}  // namespace mp

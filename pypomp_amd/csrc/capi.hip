@@ -1,0 +1,483 @@
+// extern "C" entry points of libmipomp.so (declared in include/mipomp.h):
+// argument checks, workspace carving, launch sequencing.  No C++ exception
+// crosses the boundary; errors become status codes plus a per-thread message.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mipomp.h"
+#include "engine.hpp"
+
+namespace mp {
+hipError_t launch_bits(const uint32_t* keys, int64_t nkeys, int64_t n, void* out, int mode, hipStream_t s);
+hipError_t launch_resample_rows(const float* logw, const uint32_t* keys, int64_t R, int J, int32_t* anc,
+                                uint64_t* C, hipStream_t s);
+hipError_t launch_batched(int algo, const BatchArgs& a, int block, int grid, size_t lds, hipStream_t st);
+hipError_t launch_sched(int algo, const uint32_t* keys, const SeriesSet& ser, const int32_t* unit,
+                        int64_t sched_len, uint32_t* sched, int R, hipStream_t st);
+hipError_t plain_kernel(int which, const PlainArgs& a, int ntiles, hipStream_t s);
+hipError_t plain_reset(unsigned int* flag, int ntiles, PlainState* st, hipStream_t s);
+size_t batched_small_lds(int J, int block);
+}  // namespace mp
+
+using namespace mp;
+
+static_assert(sizeof(mp_model_desc) == sizeof(ModelDesc), "model desc layout");
+static_assert(sizeof(mp_series_desc) == sizeof(SeriesSet), "series desc layout");
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* where) {
+  return fail(MP_ERR_HIP, "%s: %s", where, hipGetErrorString(e));
+}
+
+#define MP_HIP(call)                                   \
+  do {                                                 \
+    hipError_t e_ = (call);                            \
+    if (e_ != hipSuccess) return hip_fail(e_, #call);  \
+  } while (0)
+
+constexpr int kBatchedBlock = 256;
+
+// Optional per-kernel timing with HIP events recorded on the launch stream
+// (mp_profile_*).  Kinds: 0 plain propagate, 1 plain scan, 2 batched filter,
+// 3 key schedule.
+struct ProfRec {
+  int kind;
+  hipEvent_t a, b;
+};
+std::mutex g_prof_mu;
+bool g_prof_on = false;
+std::vector<ProfRec> g_prof;
+std::vector<hipEvent_t> g_event_pool;
+
+hipEvent_t take_event() {
+  if (!g_event_pool.empty()) {
+    hipEvent_t e = g_event_pool.back();
+    g_event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Runs `launch` between two recorded events when profiling is on.
+template <class F>
+hipError_t timed(int kind, hipStream_t st, F&& launch) {
+  if (!g_prof_on) return launch();
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  ProfRec r{kind, take_event(), take_event()};
+  if (!r.a || !r.b) return launch();
+  hipError_t e = hipEventRecord(r.a, st);
+  if (e != hipSuccess) return e;
+  e = launch();
+  if (e != hipSuccess) return e;
+  e = hipEventRecord(r.b, st);
+  g_prof.push_back(r);
+  return e;
+}
+
+int max_lds_bytes() {
+  static int v = -1;
+  if (v < 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 65536;
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, dev) != hipSuccess) return 65536;
+    v = (int)p.sharedMemPerBlock;
+  }
+  return v;
+}
+
+int num_cus() {
+  static int v = -1;
+  if (v < 0) {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      n = 256;
+    v = n;
+  }
+  return v;
+}
+
+int check_model(const mp_model_desc* m) {
+  if (!m) return fail(MP_ERR_ARG, "model descriptor is NULL");
+  switch (m->kind) {
+    case MP_MODEL_LG:
+      if (m->dx < 1 || m->dx > 4 || m->dy < 1 || m->dy > 4)
+        return fail(MP_ERR_ARG, "LG dimensions must be 1..4 (dx=%d, dy=%d)", m->dx, m->dy);
+      if (m->S != m->dx) return fail(MP_ERR_ARG, "LG: S must equal dx");
+      break;
+    case MP_MODEL_SV:
+      if (m->S != 2 || m->d != 6 || m->dy != 1) return fail(MP_ERR_ARG, "SV model expects S=2, d=6, dy=1");
+      break;
+    case MP_MODEL_SEIR:
+      if (m->S != 5 || m->d != 10 || m->dy != 1) return fail(MP_ERR_ARG, "SEIR model expects S=5, d=10, dy=1");
+      break;
+    default:
+      return fail(MP_ERR_ARG, "unknown model kind %d", m->kind);
+  }
+  return MP_OK;
+}
+
+int check_series(const mp_series_desc* s) {
+  if (!s) return fail(MP_ERR_ARG, "series descriptor is NULL");
+  if (s->U < 1 || s->Tmax < 1) return fail(MP_ERR_ARG, "series needs U >= 1 and Tmax >= 1");
+  if (!s->ys || !s->times || !s->nstep || !s->tidx0 || !s->dt_ext || !s->obs_off || !s->dt_off)
+    return fail(MP_ERR_ARG, "series descriptor has a NULL array");
+  return MP_OK;
+}
+
+struct BatchedPlan {
+  int W;
+  int64_t sched_len, sched_bytes;
+  int64_t per_filter;  // global scratch per filter (0 when LDS-resident)
+  bool lds_resident;
+  size_t lds;
+  size_t total;
+};
+
+BatchedPlan plan_batched(int algo, const mp_model_desc* m, const mp_series_desc* s, int R, int J) {
+  BatchedPlan p{};
+  const SeriesSet& ser = *reinterpret_cast<const SeriesSet*>(s);
+  p.W = m->S + (algo == ALGO_PER ? m->d : 0);
+  p.sched_len = sched_stride(algo, ser);
+  p.sched_bytes = align256((int64_t)R * p.sched_len * 8);
+  const FilterScratch L = filter_scratch_layout(p.W, J);
+  const size_t small = batched_small_lds(J, kBatchedBlock);
+  // Keep filters LDS-resident when two workgroups still fit per CU.
+  p.lds_resident = (int64_t)small + L.total <= (int64_t)max_lds_bytes() / 2;
+  p.per_filter = p.lds_resident ? 0 : L.total;
+  p.lds = small + (p.lds_resident ? (size_t)L.total : 0);
+  p.total = (size_t)p.sched_bytes + (size_t)R * (size_t)p.per_filter + 256;
+  return p;
+}
+
+struct PlainPlan {
+  int ntiles, rows;
+  int64_t off_sched, off_x0, off_x1, off_lw, off_c, off_pmax, off_pmin, off_psum, off_agg, off_incl, off_flag,
+      off_st, total;
+};
+
+PlainPlan plan_plain(const mp_model_desc* m, const mp_series_desc* s, int J) {
+  PlainPlan p{};
+  p.rows = 8;
+  const int tile = p.rows * 256;
+  p.ntiles = (J + tile - 1) / tile;
+  const SeriesSet& ser = *reinterpret_cast<const SeriesSet*>(s);
+  int64_t o = 0;
+  auto take = [&](int64_t bytes) {
+    int64_t r = o;
+    o += align256(bytes);
+    return r;
+  };
+  p.off_sched = take(sched_stride(ALGO_PF, ser) * 8);
+  p.off_x0 = take((int64_t)m->S * J * 4);
+  p.off_x1 = take((int64_t)m->S * J * 4);
+  p.off_lw = take((int64_t)J * 4);
+  p.off_c = take((int64_t)J * 8);
+  p.off_pmax = take((int64_t)p.ntiles * 4);
+  p.off_pmin = take((int64_t)p.ntiles * 4);
+  p.off_psum = take((int64_t)p.ntiles * 8);
+  p.off_agg = take((int64_t)p.ntiles * 8);
+  p.off_incl = take((int64_t)p.ntiles * 8);
+  p.off_flag = take((int64_t)p.ntiles * 4);
+  p.off_st = take(sizeof(PlainState));
+  p.total = o + 256;
+  return p;
+}
+
+char* align_ptr(void* p) { return reinterpret_cast<char*>((reinterpret_cast<uintptr_t>(p) + 255) & ~uintptr_t(255)); }
+
+int run_batched(int algo, const mp_model_desc* m, const mp_series_desc* s, BatchArgs a, void* ws, size_t ws_bytes,
+                hipStream_t st) {
+  const BatchedPlan p = plan_batched(algo, m, s, a.R, a.J);
+  if (ws_bytes < p.total) return fail(MP_ERR_WORKSPACE, "workspace %zu < required %zu bytes", ws_bytes, p.total);
+  char* base = align_ptr(ws);
+  uint32_t* sched = reinterpret_cast<uint32_t*>(base);
+  a.sched = sched;
+  a.sched_len = p.sched_len;
+  a.ws = base + p.sched_bytes;
+  a.ws_stride = p.per_filter;
+  a.lds_resident = p.lds_resident;
+  MP_HIP(timed(3, st, [&] { return launch_sched(algo, a.keys, a.ser, a.unit, p.sched_len, sched, a.R, st); }));
+  const int per_cu = p.lds_resident ? 2 : 1;
+  int grid = a.R < num_cus() * per_cu * 4 ? a.R : num_cus() * per_cu * 4;
+  MP_HIP(timed(2, st, [&] { return launch_batched(algo, a, kBatchedBlock, grid, p.lds, st); }));
+  return MP_OK;
+}
+
+int run_plain_pf(const mp_model_desc* m, const mp_series_desc* s, const float* theta, int64_t theta_stride,
+                 const uint32_t* keys, int R, int J, float logthresh, int flags, double* loglik, float* cll,
+                 void* ws, size_t ws_bytes, hipStream_t st) {
+  if (s->U != 1) return fail(MP_ERR_ARG, "the plain form runs single-series filters (U = 1)");
+  const PlainPlan p = plan_plain(m, s, J);
+  if (ws_bytes < (size_t)p.total) return fail(MP_ERR_WORKSPACE, "workspace %zu < required %lld bytes", ws_bytes,
+                                              (long long)p.total);
+  char* b = align_ptr(ws);
+  PlainArgs a{};
+  a.md = *reinterpret_cast<const ModelDesc*>(m);
+  a.ser = *reinterpret_cast<const SeriesSet*>(s);
+  a.unit = 0;
+  a.J = J;
+  a.T = s->Tmax;
+  a.flags = flags;
+  a.logthresh = logthresh;
+  a.sched = reinterpret_cast<uint32_t*>(b + p.off_sched);
+  a.X0 = reinterpret_cast<float*>(b + p.off_x0);
+  a.X1 = reinterpret_cast<float*>(b + p.off_x1);
+  a.lw = reinterpret_cast<float*>(b + p.off_lw);
+  a.C = reinterpret_cast<uint64_t*>(b + p.off_c);
+  a.pmax = reinterpret_cast<float*>(b + p.off_pmax);
+  a.pmin = reinterpret_cast<float*>(b + p.off_pmin);
+  a.psum = reinterpret_cast<double*>(b + p.off_psum);
+  a.agg = reinterpret_cast<uint64_t*>(b + p.off_agg);
+  a.incl = reinterpret_cast<uint64_t*>(b + p.off_incl);
+  a.flag = reinterpret_cast<unsigned int*>(b + p.off_flag);
+  a.st = reinterpret_cast<PlainState*>(b + p.off_st);
+  a.rows = p.rows;
+  const int64_t slen = sched_stride(ALGO_PF, a.ser);
+  for (int f = 0; f < R; ++f) {
+    a.f = f;
+    a.theta = theta + f * theta_stride;
+    a.fkey = keys + 2 * f;
+    a.loglik_out = loglik + f;
+    a.cll = cll ? cll + (int64_t)f * s->Tmax : nullptr;
+    MP_HIP(launch_sched(ALGO_PF, keys + 2 * f, a.ser, nullptr, slen, const_cast<uint32_t*>(a.sched), 1, st));
+    MP_HIP(plain_reset(a.flag, p.ntiles, a.st, st));
+    for (int t = 0; t < a.T; ++t) {
+      a.t = t;
+      MP_HIP(timed(0, st, [&] { return plain_kernel(0, a, p.ntiles, st); }));
+      MP_HIP(timed(1, st, [&] { return plain_kernel(1, a, p.ntiles, st); }));
+    }
+  }
+  return MP_OK;
+}
+
+BatchArgs base_args(const mp_model_desc* m, const mp_series_desc* s, const uint32_t* keys, const int32_t* unit,
+                    int R, int J) {
+  BatchArgs a{};
+  a.md = *reinterpret_cast<const ModelDesc*>(m);
+  a.ser = *reinterpret_cast<const SeriesSet*>(s);
+  a.unit = unit;
+  a.keys = keys;
+  a.R = R;
+  a.J = J;
+  return a;
+}
+
+float log_thresh(float thresh) { return thresh > 0.f ? logf(thresh) : -__builtin_inff(); }
+
+}  // namespace
+
+extern "C" {
+
+const char* mp_last_error(void) { return g_err.c_str(); }
+int32_t mp_abi_version(void) { return 1; }
+
+int32_t mp_host_threefry2x32(const uint32_t* h_keys, const uint32_t* h_ctr, uint32_t* h_out, int64_t n) {
+  if (n < 0 || (n > 0 && (!h_keys || !h_ctr || !h_out))) return fail(MP_ERR_ARG, "bad threefry arguments");
+  for (int64_t i = 0; i < n; ++i) {
+    uint32_t x0 = h_ctr[2 * i], x1 = h_ctr[2 * i + 1];
+    threefry2x32(h_keys[2 * i], h_keys[2 * i + 1], x0, x1);
+    h_out[2 * i] = x0;
+    h_out[2 * i + 1] = x1;
+  }
+  return MP_OK;
+}
+
+int32_t mp_host_split(const uint32_t* h_key, int64_t n, uint32_t* h_out) {
+  if (n < 0 || !h_key || (n > 0 && !h_out)) return fail(MP_ERR_ARG, "bad split arguments");
+  const Key k{h_key[0], h_key[1]};
+  for (int64_t i = 0; i < n; ++i) {
+    Key s = split_at(k, (uint64_t)i);
+    h_out[2 * i] = s.k0;
+    h_out[2 * i + 1] = s.k1;
+  }
+  return MP_OK;
+}
+
+static int32_t bits_common(const uint32_t* keys, int64_t nkeys, int64_t n, void* out, int mode, void* stream) {
+  if (nkeys < 0 || n < 0) return fail(MP_ERR_ARG, "negative size");
+  if (nkeys == 0 || n == 0) return MP_OK;
+  if (!keys || !out) return fail(MP_ERR_ARG, "NULL pointer");
+  MP_HIP(launch_bits(keys, nkeys, n, out, mode, (hipStream_t)stream));
+  return MP_OK;
+}
+
+int32_t mp_random_bits(const uint32_t* keys, int64_t nkeys, int64_t n, uint32_t* out, void* stream) {
+  return bits_common(keys, nkeys, n, out, 0, stream);
+}
+int32_t mp_uniform(const uint32_t* keys, int64_t nkeys, int64_t n, float* out, void* stream) {
+  return bits_common(keys, nkeys, n, out, 1, stream);
+}
+int32_t mp_normal(const uint32_t* keys, int64_t nkeys, int64_t n, float* out, void* stream) {
+  return bits_common(keys, nkeys, n, out, 2, stream);
+}
+int32_t mp_split(const uint32_t* keys, int64_t nkeys, int64_t n, uint32_t* out, void* stream) {
+  return bits_common(keys, nkeys, n, out, 3, stream);
+}
+
+size_t mp_resample_workspace_size(int64_t R, int64_t J) {
+  if (R <= 0 || J <= 0) return 256;
+  return (size_t)(R * J * 8) + 256;
+}
+
+int32_t mp_resample_systematic(const float* logw, const uint32_t* keys, int64_t R, int64_t J, int32_t* ancestors,
+                               void* workspace, size_t ws_bytes, void* stream) {
+  if (R < 0 || J < 1 || J > (int64_t)1 << 30) return fail(MP_ERR_ARG, "need R >= 0 and 1 <= J <= 2^30");
+  if (R == 0) return MP_OK;
+  if (!logw || !keys || !ancestors || !workspace) return fail(MP_ERR_ARG, "NULL pointer");
+  if (J > 65536) return fail(MP_ERR_ARG, "mp_resample_systematic handles rows of at most 65536 weights");
+  if (ws_bytes < mp_resample_workspace_size(R, J)) return fail(MP_ERR_WORKSPACE, "workspace too small");
+  MP_HIP(launch_resample_rows(logw, keys, R, (int)J, ancestors, reinterpret_cast<uint64_t*>(align_ptr(workspace)),
+                              (hipStream_t)stream));
+  return MP_OK;
+}
+
+size_t mp_filter_workspace_size(int32_t algo, const mp_model_desc* h_model, const mp_series_desc* h_series,
+                                int32_t R, int32_t J, int32_t form) {
+  if (!h_model || !h_series || R < 1 || J < 1) return 0;
+  if (form == MP_FORM_PLAIN) return (size_t)plan_plain(h_model, h_series, J).total;
+  return plan_batched(algo, h_model, h_series, R, J).total;
+}
+
+static int check_common(const mp_model_desc* m, const mp_series_desc* s, int R, int J, const void* keys) {
+  int rc = check_model(m);
+  if (rc) return rc;
+  rc = check_series(s);
+  if (rc) return rc;
+  if (R < 1) return fail(MP_ERR_ARG, "need at least one filter (R = %d)", R);
+  if (J < 1) return fail(MP_ERR_ARG, "J should be greater than 0 (J = %d)", J);
+  if (!keys) return fail(MP_ERR_ARG, "keys is NULL");
+  return MP_OK;
+}
+
+int32_t mp_pfilter(const mp_model_desc* h_model, const mp_series_desc* h_series, const float* theta,
+                   int64_t theta_stride, const uint32_t* keys, const int32_t* unit, int32_t R, int32_t J, float thresh,
+                   int32_t flags, int32_t form, double* loglik, float* cll, float* ess, float* fmean, float* pmean,
+                   void* workspace, size_t ws_bytes, void* stream) {
+  int rc = check_common(h_model, h_series, R, J, keys);
+  if (rc) return rc;
+  if (!theta || !loglik || !workspace) return fail(MP_ERR_ARG, "NULL theta/loglik/workspace");
+  if ((flags & MP_CLL) && !cll) return fail(MP_ERR_ARG, "MP_CLL set but cll is NULL");
+  if ((flags & MP_ESS) && !ess) return fail(MP_ERR_ARG, "MP_ESS set but ess is NULL");
+  if ((flags & MP_FILTER_MEAN) && !fmean) return fail(MP_ERR_ARG, "MP_FILTER_MEAN set but fmean is NULL");
+  if ((flags & MP_PREDICTION_MEAN) && !pmean) return fail(MP_ERR_ARG, "MP_PREDICTION_MEAN set but pmean is NULL");
+  hipStream_t st = (hipStream_t)stream;
+  if (form == MP_FORM_PLAIN) {
+    if (flags & (MP_ESS | MP_FILTER_MEAN | MP_PREDICTION_MEAN))
+      return fail(MP_ERR_ARG, "the plain form reports logLik and CLL only");
+    if (unit) return fail(MP_ERR_ARG, "the plain form runs single-series filters");
+    return run_plain_pf(h_model, h_series, theta, theta_stride, keys, R, J, log_thresh(thresh), flags, loglik,
+                        (flags & MP_CLL) ? cll : nullptr, workspace, ws_bytes, st);
+  }
+  BatchArgs a = base_args(h_model, h_series, keys, unit, R, J);
+  a.theta = theta;
+  a.theta_stride = theta_stride;
+  a.flags = flags;
+  a.logthresh = log_thresh(thresh);
+  a.loglik = loglik;
+  a.cll = cll;
+  a.ess = ess;
+  a.fmean = fmean;
+  a.pmean = pmean;
+  return run_batched(ALGO_PF, h_model, h_series, a, workspace, ws_bytes, st);
+}
+
+int32_t mp_mop(const mp_model_desc* h_model, const mp_series_desc* h_series, const float* theta, int64_t theta_stride,
+               const uint32_t* keys, const int32_t* unit, int32_t R, int32_t J, float alpha, int32_t form,
+               double* loglik, void* workspace, size_t ws_bytes, void* stream) {
+  int rc = check_common(h_model, h_series, R, J, keys);
+  if (rc) return rc;
+  if (!theta || !loglik || !workspace) return fail(MP_ERR_ARG, "NULL theta/loglik/workspace");
+  hipStream_t st = (hipStream_t)stream;
+  if (form == MP_FORM_PLAIN) {
+    // one filter per launch of the fused kernel
+    for (int f = 0; f < R; ++f) {
+      BatchArgs a = base_args(h_model, h_series, keys + 2 * f, unit ? unit + f : nullptr, 1, J);
+      a.theta = theta + f * theta_stride;
+      a.alpha = alpha;
+      a.loglik = loglik + f;
+      rc = run_batched(ALGO_MOP, h_model, h_series, a, workspace, ws_bytes, st);
+      if (rc) return rc;
+    }
+    return MP_OK;
+  }
+  BatchArgs a = base_args(h_model, h_series, keys, unit, R, J);
+  a.theta = theta;
+  a.theta_stride = theta_stride;
+  a.alpha = alpha;
+  a.loglik = loglik;
+  return run_batched(ALGO_MOP, h_model, h_series, a, workspace, ws_bytes, st);
+}
+
+int32_t mp_perfilter(const mp_model_desc* h_model, const mp_series_desc* h_series, const float* thetas_in,
+                     const uint32_t* keys, const int32_t* unit, int32_t R, int32_t J, float thresh, const float* sigmas,
+                     const float* sigmas_init, float cool, int32_t m, float* thetas_out, float* theta_mean,
+                     double* loglik, void* workspace, size_t ws_bytes, void* stream) {
+  int rc = check_common(h_model, h_series, R, J, keys);
+  if (rc) return rc;
+  if (!thetas_in || !thetas_out || !theta_mean || !loglik || !sigmas || !sigmas_init || !workspace)
+    return fail(MP_ERR_ARG, "NULL argument to mp_perfilter");
+  if (thetas_in == thetas_out) return fail(MP_ERR_ARG, "thetas_in and thetas_out must not alias");
+  BatchArgs a = base_args(h_model, h_series, keys, unit, R, J);
+  a.theta = thetas_in;
+  a.logthresh = log_thresh(thresh);
+  a.sigmas = sigmas;
+  a.sigmas_init = sigmas_init;
+  a.cool_factor = cool;
+  a.mif_m = m;
+  a.theta_out = thetas_out;
+  a.theta_mean = theta_mean;
+  a.loglik = loglik;
+  return run_batched(ALGO_PER, h_model, h_series, a, workspace, ws_bytes, (hipStream_t)stream);
+}
+
+int32_t mp_profile_enable(int32_t on) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  g_prof_on = on != 0;
+  return MP_OK;
+}
+
+int32_t mp_profile_read(double* h_ms, int64_t* h_count, int32_t nkinds) {
+  if (!h_ms || !h_count || nkinds < 1) return fail(MP_ERR_ARG, "bad profile buffers");
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  for (int k = 0; k < nkinds; ++k) {
+    h_ms[k] = 0.0;
+    h_count[k] = 0;
+  }
+  for (const ProfRec& r : g_prof) {
+    MP_HIP(hipEventSynchronize(r.b));
+    float ms = 0.f;
+    MP_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+    if (r.kind < nkinds) {
+      h_ms[r.kind] += ms;
+      h_count[r.kind] += 1;
+    }
+    g_event_pool.push_back(r.a);
+    g_event_pool.push_back(r.b);
+  }
+  g_prof.clear();
+  return MP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,132 @@
+// Internal launch descriptions shared by the kernel translation units and the
+// C-ABI layer.  Plain data only; every pointer is device memory unless noted.
+#pragma once
+
+#include "models.hpp"
+
+namespace mp {
+
+enum Algo : int { ALGO_PF = 0, ALGO_MOP = 1, ALGO_PER = 2 };
+
+// A set of observation series (one per panel unit; U = 1 for a single Pomp).
+// Unit u owns rows [obs_off[u], obs_off[u+1]) of ys/times/nstep/tidx0 and the
+// Euler step sizes starting at dt_ext + dt_off[u]; tidx0 is the unit-local
+// index of the first Euler step of each observation interval.
+struct SeriesSet {
+  const float* ys;       // (sum T_u, dy)
+  const float* times;    // (sum T_u)
+  const int32_t* nstep;  // (sum T_u)
+  const int32_t* tidx0;  // (sum T_u)
+  const float* dt_ext;   // (sum_u sum nstep)
+  const int64_t* obs_off;  // (U + 1)
+  const int64_t* dt_off;   // (U)
+  float t0;
+  int U;
+  int Tmax;
+  int64_t max_euler;  // max over units of the number of Euler steps
+};
+
+// Key-schedule words (keys) per filter: PF/MOP sub-step keys + one resampling
+// key per observation; IF2 adds a perturbation key per observation and two
+// initial keys.
+__host__ __device__ inline int64_t sched_stride(int algo, const SeriesSet& s) {
+  return algo == ALGO_PER ? 2 + s.max_euler + 2 * (int64_t)s.Tmax : s.max_euler + (int64_t)s.Tmax;
+}
+
+enum Flags : int {
+  F_CLL = 1,
+  F_ESS = 2,
+  F_FMEAN = 4,
+  F_PMEAN = 8,
+  F_SHOULD_TRANS = 16,  // theta is on the estimation scale (apply from_est)
+};
+
+struct BatchArgs {
+  ModelDesc md;
+  SeriesSet ser;
+  const int32_t* unit;      // (R) unit of each filter, or null (unit 0)
+  const float* theta;       // PF/MOP: (R, d) or shared (stride 0); PER: (R, J, d) est. scale
+  int64_t theta_stride;     // floats between filters' thetas (PF/MOP)
+  const uint32_t* keys;     // (R, 2) filter keys
+  const uint32_t* sched;    // key schedules, filter f at sched + 2 * f * sched_len
+  int64_t sched_len;        // keys per filter schedule
+  int R, J;
+  int flags;
+  float logthresh;          // log(thresh); -inf when thresh == 0
+  float alpha;              // MOP
+  // IF2
+  const float* sigmas;      // (d) non-initial sigmas
+  const float* sigmas_init; // (d)
+  float cool_factor;        // geometric cooling factor a^(1/50) (float32 of the double)
+  int mif_m;                // iteration index m
+  float* theta_out;         // (R, J, d) swarm after the final resample (est. scale)
+  float* theta_mean;        // (R, d) mean of theta_out
+  // outputs
+  double* loglik;           // (R)
+  float* cll;               // (R, Tmax)
+  float* ess;               // (R, Tmax)
+  float* fmean;             // (R, Tmax, S)
+  float* pmean;             // (R, Tmax, S)
+  // per-filter scratch (global) when not LDS-resident
+  char* ws;
+  int64_t ws_stride;
+  int lds_resident;
+};
+
+// Per-filter scratch layout (bytes), shared by host sizing and device use.
+struct FilterScratch {
+  int64_t off_x0, off_x1, off_lw, off_c, total;
+};
+
+__host__ __device__ inline int64_t align256(int64_t v) { return (v + 255) & ~int64_t(255); }
+
+__host__ __device__ inline FilterScratch filter_scratch_layout(int W, int J) {
+  FilterScratch s;
+  int64_t xs = (int64_t)W * J * 4;
+  s.off_x0 = 0;
+  s.off_x1 = align256(xs);
+  s.off_lw = s.off_x1 + align256(xs);
+  s.off_c = s.off_lw + align256((int64_t)J * 4);
+  s.total = s.off_c + align256((int64_t)J * 8);
+  return s;
+}
+
+struct PlainState {
+  float M, mn;         // max / min log-weight of the current step
+  float llf;           // float32 log-likelihood term of the previous step
+  float ures;          // comb uniform of the previous step's resampling
+  uint64_t total;      // sum of fixed-point weights
+  double loglik;       // running FP64 sum over t
+  int have_anc;        // previous step resampled
+  int uniform_w;       // weights are the constant wconst
+  float wconst;
+  int collapsed;       // previous step had all weights -inf
+  unsigned int ticket_prop, ticket_scan, tile_ctr, pad;
+};
+
+struct PlainArgs {
+  ModelDesc md;
+  SeriesSet ser;
+  int unit;
+  const float* theta;   // (d)
+  const uint32_t* fkey; // (2) filter key
+  const uint32_t* sched;
+  int J, t, T, flags, f;
+  float logthresh;
+  float* X0;
+  float* X1;
+  float* lw;
+  uint64_t* C;
+  float* pmax;          // (ntiles)
+  float* pmin;
+  double* psum;
+  uint64_t* agg;        // (ntiles) look-back aggregates
+  uint64_t* incl;       // (ntiles) look-back inclusive prefixes
+  unsigned int* flag;   // (ntiles) epoch<<2 | state
+  PlainState* st;
+  double* loglik_out;   // (1) written after the last step
+  float* cll;           // (T) or null
+  int rows;             // particle rows per tile (tile = rows * blockDim)
+};
+
+}  // namespace mp

@@ -1,0 +1,212 @@
+// Device model functors: the native form of the reference's rinit / rproc /
+// dmeas plugins (pypomp/core/model_mechanics.py:361-697).  Each model draws its
+// random numbers from the particle key it is handed, in the order documented
+// on the model, so that the stream matches the reference's per-particle
+// vmapped plugin (model_mechanics.py:584-588).
+#pragma once
+
+#include "common.hpp"
+
+namespace mp {
+
+enum ModelKind : int { MODEL_LG = 1, MODEL_SEIR = 2, MODEL_SV = 3 };
+
+// Host-filled description of a model instance (plain data, passed by value).
+struct ModelDesc {
+  int kind;
+  int S;            // state words per particle
+  int d;            // parameters
+  int dy;           // observation words
+  int dx;           // LG: state dimension
+  int lg_sep_init;  // LG: separate initial Cholesky factor P follows X0
+  uint32_t accum_mask;
+};
+
+__device__ __forceinline__ int tri(int d) { return d * (d + 1) / 2; }
+
+// ---------------------------------------------------------------------------
+// Linear Gaussian (pypomp/models/linear_gaussian.py:139-191), dx, dy <= MD.
+// theta = A (dx*dx, row-major) | C (dy*dx) | Q lower-tri Cholesky entries |
+//         R lower-tri Cholesky entries | X0 (dx) | [P lower-tri]  (natural scale).
+// rinit: x = X0 + L0 z, z = normal(key, (dx,))  (L0 = L_Q unless P is given)
+// step:  x = A x + L_Q z, z = normal(key, (dx,))
+// dmeas: log N(y; C x, L_R L_R^T), evaluated as the reference does through the
+//        Cholesky factor of the covariance (jax.scipy.stats.multivariate_normal).
+// Estimation scale: log of the diagonal Q and R entries.
+// All loops run to the compile-time bound MD with a guard; with EXACT the
+// dimensions are the constant MD, so every array index folds to a constant.
+// ---------------------------------------------------------------------------
+#define MP_UNROLL _Pragma("unroll")
+
+template <int MD, bool EXACT>
+struct LGT {
+  static constexpr int MAXS = MD;
+  static constexpr bool FIXED_S = EXACT;  // S == MAXS for every instance
+  static constexpr int MAXP = 2 * MD * MD + 3 * (MD * (MD + 1) / 2) + MD;
+  struct P {
+    float A[MD][MD], C[MD][MD], LQ[MD][MD], L0[MD][MD], LC[MD][MD];
+    float X0[MD];
+    float logdet, half_n_log2pi;
+    int dx, dy;
+  };
+
+  __device__ static void from_est(const ModelDesc& md, float* th) {
+    const int dx = EXACT ? MD : md.dx, dy = EXACT ? MD : md.dy;
+    // diagonal entries of the Q and R triangles
+    MP_UNROLL for (int i = 0; i < MD; ++i) if (i < dx) {
+      const int o = dx * dx + dy * dx + i * (i + 1) / 2 + i;
+      th[o] = expf(th[o]);
+    }
+    MP_UNROLL for (int i = 0; i < MD; ++i) if (i < dy) {
+      const int o = dx * dx + dy * dx + dx * (dx + 1) / 2 + i * (i + 1) / 2 + i;
+      th[o] = expf(th[o]);
+    }
+  }
+
+  __device__ static void prepare(const ModelDesc& md, const float* th, P& p) {
+    const int dx = EXACT ? MD : md.dx, dy = EXACT ? MD : md.dy;
+    p.dx = dx;
+    p.dy = dy;
+    MP_UNROLL for (int i = 0; i < MD; ++i) MP_UNROLL for (int j = 0; j < MD; ++j) {
+      p.A[i][j] = p.C[i][j] = p.LQ[i][j] = p.L0[i][j] = p.LC[i][j] = 0.f;
+    }
+    const int oC = dx * dx, oQ = oC + dy * dx, oR = oQ + dx * (dx + 1) / 2, oX = oR + dy * (dy + 1) / 2,
+              oP = oX + dx;
+    float LR[MD][MD];
+    MP_UNROLL for (int i = 0; i < MD; ++i) MP_UNROLL for (int j = 0; j < MD; ++j) {
+      if (i < dx && j < dx) p.A[i][j] = th[i * dx + j];
+      if (i < dy && j < dx) p.C[i][j] = th[oC + i * dx + j];
+      p.LQ[i][j] = (i < dx && j <= i) ? th[oQ + i * (i + 1) / 2 + j] : 0.f;
+      LR[i][j] = (i < dy && j <= i) ? th[oR + i * (i + 1) / 2 + j] : 0.f;
+      if (md.lg_sep_init) p.L0[i][j] = (i < dx && j <= i) ? th[oP + i * (i + 1) / 2 + j] : 0.f;
+      else p.L0[i][j] = p.LQ[i][j];
+    }
+    MP_UNROLL for (int i = 0; i < MD; ++i) p.X0[i] = i < dx ? th[oX + i] : 0.f;
+    // Cholesky factor of cov = LR LR^T (float32) and its log determinant.
+    float cov[MD][MD];
+    MP_UNROLL for (int i = 0; i < MD; ++i) MP_UNROLL for (int j = 0; j < MD; ++j) {
+      float s = 0.f;
+      MP_UNROLL for (int k = 0; k < MD; ++k) s += LR[i][k] * LR[j][k];
+      cov[i][j] = s;
+    }
+    float logdet = 0.f;
+    MP_UNROLL for (int j = 0; j < MD; ++j) if (j < dy) {
+      float s = cov[j][j];
+      MP_UNROLL for (int k = 0; k < MD; ++k) if (k < j) s -= p.LC[j][k] * p.LC[j][k];
+      const float djj = sqrtf(s);
+      p.LC[j][j] = djj;
+      logdet += logf(djj);
+      MP_UNROLL for (int i = 0; i < MD; ++i) if (i > j && i < dy) {
+        float t = cov[i][j];
+        MP_UNROLL for (int k = 0; k < MD; ++k) if (k < j) t -= p.LC[i][k] * p.LC[j][k];
+        p.LC[i][j] = t / djj;
+      }
+    }
+    p.logdet = logdet;
+    p.half_n_log2pi = 0.5f * (float)dy * 1.83787707f;
+  }
+
+  __device__ static void rinit(const ModelDesc&, const P& p, Key k, float, float* x) {
+    float z[MD];
+    MP_UNROLL for (int i = 0; i < MD; ++i) z[i] = i < p.dx ? normal_at(k, i) : 0.f;
+    MP_UNROLL for (int i = 0; i < MD; ++i) if (i < p.dx) {
+      float s = p.L0[i][0] * z[0];
+      MP_UNROLL for (int j = 1; j < MD; ++j) if (j < p.dx) s += p.L0[i][j] * z[j];
+      x[i] = p.X0[i] + s;
+    }
+  }
+
+  __device__ static void step(const ModelDesc&, const P& p, Key k, float, float, float* x) {
+    float z[MD], xn[MD];
+    MP_UNROLL for (int i = 0; i < MD; ++i) z[i] = i < p.dx ? normal_at(k, i) : 0.f;
+    MP_UNROLL for (int i = 0; i < MD; ++i) {
+      float a = p.A[i][0] * x[0];
+      MP_UNROLL for (int j = 1; j < MD; ++j) if (j < p.dx) a += p.A[i][j] * x[j];
+      float s = p.LQ[i][0] * z[0];
+      MP_UNROLL for (int j = 1; j < MD; ++j) if (j < p.dx) s += p.LQ[i][j] * z[j];
+      xn[i] = a + s;
+    }
+    MP_UNROLL for (int i = 0; i < MD; ++i) if (i < p.dx) x[i] = xn[i];
+  }
+
+  __device__ static float dmeas(const ModelDesc&, const P& p, const float* y, const float* x, float) {
+    float r[MD], s[MD];
+    MP_UNROLL for (int i = 0; i < MD; ++i) {
+      float m = p.C[i][0] * x[0];
+      MP_UNROLL for (int j = 1; j < MD; ++j) if (j < p.dx) m += p.C[i][j] * x[j];
+      r[i] = i < p.dy ? y[i] - m : 0.f;
+    }
+    float quad = 0.f;
+    MP_UNROLL for (int i = 0; i < MD; ++i) if (i < p.dy) {
+      float a = r[i];
+      MP_UNROLL for (int k = 0; k < MD; ++k) if (k < i) a -= p.LC[i][k] * s[k];
+      s[i] = a / p.LC[i][i];
+      quad += s[i] * s[i];
+    }
+    return -0.5f * quad - p.half_n_log2pi - p.logdet;
+  }
+};
+
+// dx = dy = 1 and dx = dy = 2 get fully constant indexing; other shapes up to
+// 4 use the generic instance with run-time dimensions.
+using LG1 = LGT<1, true>;
+using LG2 = LGT<2, true>;
+using LG4 = LGT<4, false>;
+
+// ---------------------------------------------------------------------------
+// Stochastic volatility with leverage (the MOP workload).  State (v, eta):
+//   v_0  = V0 + S0 z,                 eta_0 = 0,         z = normal(key, (1,))
+//   v_t  = mu + phi (v_{t-1} - mu) + sigma eta_t,   eta_t = normal(key, (1,))
+//   r_t | v_t, eta_t ~ N(exp(v_t/2) rho eta_t, exp(v_t) (1 - rho^2))
+// i.e. corr(eta_t, eps_t) = rho for r_t = exp(v_t/2) eps_t.
+// theta = (mu, phi, sigma, rho, V0, S0); estimation scale: phi = tanh, sigma = exp,
+// rho = tanh, S0 = exp, mu and V0 unchanged.
+// ---------------------------------------------------------------------------
+struct SV {
+  static constexpr int MAXS = 2;
+  static constexpr bool FIXED_S = true;
+  static constexpr int MAXP = 6;
+  struct P {
+    float mu, phi, sigma, rho, V0, S0;
+    float c0;       // -0.5 log(2 pi) - 0.5 log(1 - rho^2)
+    float inv1mr2;  // 1 / (1 - rho^2)
+  };
+
+  __device__ static void from_est(const ModelDesc&, float* th) {
+    th[1] = tanhf(th[1]);
+    th[2] = expf(th[2]);
+    th[3] = tanhf(th[3]);
+    th[5] = expf(th[5]);
+  }
+
+  __device__ static void prepare(const ModelDesc&, const float* th, P& p) {
+    p.mu = th[0];
+    p.phi = th[1];
+    p.sigma = th[2];
+    p.rho = th[3];
+    p.V0 = th[4];
+    p.S0 = th[5];
+    float omr2 = 1.0f - p.rho * p.rho;
+    p.inv1mr2 = 1.0f / omr2;
+    p.c0 = -0.918938533f - 0.5f * logf(omr2);
+  }
+
+  __device__ static void rinit(const ModelDesc&, const P& p, Key k, float, float* x) {
+    float z = normal_at(k, 0);
+    x[0] = p.V0 + p.S0 * z;
+    x[1] = 0.0f;
+  }
+
+  __device__ static void step(const ModelDesc&, const P& p, Key k, float, float, float* x) {
+    float z = normal_at(k, 0);
+    x[0] = p.mu + p.phi * (x[0] - p.mu) + p.sigma * z;
+    x[1] = z;
+  }
+
+  __device__ static float dmeas(const ModelDesc&, const P& p, const float* y, const float* x, float) {
+    float u = y[0] * expf(-0.5f * x[0]) - p.rho * x[1];
+    return p.c0 - 0.5f * x[0] - 0.5f * u * u * p.inv1mr2;
+  }
+};
+
+}  // namespace mp

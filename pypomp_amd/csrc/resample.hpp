@@ -1,0 +1,65 @@
+// Workgroup-level systematic resampling: fixed-point prefix sums of the
+// weights and the comb search (order-independent definition in common.hpp).
+#pragma once
+
+#include "blockops.hpp"
+#include "engine.hpp"
+
+namespace mp {
+
+constexpr int kRedBytes = 4096;  // up to 128 doubles x 4 waves of reductions
+
+// LDS bytes of the scan table of block_fixed_scan: rows * waves + 1 entries.
+__host__ __device__ inline int64_t batched_tab_bytes(int J, int block) {
+  int64_t rows = (J + block - 1) / block, nw = (block + 63) / 64;
+  return align256((rows * nw + 1) * 8);
+}
+
+// Inclusive prefix sums C[j] of fixed-point weights over a workgroup.  Rows of
+// blockDim particles: wave scans in registers, one table of wave totals in LDS,
+// one wave scans the table.  Returns the total.
+__device__ inline uint64_t block_fixed_scan(const float* lw, uint64_t* C, int J, float m, uint64_t* tab) {
+  const int B = blockDim.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int nw = (B + 63) >> 6;
+  const int rows = (J + B - 1) / B;
+  const int sh = shift_for(J);
+  for (int r = 0; r < rows; ++r) {
+    const int j = r * B + tid;
+    uint64_t e = j < J ? particle_fixed_weight(lw[j], m, sh) : 0;
+    uint64_t incl = wave_incl_scan_u64(e);
+    if (j < J) C[j] = incl;
+    if (lane == 63) tab[r * nw + w] = incl;
+  }
+  __syncthreads();
+  const int n = rows * nw;
+  if (w == 0) {
+    const int per = (n + 63) / 64;
+    const int b0 = lane * per, b1 = min(n, b0 + per);
+    uint64_t s = 0;
+    for (int i = b0; i < b1; ++i) s += tab[i];
+    uint64_t incl = wave_incl_scan_u64(s);
+    uint64_t run = incl - s;
+    for (int i = b0; i < b1; ++i) {
+      uint64_t v = tab[i];
+      tab[i] = run;
+      run += v;
+    }
+    if (lane == 63) tab[n] = incl;
+  }
+  __syncthreads();
+  for (int r = 0; r < rows; ++r) {
+    const int j = r * B + tid;
+    if (j < J) C[j] += tab[r * nw + w];
+  }
+  uint64_t total = tab[n];
+  __syncthreads();
+  return total;
+}
+
+__device__ __forceinline__ int ancestor_of(const uint64_t* C, int J, float u, uint64_t total, int j) {
+  uint64_t tj = comb_threshold(u, j, J, total);
+  int64_t a = upper_bound_u64(C, 0, J, tj);
+  return a < J ? (int)a : J - 1;
+}
+
+}  // namespace mp

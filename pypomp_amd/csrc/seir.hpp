@@ -1,0 +1,202 @@
+// Stochastic SEIR with Euler-binomial transitions (the epidemic workload),
+// written in the style of the reference's SIR model (pypomp/models/sir.py:171-249)
+// and its binomial sampler (pypomp/random/binom.py:130-522).
+//
+// State words (float32 holding exact integer counts < 2^24): S, E, I, R, H,
+// where H (weekly incidence, accumulator reset at each observation) counts
+// E -> I transitions.  Parameters (natural scale, canonical order):
+//   beta, amp, sigma, gamma, rho, k, N, S0f, E0, I0
+// Estimation scale: log for beta, sigma, gamma, k, N, E0, I0; logit for amp, rho, S0f.
+// Sub-step (rproc) with particle key `key`:
+//   (kS, kE, kI) = split(key, 3)
+//   beta_t = beta (1 + amp cos(2 pi t / 52)),   foi = beta_t I / N
+//   dSE ~ Bin(S, 1 - exp(-foi dt)), dEI ~ Bin(E, 1 - exp(-sigma dt)),
+//   dIR ~ Bin(I, 1 - exp(-gamma dt)), each drawn by fast_binomial from one
+//   uniform of its key (binom.py:211-233).
+// rinit: S = round(S0f N), E = round(E0), I = round(I0), R = N - S - E - I, H = 0.
+// dmeas: y ~ NegBin(mean mu = max(rho H, 1e-10), size k)  (sir.py:236-249 form).
+#pragma once
+
+#include "models.hpp"
+
+namespace mp {
+
+// Normal quantile for u in (0, 1): sqrt(2) * erfinv(2u - 1) with Giles'
+// single-precision erfinv, its argument w = -log((1-x)(1+x)) formed directly
+// as -log(4 u (1 - u)) so no precision is lost near the tails.
+__device__ __forceinline__ float ndtri_f32(float u) {
+  float w = -logf(4.0f * u * (1.0f - u));
+  float x = 2.0f * u - 1.0f;
+  float p;
+  if (w < 5.0f) {
+    w = w - 2.5f;
+    p = 2.81022636e-08f;
+    p = 3.43273939e-07f + p * w;
+    p = -3.5233877e-06f + p * w;
+    p = -4.39150654e-06f + p * w;
+    p = 0.00021858087f + p * w;
+    p = -0.00125372503f + p * w;
+    p = -0.00417768164f + p * w;
+    p = 0.246640727f + p * w;
+    p = 1.50140941f + p * w;
+  } else {
+    w = sqrtf(w) - 3.0f;
+    p = -0.000200214257f;
+    p = 0.000100950558f + p * w;
+    p = 0.00134934322f + p * w;
+    p = -0.00367342844f + p * w;
+    p = 0.00573950773f + p * w;
+    p = -0.0076224613f + p * w;
+    p = 0.00943887047f + p * w;
+    p = 1.00167406f + p * w;
+    p = 2.83297682f + p * w;
+  }
+  return 1.41421354f * (p * x);
+}
+
+// Giles & Beentjes (2024) approximate inverse binomial CDF, order 2, with the
+// exact bottom-up stage of at most 5 terms (binom.py:236-451).  Returns the
+// count as a float; NaN for invalid arguments.
+__device__ __forceinline__ float binominv_f32(float u, float n, float p) {
+  const float tiny = 1.17549435e-38f, eps = 1.1920929e-07f;
+  const bool invalid = (n < 0.f) || (p < 0.f) || (p > 1.f) || (u < 0.f) || (u > 1.f);
+  const bool flip = p > 0.5f;
+  float ps = flip ? 1.0f - p : p;
+  const float uf = flip ? 1.0f - u : u;
+  const float ns = (n < 0.f) ? 1.0f : n;
+  ps = fminf(fmaxf(ps, 0.f), 1.f);
+  const float us = fminf(fmaxf(uf, eps), 1.0f - eps);
+  const float q = 1.0f - ps;
+  const float w = ndtri_f32(us);
+  const float w2 = w * w;
+  const float np_ = ns * ps;
+  const float npq = np_ * q;
+  const float sq = sqrtf(fmaxf(npq, tiny));
+  const float pq = ps * q;
+  float q0 = np_ + sq * w + (2.0f + 2.0f * ps + (q - ps) * w2) / 6.0f;
+  float w3 = w2 * w;
+  float q1 = q0 + ((-2.0f + 14.0f * pq) * w + (-1.0f - 2.0f * pq) * w3) / fmaxf(72.0f * sq, tiny);
+  float w4 = w3 * w;
+  float q2 = q1 + ((ps - q) * (2.0f + pq) * (16.0f - 7.0f * w2 - 3.0f * w4)) / fmaxf(1620.0f * (sq * sq), tiny);
+  float k = fminf(fmaxf(floorf(q2), 0.f), ns);
+  k = fminf(k, np_ + 6.0f * sq + 5.0f);
+  if (k < 10.0f || np_ < 4.0f) {
+    // bottom-up exact CDF (binom.py:396-451)
+    const float ue = fminf(fmaxf(uf, 0.f), 1.f);
+    const float qc = fminf(fmaxf(1.0f - ps, tiny), 1.0f);
+    const float ratio = ps / qc;
+    const float logq = log1pf(-ps);
+    float pmf = (ns == 0.f) ? 1.0f : expf(ns * logq);
+    float cdf = pmf;
+    bool found = cdf >= ue;
+    float res = found ? 0.f : k;
+#pragma unroll
+    for (int i = 1; i < 5; ++i) {
+      float num = fmaxf(ns - (float)(i - 1), 0.f);
+      pmf = pmf * (num / (float)i) * ratio;
+      cdf = cdf + pmf;
+      bool stalled = (cdf > 1.0f - eps * 10.0f) && (pmf < eps);
+      bool fnow = (cdf >= ue) || stalled;
+      if (!found && fnow) res = (float)i;
+      found = found || fnow;
+    }
+    k = fminf(fmaxf(res, 0.f), ns);
+  }
+  float r = flip ? ns - k : k;
+  if (n == 0.f) r = 0.f;
+  if (u == 0.f) r = 0.f;
+  if (u == 1.f) r = ns;
+  if (p == 0.f) r = 0.f;
+  if (p == 1.f) r = ns;
+  r = fminf(fmaxf(r, 0.f), ns);
+  if (invalid) r = __builtin_nanf("");
+  return r;
+}
+
+// fast_binomial(key, n, p) with a scalar draw (binom.py:211-233), int result.
+__device__ __forceinline__ int fast_binomial_i(Key k, int n, float p) {
+  float u = uniform01(bits_at(k, 0));
+  u = fminf(fmaxf(u, 1.17549435e-38f), 0.99999994f);
+  float r = binominv_f32(u, (float)n, p);
+  return r != r ? -1 : (int)r;
+}
+
+struct SEIR {
+  static constexpr int MAXS = 5;
+  static constexpr bool FIXED_S = true;
+  static constexpr int MAXP = 10;
+  struct P {
+    float beta, amp, sigma, gamma, rho, k, N, S0f, E0, I0;
+    float invN, lgk;
+  };
+
+  __device__ static float logit_inv(float x) { return 1.0f / (1.0f + expf(-x)); }
+
+  __device__ static void from_est(const ModelDesc&, float* th) {
+    th[0] = expf(th[0]);
+    th[1] = logit_inv(th[1]);
+    th[2] = expf(th[2]);
+    th[3] = expf(th[3]);
+    th[4] = logit_inv(th[4]);
+    th[5] = expf(th[5]);
+    th[6] = expf(th[6]);
+    th[7] = logit_inv(th[7]);
+    th[8] = expf(th[8]);
+    th[9] = expf(th[9]);
+  }
+
+  __device__ static void prepare(const ModelDesc&, const float* th, P& p) {
+    p.beta = th[0];
+    p.amp = th[1];
+    p.sigma = th[2];
+    p.gamma = th[3];
+    p.rho = th[4];
+    p.k = th[5];
+    p.N = th[6];
+    p.S0f = th[7];
+    p.E0 = th[8];
+    p.I0 = th[9];
+    p.invN = 1.0f / p.N;
+    p.lgk = lgammaf(p.k);
+  }
+
+  __device__ static void rinit(const ModelDesc&, const P& p, Key, float, float* x) {
+    float S = rintf(p.S0f * p.N), E = rintf(p.E0), I = rintf(p.I0);
+    x[0] = S;
+    x[1] = E;
+    x[2] = I;
+    x[3] = p.N - S - E - I;
+    x[4] = 0.f;
+  }
+
+  __device__ static void step(const ModelDesc&, const P& p, Key key, float t, float dt, float* x) {
+    int S = (int)x[0], E = (int)x[1], I = (int)x[2], R = (int)x[3], H = (int)x[4];
+    const float bt = p.beta * (1.0f + p.amp * cosf(6.28318531f * t / 52.0f));
+    const float foi = bt * (float)I * p.invN;
+    const float pSE = -expm1f(-foi * dt);
+    const float pEI = -expm1f(-p.sigma * dt);
+    const float pIR = -expm1f(-p.gamma * dt);
+    const int dSE = fast_binomial_i(split_at(key, 0), S, pSE);
+    const int dEI = fast_binomial_i(split_at(key, 1), E, pEI);
+    const int dIR = fast_binomial_i(split_at(key, 2), I, pIR);
+    S -= dSE;
+    E += dSE - dEI;
+    I += dEI - dIR;
+    R += dIR;
+    H += dEI;
+    x[0] = (float)S;
+    x[1] = (float)E;
+    x[2] = (float)I;
+    x[3] = (float)R;
+    x[4] = (float)H;
+  }
+
+  __device__ static float dmeas(const ModelDesc&, const P& p, const float* y, const float* x, float) {
+    const float mu = fmaxf(x[4] * p.rho, 1e-10f);
+    const float yy = rintf(y[0]);
+    const float k = p.k;
+    return lgammaf(yy + k) - p.lgk - lgammaf(yy + 1.0f) + k * logf(k / (k + mu)) + yy * logf(mu / (k + mu));
+  }
+};
+
+}  // namespace mp

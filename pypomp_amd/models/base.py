@@ -1,0 +1,91 @@
+"""Native model specifications: the host half of each device model functor.
+
+A model carries its parameter/state/observation names, its estimation-scale
+transforms (host mirror of the device ``from_est``, float32) and the C
+descriptor handed to the library.  The plugin slots of ``Pomp``
+(rinit/rproc/dmeas/rmeas, pypomp/core/model_mechanics.py:361-760) are filled
+with ``Component`` handles of one native model.
+"""
+
+import numpy as np
+
+from .. import _native
+
+F32 = np.float32
+
+
+class Component:
+    """Handle for one plugin slot (rinit / rproc / dmeas / rmeas) of a native model."""
+
+    def __init__(self, model, role):
+        self.model = model
+        self.role = role
+
+    def __repr__(self):
+        return f"<{type(self.model).__name__}.{self.role}>"
+
+
+class NativeModel:
+    kind = 0
+    S = 0
+    dy = 1
+    dx = 0
+    lg_sep_init = 0
+    accumvars = ()
+
+    param_names: list = []
+    statenames: list = []
+    obsnames: list = []
+
+    @property
+    def d(self):
+        return len(self.param_names)
+
+    @property
+    def rinit(self):
+        return Component(self, "rinit")
+
+    @property
+    def rproc(self):
+        return Component(self, "rproc")
+
+    @property
+    def dmeas(self):
+        return Component(self, "dmeas")
+
+    @property
+    def rmeas(self):
+        return Component(self, "rmeas")
+
+    def desc(self):
+        mask = 0
+        for a in self.accumvars:
+            mask |= 1 << int(a)
+        return _native.ModelDesc(kind=self.kind, S=self.S, d=self.d, dy=self.dy, dx=self.dx,
+                                 lg_sep_init=self.lg_sep_init, accum_mask=mask)
+
+    # host mirrors of the device transforms (float32)
+    def to_est(self, theta):
+        return np.asarray(theta, dtype=F32).copy()
+
+    def from_est(self, theta):
+        return np.asarray(theta, dtype=F32).copy()
+
+    def theta_array(self, theta):
+        """dict (or sequence of dicts) of natural-scale values -> (n, d) float32."""
+        if isinstance(theta, dict):
+            theta = [theta]
+        if isinstance(theta, (list, tuple)) and theta and isinstance(theta[0], dict):
+            missing = set(self.param_names) - set(theta[0])
+            if missing:
+                raise ValueError(f"theta is missing parameters {sorted(missing)}")
+            extra = set(theta[0]) - set(self.param_names)
+            if extra:
+                raise ValueError(f"theta has unknown parameters {sorted(extra)}")
+            return np.array([[float(t[n]) for n in self.param_names] for t in theta], dtype=F32)
+        arr = np.asarray(theta, dtype=F32)
+        if arr.ndim == 1:
+            arr = arr[None, :]
+        if arr.shape[-1] != self.d:
+            raise ValueError(f"theta has {arr.shape[-1]} entries, model needs {self.d}")
+        return arr

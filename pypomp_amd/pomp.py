@@ -1,0 +1,193 @@
+"""``Pomp``: the object API of the particle-filter path (pypomp/core/pomp.py,
+core/estimation_mixin.py:101-444, core/analysis_mixin.py:22-70).
+
+Same constructor slots (ys, theta, t0, rinit/rproc/dmeas/rmeas, nstep or dt,
+accumvars) and the same key handling (``fresh_key``, core/pomp.py:391-408).
+The process/measurement plugins must be the components of one native model
+(``pypomp_amd.models``); a plain Python callable has no device form and is
+rejected at construction.
+"""
+
+import copy
+import time
+
+import numpy as np
+import pandas as pd
+
+from . import functional as F
+from . import random as prandom
+from .maths import logmeanexp, logmeanexp_se
+from .models.base import Component, NativeModel
+from .rw_sigma import RWSigma
+from .series import Series
+
+
+def _resolve_model(model, rinit, rproc, dmeas, rmeas):
+    comps = [c for c in (rinit, rproc, dmeas, rmeas) if c is not None]
+    for c in comps:
+        if not isinstance(c, Component):
+            raise TypeError(
+                f"{c!r} is a Python callable; the GPU engine runs native device models only "
+                "(use the components of a pypomp_amd.models model, e.g. LGModel(2).rproc)")
+    models = {id(c.model): c.model for c in comps}
+    if model is not None:
+        models[id(model)] = model
+    if len(models) != 1:
+        raise ValueError("rinit/rproc/dmeas must all belong to one native model")
+    m = next(iter(models.values()))
+    if not isinstance(m, NativeModel):
+        raise TypeError("model must be a pypomp_amd native model")
+    return m
+
+
+class Pomp:
+    def __init__(self, ys, theta, t0, rinit=None, rproc=None, dmeas=None, rmeas=None, times=None,
+                 nstep=None, dt=None, model=None, accumvars=None, covars=None, key=None):
+        if covars is not None:
+            raise NotImplementedError("covariates are not supported by the native models")
+        self.model = _resolve_model(model, rinit, rproc, dmeas, rmeas)
+        if accumvars is not None and tuple(accumvars) != tuple(self.model.accumvars):
+            raise ValueError("accumvars are fixed by the native model: "
+                             f"{tuple(self.model.accumvars)}")
+        if isinstance(ys, pd.DataFrame):
+            times = np.asarray(ys.index, dtype=float) if times is None else times
+            ys = ys.to_numpy(dtype=np.float32)
+        ys = np.asarray(ys, dtype=np.float32)
+        if times is None:
+            times = np.arange(1, ys.shape[0] + 1, dtype=float)
+        if nstep is None and dt is None:
+            raise ValueError("Either dt or nstep must be provided")
+        self.series = Series.build(ys, times, t0, nstep=nstep, dt=dt)
+        self.t0 = float(t0)
+        self.theta = self._theta_list(theta)
+        self.fresh_key = key
+        self.results_history = []
+
+    # -- parameters -------------------------------------------------------
+    @property
+    def canonical_param_names(self):
+        return list(self.model.param_names)
+
+    @property
+    def statenames(self):
+        return list(self.model.statenames)
+
+    @property
+    def ys(self):
+        return pd.DataFrame(self.series.ys, index=pd.Index(self.series.times.astype(float), name="time"),
+                            columns=self.model.obsnames)
+
+    def _theta_list(self, theta):
+        if isinstance(theta, dict):
+            theta = [theta]
+        theta = [dict(t) for t in theta]
+        self.model.theta_array(theta)  # validates names
+        return theta
+
+    def _theta_array(self, theta=None):
+        return self.model.theta_array(self.theta if theta is None else self._theta_list(theta))
+
+    def to_struct(self):
+        return F.PompStruct(model=self.model, units=[self.series])
+
+    def _update_fresh_key(self, key=None):
+        old = self.fresh_key if key is None else np.asarray(key, dtype=np.uint32)
+        if old is None:
+            raise ValueError("Both the key argument and the fresh_key attribute are None. "
+                             "At least one key must be given.")
+        ks = prandom.split(old)
+        self.fresh_key, new = ks[0], ks[1]
+        return new, old
+
+    # -- algorithms ---------------------------------------------------------
+    def pfilter(self, J, key=None, theta=None, thresh=0.0, reps=1, CLL=False, ESS=False,
+                filter_mean=False, prediction_mean=False, track_time=True, form="batched"):
+        """Bootstrap particle filter (estimation_mixin.py:101-272)."""
+        start = time.time()
+        thresh = float(max(0.0, thresh))
+        if J < 1:
+            raise ValueError("J should be greater than 0.")
+        thetas = self._theta_array(theta)
+        new, old = self._update_fresh_key(key)
+        n = thetas.shape[0]
+        rep_keys = prandom.split(new, n * reps).reshape(n, reps, 2)
+        res = F.pfilter(self.to_struct(), thetas, J, rep_keys, thresh, CLL, ESS, filter_mean,
+                        prediction_mean, form=form)
+        logliks = res["logLik"]
+        theta_used = self.theta if theta is None else self._theta_list(theta)
+        est = logmeanexp(logliks, axis=-1)
+        self.theta = [dict(t) for t in theta_used]
+        self.results_history.append({
+            "method": "pfilter", "execution_time": time.time() - start if track_time else None,
+            "key": old, "theta": theta_used, "logLiks": logliks, "logLik": np.atleast_1d(est),
+            "J": J, "reps": reps, "thresh": thresh,
+            **{k: res[k] for k in ("CLL", "ESS", "filter_mean", "prediction_mean") if k in res}})
+
+    def mif(self, J, M, rw_sd, key=None, theta=None, thresh=0.0, n_monitors=0, track_time=True):
+        """IF2 (estimation_mixin.py:274-444)."""
+        start = time.time()
+        if set(rw_sd.param_names) != set(self.canonical_param_names):
+            raise ValueError("rw_sd.sigmas keys must match canonical_param_names up to reordering. "
+                             f"Got {sorted(rw_sd.param_names)}, expected {sorted(self.canonical_param_names)}.")
+        if J < 1:
+            raise ValueError("J should be greater than 0.")
+        thetas = self._theta_array(theta)
+        theta_used = self.theta if theta is None else self._theta_list(theta)
+        new, old = self._update_fresh_key(key)
+        n = thetas.shape[0]
+        keys = prandom.split(new, n)
+        swarm = np.repeat(thetas[:, None, :], J, axis=1)
+        lls, traces, _ = F.mif(self.to_struct(), swarm, rw_sd, M, J, keys, thresh, n_monitors)
+        names = self.canonical_param_names
+        self.theta = [dict(zip(names, map(float, traces[i, M]))) for i in range(n)]
+        lls_nan = np.concatenate([np.full((n, 1), np.nan), lls], axis=1)
+        self.results_history.append({
+            "method": "mif", "execution_time": time.time() - start if track_time else None,
+            "key": old, "theta": theta_used, "traces": np.concatenate([lls_nan[:, :, None], traces], axis=-1),
+            "J": J, "M": M, "rw_sd": rw_sd, "thresh": thresh, "n_monitors": n_monitors})
+
+    def mop(self, J, alpha, key=None, theta=None):
+        """MOP(alpha) log-likelihood estimate at natural-scale theta (value only)."""
+        thetas = self._theta_array(theta)
+        new, _ = self._update_fresh_key(key)
+        keys = prandom.split(new, thetas.shape[0])
+        return -F.mop(self.to_struct(), self.model.to_est(thetas), J, alpha, keys)
+
+    # -- results frames (analysis_mixin.py:22-70; results/render.py) -------
+    def results(self, index=-1, ignore_nan=False):
+        r = self.results_history[index]
+        names = self.canonical_param_names
+        if r["method"] == "pfilter":
+            arr = np.asarray(r["logLiks"], dtype=float)
+            ll = np.atleast_1d(logmeanexp(arr, axis=-1, ignore_nan=ignore_nan))
+            se = (np.atleast_1d(logmeanexp_se(arr, axis=-1, ignore_nan=ignore_nan)) if arr.shape[-1] > 1
+                  else np.full_like(ll, np.nan))
+            df = pd.DataFrame({"theta_idx": np.arange(len(ll)), "logLik": ll, "se": se})
+            return pd.concat([df, pd.DataFrame(r["theta"])[names]], axis=1)
+        tr = r["traces"][:, -1, :]
+        df = pd.DataFrame(tr, columns=["logLik"] + names)
+        df.insert(0, "theta_idx", np.arange(tr.shape[0]))
+        df.insert(2, "se", np.nan)
+        return df
+
+    def traces(self):
+        frames = []
+        names = self.canonical_param_names
+        for r in self.results_history:
+            if r["method"] == "pfilter":
+                df = self.results(self.results_history.index(r))
+                df.insert(1, "iteration", 0)
+                df.insert(2, "method", "pfilter")
+            else:
+                tr = r["traces"]
+                n, M1, _ = tr.shape
+                df = pd.DataFrame(tr.reshape(n * M1, -1), columns=["logLik"] + names)
+                df.insert(0, "theta_idx", np.repeat(np.arange(n), M1))
+                df.insert(1, "iteration", np.tile(np.arange(M1), n))
+                df.insert(2, "method", r["method"])
+                df.insert(4, "se", np.nan)
+            frames.append(df)
+        return pd.concat(frames, ignore_index=True) if frames else pd.DataFrame()
+
+    def copy(self):
+        return copy.deepcopy(self)

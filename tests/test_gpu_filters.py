@@ -1,0 +1,162 @@
+"""GPU parity of the filters: the reference's regression baselines, the oracle
+restatement and the exact Kalman likelihood."""
+
+import numpy as np
+import pytest
+
+from oracle import filters as fl
+from oracle import kalman, lg as olg, threefry as tf
+from tests.helpers import assert_close_tol, lg1_data, lg1_theta, workloads
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+
+
+def lg1_struct(seed=0):
+    import pypomp_amd.functional as F
+    from pypomp_amd.models import LGModel
+    from pypomp_amd.series import Series
+    return F.PompStruct(model=LGModel(1, 1), units=[Series.build(lg1_data(seed), np.arange(1, 5.0), 0.0, nstep=1)])
+
+
+def lg2_struct(seed):
+    import pypomp_amd.functional as F
+    from pypomp_amd.models import LGModel
+    from pypomp_amd.series import Series
+    w = workloads()
+    ys = w[f"lg2_seed{seed}_ys"]
+    return (F.PompStruct(model=LGModel(2, 2, separate_init=True),
+                         units=[Series.build(ys, np.arange(1, ys.shape[0] + 1.0), 0.0, nstep=1)]),
+            w[f"lg2_seed{seed}_theta"], ys)
+
+
+@pytest.mark.parametrize("form", ["batched", "plain"])
+def test_pfilter_regression(golden, form):
+    import pypomp_amd.functional as F
+    keys = tf.split(tf.key(20260811), 1).reshape(1, 1, 2)
+    out = F.pfilter(lg1_struct(), lg1_theta()[None], 12, keys, thresh=0.0, form=form)
+    assert out["logLik"].dtype == np.float32
+    assert_close_tol(out["logLik"], golden["test_pfilter_regression"]["logLik"])
+
+
+def test_pfilter_diagnostics_regression(golden):
+    import pypomp_amd.functional as F
+    g = golden["test_pfilter_diagnostics_regression"]
+    thetas = np.stack([lg1_theta(), (lg1_theta() * F32(1.05)).astype(F32)])
+    keys = tf.split(tf.key(20260812), 2).reshape(2, 1, 2)
+    out = F.pfilter(lg1_struct(), thetas, 12, keys, CLL=True, ESS=True, filter_mean=True, prediction_mean=True)
+    assert_close_tol(out["logLik"].ravel(), g["logLik"])
+    for name in ("CLL", "ESS", "filter_mean", "prediction_mean"):
+        flat = out[name].reshape(2, -1)
+        for r in range(2):
+            for t in range(4):
+                assert_close_tol(flat[r, t], g[f"{name}_rep{r}_t{t}"])
+
+
+def test_mop_regression(golden):
+    import pypomp_amd.functional as F
+    v = F.mop(lg1_struct(), lg1_theta()[None], 12, 0.5, tf.split(tf.key(20260811), 1))
+    assert_close_tol(v, golden["test_mop_regression"]["mop"])
+
+
+def test_mif_regression(golden):
+    import pypomp_amd as pp
+    import pypomp_amd.functional as F
+    names = ["A11", "C11", "Q11", "R11", "X0_1"]
+    rw = pp.RWSigma({n: 0.02 for n in names}).geometric_cooling(0.5)
+    th = np.repeat(lg1_theta()[None, None, :], 12, axis=1)
+    lls, tr, _ = F.mif(lg1_struct(), th, rw, 3, 12, tf.split(tf.key(20260811), 1))
+    g = golden["test_mif_regression"]
+    assert_close_tol(lls[:, -1], g["final_loglik"])
+    assert_close_tol(tr[:, -1, :], g["final_theta"])
+
+
+def test_mif_full_trace_regression(golden):
+    import pypomp_amd as pp
+    import pypomp_amd.functional as F
+    names = ["A11", "C11", "Q11", "R11", "X0_1"]
+    g = golden["test_mif_full_trace_regression"]
+    rw = pp.RWSigma({n: 0.02 for n in names}).geometric_cooling(0.5)
+    thetas = np.stack([lg1_theta(), (lg1_theta() * F32(1.05)).astype(F32)])
+    th = np.repeat(thetas[:, None, :], 12, axis=1)
+    lls, tr, _ = F.mif(lg1_struct(), th, rw, 4, 12, tf.split(tf.key(20260812), 2), n_monitors=1)
+    for r in range(2):
+        for m in range(4):
+            assert_close_tol(lls[r, m], g[f"loglik_rep{r}_m{m}"])
+        for m in range(5):
+            for p, name in enumerate(names):
+                assert_close_tol(tr[r, m, p], g[f"{name}_rep{r}_m{m}"])
+
+
+def test_panel_pfilter_units_regression(golden):
+    import pypomp_amd.functional as F
+    keys = tf.split(tf.key(20260811), 2)
+    got = [F.pfilter(lg1_struct(s), lg1_theta()[None], 12, keys[u].reshape(1, 1, 2))["logLik"][0, 0]
+           for u, s in enumerate((0, 1))]
+    assert_close_tol(got, golden["test_panel_pfilter_regression"]["logLik"])
+
+
+@pytest.mark.parametrize("form", ["batched", "plain"])
+def test_lg2_matches_oracle(form):
+    import pypomp_amd.functional as F
+    struct, theta, ys = lg2_struct(1)
+    keys = tf.split(tf.key(1234), 3)
+    out = F.pfilter(struct, theta[None], 1000, keys.reshape(1, 3, 2), CLL=True, form=form)
+    model = olg.LGModel(olg.LGSpec(2, 2, separate_init=True))
+    ser = fl.make_series(ys, np.arange(1, ys.shape[0] + 1.0), 0.0, nstep=1)
+    for r in range(3):
+        o = fl.pfilter(model, theta, ser, keys[r], 1000, diagnostics=True)
+        ll = float(out["logLik64"][0, r])
+        assert abs(ll - float(o["logLik"])) <= 1e-5 * abs(float(o["logLik"])), (r, ll, o["logLik"])
+        np.testing.assert_allclose(out["CLL"][0, r], np.array(o["CLL"]), rtol=1e-4, atol=2e-4)
+
+
+def test_plain_and_batched_choose_the_same_ancestors():
+    import pypomp_amd.functional as F
+    struct, theta, _ = lg2_struct(1)
+    keys = tf.split(tf.key(99), 4).reshape(1, 4, 2)
+    a = F.pfilter(struct, theta[None], 5000, keys, CLL=True, form="batched")
+    b = F.pfilter(struct, theta[None], 5000, keys, CLL=True, form="plain")
+    # identical resampling decisions; only the FP64 sum order of exp(w) differs
+    np.testing.assert_allclose(a["logLik64"], b["logLik64"], rtol=1e-9)
+    np.testing.assert_allclose(a["CLL"], b["CLL"], rtol=1e-5, atol=1e-5)
+
+
+def test_lg2_against_kalman():
+    import pypomp_amd.functional as F
+    struct, theta, ys = lg2_struct(1)
+    A = np.array([[np.cos(0.2), -np.sin(0.2)], [np.sin(0.2), np.cos(0.2)]])
+    spec = olg.LGSpec(2, 2, separate_init=True)
+    _, C, LQ, LR, X0, L0 = spec.unpack(theta)
+    exact = kalman.kalman_loglik(ys, A, C, LQ.astype(float) @ LQ.T.astype(float),
+                                 LR.astype(float) @ LR.T.astype(float), X0.astype(float),
+                                 L0.astype(float) @ L0.T.astype(float))
+    reps = 64
+    out = F.pfilter(struct, theta[None], 1000, tf.split(tf.key(77), reps).reshape(1, reps, 2))
+    ll = out["logLik64"].ravel()
+    se = ll.std(ddof=1) / np.sqrt(reps)
+    # the particle estimate of the likelihood is unbiased; its log has a small negative bias
+    assert abs(ll.mean() - exact) < 4 * se + 0.05, (ll.mean(), exact, se)
+
+
+def test_thresh_skips_resampling_like_oracle():
+    import pypomp_amd.functional as F
+    struct, theta, ys = lg2_struct(1)
+    keys = tf.split(tf.key(5), 2)
+    out = F.pfilter(struct, theta[None], 400, keys.reshape(1, 2, 2), thresh=0.5, ESS=True)
+    model = olg.LGModel(olg.LGSpec(2, 2, separate_init=True))
+    ser = fl.make_series(ys, np.arange(1, ys.shape[0] + 1.0), 0.0, nstep=1)
+    for r in range(2):
+        o = fl.pfilter(model, theta, ser, keys[r], 400, thresh=0.5, diagnostics=True)
+        assert abs(float(out["logLik64"][0, r]) - float(o["logLik"])) <= 1e-5 * abs(float(o["logLik"]))
+        np.testing.assert_allclose(out["ESS"][0, r], np.array(o["ESS"]), rtol=1e-3)
+
+
+def test_j_equals_one_and_workspace_errors():
+    import pypomp_amd.functional as F
+    from pypomp_amd import _native
+    struct, theta, _ = lg2_struct(1)
+    out = F.pfilter(struct, theta[None], 1, tf.split(tf.key(3), 1).reshape(1, 1, 2))
+    assert np.isfinite(out["logLik64"]).all()
+    with pytest.raises(ValueError):
+        F.pfilter(struct, theta[None], 0, tf.split(tf.key(3), 1).reshape(1, 1, 2))
