@@ -131,6 +131,20 @@ int32_t mp_mop(const mp_model_desc* h_model, const mp_series_desc* h_series, con
                int64_t theta_stride, const uint32_t* keys, const int32_t* unit, int32_t R, int32_t J,
                float alpha, int32_t form, double* loglik, void* workspace, size_t ws_bytes, void* stream);
 
+/* MOP(alpha) value and its gradient d loglik / d theta_est by a hand-written
+ * adjoint: the reverse-mode derivative of _mop_internal (mop.py:19-135) that
+ * jax.grad computes through _mop_internal_mean (ad_helpers.py:6-8; the
+ * reference's gradient is -grad / T), ancestry and random numbers held fixed.
+ * The forward keeps a checkpoint every K observations; the reverse sweep
+ * recomputes each K-step block from it.  One Euler step per observation,
+ * U = 1.  loglik (R) float64, grad (R, d) float32.  form MP_FORM_PLAIN runs the
+ * filters one launch at a time. */
+size_t mp_mop_grad_workspace_size(const mp_model_desc* h_model, const mp_series_desc* h_series, int32_t R,
+                                  int32_t J, int32_t K);
+int32_t mp_mop_grad(const mp_model_desc* h_model, const mp_series_desc* h_series, const float* theta,
+                    int64_t theta_stride, const uint32_t* keys, int32_t R, int32_t J, float alpha, int32_t K,
+                    int32_t form, double* loglik, float* grad, void* workspace, size_t ws_bytes, void* stream);
+
 /* One IF2 iteration m: _perfilter_internal (core/algorithms/mif.py:88-258) for R
  * swarms.  thetas_in/thetas_out (R, J, d) on the estimation scale; sigmas and
  * sigmas_init (d) as RWSigma.sigmas_array / sigmas_init_array

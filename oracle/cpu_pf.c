@@ -1,0 +1,396 @@
+/*
+ * cpu_pf.c -- C restatement of the particle-filter hot path for the CPU
+ * baseline.  TEST INFRASTRUCTURE ONLY (built by oracle/Makefile into
+ * oracle/_ref/libcpupf.so, loaded by tests and by bench.py's cpu_baseline leg).
+ *
+ * Restates, like the NumPy oracle (oracle/filters.py):
+ *  - the JAX threefry key tree (split / random_bits / uniform / normal),
+ *  - the bootstrap filter _pfilter_internal / _pfilter_step
+ *    (pypomp/core/algorithms/pfilter.py:15-182) with thresh = 0,
+ *  - the MOP(alpha) value _mop_internal (core/algorithms/mop.py:19-135),
+ *  - the systematic resampler with fixed-point prefix sums (oracle/resample.py),
+ * for the linear-Gaussian model (dx = dy <= 4, separate initial factor) and the
+ * stochastic-volatility model.  float32 state arithmetic, float64 log-likelihood
+ * sum.  OpenMP runs one replicate filter per thread, as the reference's CPU
+ * sharding does (core/algorithms/helpers.py:494-527, 650-708).
+ * Must be compiled with -ffp-contract=off (the resampler's exp uses separate
+ * multiplies and adds).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef struct {
+  uint32_t k0, k1;
+} key_t2;
+
+static inline uint32_t rotl(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
+
+static void threefry(uint32_t k0, uint32_t k1, uint32_t* x0p, uint32_t* x1p) {
+  static const int R[8] = {13, 15, 26, 6, 17, 29, 16, 24};
+  uint32_t ks[3] = {k0, k1, k0 ^ k1 ^ 0x1BD11BDAu};
+  uint32_t x0 = *x0p + ks[0], x1 = *x1p + ks[1];
+  for (int blk = 0; blk < 5; ++blk) {
+    for (int r = 0; r < 4; ++r) {
+      x0 += x1;
+      x1 = rotl(x1, R[(blk & 1) * 4 + r]);
+      x1 ^= x0;
+    }
+    x0 += ks[(blk + 1) % 3];
+    x1 += ks[(blk + 2) % 3] + (uint32_t)(blk + 1);
+  }
+  *x0p = x0;
+  *x1p = x1;
+}
+
+static inline key_t2 split_at(key_t2 k, uint64_t i) {
+  uint32_t a = (uint32_t)(i >> 32), b = (uint32_t)i;
+  threefry(k.k0, k.k1, &a, &b);
+  key_t2 o = {a, b};
+  return o;
+}
+
+static inline uint32_t bits_at(key_t2 k, uint64_t i) {
+  uint32_t a = (uint32_t)(i >> 32), b = (uint32_t)i;
+  threefry(k.k0, k.k1, &a, &b);
+  return a ^ b;
+}
+
+static inline float unit_float(uint32_t bits) {
+  union {
+    uint32_t u;
+    float f;
+  } c;
+  c.u = (bits >> 9) | 0x3F800000u;
+  return c.f - 1.0f;
+}
+
+static inline float uniform01(uint32_t bits) {
+  float f = unit_float(bits);
+  return f > 0.f ? f : 0.f;
+}
+
+static inline float erfinv_f32(float x) {
+  float w = -log1pf(-(x * x)), p;
+  if (w < 5.0f) {
+    w = w - 2.5f;
+    p = 2.81022636e-08f;
+    p = 3.43273939e-07f + p * w;
+    p = -3.5233877e-06f + p * w;
+    p = -4.39150654e-06f + p * w;
+    p = 0.00021858087f + p * w;
+    p = -0.00125372503f + p * w;
+    p = -0.00417768164f + p * w;
+    p = 0.246640727f + p * w;
+    p = 1.50140941f + p * w;
+  } else {
+    w = sqrtf(w) - 3.0f;
+    p = -0.000200214257f;
+    p = 0.000100950558f + p * w;
+    p = 0.00134934322f + p * w;
+    p = -0.00367342844f + p * w;
+    p = 0.00573950773f + p * w;
+    p = -0.0076224613f + p * w;
+    p = 0.00943887047f + p * w;
+    p = 1.00167406f + p * w;
+    p = 2.83297682f + p * w;
+  }
+  float r = p * x;
+  if (fabsf(x) == 1.0f) r = x * 3.40282347e+38f;
+  return r;
+}
+
+static inline float normal_at(key_t2 k, uint64_t i) {
+  const float lo = -0.99999994f;
+  float u = unit_float(bits_at(k, i)) * 2.0f + lo;
+  if (!(u > lo)) u = lo;
+  return 1.41421354f * erfinv_f32(u);
+}
+
+/* ---- fixed-point systematic resampler (oracle/resample.py) ---- */
+static const double EXPC[14] = {0x1p+0, 0x1p+0, 0x1p-1, 0x1.5555555555555p-3, 0x1.5555555555555p-5,
+                                0x1.1111111111111p-7, 0x1.6c16c16c16c17p-10, 0x1.a01a01a01a01ap-13,
+                                0x1.a01a01a01a01ap-16, 0x1.71de3a556c734p-19, 0x1.27e4fb7789f5cp-22,
+                                0x1.ae64567f544e4p-26, 0x1.1eed8eff8d898p-29, 0x1.6124613a86d09p-33};
+
+static int shift_for(int64_t J) {
+  int b = 0;
+  uint64_t v = (uint64_t)J;
+  while (v) {
+    ++b;
+    v >>= 1;
+  }
+  return 63 - b;
+}
+
+static uint64_t fixed_weight(double d, int sh) {
+  const double cutoff = -(double)(sh + 2) * 0.6931471805599453;
+  if (!(d >= cutoff)) return 0;
+  double k = rint(d * 1.4426950408889634);
+  double r = d - k * 6.93147180369123816490e-01;
+  r = r - k * 1.90821492927058770002e-10;
+  double p = EXPC[13];
+  for (int n = 12; n >= 0; --n) p = p * r + EXPC[n];
+  return (uint64_t)ldexp(p, (int)k + sh);
+}
+
+static uint64_t comb_threshold(float u, int64_t j, int64_t J, uint64_t total) {
+  float f = (u + (float)j) / (float)J;
+  union {
+    float f;
+    uint32_t u;
+  } c;
+  c.f = f;
+  uint32_t e = (c.u >> 23) & 0xffu;
+  if (e == 0) return 0;
+  uint64_t mant = (uint64_t)((c.u & 0x7fffffu) | 0x800000u);
+  int sh = 150 - (int)e;
+  uint64_t p_lo = mant * (total & 0xffffffffull), p_hi = mant * (total >> 32);
+  uint64_t X = p_hi + (p_lo >> 32), lo32 = p_lo & 0xffffffffull;
+  if (sh >= 32) {
+    int s = sh - 32;
+    return s >= 64 ? 0 : (X >> s);
+  }
+  return (X << (32 - sh)) + (lo32 >> sh);
+}
+
+/* ancestors of J particles from log-weights lw (max m) and uniform u */
+static void resample(const float* lw, float m, int64_t J, float u, uint64_t* C, int32_t* anc) {
+  const int sh = shift_for(J);
+  uint64_t run = 0;
+  for (int64_t i = 0; i < J; ++i) {
+    uint64_t e;
+    if (m == -INFINITY) e = 1;
+    else if (m == INFINITY) e = lw[i] == INFINITY;
+    else e = fixed_weight((double)lw[i] - (double)m, sh);
+    run += e;
+    C[i] = run;
+  }
+  const uint64_t total = run;
+  int64_t i = 0;
+  for (int64_t j = 0; j < J; ++j) { /* merge: thresholds are non-decreasing in j */
+    uint64_t t = comb_threshold(u, j, J, total);
+    while (i < J && C[i] <= t) ++i;
+    anc[j] = (int32_t)(i < J ? i : J - 1);
+  }
+}
+
+/* ---- models ---- */
+typedef struct {
+  int dx;
+  float A[4][4], LQ[4][4], L0[4][4], C[4][4], LC[4][4], X0[4], logdet;
+} lg_t;
+
+static void lg_prepare(const float* th, int dx, lg_t* p) {
+  memset(p, 0, sizeof *p);
+  p->dx = dx;
+  int o = 0;
+  for (int i = 0; i < dx; ++i)
+    for (int j = 0; j < dx; ++j) p->A[i][j] = th[o++];
+  for (int i = 0; i < dx; ++i)
+    for (int j = 0; j < dx; ++j) p->C[i][j] = th[o++];
+  for (int i = 0; i < dx; ++i)
+    for (int j = 0; j <= i; ++j) p->LQ[i][j] = th[o++];
+  float LR[4][4] = {{0}};
+  for (int i = 0; i < dx; ++i)
+    for (int j = 0; j <= i; ++j) LR[i][j] = th[o++];
+  for (int i = 0; i < dx; ++i) p->X0[i] = th[o++];
+  for (int i = 0; i < dx; ++i)
+    for (int j = 0; j <= i; ++j) p->L0[i][j] = th[o++];
+  float cov[4][4];
+  for (int i = 0; i < dx; ++i)
+    for (int j = 0; j < dx; ++j) {
+      float s = 0.f;
+      for (int k = 0; k < dx; ++k) s += LR[i][k] * LR[j][k];
+      cov[i][j] = s;
+    }
+  float ld = 0.f;
+  for (int j = 0; j < dx; ++j) {
+    float s = cov[j][j];
+    for (int k = 0; k < j; ++k) s -= p->LC[j][k] * p->LC[j][k];
+    float d = sqrtf(s);
+    p->LC[j][j] = d;
+    ld += logf(d);
+    for (int i = j + 1; i < dx; ++i) {
+      float t = cov[i][j];
+      for (int k = 0; k < j; ++k) t -= p->LC[i][k] * p->LC[j][k];
+      p->LC[i][j] = t / d;
+    }
+  }
+  p->logdet = ld;
+}
+
+/* Bootstrap filter of the LG model with separate initial factor (thresh = 0):
+ * returns the FP64 log-likelihood; X (dx*J) and scratch provided by the caller. */
+static double lg_pfilter_one(const float* th, int dx, const float* ys, int T, int64_t J, key_t2 key, float* X,
+                             float* Xn, float* lw, uint64_t* C, int32_t* anc) {
+  lg_t p;
+  lg_prepare(th, dx, &p);
+  key_t2 run = split_at(key, 0);
+  for (int64_t j = 0; j < J; ++j) {
+    key_t2 pk = split_at(key, 1 + (uint64_t)j);
+    float z[4];
+    for (int i = 0; i < dx; ++i) z[i] = normal_at(pk, i);
+    for (int i = 0; i < dx; ++i) {
+      float s = p.L0[i][0] * z[0];
+      for (int k = 1; k < dx; ++k) s += p.L0[i][k] * z[k];
+      X[i * J + j] = p.X0[i] + s;
+    }
+  }
+  const float logJ = logf((float)J);
+  float wconst = logf(1.0f / (float)J);
+  double ll = 0.0;
+  const float hl2p = 0.5f * (float)dx * 1.83787707f;
+  for (int t = 0; t < T; ++t) {
+    key_t2 sub = split_at(run, 1);
+    run = split_at(run, 0);
+    key_t2 sk = split_at(sub, 1);
+    const float* y = ys + (int64_t)t * dx;
+    float m = -INFINITY;
+    for (int64_t j = 0; j < J; ++j) {
+      int64_t s0 = t == 0 ? j : anc[j];
+      float x[4], z[4], xn[4];
+      for (int i = 0; i < dx; ++i) x[i] = X[i * J + s0];
+      key_t2 pk = split_at(sk, (uint64_t)j);
+      for (int i = 0; i < dx; ++i) z[i] = normal_at(pk, i);
+      for (int i = 0; i < dx; ++i) {
+        float a = p.A[i][0] * x[0], s = p.LQ[i][0] * z[0];
+        for (int k = 1; k < dx; ++k) {
+          a += p.A[i][k] * x[k];
+          s += p.LQ[i][k] * z[k];
+        }
+        xn[i] = a + s;
+      }
+      float r[4], sv[4], quad = 0.f;
+      for (int i = 0; i < dx; ++i) {
+        float mm = p.C[i][0] * xn[0];
+        for (int k = 1; k < dx; ++k) mm += p.C[i][k] * xn[k];
+        r[i] = y[i] - mm;
+      }
+      for (int i = 0; i < dx; ++i) {
+        float a2 = r[i];
+        for (int k = 0; k < i; ++k) a2 -= p.LC[i][k] * sv[k];
+        sv[i] = a2 / p.LC[i][i];
+        quad += sv[i] * sv[i];
+      }
+      const float w = wconst + (-0.5f * quad - hl2p - p.logdet);
+      for (int i = 0; i < dx; ++i) Xn[i * J + j] = xn[i];
+      lw[j] = w;
+      if (w > m) m = w;
+    }
+    double se = 0.0;
+    for (int64_t j = 0; j < J; ++j) se += (double)expf(lw[j] - m);
+    ll += (double)m + log(se);
+    key_t2 rk = split_at(run, 1);
+    run = split_at(run, 0);
+    if (t < T - 1) resample(lw, m, J, uniform01(bits_at(rk, 0)), C, anc);
+    wconst = 0.0f - logJ;
+    float* tmp = X;
+    X = Xn;
+    Xn = tmp;
+  }
+  return ll;
+}
+
+/* R independent LG filters (theta shared), one per OpenMP thread. */
+int cpu_lg_pfilter(const float* theta, int dx, const float* ys, int T, int64_t J, const uint32_t* keys, int R,
+                   double* loglik) {
+  int err = 0;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int r = 0; r < R; ++r) {
+    float* X = (float*)malloc(sizeof(float) * dx * J);
+    float* Xn = (float*)malloc(sizeof(float) * dx * J);
+    float* lw = (float*)malloc(sizeof(float) * J);
+    uint64_t* C = (uint64_t*)malloc(sizeof(uint64_t) * J);
+    int32_t* anc = (int32_t*)malloc(sizeof(int32_t) * J);
+    if (!X || !Xn || !lw || !C || !anc) {
+      err = 1;
+    } else {
+      key_t2 k = {keys[2 * r], keys[2 * r + 1]};
+      loglik[r] = lg_pfilter_one(theta, dx, ys, T, J, k, X, Xn, lw, C, anc);
+    }
+    free(X);
+    free(Xn);
+    free(lw);
+    free(C);
+    free(anc);
+  }
+  return err;
+}
+
+/* MOP(alpha) value of the SV model (theta on the estimation scale). */
+static double sv_mop_one(const float* th_est, const float* ys, int T, int64_t J, float alpha, key_t2 key, float* V,
+                         float* Vn, float* E, float* lw, uint64_t* C, int32_t* anc) {
+  const float mu = th_est[0], phi = tanhf(th_est[1]), sig = expf(th_est[2]), rho = tanhf(th_est[3]);
+  const float V0 = th_est[4], S0 = expf(th_est[5]);
+  const float omr2 = 1.0f - rho * rho, k = 1.0f / omr2, c0 = -0.918938533f - 0.5f * logf(omr2);
+  key_t2 run = split_at(key, 0);
+  for (int64_t j = 0; j < J; ++j) V[j] = V0 + S0 * normal_at(split_at(key, 1 + (uint64_t)j), 0);
+  const double logJ = log((double)J);
+  double ll = 0.0;
+  (void)alpha;
+  for (int t = 0; t < T; ++t) {
+    key_t2 sub = split_at(run, 1);
+    run = split_at(run, 0);
+    key_t2 sk = split_at(sub, 1);
+    float m = -INFINITY;
+    for (int64_t j = 0; j < J; ++j) {
+      float v = V[t == 0 ? j : anc[j]];
+      float z = normal_at(split_at(sk, (uint64_t)j), 0);
+      v = mu + phi * (v - mu) + sig * z;
+      float u = ys[t] * expf(-0.5f * v) - rho * z;
+      float g = c0 - 0.5f * v - 0.5f * u * u * k;
+      Vn[j] = v;
+      E[j] = z;
+      lw[j] = g;
+      if (g > m) m = g;
+    }
+    double se = 0.0;
+    for (int64_t j = 0; j < J; ++j) se += (double)expf(lw[j] - m);
+    ll += (double)m + log(se) - logJ;
+    key_t2 rk = split_at(run, 1);
+    run = split_at(run, 0);
+    if (t < T - 1) resample(lw, m, J, uniform01(bits_at(rk, 0)), C, anc);
+    float* tmp = V;
+    V = Vn;
+    Vn = tmp;
+  }
+  return ll;
+}
+
+int cpu_sv_mop(const float* theta_est, const float* ys, int T, int64_t J, float alpha, const uint32_t* keys, int R,
+               double* loglik) {
+  int err = 0;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int r = 0; r < R; ++r) {
+    float* V = (float*)malloc(sizeof(float) * J);
+    float* Vn = (float*)malloc(sizeof(float) * J);
+    float* E = (float*)malloc(sizeof(float) * J);
+    float* lw = (float*)malloc(sizeof(float) * J);
+    uint64_t* C = (uint64_t*)malloc(sizeof(uint64_t) * J);
+    int32_t* anc = (int32_t*)malloc(sizeof(int32_t) * J);
+    if (!V || !Vn || !E || !lw || !C || !anc) {
+      err = 1;
+    } else {
+      key_t2 k = {keys[2 * r], keys[2 * r + 1]};
+      loglik[r] = sv_mop_one(theta_est, ys, T, J, alpha, k, V, Vn, E, lw, C, anc);
+    }
+    free(V);
+    free(Vn);
+    free(E);
+    free(lw);
+    free(C);
+    free(anc);
+  }
+  return err;
+}
+
+int cpu_num_threads(void) {
+#ifdef _OPENMP
+  extern int omp_get_max_threads(void);
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}

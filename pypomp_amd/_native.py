@@ -23,6 +23,7 @@ EXPORTS = [
     "mp_random_bits", "mp_uniform", "mp_normal", "mp_split",
     "mp_resample_workspace_size", "mp_resample_systematic",
     "mp_filter_workspace_size", "mp_pfilter", "mp_mop", "mp_perfilter",
+    "mp_mop_grad_workspace_size", "mp_mop_grad",
     "mp_profile_enable", "mp_profile_read",
 ]
 
@@ -75,6 +76,10 @@ def lib():
                          C.c_float, i32, f64p, vp, sz, vp]
     L.mp_perfilter.argtypes = [C.POINTER(ModelDesc), C.POINTER(SeriesDesc), f32p, u32p, vp, i32, i32,
                                C.c_float, f32p, f32p, C.c_float, i32, f32p, f32p, f64p, vp, sz, vp]
+    L.mp_mop_grad_workspace_size.argtypes = [C.POINTER(ModelDesc), C.POINTER(SeriesDesc), i32, i32, i32]
+    L.mp_mop_grad_workspace_size.restype = sz
+    L.mp_mop_grad.argtypes = [C.POINTER(ModelDesc), C.POINTER(SeriesDesc), f32p, i64, u32p, i32, i32, C.c_float,
+                              i32, i32, f64p, f32p, vp, sz, vp]
     L.mp_profile_enable.argtypes = [i32]
     L.mp_profile_read.argtypes = [vp, vp, i32]
     for n in EXPORTS:
@@ -85,6 +90,7 @@ def lib():
     L.mp_last_error.restype = C.c_char_p
     L.mp_resample_workspace_size.restype = sz
     L.mp_filter_workspace_size.restype = sz
+    L.mp_mop_grad_workspace_size.restype = sz
     _lib = L
     return L
 

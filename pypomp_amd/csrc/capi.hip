@@ -23,6 +23,11 @@ hipError_t plain_reset(unsigned int* flag, int ntiles, PlainState* st, hipStream
 size_t batched_small_lds(int J, int block);
 }  // namespace mp
 
+namespace mp {
+hipError_t launch_mop_grad(const GradArgs& a, int block, int grid, hipStream_t st);
+int64_t mop_grad_scratch_bytes(int S, int J, int T, int K);
+}  // namespace mp
+
 using namespace mp;
 
 static_assert(sizeof(mp_model_desc) == sizeof(ModelDesc), "model desc layout");
@@ -286,6 +291,52 @@ BatchArgs base_args(const mp_model_desc* m, const mp_series_desc* s, const uint3
 
 float log_thresh(float thresh) { return thresh > 0.f ? logf(thresh) : -__builtin_inff(); }
 
+struct GradPlan {
+  int64_t sched_len, sched_bytes, per_filter;
+  size_t total;
+};
+
+GradPlan plan_grad(const mp_model_desc* m, const mp_series_desc* s, int R, int J, int K) {
+  GradPlan p{};
+  const SeriesSet& ser = *reinterpret_cast<const SeriesSet*>(s);
+  p.sched_len = sched_stride(ALGO_MOP, ser);
+  p.sched_bytes = align256((int64_t)R * p.sched_len * 8);
+  p.per_filter = align256(mop_grad_scratch_bytes(m->S, J, s->Tmax, K));
+  p.total = (size_t)p.sched_bytes + (size_t)R * (size_t)p.per_filter + 256;
+  return p;
+}
+
+int run_mop_grad(const mp_model_desc* m, const mp_series_desc* s, const float* theta, int64_t theta_stride,
+                 const uint32_t* keys, int R, int J, float alpha, int K, double* loglik, float* grad, void* ws,
+                 size_t ws_bytes, hipStream_t st) {
+  const GradPlan p = plan_grad(m, s, R, J, K);
+  if (ws_bytes < p.total) return fail(MP_ERR_WORKSPACE, "workspace %zu < required %zu bytes", ws_bytes, p.total);
+  char* base = align_ptr(ws);
+  GradArgs a{};
+  a.md = *reinterpret_cast<const ModelDesc*>(m);
+  a.ser = *reinterpret_cast<const SeriesSet*>(s);
+  a.theta = theta;
+  a.theta_stride = theta_stride;
+  a.keys = keys;
+  a.sched = reinterpret_cast<uint32_t*>(base);
+  a.sched_len = p.sched_len;
+  a.R = R;
+  a.J = J;
+  a.K = K;
+  a.alpha = alpha;
+  a.loglik = loglik;
+  a.grad = grad;
+  a.ws = base + p.sched_bytes;
+  a.ws_stride = p.per_filter;
+  MP_HIP(timed(3, st, [&] { return launch_sched(ALGO_MOP, keys, a.ser, nullptr, p.sched_len,
+                                                const_cast<uint32_t*>(a.sched), R, st); }));
+  const int grid = R < num_cus() * 8 ? R : num_cus() * 8;
+  hipError_t e = timed(2, st, [&] { return launch_mop_grad(a, kBatchedBlock, grid, st); });
+  if (e == hipErrorNotSupported) return fail(MP_ERR_ARG, "model kind %d has no adjoint (not differentiable)", m->kind);
+  MP_HIP(e);
+  return MP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -428,6 +479,35 @@ int32_t mp_mop(const mp_model_desc* h_model, const mp_series_desc* h_series, con
   a.alpha = alpha;
   a.loglik = loglik;
   return run_batched(ALGO_MOP, h_model, h_series, a, workspace, ws_bytes, st);
+}
+
+size_t mp_mop_grad_workspace_size(const mp_model_desc* h_model, const mp_series_desc* h_series, int32_t R,
+                                  int32_t J, int32_t K) {
+  if (!h_model || !h_series || R < 1 || J < 1 || K < 1) return 0;
+  return plan_grad(h_model, h_series, R, J, K).total;
+}
+
+int32_t mp_mop_grad(const mp_model_desc* h_model, const mp_series_desc* h_series, const float* theta,
+                    int64_t theta_stride, const uint32_t* keys, int32_t R, int32_t J, float alpha, int32_t K,
+                    int32_t form, double* loglik, float* grad, void* workspace, size_t ws_bytes, void* stream) {
+  int rc = check_common(h_model, h_series, R, J, keys);
+  if (rc) return rc;
+  if (!theta || !loglik || !grad || !workspace) return fail(MP_ERR_ARG, "NULL theta/loglik/grad/workspace");
+  if (K < 1) return fail(MP_ERR_ARG, "checkpoint interval K must be >= 1 (K = %d)", K);
+  if (h_series->U != 1) return fail(MP_ERR_ARG, "mp_mop_grad runs single-series filters (U = 1)");
+  if (h_series->max_euler != h_series->Tmax)
+    return fail(MP_ERR_ARG, "mp_mop_grad needs exactly one Euler step per observation interval");
+  hipStream_t st = (hipStream_t)stream;
+  if (form == MP_FORM_PLAIN) {
+    for (int f = 0; f < R; ++f) {
+      rc = run_mop_grad(h_model, h_series, theta + f * theta_stride, theta_stride, keys + 2 * f, 1, J, alpha, K,
+                        loglik + f, grad + (int64_t)f * h_model->d, workspace, ws_bytes, st);
+      if (rc) return rc;
+    }
+    return MP_OK;
+  }
+  return run_mop_grad(h_model, h_series, theta, theta_stride, keys, R, J, alpha, K, loglik, grad, workspace,
+                      ws_bytes, st);
 }
 
 int32_t mp_perfilter(const mp_model_desc* h_model, const mp_series_desc* h_series, const float* thetas_in,

@@ -129,4 +129,21 @@ struct PlainArgs {
   int rows;             // particle rows per tile (tile = rows * blockDim)
 };
 
+// MOP value + adjoint gradient launch (one workgroup per filter).
+struct GradArgs {
+  ModelDesc md;
+  SeriesSet ser;
+  const float* theta;  // (R, d) estimation scale
+  int64_t theta_stride;
+  const uint32_t* keys;
+  const uint32_t* sched;
+  int64_t sched_len;
+  int R, J, K;
+  float alpha;
+  double* loglik;  // (R)
+  float* grad;     // (R, d): d loglik / d theta_est
+  char* ws;
+  int64_t ws_stride;
+};
+
 }  // namespace mp

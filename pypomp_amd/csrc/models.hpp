@@ -43,6 +43,7 @@ struct LGT {
   static constexpr int MAXS = MD;
   static constexpr bool FIXED_S = EXACT;  // S == MAXS for every instance
   static constexpr int MAXP = 2 * MD * MD + 3 * (MD * (MD + 1) / 2) + MD;
+  static constexpr bool DIFFERENTIABLE = false;
   struct P {
     float A[MD][MD], C[MD][MD], LQ[MD][MD], L0[MD][MD], LC[MD][MD];
     float X0[MD];
@@ -206,6 +207,51 @@ struct SV {
   __device__ static float dmeas(const ModelDesc&, const P& p, const float* y, const float* x, float) {
     float u = y[0] * expf(-0.5f * x[0]) - p.rho * x[1];
     return p.c0 - 0.5f * x[0] - 0.5f * u * u * p.inv1mr2;
+  }
+
+  // ---- derivatives for the MOP adjoint (natural-scale parameters) ----
+  static constexpr bool DIFFERENTIABLE = true;
+
+  // d(dmeas)/dx into gx[S], accumulated d(dmeas)/dtheta into gth[d] scaled by w.
+  __device__ static void dmeas_grad(const ModelDesc&, const P& p, const float* y, const float* x, float w,
+                                    float* gx, double* gth) {
+    const float e = expf(-0.5f * x[0]);
+    const float ry = y[0] * e;
+    const float u = ry - p.rho * x[1];
+    const float k = p.inv1mr2;
+    gx[0] = -0.5f + 0.5f * u * ry * k;
+    gx[1] = u * p.rho * k;
+    gth[3] += (double)w * (double)(p.rho * k + u * x[1] * k - u * u * p.rho * k * k);
+  }
+
+  // Vector-Jacobian product of one sub-step x_out = step(x_in): given xbar_out,
+  // writes xbar_in and accumulates theta-bar.  Redraws the step's normal.
+  __device__ static void step_vjp(const ModelDesc&, const P& p, Key k, float, float, const float* xin,
+                                  const float* xbar_out, float* xbar_in, double* gth) {
+    const float z = normal_at(k, 0);
+    const double vb = xbar_out[0];
+    xbar_in[0] = p.phi * xbar_out[0];
+    xbar_in[1] = 0.f;
+    gth[0] += vb * (double)(1.0f - p.phi);
+    gth[1] += vb * (double)(xin[0] - p.mu);
+    gth[2] += vb * (double)z;
+  }
+
+  __device__ static void rinit_vjp(const ModelDesc&, const P&, Key k, float, const float* xbar, double* gth) {
+    const float z = normal_at(k, 0);
+    gth[4] += (double)xbar[0];
+    gth[5] += (double)xbar[0] * (double)z;
+  }
+
+  // d(natural)/d(estimation) for each parameter (diagonal transform), at est.
+  __device__ static void from_est_jac(const ModelDesc&, const float* est, float* jac) {
+    const float ph = tanhf(est[1]), rh = tanhf(est[3]);
+    jac[0] = 1.f;
+    jac[1] = 1.f - ph * ph;
+    jac[2] = expf(est[2]);
+    jac[3] = 1.f - rh * rh;
+    jac[4] = 1.f;
+    jac[5] = expf(est[5]);
   }
 };
 

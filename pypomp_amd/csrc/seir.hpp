@@ -125,6 +125,7 @@ struct SEIR {
   static constexpr int MAXS = 5;
   static constexpr bool FIXED_S = true;
   static constexpr int MAXP = 10;
+  static constexpr bool DIFFERENTIABLE = false;  // integer-valued transitions
   struct P {
     float beta, amp, sigma, gamma, rho, k, N, S0f, E0, I0;
     float invN, lgk;

@@ -193,3 +193,67 @@ def mif(struct, thetas_array, rw_sd, M, J, keys, thresh=0.0, n_monitors=0, unit=
         theta = new
     traces = np.stack(traces, axis=1)
     return lls, model.from_est(traces), model.from_est(theta)
+
+
+def mop_grad(struct, thetas_array, J, alpha, keys, K=50, form="batched"):
+    """MOP(alpha) value and gradient by the native adjoint.  thetas (n, d) on the
+    estimation scale, keys (n, 2).  Returns (loglik (n,) float64, grad (n, d)
+    float32) with grad = d loglik / d theta_est.  The reference's
+    ``_mop_internal_mean`` is -loglik / T (core/algorithms/mop.py:148-158)."""
+    import torch
+    dev = _native.require_device()
+    thetas = np.ascontiguousarray(np.asarray(thetas_array, dtype=F32))
+    if thetas.ndim == 1:
+        thetas = thetas[None]
+    keys = np.asarray(keys, dtype=np.uint32).reshape(-1, 2)
+    R = keys.shape[0]
+    if thetas.shape[0] != R:
+        raise ValueError("thetas and keys disagree on the number of rows")
+    for u in struct.units:
+        if not np.all(u.nstep == 1):
+            raise ValueError("the MOP adjoint needs exactly one Euler step per observation interval")
+    ser = struct.device_series()
+    md = struct.model.desc()
+    th = torch.from_numpy(thetas).to(dev)
+    kt = _keys_tensor(keys, dev)
+    ll = torch.empty(R, dtype=torch.float64, device=dev)
+    g = torch.empty((R, struct.model.d), dtype=torch.float32, device=dev)
+    fid = _form_id(form)
+    n = _native.lib().mp_mop_grad_workspace_size(md, ser.desc, 1 if fid == _native.MP_FORM_PLAIN else R, J, K)
+    ws = torch.empty(int(n), dtype=torch.uint8, device=dev)
+    _native.check(_native.lib().mp_mop_grad(md, ser.desc, _native.ptr(th), struct.model.d, _native.ptr(kt), R, J,
+                                            float(alpha), int(K), fid, _native.ptr(ll), _native.ptr(g),
+                                            _native.ptr(ws), int(n), _native.stream_ptr()))
+    torch.cuda.current_stream().synchronize()
+    return ll.cpu().numpy(), g.cpu().numpy()
+
+
+class _MopFn:
+    """Builds the torch.autograd.Function lazily (torch is only needed on use)."""
+    fn = None
+
+    @classmethod
+    def get(cls):
+        if cls.fn is None:
+            import torch
+
+            class MopFunction(torch.autograd.Function):
+                @staticmethod
+                def forward(ctx, theta_est, struct, J, alpha, keys, K):
+                    ll, g = mop_grad(struct, theta_est.detach().cpu().numpy(), J, alpha, keys, K)
+                    ctx.save_for_backward(torch.from_numpy(g).to(theta_est.device))
+                    return torch.from_numpy(ll).to(device=theta_est.device, dtype=torch.float64)
+
+                @staticmethod
+                def backward(ctx, grad_out):
+                    (g,) = ctx.saved_tensors
+                    return (grad_out[:, None].to(g.dtype) * g, None, None, None, None, None)
+
+            cls.fn = MopFunction
+        return cls.fn
+
+
+def mop_loglik_autograd(theta_est, struct, J, alpha, keys, K=50):
+    """MOP log-likelihood as a differentiable torch function of theta_est (n, d);
+    its backward is the native adjoint (the forward runs value + gradient)."""
+    return _MopFn.get().apply(theta_est, struct, J, alpha, keys, K)
