@@ -71,8 +71,98 @@ static inline float uniform01(uint32_t bits) {
   return f > 0.f ? f : 0.f;
 }
 
+
+/* ---- deterministic elementary functions (same operation sequences as the
+ * engine and oracle/detmath.py; compiled with -ffp-contract=off) ---- */
+static inline float bf(uint32_t b) {
+  union {
+    uint32_t u;
+    float f;
+  } c;
+  c.u = b;
+  return c.f;
+}
+static inline uint32_t fb(float f) {
+  union {
+    uint32_t u;
+    float f;
+  } c;
+  c.f = f;
+  return c.u;
+}
+static inline float pow2f(int k) { return bf((uint32_t)(k + 127) << 23); }
+
+static float d_log(float x) {
+  if (!(x > 0.0f)) return x == 0.0f ? -INFINITY : NAN;
+  if (x == INFINITY) return x;
+  int e = 0;
+  if (x < 1.17549435e-38f) {
+    x = x * 8388608.0f;
+    e = -23;
+  }
+  const uint32_t b = fb(x);
+  e += (int)((b >> 23) & 0xffu) - 127;
+  float m = bf((b & 0x7fffffu) | 0x3f800000u);
+  if (m > 1.41421356f) {
+    m = m * 0.5f;
+    e += 1;
+  }
+  const float f = m - 1.0f, s = f / (2.0f + f), z = s * s;
+  float p = 0.111111112f;
+  p = p * z + 0.142857149f;
+  p = p * z + 0.2f;
+  p = p * z + 0.333333343f;
+  p = p * z + 1.0f;
+  const float lm = (2.0f * s) * p, fe = (float)e;
+  return fe * 0.693145751953125f + (lm + fe * 1.42860677e-06f);
+}
+
+static float d_log1p(float x) {
+  if (x == 0.0f) return x;
+  const float u = 1.0f + x;
+  if (u == 1.0f) return x;
+  return d_log(u) * (x / (u - 1.0f));
+}
+
+static float d_exp(float x) {
+  if (x != x) return x;
+  if (x > 88.7228394f) return INFINITY;
+  if (x < -87.3365479f) return 0.0f;
+  const float k = rintf(x * 1.44269502f);
+  const float r = (x - k * 0.693145751953125f) - k * 1.42860677e-06f;
+  float p = 1.98412701e-04f;
+  p = p * r + 1.38888892e-03f;
+  p = p * r + 8.33333377e-03f;
+  p = p * r + 4.16666679e-02f;
+  p = p * r + 0.166666672f;
+  p = p * r + 0.5f;
+  p = p * r + 1.0f;
+  p = p * r + 1.0f;
+  const int ki = (int)k;
+  if (ki < -126) return (p * pow2f(ki + 64)) * 5.42101086e-20f;
+  if (ki > 127) return (p * pow2f(ki - 64)) * 1.84467441e+19f;
+  return p * pow2f(ki);
+}
+
+static float d_expm1(float x) {
+  if (x == 0.0f) return x;
+  const float u = d_exp(x);
+  if (u == 1.0f) return x;
+  const float um1 = u - 1.0f;
+  if (um1 == -1.0f) return -1.0f;
+  if (u == INFINITY) return u;
+  return um1 * (x / d_log(u));
+}
+
+static float d_tanh(float x) {
+  const float ax = x < 0.0f ? -x : x;
+  const float t = d_expm1(-2.0f * ax);
+  const float r = -t / (t + 2.0f);
+  return x < 0.0f ? -r : r;
+}
+
 static inline float erfinv_f32(float x) {
-  float w = -log1pf(-(x * x)), p;
+  float w = -d_log1p(-(x * x)), p;
   if (w < 5.0f) {
     w = w - 2.5f;
     p = 2.81022636e-08f;
@@ -211,7 +301,7 @@ static void lg_prepare(const float* th, int dx, lg_t* p) {
     for (int k = 0; k < j; ++k) s -= p->LC[j][k] * p->LC[j][k];
     float d = sqrtf(s);
     p->LC[j][j] = d;
-    ld += logf(d);
+    ld += d_log(d);
     for (int i = j + 1; i < dx; ++i) {
       float t = cov[i][j];
       for (int k = 0; k < j; ++k) t -= p->LC[i][k] * p->LC[j][k];
@@ -238,10 +328,10 @@ static double lg_pfilter_one(const float* th, int dx, const float* ys, int T, in
       X[i * J + j] = p.X0[i] + s;
     }
   }
-  const float logJ = logf((float)J);
-  float wconst = logf(1.0f / (float)J);
+  const float logJ = d_log((float)J);
+  float wconst = d_log(1.0f / (float)J);
   double ll = 0.0;
-  const float hl2p = 0.5f * (float)dx * 1.83787707f;
+  const float hl2p = (0.5f * (float)dx) * 1.83787707f;
   for (int t = 0; t < T; ++t) {
     key_t2 sub = split_at(run, 1);
     run = split_at(run, 0);
@@ -274,7 +364,7 @@ static double lg_pfilter_one(const float* th, int dx, const float* ys, int T, in
         sv[i] = a2 / p.LC[i][i];
         quad += sv[i] * sv[i];
       }
-      const float w = wconst + (-0.5f * quad - hl2p - p.logdet);
+      const float w = wconst + ((-0.5f * quad - hl2p) - p.logdet);
       for (int i = 0; i < dx; ++i) Xn[i * J + j] = xn[i];
       lw[j] = w;
       if (w > m) m = w;
@@ -322,9 +412,9 @@ int cpu_lg_pfilter(const float* theta, int dx, const float* ys, int T, int64_t J
 /* MOP(alpha) value of the SV model (theta on the estimation scale). */
 static double sv_mop_one(const float* th_est, const float* ys, int T, int64_t J, float alpha, key_t2 key, float* V,
                          float* Vn, float* E, float* lw, uint64_t* C, int32_t* anc) {
-  const float mu = th_est[0], phi = tanhf(th_est[1]), sig = expf(th_est[2]), rho = tanhf(th_est[3]);
-  const float V0 = th_est[4], S0 = expf(th_est[5]);
-  const float omr2 = 1.0f - rho * rho, k = 1.0f / omr2, c0 = -0.918938533f - 0.5f * logf(omr2);
+  const float mu = th_est[0], phi = d_tanh(th_est[1]), sig = d_exp(th_est[2]), rho = d_tanh(th_est[3]);
+  const float V0 = th_est[4], S0 = d_exp(th_est[5]);
+  const float omr2 = 1.0f - rho * rho, k = 1.0f / omr2, c0 = -0.918938533f - 0.5f * d_log(omr2);
   key_t2 run = split_at(key, 0);
   for (int64_t j = 0; j < J; ++j) V[j] = V0 + S0 * normal_at(split_at(key, 1 + (uint64_t)j), 0);
   const double logJ = log((double)J);
@@ -339,7 +429,7 @@ static double sv_mop_one(const float* th_est, const float* ys, int T, int64_t J,
       float v = V[t == 0 ? j : anc[j]];
       float z = normal_at(split_at(sk, (uint64_t)j), 0);
       v = mu + phi * (v - mu) + sig * z;
-      float u = ys[t] * expf(-0.5f * v) - rho * z;
+      float u = ys[t] * d_exp(-0.5f * v) - rho * z;
       float g = c0 - 0.5f * v - 0.5f * u * u * k;
       Vn[j] = v;
       E[j] = z;

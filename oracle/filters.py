@@ -18,6 +18,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import threefry as tf
+from .detmath import d_log, d_pow
 from .resample import resample_fixed_fast, resample_reference
 
 F32 = np.float32
@@ -88,7 +89,7 @@ def normalize_weights(w):
     J = w.shape[-1]
     mw = np.max(w)
     if mw == -np.inf:
-        return np.full(J, -F32(np.log(F32(J))), dtype=F32), F32(-np.inf)
+        return np.full(J, -d_log(F32(J)), dtype=F32), F32(-np.inf)
     with np.errstate(invalid="ignore", over="ignore"):
         ll = F32(mw + np.log(np.nansum(np.exp((w - mw).astype(F32)), dtype=F32)))
         nw = (w - ll).astype(F32)
@@ -113,12 +114,12 @@ def pfilter(model, theta, series, key, J, thresh=0.0, resampler="fixed",
     K = tf.split(key, J + 1)
     key, pkeys = K[0], K[1:]
     X = model.rinit(th, pkeys, series.t0)
-    nw = np.log(np.ones(J, dtype=F32) / F32(J)).astype(F32)
+    nw = np.full(J, d_log(F32(1.0) / F32(J)), dtype=F32)
     loglik = F32(0.0)
     t, t_idx = F32(series.t0), 0
     out = {"CLL": [], "ESS": [], "filter_mean": [], "prediction_mean": [],
            "ancestors": [], "weights": [], "states": []}
-    logthresh = np.log(F32(thresh)) if thresh > 0 else -np.inf
+    logthresh = d_log(F32(thresh)) if thresh > 0 else -np.inf
     for i in range(series.T):
         k2 = tf.split(key)
         key, sub = k2[0], k2[1]
@@ -143,7 +144,7 @@ def pfilter(model, theta, series, key, J, thresh=0.0, resampler="fixed",
         if do_res:
             a = _resample(resampler, nw, w, sub)
             X = X[a]
-            nw = np.full(J, F32(0.0) - F32(np.log(F32(J))), dtype=F32)
+            nw = np.full(J, F32(0.0) - d_log(F32(J)), dtype=F32)
         else:
             a = np.arange(J, dtype=np.int32)
         if return_trace:
@@ -160,7 +161,7 @@ def mop(model, theta_est, series, key, J, alpha, resampler="fixed", return_trace
     K = tf.split(key, J + 1)
     key, pkeys = K[0], K[1:]
     X = model.rinit(th, pkeys, series.t0)
-    wF = np.log(np.ones(J, dtype=F32) / F32(J)).astype(F32)
+    wF = np.full(J, d_log(F32(1.0) / F32(J)), dtype=F32)
     loglik = F32(0.0)
     t, t_idx = F32(series.t0), 0
     trace = {"ancestors": [], "meas": [], "states": []}
@@ -215,7 +216,7 @@ class RWSigma:
     def cooling_factor(self, nt, m, ntimes):
         frac = F32(F32(nt) / F32(ntimes) + F32(m))
         factor = F32(self.a ** (1.0 / 50.0))
-        return F32(np.power(factor, frac, dtype=F32))
+        return F32(d_pow(factor, frac))
 
 
 def perfilter(model, m, thetas, series, key, J, rw, thresh=0.0, resampler="fixed"):
@@ -224,7 +225,7 @@ def perfilter(model, m, thetas, series, key, J, rw, thresh=0.0, resampler="fixed
     thetas = np.asarray(thetas, dtype=F32).copy()
     d = thetas.shape[1]
     T = series.T
-    nw = np.full(J, -F32(np.log(F32(J))), dtype=F32)
+    nw = np.full(J, -d_log(F32(J)), dtype=F32)
     loglik = F32(0.0)
     cf = [rw.cooling_factor(i, m, T) for i in range(T)]
     k2 = tf.split(key)
@@ -236,8 +237,8 @@ def perfilter(model, m, thetas, series, key, J, rw, thresh=0.0, resampler="fixed
     X = model.rinit(model.from_est(thetas), pkeys, series.t0)
     step_keys = tf.split(key, T + 1)[1:]
     t, t_idx = F32(series.t0), 0
-    logthresh = np.log(F32(thresh)) if thresh > 0 else -np.inf
-    nan_sub = F32(np.log(F32(1e-18)))
+    logthresh = d_log(F32(thresh)) if thresh > 0 else -np.inf
+    nan_sub = d_log(F32(1e-18))
     sig = rw.sigmas_array
     for i in range(T):
         k3 = tf.split(step_keys[i], 3)
@@ -257,7 +258,7 @@ def perfilter(model, m, thetas, series, key, J, rw, thresh=0.0, resampler="fixed
             a = _resample(resampler, nw, w, kres)
             X = X[a]
             thetas = thetas[a]
-            nw = np.full(J, F32(0.0) - F32(np.log(F32(J))), dtype=F32)
+            nw = np.full(J, F32(0.0) - d_log(F32(J)), dtype=F32)
     return thetas, loglik
 
 

@@ -19,6 +19,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import threefry as tf
+from .detmath import d_exp, d_log
 
 F32 = np.float32
 
@@ -89,14 +90,42 @@ class LGSpec:
     def to_est(self, theta):
         theta = np.array(theta, dtype=F32, copy=True)
         m = self._diag_mask()
-        theta[..., m] = np.log(theta[..., m])
+        theta[..., m] = d_log(theta[..., m])
         return theta
 
     def from_est(self, theta):
         theta = np.array(theta, dtype=F32, copy=True)
         m = self._diag_mask()
-        theta[..., m] = np.exp(theta[..., m])
+        theta[..., m] = d_exp(theta[..., m])
         return theta
+
+
+def chol_logdet(LR, d):
+    """float32 Cholesky factor of LR LR^T and the sum of log of its diagonal, in the
+    engine's operation order (sequential sums, deterministic log)."""
+    lead = LR.shape[:-2]
+    cov = np.zeros(lead + (d, d), dtype=F32)
+    for i in range(d):
+        for j in range(d):
+            s = (LR[..., i, 0] * LR[..., j, 0]).astype(F32)
+            for k in range(1, d):
+                s = (s + (LR[..., i, k] * LR[..., j, k]).astype(F32)).astype(F32)
+            cov[..., i, j] = s
+    L = np.zeros(lead + (d, d), dtype=F32)
+    logdet = np.zeros(lead, dtype=F32)
+    for j in range(d):
+        s = cov[..., j, j]
+        for k in range(j):
+            s = (s - (L[..., j, k] * L[..., j, k]).astype(F32)).astype(F32)
+        djj = np.sqrt(s).astype(F32)
+        L[..., j, j] = djj
+        logdet = (logdet + d_log(djj)).astype(F32)
+        for i in range(j + 1, d):
+            t = cov[..., i, j]
+            for k in range(j):
+                t = (t - (L[..., i, k] * L[..., j, k]).astype(F32)).astype(F32)
+            L[..., i, j] = (t / djj).astype(F32)
+    return L, logdet
 
 
 def _matvec(M, v):
@@ -106,7 +135,7 @@ def _matvec(M, v):
     for i in range(r):
         acc = (M[..., i, 0] * v[..., 0]).astype(F32)
         for k in range(1, c):
-            acc = (acc + M[..., i, k] * v[..., k]).astype(F32)
+            acc = (acc + (M[..., i, k] * v[..., k]).astype(F32)).astype(F32)
         out[..., i] = acc
     return out
 
@@ -136,12 +165,13 @@ class LGModel:
         return (_matvec(A, X) + _matvec(LQ, z)).astype(F32)
 
     def dmeas(self, y, X, theta, t):
-        """multivariate normal log density of y given C x, cov L_R L_R^T."""
+        """multivariate normal log density of y given C x, cov L_R L_R^T, through the
+        float32 Cholesky factor of the covariance (the reference's
+        jax.scipy.stats.multivariate_normal.logpdf path)."""
         A, C, LQ, LR, X0, L0 = self.spec.unpack(theta)
         dy = self.spec.dy
         mean = _matvec(C, X)
-        cov = np.einsum("...ik,...jk->...ij", LR.astype(F32), LR.astype(F32)).astype(F32)
-        L = np.linalg.cholesky(cov.astype(np.float64)).astype(F32)
+        L, logdet = chol_logdet(LR, dy)
         r = (np.asarray(y, dtype=F32) - mean).astype(F32)
         # forward substitution L s = r
         s = np.zeros_like(r)
@@ -152,11 +182,9 @@ class LGModel:
             s[..., i] = (acc / L[..., i, i]).astype(F32)
         quad = np.zeros(s.shape[:-1], dtype=F32)
         for i in range(dy):
-            quad = (quad + s[..., i] * s[..., i]).astype(F32)
-        logdet = np.zeros(np.shape(L)[:-2], dtype=F32)
-        for i in range(dy):
-            logdet = (logdet + np.log(L[..., i, i])).astype(F32)
-        return (F32(-0.5) * quad - F32(dy / 2.0) * F32(np.log(2 * np.pi)) - logdet).astype(F32)
+            quad = (quad + (s[..., i] * s[..., i]).astype(F32)).astype(F32)
+        hn = ((F32(0.5) * F32(dy)).astype(F32) * F32(1.83787707)).astype(F32)
+        return (((F32(-0.5) * quad).astype(F32) - hn).astype(F32) - logdet).astype(F32)
 
     def rmeas(self, X, theta, keys, t):
         A, C, LQ, LR, X0, L0 = self.spec.unpack(theta)

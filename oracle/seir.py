@@ -21,10 +21,11 @@ theta = (beta, amp, sigma, gamma, rho, k, N, S0f, E0, I0), natural scale.
 """
 
 import numpy as np
-from scipy import special, stats
+from scipy import stats
 
 from . import filters as fl
 from . import threefry as tf
+from .detmath import d_cos, d_exp, d_expm1, d_log, d_log1p, dd_lgamma, dd_log
 
 F32 = np.float32
 _C = tf._ERFINV_CENTRAL
@@ -40,7 +41,7 @@ _LOGIT = [1, 4, 7]
 def ndtri_f32(u):
     u = np.asarray(u, dtype=F32)
     with np.errstate(divide="ignore", invalid="ignore"):
-        w = (-np.log((F32(4.0) * u * (F32(1.0) - u)).astype(F32))).astype(F32)
+        w = (-d_log((F32(4.0) * u * (F32(1.0) - u)).astype(F32))).astype(F32)
         x = (F32(2.0) * u - F32(1.0)).astype(F32)
         lt = w < F32(5.0)
         wc = np.where(lt, w - F32(2.5), np.sqrt(w) - F32(3.0)).astype(F32)
@@ -85,8 +86,8 @@ def binominv_f32(u, n, p):
         ue = np.clip(uf, F32(0), F32(1)).astype(F32)
         qc = np.clip(F32(1) - ps, tiny, F32(1)).astype(F32)
         ratio = (ps / qc).astype(F32)
-        logq = np.log1p(-ps).astype(F32)
-        pmf = np.where(ns == 0, F32(1), np.exp((ns * logq).astype(F32))).astype(F32)
+        logq = d_log1p(-ps)
+        pmf = np.where(ns == 0, F32(1), d_exp((ns * logq).astype(F32))).astype(F32)
         cdf = pmf.copy()
         found = cdf >= ue
         res = np.where(found, F32(0), k).astype(F32)
@@ -127,8 +128,8 @@ class SEIRModel:
 
     def from_est(self, th):
         th = np.array(th, dtype=F32, copy=True)
-        th[..., _LOG] = np.exp(th[..., _LOG])
-        th[..., _LOGIT] = F32(1) / (F32(1) + np.exp(-th[..., _LOGIT]))
+        th[..., _LOG] = d_exp(th[..., _LOG])
+        th[..., _LOGIT] = F32(1) / (F32(1) + d_exp(-th[..., _LOGIT]))
         return th.astype(F32)
 
     def to_est(self, th):
@@ -159,12 +160,12 @@ class SEIRModel:
         beta, amp, sig, gam, rho, k, N, S0f, E0, I0 = self._cols(theta, J)
         S, E, I, R, H = (X[:, i] for i in range(5))
         invN = (F32(1) / N).astype(F32)
-        bt = (beta * (F32(1) + amp * np.cos((F32(6.28318531) * F32(t) / F32(52)).astype(F32)))).astype(F32)
+        bt = (beta * (F32(1) + (amp * d_cos((F32(6.28318531) * F32(t) / F32(52)).astype(F32))).astype(F32))).astype(F32)
         foi = (bt * I * invN).astype(F32)
         dt = F32(dt)
-        pSE = (-np.expm1(-(foi * dt).astype(F32))).astype(F32)
-        pEI = (-np.expm1(-(sig * dt).astype(F32))).astype(F32)
-        pIR = (-np.expm1(-(gam * dt).astype(F32))).astype(F32)
+        pSE = (-d_expm1(-(foi * dt).astype(F32))).astype(F32)
+        pEI = (-d_expm1(-(sig * dt).astype(F32))).astype(F32)
+        pIR = (-d_expm1(-(gam * dt).astype(F32))).astype(F32)
         k3 = tf.split_many(keys, 3)
         dSE = fast_binomial(k3[:, 0], S, pSE)
         dEI = fast_binomial(k3[:, 1], E, pEI)
@@ -179,11 +180,14 @@ class SEIRModel:
     def dmeas(self, y, X, theta, t):
         J = X.shape[0]
         beta, amp, sig, gam, rho, k, N, S0f, E0, I0 = self._cols(theta, J)
-        mu = np.maximum((X[:, 4] * rho).astype(F32), F32(1e-10)).astype(F32)
-        yy = np.rint(np.asarray(y, dtype=F32)[0])
-        k64, mu64 = k.astype(np.float64), mu.astype(np.float64)
-        v = (special.gammaln(yy + k64) - special.gammaln(k64) - special.gammaln(yy + 1.0)
-             + k64 * np.log(k64 / (k64 + mu64)) + yy * np.log(mu64 / (k64 + mu64)))
+        mu = np.maximum((X[:, 4] * rho).astype(F32), F32(1e-10)).astype(F32).astype(np.float64)
+        yy = float(np.rint(np.asarray(y, dtype=F32)[0]))
+        k64 = k.astype(np.float64)
+        ku, kinv = np.unique(k64, return_inverse=True)
+        lgk = dd_lgamma(ku)[kinv]
+        lgyk = dd_lgamma(ku + yy)[kinv]
+        lgy1 = dd_lgamma(np.array([yy + 1.0]))[0]
+        v = (((lgyk - lgk) - lgy1) + k64 * dd_log(k64 / (k64 + mu))) + yy * dd_log(mu / (k64 + mu))
         return v.astype(F32)
 
     def rmeas(self, X, theta, keys, t):

@@ -17,6 +17,7 @@ import numpy as np
 
 from . import filters as fl
 from . import threefry as tf
+from .detmath import d_exp, d_log, d_tanh
 
 F32 = np.float32
 PARAM_NAMES = ["mu", "phi", "sigma", "rho", "V0", "S0"]
@@ -30,10 +31,10 @@ class SVModel:
 
     def from_est(self, th):
         th = np.array(th, dtype=F32, copy=True)
-        th[..., 1] = np.tanh(th[..., 1])
-        th[..., 2] = np.exp(th[..., 2])
-        th[..., 3] = np.tanh(th[..., 3])
-        th[..., 5] = np.exp(th[..., 5])
+        th[..., 1] = d_tanh(th[..., 1])
+        th[..., 2] = d_exp(th[..., 2])
+        th[..., 3] = d_tanh(th[..., 3])
+        th[..., 5] = d_exp(th[..., 5])
         return th
 
     def to_est(self, th):
@@ -67,10 +68,13 @@ class SVModel:
     def dmeas(self, y, X, theta, t):
         J = X.shape[0]
         mu, phi, sig, rho, V0, S0 = self._cols(theta, J)
-        omr2 = (F32(1) - rho * rho).astype(F32)
-        c0 = (F32(-0.918938533) - F32(0.5) * np.log(omr2)).astype(F32)
-        u = (F32(np.asarray(y, dtype=F32)[0]) * np.exp(F32(-0.5) * X[:, 0]) - rho * X[:, 1]).astype(F32)
-        return (c0 - F32(0.5) * X[:, 0] - F32(0.5) * u * u / omr2).astype(F32)
+        omr2 = (F32(1) - (rho * rho).astype(F32)).astype(F32)
+        c0 = (F32(-0.918938533) - (F32(0.5) * d_log(omr2)).astype(F32)).astype(F32)
+        inv = (F32(1) / omr2).astype(F32)
+        yv = F32(np.asarray(y, dtype=F32)[0])
+        u = ((yv * d_exp((F32(-0.5) * X[:, 0]).astype(F32))).astype(F32) - (rho * X[:, 1]).astype(F32)).astype(F32)
+        return ((c0 - (F32(0.5) * X[:, 0]).astype(F32)).astype(F32)
+                - (((F32(0.5) * u).astype(F32) * u).astype(F32) * inv).astype(F32)).astype(F32)
 
     def rmeas(self, X, theta, keys, t):
         J = X.shape[0]

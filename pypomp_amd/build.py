@@ -25,6 +25,9 @@ HIPCC_FLAGS = [
     "-Wall",
     "-Wno-unused-function",
     "-munsafe-fp-atomics",
+    # no fused multiply-add contraction: the state/weight path is a fixed
+    # sequence of correctly rounded operations (reproducible by the oracle)
+    "-ffp-contract=off",
     # packed FP32 ops stay off: the compiler must not form v_pk_* f32 ops
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]

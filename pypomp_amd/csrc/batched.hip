@@ -128,7 +128,7 @@ __device__ void run_filter(const BatchArgs& a, const int f, char* buf, double* r
       isub = ld_key(sch, 0);
       rbase = ld_key(sch, 1);
       float fr = 0.0f / (float)T + (float)a.mif_m;
-      c0 = powf(a.cool_factor, fr);
+      c0 = d_pow(a.cool_factor, fr);
     } else {
       rbase = ld_key(a.keys, f);
     }
@@ -157,13 +157,13 @@ __device__ void run_filter(const BatchArgs& a, const int f, char* buf, double* r
 
   int cur = 0;
   bool have_anc = false, uniform_w = true;
-  float wconst = (ALGO == ALGO_PER) ? -logf(fJ) : logf(1.0f / fJ);
-  float mop_c = logf(1.0f / fJ);
+  float wconst = (ALGO == ALGO_PER) ? -d_log(fJ) : d_log(1.0f / fJ);
+  float mop_c = d_log(1.0f / fJ);
   uint64_t total = 0;
   float ures = 0.f;
   double loglik = 0.0;
-  const float logJ = logf(fJ);
-  const float nan_sub = logf(1e-18f);
+  const float logJ = d_log(fJ);
+  const float nan_sub = d_log(1e-18f);
 
   for (int t = 0; t < T; ++t) {
     const int nst = nstep[t];
@@ -177,7 +177,7 @@ __device__ void run_filter(const BatchArgs& a, const int f, char* buf, double* r
     Key kpert{0, 0};
     if (ALGO == ALGO_PER) {
       float fr = (float)t / (float)T + (float)a.mif_m;
-      cft = powf(a.cool_factor, fr);
+      cft = d_pow(a.cool_factor, fr);
       kpert = ld_key(sch, base - 1);
     }
     float lmax = -__builtin_inff(), lmin = __builtin_inff();

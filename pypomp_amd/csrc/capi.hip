@@ -289,7 +289,7 @@ BatchArgs base_args(const mp_model_desc* m, const mp_series_desc* s, const uint3
   return a;
 }
 
-float log_thresh(float thresh) { return thresh > 0.f ? logf(thresh) : -__builtin_inff(); }
+float log_thresh(float thresh) { return thresh > 0.f ? d_log(thresh) : -__builtin_inff(); }
 
 struct GradPlan {
   int64_t sched_len, sched_bytes, per_filter;

@@ -18,6 +18,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "detmath.hpp"
+
 #define MP_HD __host__ __device__ __forceinline__
 
 namespace mp {
@@ -74,9 +76,10 @@ MP_HD float uniform01(uint32_t bits) {
   return f > 0.0f ? f : 0.0f;
 }
 
-// Giles' single-precision erfinv; w = -log1p(-x*x) as XLA evaluates it.
+// Giles' single-precision erfinv; w = -log1p(-x*x) as XLA evaluates it
+// (deterministic log1p, see detmath.hpp).
 __device__ __forceinline__ float erfinv_f32(float x) {
-  float w = -log1pf(-(x * x));
+  float w = -d_log1p(-(x * x));
   float p;
   if (w < 5.0f) {
     w = w - 2.5f;

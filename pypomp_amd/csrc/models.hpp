@@ -56,11 +56,11 @@ struct LGT {
     // diagonal entries of the Q and R triangles
     MP_UNROLL for (int i = 0; i < MD; ++i) if (i < dx) {
       const int o = dx * dx + dy * dx + i * (i + 1) / 2 + i;
-      th[o] = expf(th[o]);
+      th[o] = d_exp(th[o]);
     }
     MP_UNROLL for (int i = 0; i < MD; ++i) if (i < dy) {
       const int o = dx * dx + dy * dx + dx * (dx + 1) / 2 + i * (i + 1) / 2 + i;
-      th[o] = expf(th[o]);
+      th[o] = d_exp(th[o]);
     }
   }
 
@@ -96,7 +96,7 @@ struct LGT {
       MP_UNROLL for (int k = 0; k < MD; ++k) if (k < j) s -= p.LC[j][k] * p.LC[j][k];
       const float djj = sqrtf(s);
       p.LC[j][j] = djj;
-      logdet += logf(djj);
+      logdet += d_log(djj);
       MP_UNROLL for (int i = 0; i < MD; ++i) if (i > j && i < dy) {
         float t = cov[i][j];
         MP_UNROLL for (int k = 0; k < MD; ++k) if (k < j) t -= p.LC[i][k] * p.LC[j][k];
@@ -174,10 +174,10 @@ struct SV {
   };
 
   __device__ static void from_est(const ModelDesc&, float* th) {
-    th[1] = tanhf(th[1]);
-    th[2] = expf(th[2]);
-    th[3] = tanhf(th[3]);
-    th[5] = expf(th[5]);
+    th[1] = d_tanh(th[1]);
+    th[2] = d_exp(th[2]);
+    th[3] = d_tanh(th[3]);
+    th[5] = d_exp(th[5]);
   }
 
   __device__ static void prepare(const ModelDesc&, const float* th, P& p) {
@@ -189,7 +189,7 @@ struct SV {
     p.S0 = th[5];
     float omr2 = 1.0f - p.rho * p.rho;
     p.inv1mr2 = 1.0f / omr2;
-    p.c0 = -0.918938533f - 0.5f * logf(omr2);
+    p.c0 = -0.918938533f - 0.5f * d_log(omr2);
   }
 
   __device__ static void rinit(const ModelDesc&, const P& p, Key k, float, float* x) {
@@ -205,7 +205,7 @@ struct SV {
   }
 
   __device__ static float dmeas(const ModelDesc&, const P& p, const float* y, const float* x, float) {
-    float u = y[0] * expf(-0.5f * x[0]) - p.rho * x[1];
+    float u = y[0] * d_exp(-0.5f * x[0]) - p.rho * x[1];
     return p.c0 - 0.5f * x[0] - 0.5f * u * u * p.inv1mr2;
   }
 
@@ -215,7 +215,7 @@ struct SV {
   // d(dmeas)/dx into gx[S], accumulated d(dmeas)/dtheta into gth[d] scaled by w.
   __device__ static void dmeas_grad(const ModelDesc&, const P& p, const float* y, const float* x, float w,
                                     float* gx, double* gth) {
-    const float e = expf(-0.5f * x[0]);
+    const float e = d_exp(-0.5f * x[0]);
     const float ry = y[0] * e;
     const float u = ry - p.rho * x[1];
     const float k = p.inv1mr2;
@@ -245,13 +245,13 @@ struct SV {
 
   // d(natural)/d(estimation) for each parameter (diagonal transform), at est.
   __device__ static void from_est_jac(const ModelDesc&, const float* est, float* jac) {
-    const float ph = tanhf(est[1]), rh = tanhf(est[3]);
+    const float ph = d_tanh(est[1]), rh = d_tanh(est[3]);
     jac[0] = 1.f;
     jac[1] = 1.f - ph * ph;
-    jac[2] = expf(est[2]);
+    jac[2] = d_exp(est[2]);
     jac[3] = 1.f - rh * rh;
     jac[4] = 1.f;
-    jac[5] = expf(est[5]);
+    jac[5] = d_exp(est[5]);
   }
 };
 

@@ -94,7 +94,7 @@ __device__ void mop_grad_filter(const GradArgs& a, const int f, char* buf, doubl
   const float* dts = a.ser.dt_ext;
   const uint32_t* sch = a.sched + 2 * (int64_t)f * a.sched_len;
   const Key fkey{a.keys[2 * f], a.keys[2 * f + 1]};
-  const float fJ = (float)J, logJ = logf(fJ), invJ = 1.0f / fJ;
+  const float fJ = (float)J, logJ = d_log(fJ), invJ = 1.0f / fJ;
 
   float th_est[M::MAXP], th[M::MAXP];
   MP_UNROLL for (int p = 0; p < M::MAXP; ++p) if (p < d) {

@@ -37,7 +37,7 @@ __global__ __launch_bounds__(256) void k_plain_prop(const PlainArgs a) {
   const float* y = a.ser.ys + (o0 + t) * md.dy;
   const int64_t base = ti0 + (int64_t)t;
   const float fJ = (float)J;
-  const float logJ = logf(fJ);
+  const float logJ = d_log(fJ);
 
   float th[M::MAXP];
   MP_UNROLL for (int p = 0; p < M::MAXP; ++p) if (p < md.d) th[p] = a.theta[p];
@@ -48,7 +48,7 @@ __global__ __launch_bounds__(256) void k_plain_prop(const PlainArgs a) {
   const PlainState st = *a.st;
   const float* Xc = (t & 1) ? a.X1 : a.X0;
   float* Xn = (t & 1) ? a.X0 : a.X1;
-  const float wconst = t == 0 ? logf(1.0f / fJ) : st.wconst;
+  const float wconst = t == 0 ? d_log(1.0f / fJ) : st.wconst;
   const bool uniform_w = t == 0 ? true : st.uniform_w;
   const bool have_anc = t == 0 ? false : st.have_anc;
 
@@ -199,7 +199,7 @@ __global__ __launch_bounds__(256) void k_plain_scan(const PlainArgs a) {
   s = block_sum(s, red);
   if (tid == 0) {
     PlainState* st = a.st;
-    const float fJ = (float)J, logJ = logf(fJ);
+    const float fJ = (float)J, logJ = d_log(fJ);
     const bool collapsed = (m == -__builtin_inff());
     const double llt = collapsed ? -__builtin_inf() : (double)m + log(s);
     const float llf = (float)llt;

@@ -25,7 +25,7 @@ namespace mp {
 // single-precision erfinv, its argument w = -log((1-x)(1+x)) formed directly
 // as -log(4 u (1 - u)) so no precision is lost near the tails.
 __device__ __forceinline__ float ndtri_f32(float u) {
-  float w = -logf(4.0f * u * (1.0f - u));
+  float w = -d_log(4.0f * u * (1.0f - u));
   float x = 2.0f * u - 1.0f;
   float p;
   if (w < 5.0f) {
@@ -85,8 +85,8 @@ __device__ __forceinline__ float binominv_f32(float u, float n, float p) {
     const float ue = fminf(fmaxf(uf, 0.f), 1.f);
     const float qc = fminf(fmaxf(1.0f - ps, tiny), 1.0f);
     const float ratio = ps / qc;
-    const float logq = log1pf(-ps);
-    float pmf = (ns == 0.f) ? 1.0f : expf(ns * logq);
+    const float logq = d_log1p(-ps);
+    float pmf = (ns == 0.f) ? 1.0f : d_exp(ns * logq);
     float cdf = pmf;
     bool found = cdf >= ue;
     float res = found ? 0.f : k;
@@ -128,22 +128,23 @@ struct SEIR {
   static constexpr bool DIFFERENTIABLE = false;  // integer-valued transitions
   struct P {
     float beta, amp, sigma, gamma, rho, k, N, S0f, E0, I0;
-    float invN, lgk;
+    float invN;
+    double lgk;
   };
 
-  __device__ static float logit_inv(float x) { return 1.0f / (1.0f + expf(-x)); }
+  __device__ static float logit_inv(float x) { return 1.0f / (1.0f + d_exp(-x)); }
 
   __device__ static void from_est(const ModelDesc&, float* th) {
-    th[0] = expf(th[0]);
+    th[0] = d_exp(th[0]);
     th[1] = logit_inv(th[1]);
-    th[2] = expf(th[2]);
-    th[3] = expf(th[3]);
+    th[2] = d_exp(th[2]);
+    th[3] = d_exp(th[3]);
     th[4] = logit_inv(th[4]);
-    th[5] = expf(th[5]);
-    th[6] = expf(th[6]);
+    th[5] = d_exp(th[5]);
+    th[6] = d_exp(th[6]);
     th[7] = logit_inv(th[7]);
-    th[8] = expf(th[8]);
-    th[9] = expf(th[9]);
+    th[8] = d_exp(th[8]);
+    th[9] = d_exp(th[9]);
   }
 
   __device__ static void prepare(const ModelDesc&, const float* th, P& p) {
@@ -158,7 +159,7 @@ struct SEIR {
     p.E0 = th[8];
     p.I0 = th[9];
     p.invN = 1.0f / p.N;
-    p.lgk = lgammaf(p.k);
+    p.lgk = dd_lgamma((double)p.k);
   }
 
   __device__ static void rinit(const ModelDesc&, const P& p, Key, float, float* x) {
@@ -172,11 +173,11 @@ struct SEIR {
 
   __device__ static void step(const ModelDesc&, const P& p, Key key, float t, float dt, float* x) {
     int S = (int)x[0], E = (int)x[1], I = (int)x[2], R = (int)x[3], H = (int)x[4];
-    const float bt = p.beta * (1.0f + p.amp * cosf(6.28318531f * t / 52.0f));
+    const float bt = p.beta * (1.0f + p.amp * d_cos(6.28318531f * t / 52.0f));
     const float foi = bt * (float)I * p.invN;
-    const float pSE = -expm1f(-foi * dt);
-    const float pEI = -expm1f(-p.sigma * dt);
-    const float pIR = -expm1f(-p.gamma * dt);
+    const float pSE = -d_expm1(-foi * dt);
+    const float pEI = -d_expm1(-p.sigma * dt);
+    const float pIR = -d_expm1(-p.gamma * dt);
     const int dSE = fast_binomial_i(split_at(key, 0), S, pSE);
     const int dEI = fast_binomial_i(split_at(key, 1), E, pEI);
     const int dIR = fast_binomial_i(split_at(key, 2), I, pIR);
@@ -192,11 +193,15 @@ struct SEIR {
     x[4] = (float)H;
   }
 
+  // NegBin log pmf evaluated in float64 (its three log-gamma terms cancel to
+  // far below float32 resolution at counts ~1e4-1e5).
   __device__ static float dmeas(const ModelDesc&, const P& p, const float* y, const float* x, float) {
-    const float mu = fmaxf(x[4] * p.rho, 1e-10f);
-    const float yy = rintf(y[0]);
-    const float k = p.k;
-    return lgammaf(yy + k) - p.lgk - lgammaf(yy + 1.0f) + k * logf(k / (k + mu)) + yy * logf(mu / (k + mu));
+    const double mu = (double)fmaxf(x[4] * p.rho, 1e-10f);
+    const double yy = (double)rintf(y[0]);
+    const double k = (double)p.k;
+    const double v = dd_lgamma(yy + k) - p.lgk - dd_lgamma(yy + 1.0) + k * dd_log(k / (k + mu)) +
+                     yy * dd_log(mu / (k + mu));
+    return (float)v;
   }
 };
 

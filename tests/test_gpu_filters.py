@@ -101,14 +101,20 @@ def test_lg2_matches_oracle(form):
     import pypomp_amd.functional as F
     struct, theta, ys = lg2_struct(1)
     keys = tf.split(tf.key(1234), 3)
-    out = F.pfilter(struct, theta[None], 1000, keys.reshape(1, 3, 2), CLL=True, form=form)
+    flags = dict(CLL=True) if form == "plain" else dict(CLL=True, filter_mean=True, prediction_mean=True)
+    out = F.pfilter(struct, theta[None], 1000, keys.reshape(1, 3, 2), form=form, **flags)
     model = olg.LGModel(olg.LGSpec(2, 2, separate_init=True))
     ser = fl.make_series(ys, np.arange(1, ys.shape[0] + 1.0), 0.0, nstep=1)
     for r in range(3):
         o = fl.pfilter(model, theta, ser, keys[r], 1000, diagnostics=True)
         ll = float(out["logLik64"][0, r])
         assert abs(ll - float(o["logLik"])) <= 1e-5 * abs(float(o["logLik"])), (r, ll, o["logLik"])
-        np.testing.assert_allclose(out["CLL"][0, r], np.array(o["CLL"]), rtol=1e-4, atol=2e-4)
+        np.testing.assert_allclose(out["CLL"][0, r], np.array(o["CLL"]), rtol=1e-5, atol=1e-5)
+        if form == "batched":
+            # identical particle trajectories: same predicted and filtered means
+            np.testing.assert_allclose(out["prediction_mean"][0, r], np.array(o["prediction_mean"]),
+                                       rtol=1e-4, atol=1e-5)
+            np.testing.assert_allclose(out["filter_mean"][0, r], np.array(o["filter_mean"]), rtol=1e-4, atol=1e-5)
 
 
 def test_plain_and_batched_choose_the_same_ancestors():

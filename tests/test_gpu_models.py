@@ -67,7 +67,8 @@ def test_seir_pfilter_matches_oracle():
     for r in range(2):
         o = fl.pfilter(model, theta, ser, keys[r], 256, diagnostics=True)
         assert abs(float(out["logLik64"][0, r]) - float(o["logLik"])) <= 1e-5 * abs(float(o["logLik"]))
-        np.testing.assert_allclose(out["prediction_mean"][0, r], np.array(o["prediction_mean"]), rtol=1e-4)
+        # integer counts follow the same trajectories
+        np.testing.assert_allclose(out["prediction_mean"][0, r], np.array(o["prediction_mean"]), rtol=1e-6)
 
 
 def test_seir_counts_are_conserved():
