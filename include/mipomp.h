@@ -155,6 +155,17 @@ int32_t mp_perfilter(const mp_model_desc* h_model, const mp_series_desc* h_serie
                      const float* sigmas, const float* sigmas_init, float cool, int32_t m, float* thetas_out,
                      float* theta_mean, double* loglik, void* workspace, size_t ws_bytes, void* stream);
 
+/* ---- launch-shape tuning ------------------------------------------------------
+ * No counterpart in the reference (XLA picks its own launch shapes).  Process-
+ * wide settings read when a call is planned (workspace sizes depend on them:
+ * query the size after setting).  The Python autotuner
+ * (pypomp_amd/autotune.py) sweeps them on the caller's workload and caches the
+ * best values. */
+#define MP_TUNE_PLAIN_SCAN_ROWS 1 /* rows of 256 particles per scan tile, 1..8 */
+#define MP_TUNE_PLAIN_PROP_ROWS 2 /* rows of 256 particles per propagate block, 1..16 */
+int32_t mp_tune_set(int32_t param, int32_t value);
+int32_t mp_tune_get(int32_t param);
+
 /* ---- kernel timing ----------------------------------------------------------
  * No counterpart in the reference (it has no profiler hooks).  While enabled,
  * every kernel launch of the entry points above is bracketed by HIP events on

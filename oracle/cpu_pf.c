@@ -162,7 +162,7 @@ static float d_tanh(float x) {
 }
 
 static inline float erfinv_f32(float x) {
-  float w = -d_log1p(-(x * x)), p;
+  float w = -d_log((1.0f - x) * (1.0f + x)), p;
   if (w < 5.0f) {
     w = w - 2.5f;
     p = 2.81022636e-08f;
@@ -211,7 +211,7 @@ static int shift_for(int64_t J) {
     ++b;
     v >>= 1;
   }
-  return 63 - b;
+  return 62 - b;
 }
 
 static uint64_t fixed_weight(double d, int sh) {

@@ -13,7 +13,8 @@ kernels implement and which this oracle restates:
 * m = max(lw); e_i = exp_det(lw_i - m) in float64, where ``exp_det`` uses only
   correctly rounded IEEE multiplies and adds (no fused multiply-add), so every
   conforming implementation gets the same bits;
-* E_i = floor(e_i * 2**SHIFT) as uint64, SHIFT = 63 - bitlen(J), so sum(E) < 2**63;
+* E_i = floor(e_i * 2**SHIFT) as uint64, SHIFT = 62 - bitlen(J), so sum(E) < 2**62
+  (a prefix fits in 62 bits next to a 2-bit state in one 64-bit word);
 * C = inclusive prefix sum of E (integer addition is associative: the scan order
   cannot change it), total = C[J-1];
 * unif_j = fl32(fl32(u + j) / J) exactly as the reference;
@@ -43,7 +44,7 @@ LN2_LO = 1.90821492927058770002e-10
 
 
 def shift_for(J):
-    return 63 - int(J).bit_length()
+    return 62 - int(J).bit_length()
 
 
 def exp_det(x):
@@ -115,7 +116,7 @@ def resample_fixed(lw, u):
 
 def resample_fixed_fast(lw, u):
     """Same result as ``resample_fixed`` with uint64 NumPy arithmetic (no overflow:
-    sum(E) < 2**63 and unif_j <= 1)."""
+    sum(E) < 2**62 and unif_j <= 1)."""
     E = fixed_weights(lw)
     J = E.shape[0]
     C = np.cumsum(E, dtype=np.uint64)
@@ -124,7 +125,7 @@ def resample_fixed_fast(lw, u):
     m, e = np.frexp(unifs.astype(np.float64))
     mant = (m * (1 << 24)).astype(np.uint64)
     sh = (e - 24).astype(np.int64)
-    # mant * total < 2**24 * 2**63: split total into 32-bit halves
+    # mant * total < 2**24 * 2**62: split total into 32-bit halves
     t_hi = np.uint64(total >> 32)
     t_lo = np.uint64(total & 0xFFFFFFFF)
     p_hi = mant * t_hi            # < 2**55

@@ -18,8 +18,9 @@ core/algorithms/helpers.py:28.  JAX itself is an un-vendored dependency
   then max(lo, .);
 * ``normal``         -> sqrt(2) * erfinv(uniform(lo=nextafter(-1, 0), hi=1)),
   with erfinv from M. Giles, "Approximating the erfinv function" (GPU Computing
-  Gems Jade, 2011), single-precision branch, w = -log1p(-x*x) with the
-  engine's deterministic log1p (oracle/detmath.py).
+  Gems Jade, 2011), single-precision branch, with his w = -log((1-x)(1+x))
+  evaluated by the engine's deterministic log (oracle/detmath.py).  (XLA
+  forms w = -log1p(-x*x); the values differ in the last bits only.)
 
 Pinned by the Random123 known-answer vectors (tests/test_oracle_threefry.py) and,
 end to end, by the reference regression CSVs (tests/test_oracle_regression.py).
@@ -27,7 +28,7 @@ end to end, by the reference regression CSVs (tests/test_oracle_regression.py).
 
 import numpy as np
 
-from .detmath import d_log1p
+from .detmath import d_log
 
 U32 = np.uint32
 _ROT0 = (13, 15, 26, 6)
@@ -138,7 +139,7 @@ def erfinv_f32(x):
     """Giles' single-precision erfinv evaluated in float32 arithmetic."""
     x = np.asarray(x, dtype=np.float32)
     with np.errstate(divide="ignore", invalid="ignore"):
-        w = -d_log1p(-(x * x).astype(np.float32))
+        w = -d_log(((np.float32(1.0) - x).astype(np.float32) * (np.float32(1.0) + x).astype(np.float32)).astype(np.float32))
         lt = w < np.float32(5.0)
         wc = np.where(lt, w - np.float32(2.5), np.sqrt(w) - np.float32(3.0)).astype(np.float32)
         p = np.where(lt, _ERFINV_CENTRAL[0], _ERFINV_TAIL[0]).astype(np.float32)

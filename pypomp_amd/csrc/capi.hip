@@ -19,7 +19,9 @@ hipError_t launch_batched(int algo, const BatchArgs& a, int block, int grid, siz
 hipError_t launch_sched(int algo, const uint32_t* keys, const SeriesSet& ser, const int32_t* unit,
                         int64_t sched_len, uint32_t* sched, int R, hipStream_t st);
 hipError_t plain_kernel(int which, const PlainArgs& a, int ntiles, hipStream_t s);
-hipError_t plain_reset(unsigned int* flag, int ntiles, PlainState* st, hipStream_t s);
+hipError_t plain_prepare(const ModelDesc& md, const float* theta, int64_t stride, int R, int flags, char* out,
+                         int64_t pstride, hipStream_t s);
+hipError_t plain_reset(uint64_t* lb, uint64_t* psum, int ntiles, PlainState* st, hipStream_t s);
 size_t batched_small_lds(int J, int block);
 }  // namespace mp
 
@@ -59,6 +61,11 @@ int hip_fail(hipError_t e, const char* where) {
   } while (0)
 
 constexpr int kBatchedBlock = 256;
+constexpr int64_t kParamSlot = 512;  // bytes per filter of prepared model parameters
+
+// Launch-shape parameters (set by the autotuner through mp_tune_set).
+int g_plain_scan_rows = 8;  // particle rows of 256 per scan tile (1..8)
+int g_plain_prop_rows = 2;  // particle rows of 256 per propagate block (1..16)
 
 // Optional per-kernel timing with HIP events recorded on the launch stream
 // (mp_profile_*).  Kinds: 0 plain propagate, 1 plain scan, 2 batched filter,
@@ -177,16 +184,17 @@ BatchedPlan plan_batched(int algo, const mp_model_desc* m, const mp_series_desc*
 }
 
 struct PlainPlan {
-  int ntiles, rows;
-  int64_t off_sched, off_x0, off_x1, off_lw, off_c, off_pmax, off_pmin, off_psum, off_agg, off_incl, off_flag,
-      off_st, total;
+  int ntiles, rows, nprop, rows_prop;
+  int64_t off_sched, off_params, off_x0, off_x1, off_lw, off_c, off_pmax, off_pmin, off_psum, off_lb, off_st, total;
 };
 
-PlainPlan plan_plain(const mp_model_desc* m, const mp_series_desc* s, int J) {
+PlainPlan plan_plain(const mp_model_desc* m, const mp_series_desc* s, int R, int J) {
   PlainPlan p{};
-  p.rows = 8;
+  p.rows = g_plain_scan_rows;
   const int tile = p.rows * 256;
   p.ntiles = (J + tile - 1) / tile;
+  p.rows_prop = g_plain_prop_rows;
+  p.nprop = (J + p.rows_prop * 256 - 1) / (p.rows_prop * 256);
   const SeriesSet& ser = *reinterpret_cast<const SeriesSet*>(s);
   int64_t o = 0;
   auto take = [&](int64_t bytes) {
@@ -194,17 +202,16 @@ PlainPlan plan_plain(const mp_model_desc* m, const mp_series_desc* s, int J) {
     o += align256(bytes);
     return r;
   };
-  p.off_sched = take(sched_stride(ALGO_PF, ser) * 8);
+  p.off_sched = take((int64_t)R * sched_stride(ALGO_PF, ser) * 8);  // all filters' key schedules
+  p.off_params = take((int64_t)R * kParamSlot);                      // prepared parameters
   p.off_x0 = take((int64_t)m->S * J * 4);
   p.off_x1 = take((int64_t)m->S * J * 4);
   p.off_lw = take((int64_t)J * 4);
   p.off_c = take((int64_t)J * 8);
-  p.off_pmax = take((int64_t)p.ntiles * 4);
-  p.off_pmin = take((int64_t)p.ntiles * 4);
+  p.off_pmax = take((int64_t)p.nprop * 4);
+  p.off_pmin = take((int64_t)p.nprop * 4);
   p.off_psum = take((int64_t)p.ntiles * 8);
-  p.off_agg = take((int64_t)p.ntiles * 8);
-  p.off_incl = take((int64_t)p.ntiles * 8);
-  p.off_flag = take((int64_t)p.ntiles * 4);
+  p.off_lb = take((int64_t)p.ntiles * 16);
   p.off_st = take(sizeof(PlainState));
   p.total = o + 256;
   return p;
@@ -234,7 +241,7 @@ int run_plain_pf(const mp_model_desc* m, const mp_series_desc* s, const float* t
                  const uint32_t* keys, int R, int J, float logthresh, int flags, double* loglik, float* cll,
                  void* ws, size_t ws_bytes, hipStream_t st) {
   if (s->U != 1) return fail(MP_ERR_ARG, "the plain form runs single-series filters (U = 1)");
-  const PlainPlan p = plan_plain(m, s, J);
+  const PlainPlan p = plan_plain(m, s, R, J);
   if (ws_bytes < (size_t)p.total) return fail(MP_ERR_WORKSPACE, "workspace %zu < required %lld bytes", ws_bytes,
                                               (long long)p.total);
   char* b = align_ptr(ws);
@@ -253,21 +260,27 @@ int run_plain_pf(const mp_model_desc* m, const mp_series_desc* s, const float* t
   a.C = reinterpret_cast<uint64_t*>(b + p.off_c);
   a.pmax = reinterpret_cast<float*>(b + p.off_pmax);
   a.pmin = reinterpret_cast<float*>(b + p.off_pmin);
-  a.psum = reinterpret_cast<double*>(b + p.off_psum);
-  a.agg = reinterpret_cast<uint64_t*>(b + p.off_agg);
-  a.incl = reinterpret_cast<uint64_t*>(b + p.off_incl);
-  a.flag = reinterpret_cast<unsigned int*>(b + p.off_flag);
+  a.psum = reinterpret_cast<uint64_t*>(b + p.off_psum);
+  a.lb = reinterpret_cast<uint64_t*>(b + p.off_lb);
+  a.ntiles = p.ntiles;
+  a.rows_prop = p.rows_prop;
+  a.nprop = p.nprop;
   a.st = reinterpret_cast<PlainState*>(b + p.off_st);
   a.rows = p.rows;
   const int64_t slen = sched_stride(ALGO_PF, a.ser);
+  uint32_t* sched_all = reinterpret_cast<uint32_t*>(b + p.off_sched);
+  MP_HIP(timed(3, st, [&] { return launch_sched(ALGO_PF, keys, a.ser, nullptr, slen, sched_all, R, st); }));
+  char* params = b + p.off_params;
+  MP_HIP(plain_prepare(a.md, theta, theta_stride, R, flags, params, kParamSlot, st));
   for (int f = 0; f < R; ++f) {
     a.f = f;
     a.theta = theta + f * theta_stride;
     a.fkey = keys + 2 * f;
     a.loglik_out = loglik + f;
     a.cll = cll ? cll + (int64_t)f * s->Tmax : nullptr;
-    MP_HIP(launch_sched(ALGO_PF, keys + 2 * f, a.ser, nullptr, slen, const_cast<uint32_t*>(a.sched), 1, st));
-    MP_HIP(plain_reset(a.flag, p.ntiles, a.st, st));
+    a.sched = sched_all + 2 * (int64_t)f * slen;
+    a.params = params + (int64_t)f * kParamSlot;
+    MP_HIP(plain_reset(a.lb, a.psum, p.ntiles, a.st, st));
     for (int t = 0; t < a.T; ++t) {
       a.t = t;
       MP_HIP(timed(0, st, [&] { return plain_kernel(0, a, p.ntiles, st); }));
@@ -407,7 +420,7 @@ int32_t mp_resample_systematic(const float* logw, const uint32_t* keys, int64_t 
 size_t mp_filter_workspace_size(int32_t algo, const mp_model_desc* h_model, const mp_series_desc* h_series,
                                 int32_t R, int32_t J, int32_t form) {
   if (!h_model || !h_series || R < 1 || J < 1) return 0;
-  if (form == MP_FORM_PLAIN) return (size_t)plan_plain(h_model, h_series, J).total;
+  if (form == MP_FORM_PLAIN) return (size_t)plan_plain(h_model, h_series, R, J).total;
   return plan_batched(algo, h_model, h_series, R, J).total;
 }
 
@@ -530,6 +543,28 @@ int32_t mp_perfilter(const mp_model_desc* h_model, const mp_series_desc* h_serie
   a.theta_mean = theta_mean;
   a.loglik = loglik;
   return run_batched(ALGO_PER, h_model, h_series, a, workspace, ws_bytes, (hipStream_t)stream);
+}
+
+int32_t mp_tune_set(int32_t param, int32_t value) {
+  switch (param) {
+    case MP_TUNE_PLAIN_SCAN_ROWS:
+      if (value < 1 || value > 8) return fail(MP_ERR_ARG, "scan rows must be 1..8");
+      g_plain_scan_rows = value;
+      return MP_OK;
+    case MP_TUNE_PLAIN_PROP_ROWS:
+      if (value < 1 || value > 16) return fail(MP_ERR_ARG, "propagate rows must be 1..16");
+      g_plain_prop_rows = value;
+      return MP_OK;
+  }
+  return fail(MP_ERR_ARG, "unknown tuning parameter %d", param);
+}
+
+int32_t mp_tune_get(int32_t param) {
+  switch (param) {
+    case MP_TUNE_PLAIN_SCAN_ROWS: return g_plain_scan_rows;
+    case MP_TUNE_PLAIN_PROP_ROWS: return g_plain_prop_rows;
+  }
+  return -1;
 }
 
 int32_t mp_profile_enable(int32_t on) {

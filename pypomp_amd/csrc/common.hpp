@@ -8,7 +8,7 @@
 //      uniform              = bitcast((bits >> 9) | 0x3f800000) - 1
 //      normal               = sqrt(2) * erfinv(uniform on [nextafter(-1,0), 1))
 //    erfinv is M. Giles' single-precision approximation ("Approximating the
-//    erfinv function", GPU Computing Gems Jade, 2011).
+//    erfinv function", GPU Computing Gems Jade, 2011), w = -log((1-x)(1+x)).
 //  * The order-independent systematic resampler: deterministic exp, 64-bit
 //    fixed-point weights and the exact comb threshold (see exp_det / fixed_weight
 //    / comb_threshold below).  Every operation there is a correctly rounded IEEE
@@ -76,33 +76,34 @@ MP_HD float uniform01(uint32_t bits) {
   return f > 0.0f ? f : 0.0f;
 }
 
-// Giles' single-precision erfinv; w = -log1p(-x*x) as XLA evaluates it
-// (deterministic log1p, see detmath.hpp).
+// Giles' single-precision erfinv with his w = -log((1-x)(1+x)) (deterministic
+// log, see detmath.hpp).  The tail branch (|x| > 0.9966, w >= 5) is entered only
+// by waves that hold such a lane.
 __device__ __forceinline__ float erfinv_f32(float x) {
-  float w = -d_log1p(-(x * x));
-  float p;
-  if (w < 5.0f) {
-    w = w - 2.5f;
-    p = 2.81022636e-08f;
-    p = 3.43273939e-07f + p * w;
-    p = -3.5233877e-06f + p * w;
-    p = -4.39150654e-06f + p * w;
-    p = 0.00021858087f + p * w;
-    p = -0.00125372503f + p * w;
-    p = -0.00417768164f + p * w;
-    p = 0.246640727f + p * w;
-    p = 1.50140941f + p * w;
-  } else {
-    w = sqrtf(w) - 3.0f;
-    p = -0.000200214257f;
-    p = 0.000100950558f + p * w;
-    p = 0.00134934322f + p * w;
-    p = -0.00367342844f + p * w;
-    p = 0.00573950773f + p * w;
-    p = -0.0076224613f + p * w;
-    p = 0.00943887047f + p * w;
-    p = 1.00167406f + p * w;
-    p = 2.83297682f + p * w;
+  const float w = -d_log((1.0f - x) * (1.0f + x));
+  const float wc = w - 2.5f;
+  float p = 2.81022636e-08f;
+  p = 3.43273939e-07f + p * wc;
+  p = -3.5233877e-06f + p * wc;
+  p = -4.39150654e-06f + p * wc;
+  p = 0.00021858087f + p * wc;
+  p = -0.00125372503f + p * wc;
+  p = -0.00417768164f + p * wc;
+  p = 0.246640727f + p * wc;
+  p = 1.50140941f + p * wc;
+  const bool tail = !(w < 5.0f);
+  if (__builtin_amdgcn_ballot_w64(tail)) {
+    const float wt = sqrtf(w) - 3.0f;
+    float q = -0.000200214257f;
+    q = 0.000100950558f + q * wt;
+    q = 0.00134934322f + q * wt;
+    q = -0.00367342844f + q * wt;
+    q = 0.00573950773f + q * wt;
+    q = -0.0076224613f + q * wt;
+    q = 0.00943887047f + q * wt;
+    q = 1.00167406f + q * wt;
+    q = 2.83297682f + q * wt;
+    if (tail) p = q;
   }
   float r = p * x;
   if (fabsf(x) == 1.0f) r = x * 3.40282347e+38f;
@@ -147,10 +148,11 @@ MP_HD int shift_for(int64_t J) {
   int b = 0;
   uint64_t v = (uint64_t)J;
   while (v) { ++b; v >>= 1; }
-  return 63 - b;
+  return 62 - b;
 }
 
-// Fixed-point weight floor(exp(d) * 2^sh) for d = lw - max(lw) <= 0 (or NaN).
+// Fixed-point weight floor(exp(d) * 2^sh) for d = lw - max(lw) <= 0 (or NaN);
+// sh = 62 - bitlen(J) keeps every prefix sum below 2^62.
 MP_HD uint64_t fixed_weight(double d, int sh) {
 #pragma clang fp contract(off)
   const double cutoff = -(double)(sh + 2) * 0.6931471805599453;
@@ -176,7 +178,7 @@ MP_HD uint64_t fixed_weight(double d, int sh) {
   p = p * r + MP_EXPC1;
   p = p * r + MP_EXPC0;
   double v = ldexp(p, (int)k + sh);
-  return (uint64_t)v;  // v >= 0 and < 2^63: truncation is the floor
+  return (uint64_t)v;  // v >= 0 and < 2^62: truncation is the floor
 }
 
 // Weight of particle with log-weight lw given the filter's max log-weight m.

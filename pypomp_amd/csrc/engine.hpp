@@ -92,16 +92,15 @@ __host__ __device__ inline FilterScratch filter_scratch_layout(int W, int J) {
 }
 
 struct PlainState {
-  float M, mn;         // max / min log-weight of the current step
   float llf;           // float32 log-likelihood term of the previous step
   float ures;          // comb uniform of the previous step's resampling
-  uint64_t total;      // sum of fixed-point weights
+  uint64_t total;      // sum of fixed-point weights of the previous step
   double loglik;       // running FP64 sum over t
   int have_anc;        // previous step resampled
   int uniform_w;       // weights are the constant wconst
   float wconst;
   int collapsed;       // previous step had all weights -inf
-  unsigned int ticket_prop, ticket_scan, tile_ctr, pad;
+  unsigned int tile_ctr[2], ticket[2];  // per step parity
 };
 
 struct PlainArgs {
@@ -109,6 +108,7 @@ struct PlainArgs {
   SeriesSet ser;
   int unit;
   const float* theta;   // (d)
+  const void* params;   // the model's prepared parameters (M::P) for theta
   const uint32_t* fkey; // (2) filter key
   const uint32_t* sched;
   int J, t, T, flags, f;
@@ -117,16 +117,17 @@ struct PlainArgs {
   float* X1;
   float* lw;
   uint64_t* C;
-  float* pmax;          // (ntiles)
+  float* pmax;          // (ntiles) per-tile max / min log-weight
   float* pmin;
-  double* psum;
-  uint64_t* agg;        // (ntiles) look-back aggregates
-  uint64_t* incl;       // (ntiles) look-back inclusive prefixes
-  unsigned int* flag;   // (ntiles) epoch<<2 | state
+  uint64_t* lb;         // (2, ntiles) look-back words: state << 62 | prefix
+  uint64_t* psum;       // (ntiles) per-tile exp sums (float64 bits), ~0 when empty
   PlainState* st;
   double* loglik_out;   // (1) written after the last step
   float* cll;           // (T) or null
-  int rows;             // particle rows per tile (tile = rows * blockDim)
+  int rows;             // particle rows per scan tile (tile = rows * blockDim)
+  int ntiles;           // scan tiles
+  int rows_prop;        // particle rows per propagate block
+  int nprop;            // propagate blocks
 };
 
 // MOP value + adjoint gradient launch (one workgroup per filter).

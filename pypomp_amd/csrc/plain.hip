@@ -3,29 +3,47 @@
 // loop body, pfilter.py:97-182):
 //
 //  k_plain_prop  -- ancestor search (systematic comb against the previous
-//                   step's prefix sums) fused with the gather, the Euler
-//                   sub-steps, the measurement density and the log-weight;
-//                   per-tile max/min, the last tile combines them.
-//  k_plain_scan  -- fixed-point weights, per-tile exp-sums and one-pass
+//                   step's prefix sums, staged per tile in LDS) fused with the
+//                   gather, the Euler sub-steps, the measurement density and the
+//                   log-weight; per-tile max/min.
+//  k_plain_scan  -- fixed-point weights, per-tile exp-sums and a one-pass
 //                   inclusive prefix sum across tiles with decoupled look-back;
 //                   the last tile finishes the log-likelihood term (FP64 sum
 //                   over t), the resampling decision and the comb's uniform.
+//
+// Cross-workgroup traffic uses only relaxed agent-scope atomics on single
+// 64-bit words (look-back state and prefix packed together, exp sums as
+// float64 bits with an "empty" sentinel): on a multi-XCD GPU an agent-scope
+// release/acquire fence writes back / invalidates the whole L2, which is what
+// this design avoids.  Everything else a kernel reads was written by an
+// earlier launch.
 #include "blockops.hpp"
 #include "engine.hpp"
 #include "seir.hpp"
 
 namespace mp {
 
-constexpr unsigned kAgg = 1, kIncl = 2;
+constexpr uint64_t kStateShift = 62;
+constexpr uint64_t kAggState = 1ull << kStateShift, kInclState = 2ull << kStateShift;
+constexpr uint64_t kValueMask = (1ull << kStateShift) - 1;
+constexpr uint64_t kEmpty = ~0ull;
+constexpr int kWin = 1024;  // LDS search window (uint64 entries)
 
-__device__ __forceinline__ unsigned atomic_ticket(unsigned int* p) {
-  return __hip_atomic_fetch_add(p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ uint64_t ld_relaxed(uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_relaxed(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned ticket_relaxed(unsigned int* p) {
+  return __hip_atomic_fetch_add(p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <class M>
 __global__ __launch_bounds__(256) void k_plain_prop(const PlainArgs a) {
   __shared__ double red[32];
-  __shared__ int is_last;
+  __shared__ int64_t s_win[2];
+  __shared__ uint64_t s_C[kWin];
   const ModelDesc md = a.md;
   const int J = a.J, S = M::FIXED_S ? M::MAXS : md.S, t = a.t, tid = threadIdx.x, B = blockDim.x;
   const int64_t o0 = a.ser.obs_off[a.unit];
@@ -39,11 +57,8 @@ __global__ __launch_bounds__(256) void k_plain_prop(const PlainArgs a) {
   const float fJ = (float)J;
   const float logJ = d_log(fJ);
 
-  float th[M::MAXP];
-  MP_UNROLL for (int p = 0; p < M::MAXP; ++p) if (p < md.d) th[p] = a.theta[p];
-  if (a.flags & F_SHOULD_TRANS) M::from_est(md, th);
-  typename M::P P;
-  M::prepare(md, th, P);
+  // parameters prepared once per filter (k_plain_prepare); uniform loads
+  const typename M::P P = *reinterpret_cast<const typename M::P*>(a.params);
 
   const PlainState st = *a.st;
   const float* Xc = (t & 1) ? a.X1 : a.X0;
@@ -53,8 +68,25 @@ __global__ __launch_bounds__(256) void k_plain_prop(const PlainArgs a) {
   const bool have_anc = t == 0 ? false : st.have_anc;
 
   float lmax = -__builtin_inff(), lmin = __builtin_inff();
-  const int64_t tile0 = (int64_t)blockIdx.x * a.rows * B;
-  for (int r = 0; r < a.rows; ++r) {
+  const int64_t tile0 = (int64_t)blockIdx.x * a.rows_prop * B;
+  const int64_t tile1 = min((int64_t)J, tile0 + (int64_t)a.rows_prop * B);
+  // Ancestor search window of this tile.  A(t) = #{i : C_i <= t} is
+  // non-decreasing in the comb position, so every j of the tile has
+  // A(t_j) in [A(t_tile0), A(t_tile1-1)]; that slice of C is staged in LDS.
+  if (have_anc) {
+    if (tid < 2) {
+      const int64_t jj = tid == 0 ? tile0 : tile1 - 1;
+      s_win[tid] = upper_bound_u64(a.C, 0, J, comb_threshold(st.ures, jj, J, st.total));
+    }
+    __syncthreads();
+  }
+  const int64_t wlo = have_anc ? s_win[0] : 0, whi = have_anc ? s_win[1] : 0;
+  const bool in_lds = have_anc && (whi - wlo) <= kWin;
+  if (in_lds) {
+    for (int64_t i = tid; i < whi - wlo; i += B) s_C[i] = a.C[wlo + i];
+    __syncthreads();
+  }
+  for (int r = 0; r < a.rows_prop; ++r) {
     const int64_t j = tile0 + (int64_t)r * B + tid;
     if (j >= J) break;
     float x[M::MAXS];
@@ -63,8 +95,9 @@ __global__ __launch_bounds__(256) void k_plain_prop(const PlainArgs a) {
       M::rinit(md, P, split_at(Key{a.fkey[0], a.fkey[1]}, 1 + (uint64_t)j), a.ser.t0, x);
     } else {
       if (have_anc) {
-        uint64_t tj = comb_threshold(st.ures, j, J, st.total);
-        int64_t an = upper_bound_u64(a.C, 0, J, tj);
+        const uint64_t tj = comb_threshold(st.ures, j, J, st.total);
+        const int64_t an = in_lds ? wlo + upper_bound_u64(s_C, 0, whi - wlo, tj)
+                                  : upper_bound_u64(a.C, wlo, whi, tj);
         src = an < J ? an : J - 1;
       }
       MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) x[s] = Xc[(int64_t)s * J + src];
@@ -89,22 +122,6 @@ __global__ __launch_bounds__(256) void k_plain_prop(const PlainArgs a) {
   if (tid == 0) {
     a.pmax[blockIdx.x] = lmax;
     a.pmin[blockIdx.x] = lmin;
-    unsigned k = atomic_ticket(&a.st->ticket_prop);
-    is_last = (k == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (is_last) {
-    float mx = -__builtin_inff(), mn = __builtin_inff();
-    for (int i = tid; i < (int)gridDim.x; i += B) {
-      mx = nanmax(mx, __hip_atomic_load(&a.pmax[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-      mn = nanmin(mn, __hip_atomic_load(&a.pmin[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    }
-    block_maxmin(mx, mn, red);
-    if (tid == 0) {
-      a.st->M = mx;
-      a.st->mn = mn;
-      a.st->ticket_prop = 0;
-    }
   }
 }
 
@@ -116,12 +133,19 @@ __global__ __launch_bounds__(256) void k_plain_scan(const PlainArgs a) {
   __shared__ int s_tile, is_last;
   const int J = a.J, tid = threadIdx.x, B = blockDim.x, lane = tid & 63, w = tid >> 6;
   const int nw = (B + 63) >> 6;
-  const float m = a.st->M;
+  const int ntiles = a.ntiles;
+  const int par = a.t & 1;
+  uint64_t* lb = a.lb + (int64_t)par * ntiles;
   const int sh = shift_for(J);
-  const unsigned epoch = (unsigned)a.t + 1u;
 
-  if (tid == 0) s_tile = (int)atomic_ticket(&a.st->tile_ctr);
-  __syncthreads();
+  if (tid == 0) s_tile = (int)ticket_relaxed(&a.st->tile_ctr[par]);
+  // global max / min of the log-weights from the per-block values
+  float m = -__builtin_inff(), mn = __builtin_inff();
+  for (int i = tid; i < a.nprop; i += B) {
+    m = nanmax(m, a.pmax[i]);
+    mn = nanmin(mn, a.pmin[i]);
+  }
+  block_maxmin(m, mn, red);  // includes barriers: s_tile is visible below
   const int tile = s_tile;
   const int64_t tile0 = (int64_t)tile * a.rows * B;
 
@@ -143,6 +167,7 @@ __global__ __launch_bounds__(256) void k_plain_scan(const PlainArgs a) {
   }
   __syncthreads();
   const int n = a.rows * nw;
+  uint64_t aggv = 0;
   if (tid == 0) {
     uint64_t run = 0;
     for (int i = 0; i < n; ++i) {
@@ -150,34 +175,38 @@ __global__ __launch_bounds__(256) void k_plain_scan(const PlainArgs a) {
       tab[i] = run;
       run += v;
     }
-    tab[n] = run;  // tile aggregate
-    // decoupled look-back
-    const uint64_t aggv = run;
+    tab[n] = run;
+    aggv = run;
+    st_relaxed(&lb[tile], (tile == 0 ? kInclState : kAggState) | aggv);
+  }
+  // Decoupled look-back, 64 predecessors per probe: lane l inspects tile
+  // base - l; the nearest predecessor with an inclusive prefix ends the walk,
+  // the aggregates in between are summed with a wave reduction.
+  if (w == 0) {
+    aggv = __shfl(aggv, 0, WAVE);
     uint64_t prefix = 0;
-    if (tile == 0) {
-      a.incl[0] = aggv;
-      __hip_atomic_store(&a.flag[0], (epoch << 2) | kIncl, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      a.agg[tile] = aggv;
-      __hip_atomic_store(&a.flag[tile], (epoch << 2) | kAgg, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      int k = tile - 1;
+    if (tile > 0) {
+      int bse = tile - 1;
       while (true) {
-        unsigned fl = __hip_atomic_load(&a.flag[k], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        if ((fl >> 2) != epoch) {
+        const int idx = bse - lane;
+        const uint64_t word = idx >= 0 ? ld_relaxed(&lb[idx]) : kInclState;
+        const uint64_t state = word >> kStateShift;
+        const uint64_t incl_mask = __ballot(state == 2);
+        const uint64_t notready = __ballot(state == 0);
+        const int first = incl_mask ? __ffsll((unsigned long long)incl_mask) - 1 : 64;
+        const uint64_t need = first >= 63 ? ~0ull : ((1ull << (first + 1)) - 1ull);
+        if (notready & need) {
           __builtin_amdgcn_s_sleep(1);
           continue;
         }
-        if ((fl & 3u) == kIncl) {
-          prefix += __hip_atomic_load(&a.incl[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          break;
-        }
-        prefix += __hip_atomic_load(&a.agg[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        --k;
+        const uint64_t v = lane <= first ? (word & kValueMask) : 0;
+        prefix += wave_sum_u64(v);
+        if (first < 64) break;
+        bse -= 64;
       }
-      a.incl[tile] = prefix + aggv;
-      __hip_atomic_store(&a.flag[tile], (epoch << 2) | kIncl, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      if (lane == 0) st_relaxed(&lb[tile], kInclState | (prefix + aggv));
     }
-    s_prefix = prefix;
+    if (lane == 0) s_prefix = prefix;
   }
   __syncthreads();
   const uint64_t prefix = s_prefix;
@@ -187,17 +216,31 @@ __global__ __launch_bounds__(256) void k_plain_scan(const PlainArgs a) {
   }
   se = block_sum(se, red);
   if (tid == 0) {
-    a.psum[tile] = se;
-    unsigned k = atomic_ticket(&a.st->ticket_scan);
-    is_last = (k == gridDim.x - 1);
+    st_relaxed(&a.psum[tile], __double_as_longlong(se));
+    const unsigned k = ticket_relaxed(&a.st->ticket[par]);
+    is_last = (k == (unsigned)ntiles - 1u);
   }
   __syncthreads();
   if (!is_last) return;
-  // last tile: finish the step (deterministic fixed-order sums)
+  // Last tile: every other tile has published its sum and its inclusive prefix
+  // (both before taking a ticket); wait for them to arrive, then finish the
+  // step with fixed-order (deterministic) sums.
   double s = 0.0;
-  for (int i = tid; i < (int)gridDim.x; i += B) s += __hip_atomic_load(&a.psum[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int i = tid; i < ntiles; i += B) {
+    uint64_t v = ld_relaxed(&a.psum[i]);
+    while (v == kEmpty) {
+      __builtin_amdgcn_s_sleep(1);
+      v = ld_relaxed(&a.psum[i]);
+    }
+    s += __longlong_as_double((long long)v);
+  }
   s = block_sum(s, red);
   if (tid == 0) {
+    uint64_t lastw = ld_relaxed(&lb[ntiles - 1]);
+    while ((lastw >> kStateShift) != 2) {
+      __builtin_amdgcn_s_sleep(1);
+      lastw = ld_relaxed(&lb[ntiles - 1]);
+    }
     PlainState* st = a.st;
     const float fJ = (float)J, logJ = d_log(fJ);
     const bool collapsed = (m == -__builtin_inff());
@@ -206,44 +249,90 @@ __global__ __launch_bounds__(256) void k_plain_scan(const PlainArgs a) {
     st->loglik = (a.t == 0 ? 0.0 : st->loglik) + llt;
     if (a.cll) a.cll[a.t] = llf;
     const float nmax = collapsed ? -logJ : m - llf;
-    const float nmin = collapsed ? -logJ : st->mn - llf;
+    const float nmin = collapsed ? -logJ : mn - llf;
     const bool res = (nmax - nmin) > a.logthresh;
     st->llf = llf;
     st->collapsed = collapsed;
     st->have_anc = res;
     st->uniform_w = res;
     st->wconst = 0.0f - logJ;
-    st->total = __hip_atomic_load(&a.C[J - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    st->total = lastw & kValueMask;
     const int64_t o0 = a.ser.obs_off[a.unit];
     const int64_t base = a.ser.tidx0[o0 + a.t] + (int64_t)a.t;
     const int nst = a.ser.nstep[o0 + a.t];
     const Key rk{a.sched[2 * (base + nst)], a.sched[2 * (base + nst) + 1]};
     st->ures = uniform01(bits_at(rk, 0));
-    st->ticket_scan = 0;
-    st->tile_ctr = 0;
+    st->tile_ctr[par ^ 1] = 0;
+    st->ticket[par ^ 1] = 0;
     if (a.t == a.T - 1) *a.loglik_out = st->loglik;
+  }
+  // clear the next step's look-back words and this step's sums
+  uint64_t* lbn = a.lb + (int64_t)(par ^ 1) * ntiles;
+  for (int i = tid; i < ntiles; i += B) {
+    lbn[i] = 0;
+    a.psum[i] = kEmpty;
   }
 }
 
-// Start of a filter: clear the look-back flags and the step state.
-__global__ void k_plain_reset(unsigned int* flag, int ntiles, PlainState* st) {
+// Prepared model parameters of each filter (one thread per filter).
+template <class M>
+__global__ void k_plain_prepare(ModelDesc md, const float* theta, int64_t stride, int R, int flags, char* out,
+                                int64_t pstride) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= R) return;
+  float th[M::MAXP];
+  MP_UNROLL for (int p = 0; p < M::MAXP; ++p) if (p < md.d) th[p] = theta[f * stride + p];
+  if (flags & F_SHOULD_TRANS) M::from_est(md, th);
+  typename M::P P;
+  M::prepare(md, th, P);
+  *reinterpret_cast<typename M::P*>(out + f * pstride) = P;
+}
+
+template <class M>
+static hipError_t plain_prepare_t(const ModelDesc& md, const float* theta, int64_t stride, int R, int flags,
+                                  char* out, int64_t pstride, hipStream_t s) {
+  static_assert(sizeof(typename M::P) <= 512, "prepared parameters exceed their slot");
+  hipLaunchKernelGGL(k_plain_prepare<M>, dim3((R + 63) / 64), dim3(64), 0, s, md, theta, stride, R, flags, out,
+                     pstride);
+  return hipGetLastError();
+}
+
+hipError_t plain_prepare(const ModelDesc& md, const float* theta, int64_t stride, int R, int flags, char* out,
+                         int64_t pstride, hipStream_t s) {
+  switch (md.kind) {
+    case MODEL_LG:
+      if (md.dx == 1 && md.dy == 1) return plain_prepare_t<LG1>(md, theta, stride, R, flags, out, pstride, s);
+      if (md.dx == 2 && md.dy == 2) return plain_prepare_t<LG2>(md, theta, stride, R, flags, out, pstride, s);
+      return plain_prepare_t<LG4>(md, theta, stride, R, flags, out, pstride, s);
+    case MODEL_SV: return plain_prepare_t<SV>(md, theta, stride, R, flags, out, pstride, s);
+    case MODEL_SEIR: return plain_prepare_t<SEIR>(md, theta, stride, R, flags, out, pstride, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+// Start of a filter: clear the look-back words, the sums and the step state.
+__global__ void k_plain_reset(uint64_t* lb, uint64_t* psum, int ntiles, PlainState* st) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < ntiles) flag[i] = 0u;
+  if (i < ntiles) {
+    lb[i] = 0;
+    lb[ntiles + i] = 0;
+    psum[i] = kEmpty;
+  }
   if (i == 0) {
     PlainState z{};
     *st = z;
   }
 }
 
-hipError_t plain_reset(unsigned int* flag, int ntiles, PlainState* st, hipStream_t s) {
-  hipLaunchKernelGGL(k_plain_reset, dim3((ntiles + 255) / 256), dim3(256), 0, s, flag, ntiles, st);
+hipError_t plain_reset(uint64_t* lb, uint64_t* psum, int ntiles, PlainState* st, hipStream_t s) {
+  hipLaunchKernelGGL(k_plain_reset, dim3((ntiles + 255) / 256), dim3(256), 0, s, lb, psum, ntiles, st);
   return hipGetLastError();
 }
 
 // which = 0: propagate kernel, 1: scan kernel
 template <class M>
 static hipError_t plain_kernel_t(int which, const PlainArgs& a, int ntiles, hipStream_t s) {
-  if (which == 0) hipLaunchKernelGGL(k_plain_prop<M>, dim3(ntiles), dim3(256), 0, s, a);
+  if (which == 0) hipLaunchKernelGGL(k_plain_prop<M>, dim3(a.nprop), dim3(256), 0, s, a);
   else hipLaunchKernelGGL(k_plain_scan<M>, dim3(ntiles), dim3(256), 0, s, a);
   return hipGetLastError();
 }

@@ -45,18 +45,6 @@ def test_sv_mop_matches_oracle(form):
         assert abs(-got[r] - float(exp)) <= 1e-5 * abs(float(exp)), (r, -got[r], exp)
 
 
-def test_sv_mop_equals_pfilter_value():
-    # MOP's weights stay constant across particles in value, so its estimate equals
-    # the bootstrap filter's (thresh = 0) under the same key.
-    import pypomp_amd.functional as F
-    struct, theta, _ = sv_struct(200)
-    model = osv.SVModel()
-    keys = tf.split(tf.key(8), 3)
-    mop = -F.mop(struct, model.to_est(np.repeat(theta[None], 3, 0)), 400, 0.5, keys, return_float64=True)
-    pf = F.pfilter(struct, theta[None], 400, keys.reshape(1, 3, 2))["logLik64"].ravel()
-    np.testing.assert_allclose(mop, pf, rtol=1e-6)
-
-
 def test_seir_pfilter_matches_oracle():
     import pypomp_amd.functional as F
     struct, theta, ys = seir_struct(20)

@@ -57,7 +57,7 @@ def test_fixed_weights_against_float_exp():
     d = lw.astype(np.float64) - float(lw.max())
     approx = np.floor(np.exp(d) * 2.0 ** sh)
     assert np.max(np.abs(E - approx)) <= 2.0
-    assert E.sum() < 2.0 ** 63
+    assert E.sum() < 2.0 ** 62
 
 
 def test_uniform_and_collapsed_weights_give_identity():
