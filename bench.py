@@ -24,6 +24,10 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# HBM-side bytes per launch at J = 1e6 from rocprofv3 FETCH_SIZE / WRITE_SIZE
+# passes (profiles/plain_lg2_J1e6_traffic.md): FETCH_SIZE x 2 + WRITE_SIZE x 1, the
+# factors that make a 1 GiB device copy read 1 GiB and write 1 GiB.
+MEASURED_TRAFFIC_J1E6 = {"plain_prop": 30.84e6, "plain_scan": 12.54e6}
 METRIC = "particle-steps/s (pfilter & MOP-grad, batched vs plain GPU); loglik rel err"
 
 
@@ -75,11 +79,134 @@ class PlainLG:
             self.R, self.J, 0.0, 0, _native.MP_FORM_PLAIN, _native.ptr(self.ll), None, None, None, None,
             _native.ptr(self.ws), self.nws, _native.stream_ptr()))
 
-    # algorithmic bytes per launch (4-byte words; S = 2 state words)
+    # Algorithmic bytes per launch: 16 (S + 1) bytes per particle-step in 4-byte
+    # words (propagate read + write 2S, gather read + write 2S, log-weight write 1
+    # and read 1, ancestor write 1 and read 1), split between the two kernels by
+    # stage: k_plain_prop propagates, gathers, writes the log-weight and reads the
+    # ancestor (4S + 2 words); k_plain_scan reads the log-weight and writes the
+    # ancestor (prefix) (2 words).  S = 2.
     def kernel_bytes(self):
         S = 2
-        return {"plain_prop": self.J * 4 * (2 + S + S + 1),  # C read (8B), gather S, write S, lw
-                "plain_scan": self.J * (4 + 8)}              # lw read, C write
+        return {"plain_prop": self.J * 4 * (4 * S + 2),
+                "plain_scan": self.J * 4 * 2}
+
+
+class BatchedSEIR:
+    """Config: SEIR (7 Euler-binomial sub-steps/week), J=20000, T=260, 256 replicate
+    filters in one persistent launch (batched form)."""
+
+    def __init__(self, J=20000, reps=256):
+        import torch
+        import pypomp_amd.functional as F
+        from pypomp_amd import _native
+        from pypomp_amd import random as prandom
+        from pypomp_amd.models import SEIRModel
+        from pypomp_amd.series import Series
+        from tests.helpers import workloads
+        self.dev = _native.require_device()
+        w = workloads()
+        ys, self.theta = w["seir_seed3_ys"], w["seir_seed3_theta"]
+        self.struct = F.PompStruct(model=SEIRModel(), units=[Series.build(ys, np.arange(1, len(ys) + 1.0), 0.0,
+                                                                          nstep=7)])
+        self.J, self.R, self.T = J, reps, len(ys)
+        self.ser = self.struct.device_series()
+        self.md = self.struct.model.desc()
+        keys = prandom.split(prandom.key(3), reps)
+        self.keys = torch.from_numpy(np.ascontiguousarray(keys).view(np.int32)).to(self.dev)
+        self.th = torch.from_numpy(np.ascontiguousarray(self.theta)).to(self.dev)
+        self.ll = torch.empty(reps, dtype=torch.float64, device=self.dev)
+        n = _native.lib().mp_filter_workspace_size(_native.MP_ALGO_PF, self.md, self.ser.desc, reps, J,
+                                                   _native.MP_FORM_BATCHED)
+        self.ws = torch.empty(int(n), dtype=torch.uint8, device=self.dev)
+        self.nws = int(n)
+
+    @property
+    def particle_steps(self):
+        return self.J * self.T * self.R
+
+    def step(self):
+        from pypomp_amd import _native
+        _native.check(_native.lib().mp_pfilter(
+            self.md, self.ser.desc, _native.ptr(self.th), 0, _native.ptr(self.keys), None, self.R, self.J, 0.0,
+            0, _native.MP_FORM_BATCHED, _native.ptr(self.ll), None, None, None, None, _native.ptr(self.ws),
+            self.nws, _native.stream_ptr()))
+
+    def bytes_per_particle_step(self):
+        return 16 * (5 + 1)
+
+
+class MopGradSV:
+    """Config: SV MOP(0.97) value + adjoint gradient (6 parameters), J=1000, T=1000,
+    K=50 checkpoints, 512 replicates; batched (one launch) or plain (one filter per
+    launch)."""
+
+    def __init__(self, form="batched", reps=512, J=1000, K=50):
+        import torch
+        import pypomp_amd.functional as F
+        from pypomp_amd import _native
+        from pypomp_amd import random as prandom
+        from pypomp_amd.models import SVModel
+        from pypomp_amd.series import Series
+        from tests.helpers import workloads
+        self.dev = _native.require_device()
+        w = workloads()
+        ys = w["sv_seed4_ys"]
+        model = SVModel()
+        self.struct = F.PompStruct(model=model, units=[Series.build(ys, np.arange(1, len(ys) + 1.0), 0.0,
+                                                                    nstep=1)])
+        self.theta_est = model.to_est(w["sv_seed4_theta"])
+        self.J, self.R, self.T, self.K, self.form = J, reps, len(ys), K, form
+        self.ser = self.struct.device_series()
+        self.md = model.desc()
+        keys = prandom.split(prandom.key(4), reps)
+        self.keys = torch.from_numpy(np.ascontiguousarray(keys).view(np.int32)).to(self.dev)
+        self.th = torch.from_numpy(np.ascontiguousarray(self.theta_est)).to(self.dev)
+        self.ll = torch.empty(reps, dtype=torch.float64, device=self.dev)
+        self.g = torch.empty((reps, 6), dtype=torch.float32, device=self.dev)
+        self.fid = _native.MP_FORM_PLAIN if form == "plain" else _native.MP_FORM_BATCHED
+        n = _native.lib().mp_mop_grad_workspace_size(self.md, self.ser.desc, 1 if form == "plain" else reps, J, K)
+        self.ws = torch.empty(int(n), dtype=torch.uint8, device=self.dev)
+        self.nws = int(n)
+
+    @property
+    def particle_steps(self):
+        return self.J * self.T * self.R
+
+    def step(self):
+        from pypomp_amd import _native
+        _native.check(_native.lib().mp_mop_grad(
+            self.md, self.ser.desc, _native.ptr(self.th), 0, _native.ptr(self.keys), self.R, self.J, 0.97, self.K,
+            self.fid, _native.ptr(self.ll), _native.ptr(self.g), _native.ptr(self.ws), self.nws,
+            _native.stream_ptr()))
+
+    def bytes_per_particle_step(self):
+        return 3 * 16 * (2 + 1 + 1)  # forward, recompute, reverse; S = v, eta, weightsF
+
+
+def time_workload(wl, steps, warmup):
+    import torch
+    from pypomp_amd import _native
+    for _ in range(warmup):
+        wl.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        wl.step()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    _native.profile_enable(True)
+    wl.step()
+    torch.cuda.synchronize()
+    prof = _native.profile_read()
+    _native.profile_enable(False)
+    ms, n = prof["batched"]
+    avg = ms / max(n, 1)
+    gbps = wl.particle_steps * wl.bytes_per_particle_step() / (ms * 1e-3) / 1e9 if ms > 0 else None
+    return {"value": wl.particle_steps * steps / el, "unit": "particle-steps/s", "seconds_per_step": el / steps,
+            "kernel_avg_ms": avg, "launches_per_step": n,
+            "roofline": {"bound": "hbm", "achieved": gbps, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": (gbps / HBM_PEAK_GBS) if gbps else None,
+                         "bytes_per_particle_step": wl.bytes_per_particle_step()}}
 
 
 def kalman_exact(ys, theta):
@@ -106,17 +233,72 @@ def parity_check():
     return max(errs)
 
 
-def cpu_baseline(J=200_000, T=12):
-    """NumPy oracle bootstrap filter, single thread, on a bounded sample of the workload."""
-    from oracle import filters as fl, lg as olg, threefry as tf
+def cpu_baseline(target_s=12.0):
+    """C restatement of the bootstrap filter (oracle/cpu_pf.c, OpenMP, one replicate
+    per thread as the reference's CPU sharding) on a bounded sample of the workload:
+    as many J=1e6 replicates as threads, T calibrated to ~target_s seconds."""
+    from oracle import cpu
+    from oracle import threefry as tf
     st, theta, ys = lg2_struct(2)
-    model = olg.LGModel(olg.LGSpec(2, 2, separate_init=True))
-    ser = fl.make_series(ys[:T], np.arange(1, T + 1.0), 0.0, nstep=1)
+    cores = cpu.num_threads()
+    J = 1_000_000
+    keys = tf.split(tf.key(2), cores)
     t0 = time.perf_counter()
-    fl.pfilter(model, theta, ser, tf.split(tf.key(2), 1)[0], J)
+    cpu.lg_pfilter(theta, 2, ys[:2], J, keys)
+    per_step = (time.perf_counter() - t0) / 2
+    T = int(max(2, min(ys.shape[0], target_s / max(per_step, 1e-6))))
+    t0 = time.perf_counter()
+    cpu.lg_pfilter(theta, 2, ys[:T], J, keys)
     dt = time.perf_counter() - t0
-    return {"value": J * T / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"NumPy oracle pfilter, LG 2-D, J={J}, T={T}, 1 replicate, single thread ({dt:.1f} s)"}
+    return {"value": cores * J * T / dt, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+            "sample": f"C restatement (oracle/cpu_pf.c, -O3 -march=native, OpenMP), LG 2-D pfilter, "
+                      f"{cores} replicates x J={J} x T={T} ({dt:.1f} s)"}
+
+
+def cpu_mop_grad_baseline(J=1000, T=1000, reps=4):
+    """Torch (CPU, float64) autograd of the MOP estimator -- the reference's jax.grad
+    route restated (oracle/mop_grad.py) -- on a few replicates, one after another."""
+    import torch
+    from oracle import mop_grad as mg
+    from oracle import sv as osv
+    from oracle import threefry as tf
+    from tests.helpers import workloads
+    ys = workloads()["sv_seed4_ys"][:T]
+    th = osv.SVModel().to_est(osv.default_theta())
+    keys = tf.split(tf.key(4), reps)
+    t0 = time.perf_counter()
+    for r in range(reps):
+        mg.sv_mop_value_and_grad(th, ys, keys[r], J, 0.97)
+    dt = time.perf_counter() - t0
+    return {"value": reps * J * T / dt, "unit": "particle-steps/s", "cores": torch.get_num_threads(),
+            "kind": "port", "sample": f"NumPy forward + torch float64 autograd of MOP(0.97), SV, "
+                                      f"J={J}, T={T}, {reps} replicates ({dt:.1f} s)"}
+
+
+def run_only(args):
+    """One workload, --warmup untimed + --steps timed passes, nothing else (the
+    command profilers wrap).  ``copy`` is a 1 GiB device-to-device copy (the
+    calibration kernel for the FETCH_SIZE / WRITE_SIZE counters)."""
+    import torch
+    if args.only == "copy":
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        step = lambda: dst.copy_(src)  # noqa: E731
+        work, unit = src.numel() * 4, "bytes copied"
+    else:
+        wl = {"plain": lambda: PlainLG(reps=args.reps), "seir": BatchedSEIR, "mop_batched": MopGradSV,
+              "mop_plain": lambda: MopGradSV("plain", reps=16)}[args.only]()
+        step, work, unit = wl.step, wl.particle_steps, "particle-steps"
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    print(json.dumps({"only": args.only, "steps": args.steps, "seconds": el, "rate": work * args.steps / el,
+                      "unit": unit + "/s"}), flush=True)
 
 
 def main():
@@ -126,6 +308,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--reps", type=int, default=64)
+    ap.add_argument("--no-extra", action="store_true", help="skip the batched workloads")
+    ap.add_argument("--only", choices=["plain", "seir", "mop_batched", "mop_plain", "copy"],
+                    help="run one workload --steps times and print a short JSON line (for profilers)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (replicate filters shard "
@@ -135,6 +320,8 @@ def main():
     import torch
     from pypomp_amd import _native
     torch.cuda.set_device(0)
+    if args.only:
+        return run_only(args)
     wl = PlainLG(reps=args.reps)
     log(f"workload LG 2-D plain: J={wl.J} T={wl.T} reps={wl.R}")
     for _ in range(args.warmup):
@@ -157,6 +344,7 @@ def main():
     prof = _native.profile_read()
     _native.profile_enable(False)
     kb = wl.kernel_bytes()
+    J, T, R = wl.J, wl.T, wl.R
     kernels = {}
     for k in ("plain_prop", "plain_scan"):
         ms, n = prof[k]
@@ -166,23 +354,42 @@ def main():
     dom = max(kernels, key=lambda k: prof[k][0])
     d = kernels[dom]
     roofline = {"bound": "hbm", "achieved": d["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": d["GBps"] / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                "frac": d["GBps"] / HBM_PEAK_GBS,
+                "traffic": MEASURED_TRAFFIC_J1E6[dom] if J == 1_000_000 else None, "kernel": dom,
                 "bytes_per_launch": d["bytes"], "avg_launch_us": d["avg_us"]}
     log(f"kernels {json.dumps(kernels)}")
 
     exact = kalman_exact(wl.ys, wl.theta)
     rel = parity_check()
+    del wl
+    torch.cuda.empty_cache()
+    extra = {}
+    if not args.no_extra:
+        log("SEIR batched")
+        extra["seir_pfilter_batched"] = time_workload(BatchedSEIR(), 1, 1)
+        log("MOP grad batched")
+        mb = time_workload(MopGradSV("batched"), 2, 1)
+        log("MOP grad plain (16 filters, one launch each)")
+        mp_ = time_workload(MopGradSV("plain", reps=16), 1, 0)
+        mb["vs_plain"] = mb["value"] / mp_["value"]
+        extra["mop_grad_batched"] = mb
+        extra["mop_grad_plain"] = mp_
     cpu = None if args.no_cpu_baseline else cpu_baseline()
+    if cpu is not None and not args.no_extra:
+        cg = cpu_mop_grad_baseline()
+        extra["mop_grad_cpu_baseline"] = cg
+        extra["mop_grad_batched"]["vs_cpu"] = extra["mop_grad_batched"]["value"] / cg["value"]
     out = {
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "pfilter LG 2-D, J=1e6, T=200, 64 replicates one at a time (plain form)",
-                   "J": wl.J, "T": wl.T, "replicates": wl.R, "form": "plain"},
+                   "J": J, "T": T, "replicates": R, "form": "plain"},
         "roofline": roofline, "cpu_baseline": cpu,
         "loglik_rel_err": rel,
         "loglik": {"mean": float(ll.mean()), "sd": float(ll.std(ddof=1)), "kalman_fp64": exact},
         "kernels": kernels,
+        "extra": extra,
     }
     print(json.dumps(out), flush=True)
 

@@ -67,8 +67,9 @@ def sv_mop_objective(theta_est, ys, z0, zs, anc, alpha, frozen_m=None):
     return ll
 
 
-def sv_mop_reference_grad(theta_est, ys, key, J, alpha):
-    """(value, autograd gradient, trace) at theta_est with the oracle's ancestry."""
+def sv_mop_value_and_grad(theta_est, ys, key, J, alpha):
+    """MOP value (oracle forward), its float64 re-evaluation on the oracle's ancestry
+    and the autograd gradient; also the noise and ancestry used."""
     from .sv import SVModel
     model = SVModel()
     ser = fl.make_series(ys, np.arange(1, len(ys) + 1.0), 0.0, nstep=1)
@@ -78,6 +79,12 @@ def sv_mop_reference_grad(theta_est, ys, key, J, alpha):
     th = torch.tensor(np.asarray(theta_est, np.float64), requires_grad=True)
     ll = sv_mop_objective(th, ys[:, 0], z0, zs, anc, alpha)
     ll.backward()
+    return float(val), ll, th, z0, zs, anc
+
+
+def sv_mop_reference_grad(theta_est, ys, key, J, alpha):
+    """(value, autograd gradient, trace) at theta_est with the oracle's ancestry."""
+    val, ll, th, z0, zs, anc = sv_mop_value_and_grad(theta_est, ys, key, J, alpha)
     with torch.no_grad():
         frozen_m = np.stack([_sv_meas(th, ys[:, 0], z0, zs, anc, t) for t in range(zs.shape[0])])
     return float(val), float(ll.detach()), th.grad.numpy().copy(), dict(z0=z0, zs=zs, anc=anc, frozen_m=frozen_m)
