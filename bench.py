@@ -183,6 +183,64 @@ class MopGradSV:
         return 3 * 16 * (2 + 1 + 1)  # forward, recompute, reverse; S = v, eta, weightsF
 
 
+class PanelIF2:
+    """Config: IF2 on a panel of 100 SEIR units of uneven length (T_u in
+    [52, 520] weeks, seed-5 data, unit-specific beta), J=2000, 8 chains per unit,
+    M=50 iterations, random-walk sd 0.02 on the estimation scale (S0f, E0, I0
+    perturbed at t0 only), geometric cooling 0.5.  One step = the whole M=50
+    run: 50 launches of 800 swarms, recorded as one HIP graph and replayed."""
+
+    def __init__(self, J=2000, chains=8, M=50):
+        import torch
+        import pypomp_amd as pp
+        import pypomp_amd.functional as F
+        from pypomp_amd import random as prandom
+        from pypomp_amd.models import SEIRModel
+        from pypomp_amd.series import Series
+        from tests.helpers import workloads
+        w = workloads()
+        off, beta, ys = w["panel_off"], w["panel_beta"], w["panel_ys"]
+        U = len(beta)
+        model = SEIRModel()
+        units = [Series.build(ys[off[u]:off[u + 1]][:, None], np.arange(1, off[u + 1] - off[u] + 1.0), 0.0, nstep=7)
+                 for u in range(U)]
+        self.struct = F.PompStruct(model=model, units=units)
+        names = model.param_names
+        rw = pp.RWSigma({p: 0.02 for p in names}, init_names=["S0f", "E0", "I0"]).geometric_cooling(0.5)
+        rw = rw.canonicalize(names)
+        theta = np.array([pp.models.SEIR_DEFAULTS[p] for p in names], dtype=np.float32)
+        th = np.repeat(theta[None], U, 0)
+        th[:, 0] = beta
+        th0 = np.repeat(model.to_est(th)[None, :, None, :], chains, 0)          # (chains, U, 1, d)
+        th0 = np.ascontiguousarray(np.broadcast_to(th0, (chains, U, J, len(names))).reshape(chains * U, J, -1))
+        unit = np.tile(np.arange(U, dtype=np.int32), chains)
+        R = chains * U
+        keys = prandom.split(prandom.key(5), chains)
+        ik = np.stack([prandom.split(k, M + 1)[1:] for k in keys], axis=1)      # (M, chains, 2)
+        self.mkeys = np.stack([[prandom.split(k, U) for k in row] for row in ik]).reshape(M, R, 2)
+        self.run = F._PerRunner(self.struct, R, J, rw, 0.0, unit)
+        self.t0 = torch.from_numpy(th0).to(self.run.dev)
+        self.start = self.t0.clone()
+        self.J, self.R, self.M, self.U = J, R, M, U
+        self.sumT = int(off[-1])
+        self.graph = None
+
+    @property
+    def particle_steps(self):
+        return self.J * self.M * (self.R // self.U) * self.sumT
+
+    def step(self):
+        if self.graph is None:
+            self.out = self.run.iterate(self.start, self.mkeys, graph=True)
+            self.graph = self.run.graph
+        else:
+            self.start.copy_(self.t0)
+            self.graph.replay()
+
+    def bytes_per_particle_step(self):
+        return 16 * (5 + 10 + 1)
+
+
 def time_workload(wl, steps, warmup):
     import torch
     from pypomp_amd import _native
@@ -200,6 +258,8 @@ def time_workload(wl, steps, warmup):
     prof = _native.profile_read()
     _native.profile_enable(False)
     ms, n = prof["batched"]
+    if n == 0:  # launches inside a replayed graph are not event-bracketed
+        ms, n = el / steps * 1e3, getattr(wl, "M", 1)
     avg = ms / max(n, 1)
     gbps = wl.particle_steps * wl.bytes_per_particle_step() / (ms * 1e-3) / 1e9 if ms > 0 else None
     return {"value": wl.particle_steps * steps / el, "unit": "particle-steps/s", "seconds_per_step": el / steps,
@@ -287,7 +347,7 @@ def run_only(args):
         work, unit = src.numel() * 4, "bytes copied"
     else:
         wl = {"plain": lambda: PlainLG(reps=args.reps), "seir": BatchedSEIR, "mop_batched": MopGradSV,
-              "mop_plain": lambda: MopGradSV("plain", reps=16)}[args.only]()
+              "mop_plain": lambda: MopGradSV("plain", reps=16), "panel": PanelIF2}[args.only]()
         step, work, unit = wl.step, wl.particle_steps, "particle-steps"
     for _ in range(args.warmup):
         step()
@@ -309,7 +369,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--reps", type=int, default=64)
     ap.add_argument("--no-extra", action="store_true", help="skip the batched workloads")
-    ap.add_argument("--only", choices=["plain", "seir", "mop_batched", "mop_plain", "copy"],
+    ap.add_argument("--only", choices=["plain", "seir", "mop_batched", "mop_plain", "panel", "copy"],
                     help="run one workload --steps times and print a short JSON line (for profilers)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -374,6 +434,8 @@ def main():
         mb["vs_plain"] = mb["value"] / mp_["value"]
         extra["mop_grad_batched"] = mb
         extra["mop_grad_plain"] = mp_
+        log("panel IF2 (100 SEIR units, 8 chains, J=2000, M=50, one HIP graph)")
+        extra["panel_if2_graph"] = time_workload(PanelIF2(), 1, 1)
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     if cpu is not None and not args.no_extra:
         cg = cpu_mop_grad_baseline()

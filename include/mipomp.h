@@ -57,6 +57,9 @@ typedef struct mp_model_desc {
   int32_t lg_sep_init; /* LG: initial Cholesky factor P follows X0 in theta */
   uint32_t accum_mask; /* bit s: state word s is reset to 0 at each observation
                           interval (reference: accumvars, model_mechanics.py:921) */
+  int32_t identity_trans; /* 1: the estimation scale is the natural scale (the
+                          reference's ParTrans() with no transforms,
+                          core/par_trans.py:70-111); 0: the model's own transforms */
 } mp_model_desc;
 
 /* Observation series of U units (U = 1 for a single Pomp).  Unit u owns rows
@@ -149,11 +152,18 @@ int32_t mp_mop_grad(const mp_model_desc* h_model, const mp_series_desc* h_series
  * swarms.  thetas_in/thetas_out (R, J, d) on the estimation scale; sigmas and
  * sigmas_init (d) as RWSigma.sigmas_array / sigmas_init_array
  * (core/rw_sigma.py:184-191); geometric cooling factor cool = a^(1/50)
- * (rw_sigma.py:210-213).  theta_mean (R, d): mean of thetas_out over J. */
+ * (rw_sigma.py:210-213).  theta_mean (R, d): mean of thetas_out over J.
+ * ancestry (R, J) int32 or NULL: the composed resampling ancestry of the final
+ * particles (mif.py:237-245, return_ancestry; panel IF2 with block=False,
+ * core/algorithms/panel_mif.py:224-231).  Panel units of one call (unit[f])
+ * are independent filters; the panel driver (panel_mif.py:17-247) runs units
+ * that share parameters one call after another.  Filters are taken longest
+ * unit first from a device work queue, so uneven units balance over the CUs. */
 int32_t mp_perfilter(const mp_model_desc* h_model, const mp_series_desc* h_series, const float* thetas_in,
                      const uint32_t* keys, const int32_t* unit, int32_t R, int32_t J, float thresh,
                      const float* sigmas, const float* sigmas_init, float cool, int32_t m, float* thetas_out,
-                     float* theta_mean, double* loglik, void* workspace, size_t ws_bytes, void* stream);
+                     float* theta_mean, double* loglik, int32_t* ancestry, void* workspace, size_t ws_bytes,
+                     void* stream);
 
 /* ---- launch-shape tuning ------------------------------------------------------
  * No counterpart in the reference (XLA picks its own launch shapes).  Process-

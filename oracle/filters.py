@@ -219,9 +219,10 @@ class RWSigma:
         return F32(d_pow(factor, frac))
 
 
-def perfilter(model, m, thetas, series, key, J, rw, thresh=0.0, resampler="fixed"):
+def perfilter(model, m, thetas, series, key, J, rw, thresh=0.0, resampler="fixed", return_ancestry=False):
     """``_perfilter_internal`` + ``_perfilter_scan_body`` (core/algorithms/mif.py:88-258).
-    ``thetas`` (J, d) on the estimation scale.  Returns (thetas, loglik)."""
+    ``thetas`` (J, d) on the estimation scale.  Returns (thetas, loglik), plus the
+    composed resampling ancestry (mif.py:237-245) when ``return_ancestry``."""
     thetas = np.asarray(thetas, dtype=F32).copy()
     d = thetas.shape[1]
     T = series.T
@@ -240,6 +241,7 @@ def perfilter(model, m, thetas, series, key, J, rw, thresh=0.0, resampler="fixed
     logthresh = d_log(F32(thresh)) if thresh > 0 else -np.inf
     nan_sub = d_log(F32(1e-18))
     sig = rw.sigmas_array
+    anc = np.arange(J)
     for i in range(T):
         k3 = tf.split(step_keys[i], 3)
         kpert, kproc, kres = k3[0], k3[1], k3[2]
@@ -258,8 +260,9 @@ def perfilter(model, m, thetas, series, key, J, rw, thresh=0.0, resampler="fixed
             a = _resample(resampler, nw, w, kres)
             X = X[a]
             thetas = thetas[a]
+            anc = anc[a]
             nw = np.full(J, F32(0.0) - d_log(F32(J)), dtype=F32)
-    return thetas, loglik
+    return (thetas, loglik, anc) if return_ancestry else (thetas, loglik)
 
 
 def mif(model, thetas_nat, series, key, J, M, rw, thresh=0.0, n_monitors=0,
@@ -286,6 +289,84 @@ def mif(model, thetas_nat, series, key, J, M, rw, thresh=0.0, n_monitors=0,
         theta = new
     traces = np.stack(traces).astype(F32)
     return (np.array(lls, dtype=F32), model.from_est(traces), model.from_est(theta))
+
+
+def panel_mif(model, shared_nat, unit_nat, series_list, key, J, M, rw, perm, thresh=0.0, block=True,
+              resampler="fixed", par_trans=None):
+    """``_panel_mif_internal`` / ``_panel_mif_iter_body`` / ``_panel_mif_unit_scan_fn``
+    (core/algorithms/panel_mif.py:17-247) for one replicate, no monitors, with the
+    scale handling of ``F.panel_mif`` (functional/mif.py:212-294).
+
+    shared_nat (J, ns), unit_nat (J, U, nu) natural scale; ``perm`` gives, for
+    each model-order parameter, its index in the panel order (shared names,
+    then unit-specific names); ``rw`` is in model order.  ``par_trans`` (default:
+    the model) supplies the panel-level to_est/from_est (PanelPompStruct.par_trans,
+    panel/panel.py:146); the filter itself always applies the model's own
+    transforms, which the reference binds into the mechanics when the unit Pomp
+    is built (core/pomp.py:271-313).  Returns
+    (shared_traces (M+1, ns+1), unit_traces (M+1, U, nu+1), final shared, final
+    unit), natural scale, log-likelihood in column 0 (NaN at m = 0)."""
+    shared_nat = np.asarray(shared_nat, dtype=F32)
+    unit_nat = np.asarray(unit_nat, dtype=F32)
+    ns, U, nu = shared_nat.shape[1], unit_nat.shape[1], unit_nat.shape[2]
+    perm = np.asarray(perm)
+    inv = np.argsort(perm)
+    pt = model if par_trans is None else par_trans
+
+    def to_model(shared, unit):  # (..., ns), (..., U, nu) -> (..., U, d) model order
+        lead = unit.shape[:-2]
+        full = np.concatenate([np.broadcast_to(shared[..., None, :], lead + (U, ns)), unit], axis=-1)
+        return full[..., perm]
+
+    def to_panel(full):          # (..., U, d) model order -> shared (..., ns), unit (..., U, nu)
+        p = full[..., inv]
+        return p[..., 0, :ns], p[..., ns:]
+
+    sh, un = to_panel(pt.to_est(to_model(shared_nat, unit_nat)))
+    sh, un = sh.astype(F32).copy(), un.astype(F32).copy()
+    sh_tr = [np.concatenate([[np.nan], sh.mean(0, dtype=F32)]).astype(F32)]
+    un_tr = [np.concatenate([np.full((U, 1), np.nan, F32), un.mean(0, dtype=F32)], axis=1)]
+    mkeys = tf.split(key, M + 1)[1:]
+    for m in range(M):
+        ukeys = tf.split(mkeys[m], U)
+        rows, total = [], F32(0.0)
+        for u in range(U):
+            th = np.concatenate([sh, un[:, u, :]], axis=1)[:, perm]
+            new, ll, anc = perfilter(model, m, th, series_list[u], ukeys[u], J, rw, thresh, resampler,
+                                     return_ancestry=True)
+            upd = new[:, inv]
+            if ns:
+                sh = upd[:, :ns].copy()
+            if not block:
+                un = un[anc]
+            un = un.copy()
+            un[:, u, :] = upd[:, ns:]
+            rows.append(np.concatenate([[F32(ll)], upd[:, ns:].mean(0, dtype=F32)]))
+            total = F32(total + F32(ll))
+        un_tr.append(np.stack(rows).astype(F32))
+        sh_tr.append(np.concatenate([[total], sh.mean(0, dtype=F32)]).astype(F32))
+    sh_tr, un_tr = np.stack(sh_tr), np.stack(un_tr)
+    s_nat, u_nat = to_panel(pt.from_est(to_model(sh_tr[:, 1:], un_tr[..., 1:])))
+    sh_tr[:, 1:], un_tr[..., 1:] = s_nat, u_nat
+    fs, fu = to_panel(pt.from_est(to_model(sh, un)))
+    return sh_tr, un_tr, fs, fu
+
+
+class IdentityTrans:
+    """A model whose estimation scale is its natural scale (the reference's
+    ``ParTrans()`` with no transforms, core/par_trans.py:70-111)."""
+
+    def __init__(self, model):
+        self._m = model
+
+    def __getattr__(self, name):
+        return getattr(self._m, name)
+
+    def to_est(self, theta):
+        return np.array(theta, dtype=F32, copy=True)
+
+    def from_est(self, theta):
+        return np.array(theta, dtype=F32, copy=True)
 
 
 def simulate(model, theta, series, key, nsim=1):

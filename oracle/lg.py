@@ -90,7 +90,7 @@ class LGSpec:
     def to_est(self, theta):
         theta = np.array(theta, dtype=F32, copy=True)
         m = self._diag_mask()
-        theta[..., m] = d_log(theta[..., m])
+        theta[..., m] = np.log(theta[..., m])  # host-side transform, as the host layer
         return theta
 
     def from_est(self, theta):

@@ -31,7 +31,8 @@ MP_TUNE_PLAIN_SCAN_ROWS, MP_TUNE_PLAIN_PROP_ROWS = 1, 2
 
 class ModelDesc(C.Structure):
     _fields_ = [("kind", C.c_int32), ("S", C.c_int32), ("d", C.c_int32), ("dy", C.c_int32),
-                ("dx", C.c_int32), ("lg_sep_init", C.c_int32), ("accum_mask", C.c_uint32)]
+                ("dx", C.c_int32), ("lg_sep_init", C.c_int32), ("accum_mask", C.c_uint32),
+                ("identity_trans", C.c_int32)]
 
 
 class SeriesDesc(C.Structure):
@@ -76,7 +77,7 @@ def lib():
     L.mp_mop.argtypes = [C.POINTER(ModelDesc), C.POINTER(SeriesDesc), f32p, i64, u32p, vp, i32, i32,
                          C.c_float, i32, f64p, vp, sz, vp]
     L.mp_perfilter.argtypes = [C.POINTER(ModelDesc), C.POINTER(SeriesDesc), f32p, u32p, vp, i32, i32,
-                               C.c_float, f32p, f32p, C.c_float, i32, f32p, f32p, f64p, vp, sz, vp]
+                               C.c_float, f32p, f32p, C.c_float, i32, f32p, f32p, f64p, vp, vp, sz, vp]
     L.mp_mop_grad_workspace_size.argtypes = [C.POINTER(ModelDesc), C.POINTER(SeriesDesc), i32, i32, i32]
     L.mp_mop_grad_workspace_size.restype = sz
     L.mp_mop_grad.argtypes = [C.POINTER(ModelDesc), C.POINTER(SeriesDesc), f32p, i64, u32p, i32, i32, C.c_float,

@@ -84,9 +84,11 @@ __global__ void k_sched_per(const uint32_t* __restrict__ keys, SeriesSet ser, co
   st_key(out, pos, split_at(sk, 2));
 }
 
+// Per-particle words carried by a perturbed filter: state, parameters and
+// one ancestry word (the particle's original index, bit-cast to float).
 template <class M>
 struct WMax {
-  static constexpr int v = M::MAXS + M::MAXP;
+  static constexpr int v = M::MAXS + M::MAXP + 1;
 };
 
 template <class M, int ALGO>
@@ -94,7 +96,8 @@ __device__ void run_filter(const BatchArgs& a, const int f, char* buf, double* r
   constexpr int WM = (ALGO == ALGO_PER) ? WMax<M>::v : M::MAXS;
   const ModelDesc md = a.md;
   const int J = a.J, S = M::FIXED_S ? M::MAXS : md.S, d = md.d, dy = md.dy;
-  const int W = S + (ALGO == ALGO_PER ? d : 0);
+  const bool track = (ALGO == ALGO_PER) && a.ancestry != nullptr;
+  const int W = S + (ALGO == ALGO_PER ? d + (track ? 1 : 0) : 0);
   const int tid = threadIdx.x, B = blockDim.x;
   const int u = a.unit ? a.unit[f] : 0;
   const int64_t o0 = a.ser.obs_off[u];
@@ -116,7 +119,7 @@ __device__ void run_filter(const BatchArgs& a, const int f, char* buf, double* r
   if (ALGO != ALGO_PER) {
     float th[M::MAXP];
     MP_UNROLL for (int p = 0; p < M::MAXP; ++p) if (p < d) th[p] = a.theta[f * a.theta_stride + p];
-    if (trans) M::from_est(md, th);
+    if (trans && !md.identity_trans) M::from_est(md, th);
     M::prepare(md, th, P);
   }
 
@@ -143,7 +146,8 @@ __device__ void run_filter(const BatchArgs& a, const int f, char* buf, double* r
           x[S + p] = tin[p] + (c0 * a.sigmas_init[p]) * z;
           th[p] = x[S + p];
         }
-        M::from_est(md, th);
+        if (track) x[S + d] = __int_as_float(j);
+        if (!md.identity_trans) M::from_est(md, th);
         typename M::P Pj;
         M::prepare(md, th, Pj);
         M::rinit(md, Pj, pk, a.ser.t0, x);
@@ -201,7 +205,7 @@ __device__ void run_filter(const BatchArgs& a, const int f, char* buf, double* r
           x[S + p] = x[S + p] + (cft * a.sigmas[p]) * z;
           th[p] = x[S + p];
         }
-        M::from_est(md, th);
+        if (!md.identity_trans) M::from_est(md, th);
         M::prepare(md, th, Pj);
       } else {
         Pj = P;
@@ -315,6 +319,11 @@ __device__ void run_filter(const BatchArgs& a, const int f, char* buf, double* r
         mean[p] += (double)v;
       }
     }
+    if (track)
+      for (int j = tid; j < J; j += B) {
+        int src = have_anc ? ancestor_of(Cc, J, ures, total, j) : j;
+        a.ancestry[(int64_t)f * J + j] = __float_as_int(X[cur][(int64_t)(S + d) * J + src]);
+      }
     block_sum_n<M::MAXP>(mean, red);
     if (tid == 0)
       MP_UNROLL for (int p = 0; p < M::MAXP; ++p) if (p < d) a.theta_mean[(int64_t)f * d + p] = (float)(mean[p] / (double)J);
@@ -323,16 +332,50 @@ __device__ void run_filter(const BatchArgs& a, const int f, char* buf, double* r
   __syncthreads();
 }
 
+// Persistent workgroups take filters from the queue, longest unit first
+// (order[] from k_order), so the uneven units of a panel balance over the CUs.
+// A filter's result does not depend on which workgroup runs it.  Global
+// scratch is indexed by the filter's queue position, which every workgroup
+// takes once.
 template <class M, int ALGO>
 __global__ __launch_bounds__(256) void k_batched(const BatchArgs a) {
   extern __shared__ __align__(256) char smem[];
+  __shared__ int next;
   double* red = reinterpret_cast<double*>(smem);
   uint64_t* tab = reinterpret_cast<uint64_t*>(smem + kRedBytes);
   char* ldsbuf = smem + kRedBytes + batched_tab_bytes(a.J, blockDim.x);
-  for (int f = blockIdx.x; f < a.R; f += gridDim.x) {
-    char* buf = a.lds_resident ? ldsbuf : a.ws + (int64_t)f * a.ws_stride;
+  for (;;) {
+    if (threadIdx.x == 0) next = atomicAdd(a.queue, 1);
+    __syncthreads();
+    const int q = next;
+    __syncthreads();
+    if (q >= a.R) break;
+    const int f = a.order[q];
+    char* buf = a.lds_resident ? ldsbuf : a.ws + (int64_t)q * a.ws_stride;
     run_filter<M, ALGO>(a, f, buf, red, tab);
   }
+}
+
+// Filter order for the work queue: filters sorted by their unit's length,
+// longest first, ties by index (a rank per filter; R is at most a few
+// thousand).  Also resets the queue counter, so a captured launch sequence
+// needs no memset.
+__global__ void k_order(SeriesSet ser, const int32_t* __restrict__ unit, int R, int32_t* __restrict__ order,
+                        int32_t* __restrict__ queue) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f == 0) *queue = 0;
+  if (f >= R) return;
+  auto len = [&](int g) {
+    const int u = unit ? unit[g] : 0;
+    return ser.obs_off[u + 1] - ser.obs_off[u];
+  };
+  const int64_t Tf = len(f);
+  int rank = 0;
+  for (int g = 0; g < R; ++g) {
+    const int64_t Tg = len(g);
+    rank += (Tg > Tf) || (Tg == Tf && g < f);
+  }
+  order[rank] = f;
 }
 
 
@@ -364,6 +407,12 @@ hipError_t launch_batched(int algo, const BatchArgs& a, int block, int grid, siz
     case ALGO_PER: return launch_batched_algo<ALGO_PER>(a, block, grid, lds, st);
   }
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_order(const SeriesSet& ser, const int32_t* unit, int R, int32_t* order, int32_t* queue,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(k_order, dim3((R + 255) / 256), dim3(256), 0, st, ser, unit, R, order, queue);
+  return hipGetLastError();
 }
 
 hipError_t launch_sched(int algo, const uint32_t* keys, const SeriesSet& ser, const int32_t* unit,

@@ -23,6 +23,8 @@ hipError_t plain_prepare(const ModelDesc& md, const float* theta, int64_t stride
                          int64_t pstride, hipStream_t s);
 hipError_t plain_reset(uint64_t* lb, uint64_t* psum, int ntiles, PlainState* st, hipStream_t s);
 size_t batched_small_lds(int J, int block);
+hipError_t launch_order(const SeriesSet& ser, const int32_t* unit, int R, int32_t* order, int32_t* queue,
+                        hipStream_t st);
 }  // namespace mp
 
 namespace mp {
@@ -160,7 +162,7 @@ int check_series(const mp_series_desc* s) {
 
 struct BatchedPlan {
   int W;
-  int64_t sched_len, sched_bytes;
+  int64_t sched_len, sched_bytes, order_bytes;
   int64_t per_filter;  // global scratch per filter (0 when LDS-resident)
   bool lds_resident;
   size_t lds;
@@ -170,16 +172,18 @@ struct BatchedPlan {
 BatchedPlan plan_batched(int algo, const mp_model_desc* m, const mp_series_desc* s, int R, int J) {
   BatchedPlan p{};
   const SeriesSet& ser = *reinterpret_cast<const SeriesSet*>(s);
-  p.W = m->S + (algo == ALGO_PER ? m->d : 0);
+  // a perturbed filter also carries d parameters and one ancestry word
+  p.W = m->S + (algo == ALGO_PER ? m->d + 1 : 0);
   p.sched_len = sched_stride(algo, ser);
   p.sched_bytes = align256((int64_t)R * p.sched_len * 8);
+  p.order_bytes = align256((int64_t)R * 4) + 256;  // filter order + queue counter
   const FilterScratch L = filter_scratch_layout(p.W, J);
   const size_t small = batched_small_lds(J, kBatchedBlock);
   // Keep filters LDS-resident when two workgroups still fit per CU.
   p.lds_resident = (int64_t)small + L.total <= (int64_t)max_lds_bytes() / 2;
   p.per_filter = p.lds_resident ? 0 : L.total;
   p.lds = small + (p.lds_resident ? (size_t)L.total : 0);
-  p.total = (size_t)p.sched_bytes + (size_t)R * (size_t)p.per_filter + 256;
+  p.total = (size_t)p.sched_bytes + (size_t)p.order_bytes + (size_t)R * (size_t)p.per_filter + 256;
   return p;
 }
 
@@ -227,9 +231,13 @@ int run_batched(int algo, const mp_model_desc* m, const mp_series_desc* s, Batch
   uint32_t* sched = reinterpret_cast<uint32_t*>(base);
   a.sched = sched;
   a.sched_len = p.sched_len;
-  a.ws = base + p.sched_bytes;
+  int32_t* order = reinterpret_cast<int32_t*>(base + p.sched_bytes);
+  a.order = order;
+  a.queue = reinterpret_cast<int32_t*>(base + p.sched_bytes + p.order_bytes - 256);
+  a.ws = base + p.sched_bytes + p.order_bytes;
   a.ws_stride = p.per_filter;
   a.lds_resident = p.lds_resident;
+  MP_HIP(timed(3, st, [&] { return launch_order(a.ser, a.unit, a.R, order, a.queue, st); }));
   MP_HIP(timed(3, st, [&] { return launch_sched(algo, a.keys, a.ser, a.unit, p.sched_len, sched, a.R, st); }));
   const int per_cu = p.lds_resident ? 2 : 1;
   int grid = a.R < num_cus() * per_cu * 4 ? a.R : num_cus() * per_cu * 4;
@@ -355,7 +363,7 @@ int run_mop_grad(const mp_model_desc* m, const mp_series_desc* s, const float* t
 extern "C" {
 
 const char* mp_last_error(void) { return g_err.c_str(); }
-int32_t mp_abi_version(void) { return 1; }
+int32_t mp_abi_version(void) { return 2; }
 
 int32_t mp_host_threefry2x32(const uint32_t* h_keys, const uint32_t* h_ctr, uint32_t* h_out, int64_t n) {
   if (n < 0 || (n > 0 && (!h_keys || !h_ctr || !h_out))) return fail(MP_ERR_ARG, "bad threefry arguments");
@@ -420,6 +428,9 @@ int32_t mp_resample_systematic(const float* logw, const uint32_t* keys, int64_t 
 size_t mp_filter_workspace_size(int32_t algo, const mp_model_desc* h_model, const mp_series_desc* h_series,
                                 int32_t R, int32_t J, int32_t form) {
   if (!h_model || !h_series || R < 1 || J < 1) return 0;
+  // device queries are cached here, before any stream capture of the launches
+  (void)num_cus();
+  (void)max_lds_bytes();
   if (form == MP_FORM_PLAIN) return (size_t)plan_plain(h_model, h_series, R, J).total;
   return plan_batched(algo, h_model, h_series, R, J).total;
 }
@@ -526,7 +537,7 @@ int32_t mp_mop_grad(const mp_model_desc* h_model, const mp_series_desc* h_series
 int32_t mp_perfilter(const mp_model_desc* h_model, const mp_series_desc* h_series, const float* thetas_in,
                      const uint32_t* keys, const int32_t* unit, int32_t R, int32_t J, float thresh, const float* sigmas,
                      const float* sigmas_init, float cool, int32_t m, float* thetas_out, float* theta_mean,
-                     double* loglik, void* workspace, size_t ws_bytes, void* stream) {
+                     double* loglik, int32_t* ancestry, void* workspace, size_t ws_bytes, void* stream) {
   int rc = check_common(h_model, h_series, R, J, keys);
   if (rc) return rc;
   if (!thetas_in || !thetas_out || !theta_mean || !loglik || !sigmas || !sigmas_init || !workspace)
@@ -542,6 +553,7 @@ int32_t mp_perfilter(const mp_model_desc* h_model, const mp_series_desc* h_serie
   a.theta_out = thetas_out;
   a.theta_mean = theta_mean;
   a.loglik = loglik;
+  a.ancestry = ancestry;
   return run_batched(ALGO_PER, h_model, h_series, a, workspace, ws_bytes, (hipStream_t)stream);
 }
 

@@ -61,6 +61,7 @@ struct BatchArgs {
   int mif_m;                // iteration index m
   float* theta_out;         // (R, J, d) swarm after the final resample (est. scale)
   float* theta_mean;        // (R, d) mean of theta_out
+  int32_t* ancestry;        // (R, J) composed final ancestry, or null
   // outputs
   double* loglik;           // (R)
   float* cll;               // (R, Tmax)
@@ -71,6 +72,9 @@ struct BatchArgs {
   char* ws;
   int64_t ws_stride;
   int lds_resident;
+  // work queue: filters are taken in order[] (longest unit first) via *queue
+  const int32_t* order;
+  int32_t* queue;
 };
 
 // Per-filter scratch layout (bytes), shared by host sizing and device use.

@@ -20,6 +20,7 @@ struct ModelDesc {
   int dx;           // LG: state dimension
   int lg_sep_init;  // LG: separate initial Cholesky factor P follows X0
   uint32_t accum_mask;
+  int identity_trans;  // estimation scale == natural scale
 };
 
 __device__ __forceinline__ int tri(int d) { return d * (d + 1) / 2; }

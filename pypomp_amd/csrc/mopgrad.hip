@@ -101,7 +101,7 @@ __device__ void mop_grad_filter(const GradArgs& a, const int f, char* buf, doubl
     th_est[p] = a.theta[f * a.theta_stride + p];
     th[p] = th_est[p];
   }
-  M::from_est(md, th);
+  if (!md.identity_trans) M::from_est(md, th);
   typename M::P P;
   M::prepare(md, th, P);
 
@@ -281,7 +281,11 @@ __device__ void mop_grad_filter(const GradArgs& a, const int f, char* buf, doubl
   block_sum_n<M::MAXP>(gth, red);
   if (tid == 0) {
     float jac[M::MAXP];
-    M::from_est_jac(md, th_est, jac);
+    if (md.identity_trans) {
+      MP_UNROLL for (int p = 0; p < M::MAXP; ++p) jac[p] = 1.0f;
+    } else {
+      M::from_est_jac(md, th_est, jac);
+    }
     MP_UNROLL for (int p = 0; p < M::MAXP; ++p) if (p < d)
       a.grad[(int64_t)f * d + p] = (float)(gth[p] * (double)jac[p]);
     a.loglik[f] = loglik;

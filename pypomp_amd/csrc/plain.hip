@@ -282,7 +282,7 @@ __global__ void k_plain_prepare(ModelDesc md, const float* theta, int64_t stride
   if (f >= R) return;
   float th[M::MAXP];
   MP_UNROLL for (int p = 0; p < M::MAXP; ++p) if (p < md.d) th[p] = theta[f * stride + p];
-  if (flags & F_SHOULD_TRANS) M::from_est(md, th);
+  if ((flags & F_SHOULD_TRANS) && !md.identity_trans) M::from_est(md, th);
   typename M::P P;
   M::prepare(md, th, P);
   *reinterpret_cast<typename M::P*>(out + f * pstride) = P;

@@ -142,57 +142,276 @@ def mop(struct, thetas_array, J, alpha, keys, form="batched", return_float64=Fal
     return v if return_float64 else v.astype(F32)
 
 
-def perfilter(struct, thetas_est, keys, J, rw_sd, m, thresh=0.0, unit=None):
+class _PerRunner:
+    """IF2 iterations on device-resident swarms: one ``mp_perfilter`` launch per
+    iteration over R swarms (replicates x units), the (R, J, d) parameter
+    swarm staying in HBM between iterations."""
+
+    def __init__(self, struct, R, J, rw, thresh=0.0, unit=None, ancestry=False):
+        import torch
+        self.dev = dev = _native.require_device()
+        self.ser = struct.device_series()
+        self.md = struct.model.desc()
+        self.R, self.J, self.d = int(R), int(J), struct.model.d
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(np.asarray(a, dtype=F32))).to(dev)  # noqa: E731
+        self.sig, self.sig0 = t(rw.sigmas_array), t(rw.sigmas_init_array)
+        self.factor = float(rw.factor)
+        self.thresh = float(max(0.0, thresh))
+        self.unit = None if unit is None else torch.from_numpy(np.ascontiguousarray(unit, dtype=np.int32)).to(dev)
+        self.ws, self.nws = _workspace(_native.MP_ALGO_PER, self.md, self.ser, self.R, self.J,
+                                       _native.MP_FORM_BATCHED, dev)
+        self.anc = torch.empty((self.R, self.J), dtype=torch.int32, device=dev) if ancestry else None
+
+    def run(self, tin, tout, keys, m, tmean, ll):
+        """One iteration: tin -> tout (R, J, d), keys (R, 2) int32 view, means (R, d), loglik (R)."""
+        _native.check(_native.lib().mp_perfilter(
+            self.md, self.ser.desc, _native.ptr(tin), _native.ptr(keys), _native.ptr(self.unit), self.R, self.J,
+            self.thresh, _native.ptr(self.sig), _native.ptr(self.sig0), self.factor, int(m), _native.ptr(tout),
+            _native.ptr(tmean), _native.ptr(ll), _native.ptr(self.anc), _native.ptr(self.ws), self.nws,
+            _native.stream_ptr()))
+
+    def iterate(self, theta0, mkeys, graph=False):
+        """M iterations from theta0 (R, J, d) est. scale (device tensor), keys
+        mkeys (M, R, 2).  Returns (final swarm, means (M, R, d), logliks (M, R)),
+        device tensors.  graph=True records the M launches as one HIP graph
+        and replays it once."""
+        import torch
+        M = int(mkeys.shape[0])
+        kt = torch.from_numpy(np.ascontiguousarray(np.asarray(mkeys, dtype=np.uint32)).view(np.int32)).to(self.dev)
+        bufs = [theta0.contiguous(), torch.empty_like(theta0)]
+        means = torch.empty((M, self.R, self.d), dtype=torch.float32, device=self.dev)
+        lls = torch.empty((M, self.R), dtype=torch.float64, device=self.dev)
+
+        def body():
+            for m in range(M):
+                self.run(bufs[m % 2], bufs[(m + 1) % 2], kt[m], m, means[m], lls[m])
+
+        if graph and M > 0:
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                body()
+            # the graph holds raw addresses: keep every buffer it reads or
+            # writes alive for as long as the graph may be replayed
+            self.graph, self.graph_buffers = g, (kt, bufs, means, lls)
+            g.replay()
+        else:
+            body()
+        return bufs[M % 2], means, lls
+
+
+def perfilter(struct, thetas_est, keys, J, rw_sd, m, thresh=0.0, unit=None, ancestry=False):
     """One IF2 iteration for R swarms: thetas_est (R, J, d), keys (R, 2).
-    Returns (thetas_out (R, J, d), theta_mean (R, d), loglik (R,)), all est. scale."""
+    Returns (thetas_out (R, J, d), theta_mean (R, d), loglik (R,)), all est. scale,
+    plus the final ancestry (R, J) when ``ancestry`` (mif.py:237-245)."""
     import torch
-    dev = _native.require_device()
     thetas = np.ascontiguousarray(np.asarray(thetas_est, dtype=F32))
     R = thetas.shape[0]
-    ser = struct.device_series()
-    md = struct.model.desc()
-    tin = torch.from_numpy(thetas).to(dev)
+    run = _PerRunner(struct, R, J, rw_sd, thresh, unit, ancestry)
+    tin = torch.from_numpy(thetas).to(run.dev)
     tout = torch.empty_like(tin)
-    tmean = torch.empty((R, struct.model.d), dtype=torch.float32, device=dev)
-    ll = torch.empty(R, dtype=torch.float64, device=dev)
-    kt = _keys_tensor(keys, dev)
-    sig = torch.from_numpy(np.ascontiguousarray(rw_sd.sigmas_array)).to(dev)
-    sig0 = torch.from_numpy(np.ascontiguousarray(rw_sd.sigmas_init_array)).to(dev)
-    ut = None if unit is None else torch.from_numpy(np.asarray(unit, dtype=np.int32)).to(dev)
-    ws, nws = _workspace(_native.MP_ALGO_PER, md, ser, R, J, _native.MP_FORM_BATCHED, dev)
-    _native.check(_native.lib().mp_perfilter(
-        md, ser.desc, _native.ptr(tin), _native.ptr(kt), _native.ptr(ut), R, J, float(max(0.0, thresh)),
-        _native.ptr(sig), _native.ptr(sig0), float(rw_sd.factor), int(m), _native.ptr(tout),
-        _native.ptr(tmean), _native.ptr(ll), _native.ptr(ws), nws, _native.stream_ptr()))
+    tmean = torch.empty((R, run.d), dtype=torch.float32, device=run.dev)
+    ll = torch.empty(R, dtype=torch.float64, device=run.dev)
+    run.run(tin, tout, _keys_tensor(keys, run.dev), m, tmean, ll)
     torch.cuda.current_stream().synchronize()
-    return tout.cpu().numpy(), tmean.cpu().numpy(), ll.cpu().numpy()
+    out = (tout.cpu().numpy(), tmean.cpu().numpy(), ll.cpu().numpy())
+    return out + (run.anc.cpu().numpy(),) if ancestry else out
 
 
-def mif(struct, thetas_array, rw_sd, M, J, keys, thresh=0.0, n_monitors=0, unit=None):
-    """IF2 (functional/mif.py:15-122): thetas (n, J, d) natural scale, keys (n, 2).
-    Returns (logliks (n, M), traces (n, M+1, d) natural, final swarm (n, J, d) natural)."""
+def mif(struct, thetas_array, rw_sd, M, J, keys, thresh=0.0, n_monitors=0, unit=None, graph=False):
+    """IF2 (functional/mif.py:15-122; core/algorithms/mif.py:16-85): thetas (n, J, d)
+    natural scale, keys (n, 2).  Returns (logliks (n, M), traces (n, M+1, d)
+    natural, final swarm (n, J, d) natural).  The swarms stay on the device for
+    all M iterations; ``graph`` records them as one HIP graph (n_monitors = 0)."""
+    import torch
     model = struct.model
     thetas = np.asarray(thetas_array, dtype=F32)
     keys = np.asarray(keys, dtype=np.uint32).reshape(-1, 2)
     n = keys.shape[0]
+    if thetas.shape[0] != n:
+        raise ValueError("thetas and keys disagree on the number of rows")
     rw = rw_sd.canonicalize(model.param_names)
     theta = model.to_est(thetas)
-    traces = [theta.mean(axis=1, dtype=F32)]
-    lls = np.zeros((n, M), dtype=F32)
-    mkeys = np.stack([prandom.split(k, M + 1)[1:] for k in keys])  # (n, M, 2)
-    for m in range(M):
-        new, mean, ll_per = perfilter(struct, theta, mkeys[:, m], J, rw, m, thresh, unit=unit)
-        if n_monitors >= 1:
-            means = theta.mean(axis=1, dtype=F32)
-            mon = np.stack([prandom.split(k, n_monitors) for k in mkeys[:, m]])
-            res = pfilter(struct, means, J, mon, thresh=thresh, should_trans=True)
+    mkeys = np.stack([prandom.split(k, M + 1)[1:] for k in keys], axis=1)  # (M, n, 2)
+    run = _PerRunner(struct, n, J, rw, thresh, unit)
+    t0 = torch.from_numpy(np.ascontiguousarray(theta)).to(run.dev)
+    if n_monitors >= 1:
+        lls = np.zeros((n, M), dtype=F32)
+        means = np.zeros((M, n, model.d), dtype=F32)
+        cur, nxt = t0, torch.empty_like(t0)
+        tm = torch.empty((n, model.d), dtype=torch.float32, device=run.dev)
+        llt = torch.empty(n, dtype=torch.float64, device=run.dev)
+        kt = _keys_tensor(mkeys.reshape(-1, 2), run.dev).view(M, n, 2)
+        for m in range(M):
+            mean_now = cur.cpu().numpy().mean(axis=1, dtype=F32)
+            mon = np.stack([prandom.split(k, n_monitors) for k in mkeys[m]])
+            res = pfilter(struct, mean_now, J, mon, thresh=thresh, should_trans=True)
             lls[:, m] = res["logLik"].mean(axis=1, dtype=F32)
-        else:
-            lls[:, m] = ll_per.astype(F32)
-        traces.append(mean.astype(F32))
-        theta = new
-    traces = np.stack(traces, axis=1)
-    return lls, model.from_est(traces), model.from_est(theta)
+            run.run(cur, nxt, kt[m], m, tm, llt)
+            means[m] = tm.cpu().numpy()
+            cur, nxt = nxt, cur
+        final = cur
+    else:
+        final, mt, lt = run.iterate(t0, mkeys, graph=graph)
+        torch.cuda.current_stream().synchronize()
+        means = mt.cpu().numpy()
+        lls = lt.cpu().numpy().T.astype(F32)
+    traces = np.concatenate([theta.mean(axis=1, dtype=F32)[:, None], np.transpose(means, (1, 0, 2))], axis=1)
+    return lls, model.from_est(traces), model.from_est(final.cpu().numpy())
+
+
+@dataclass
+class PanelStruct:
+    """A panel of units sharing one native model (functional/structs.py,
+    PanelPompStruct): per-unit series, shared and unit-specific parameter names
+    (the panel order is shared names, then unit-specific names).  Units may have
+    different lengths."""
+    model: object
+    units: list
+    shared_param_names: list
+    unit_param_names: list
+    unit_names: list = None
+    # panel-level estimation-scale transforms (PanelPompStruct.par_trans,
+    # panel/panel.py:146); None: the model's own.  The filter kernels always
+    # apply the model's transforms (the reference binds them into the unit
+    # mechanics when the Pomp is built, core/pomp.py:271-313).
+    par_trans: object = None
+    _dev: dict = field(default_factory=dict, repr=False)
+
+    def __post_init__(self):
+        names = list(self.shared_param_names) + list(self.unit_param_names)
+        if sorted(names) != sorted(self.model.param_names) or len(set(names)) != len(names):
+            raise ValueError("shared + unit-specific parameter names must be the model's parameter names")
+        if self.unit_names is None:
+            self.unit_names = [f"unit{u + 1}" for u in range(len(self.units))]
+
+    @property
+    def U(self):
+        return len(self.units)
+
+    @property
+    def param_names(self):
+        return list(self.shared_param_names) + list(self.unit_param_names)
+
+    @property
+    def perm(self):
+        """Model-order index into the panel order (unit_param_permutations)."""
+        names = self.param_names
+        return np.array([names.index(n) for n in self.model.param_names], dtype=np.int64)
+
+    def device_series(self):
+        return PompStruct.device_series(self)
+
+
+def panel_mif(struct, shared_array, unit_array, rw_sd, M, J, keys, thresh=0.0, n_monitors=0, block=True,
+              graph=False):
+    """Panel IF2 (functional/mif.py:125-294; core/algorithms/panel_mif.py:17-260).
+    shared_array (n, J, n_shared) and unit_array (n, J, U, n_spec) on the natural
+    scale, keys (n, 2).  Returns (shared_traces (n, M+1, n_shared+1),
+    unit_traces (n, M+1, U, n_spec+1), final shared swarm, final unit swarm),
+    natural scale; column 0 of the traces is the log-likelihood (NaN at m = 0).
+
+    Per iteration the unit keys are split(iter_key, U) (panel_mif.py:113) and
+    unit u runs _perfilter_internal on [shared, unit u] in the model's order.
+    With no shared parameters and block=True the units are independent: every
+    (replicate, unit) swarm of an iteration runs in one launch, and ``graph``
+    records the M launches as one HIP graph.  Otherwise units run one launch
+    after another, carrying the shared swarm (and, for block=False, reordering
+    the other units' swarms by the unit's ancestry, panel_mif.py:224-231)."""
+    import torch
+    model = struct.model
+    dev = _native.require_device()
+    U, ns, nu = struct.U, len(struct.shared_param_names), len(struct.unit_param_names)
+    sh = np.asarray(shared_array, dtype=F32)
+    un = np.asarray(unit_array, dtype=F32)
+    keys = np.asarray(keys, dtype=np.uint32).reshape(-1, 2)
+    n = keys.shape[0]
+    if sh.shape[:2] != (n, J) or sh.shape[2] != ns or un.shape != (n, J, U, nu):
+        raise ValueError("shared_array must be (n, J, n_shared) and unit_array (n, J, U, n_spec)")
+    if struct.model.d != ns + nu:
+        raise ValueError("parameter count mismatch")
+    perm = struct.perm
+    inv = np.argsort(perm)
+    pt = model if struct.par_trans is None else struct.par_trans
+    rw = rw_sd.canonicalize(model.param_names)  # model order
+    # natural -> estimation scale (panel order)
+    full = np.concatenate([np.broadcast_to(sh[:, :, None, :], (n, J, U, ns)), un], axis=3)
+    full_est = pt.to_est(full[..., perm])[..., inv]
+    sh_est = full_est[:, :, 0, :ns].copy() if ns else np.zeros((n, J, 0), F32)
+    un_est = full_est[:, :, :, ns:].copy()
+    iter_keys = np.stack([prandom.split(k, M + 1)[1:] for k in keys], axis=1)            # (M, n, 2)
+    unit_keys = np.stack([[prandom.split(k, U) for k in row] for row in iter_keys])     # (M, n, U, 2)
+    sh_tr = np.full((n, M + 1, ns + 1), np.nan, dtype=F32)
+    un_tr = np.full((n, M + 1, U, nu + 1), np.nan, dtype=F32)
+    sh_tr[:, 0, 1:] = sh_est.mean(axis=1, dtype=F32)
+    un_tr[:, 0, :, 1:] = un_est.mean(axis=1, dtype=F32)
+
+    if ns == 0 and block and n_monitors == 0:
+        # independent units: filter r * U + u, one launch per iteration
+        unit = np.tile(np.arange(U, dtype=np.int32), n)
+        th0 = np.ascontiguousarray(np.transpose(un_est, (0, 2, 1, 3))[..., perm].reshape(n * U, J, -1))
+        run = _PerRunner(struct, n * U, J, rw, thresh, unit)
+        final, mt, lt = run.iterate(torch.from_numpy(th0).to(dev), unit_keys.reshape(M, n * U, 2), graph=graph)
+        torch.cuda.current_stream().synchronize()
+        means = mt.cpu().numpy()[..., inv].reshape(M, n, U, nu)
+        lls = lt.cpu().numpy().reshape(M, n, U).astype(F32)
+        un_tr[:, 1:, :, 0] = np.transpose(lls, (1, 0, 2))
+        un_tr[:, 1:, :, 1:] = np.transpose(means, (1, 0, 2, 3))
+        sh_tr[:, 1:, 0] = lls.sum(axis=2, dtype=F32).T
+        un_est = np.transpose(final.cpu().numpy()[..., inv].reshape(n, U, J, nu), (0, 2, 1, 3))
+    else:
+        perm_t = torch.from_numpy(perm).to(dev)
+        inv_t = torch.from_numpy(inv).to(dev)
+        sh_t = torch.from_numpy(np.ascontiguousarray(sh_est)).to(dev)
+        un_t = torch.from_numpy(np.ascontiguousarray(un_est)).to(dev)
+        units = [_PerRunner(struct, n, J, rw, thresh, np.full(n, u, np.int32), ancestry=not block)
+                 for u in range(U)]
+        tm = torch.empty((n, model.d), dtype=torch.float32, device=dev)
+        llt = torch.empty(n, dtype=torch.float64, device=dev)
+        for m in range(M):
+            total = np.zeros(n, dtype=F32)
+            for u in range(U):
+                th = torch.cat([sh_t, un_t[:, :, u, :]], dim=2).index_select(2, perm_t).contiguous()
+                kt = _keys_tensor(unit_keys[m, :, u], dev)
+                if n_monitors >= 1:
+                    mean_now = th.mean(dim=1).cpu().numpy()
+                    mon = np.stack([prandom.split(k, n_monitors) for k in unit_keys[m, :, u]])
+                    sub = PompStruct(model=model, units=[struct.units[u]])
+                    res = pfilter(sub, mean_now, J, mon, thresh=thresh, should_trans=True)
+                    ll_u = res["logLik"].mean(axis=1, dtype=F32)
+                out = torch.empty_like(th)
+                units[u].run(th, out, kt, m, tm, llt)
+                upd = out.index_select(2, inv_t)
+                if n_monitors < 1:
+                    ll_u = llt.cpu().numpy().astype(F32)
+                if ns:
+                    sh_t = upd[:, :, :ns].contiguous()
+                if not block:
+                    anc = units[u].anc.long()
+                    un_t = torch.stack([un_t[r].index_select(0, anc[r]) for r in range(n)])
+                un_t = un_t.clone()
+                un_t[:, :, u, :] = upd[:, :, ns:]
+                un_tr[:, m + 1, u, 0] = ll_u
+                un_tr[:, m + 1, u, 1:] = upd[:, :, ns:].mean(dim=1).cpu().numpy()
+                total = (total + ll_u).astype(F32)
+            sh_tr[:, m + 1, 0] = total
+            if ns:
+                sh_tr[:, m + 1, 1:] = sh_t.mean(dim=1).cpu().numpy()
+        torch.cuda.current_stream().synchronize()
+        sh_est = sh_t.cpu().numpy()
+        un_est = un_t.cpu().numpy()
+
+    def nat(shared_part, unit_part):
+        """est -> natural for arrays (..., ns) and (..., U, nu) in panel order."""
+        lead = unit_part.shape[:-2]
+        fullp = np.concatenate([np.broadcast_to(shared_part[..., None, :], lead + (U, ns)), unit_part], axis=-1)
+        natp = pt.from_est(fullp[..., perm])[..., inv]
+        return natp[..., 0, :ns], natp[..., ns:]
+
+    s_nat, u_nat = nat(sh_tr[..., 1:], un_tr[..., 1:])
+    sh_tr[..., 1:] = s_nat
+    un_tr[..., 1:] = u_nat
+    fs, fu = nat(sh_est, un_est)
+    return sh_tr, un_tr, np.ascontiguousarray(fs), np.ascontiguousarray(fu)
 
 
 def mop_grad(struct, thetas_array, J, alpha, keys, K=50, form="batched"):

@@ -32,6 +32,9 @@ class NativeModel:
     dx = 0
     lg_sep_init = 0
     accumvars = ()
+    # True: parameters are estimated on their natural scale (the reference's
+    # ParTrans() with no transforms, core/par_trans.py:70-111)
+    identity_trans = False
 
     param_names: list = []
     statenames: list = []
@@ -62,13 +65,22 @@ class NativeModel:
         for a in self.accumvars:
             mask |= 1 << int(a)
         return _native.ModelDesc(kind=self.kind, S=self.S, d=self.d, dy=self.dy, dx=self.dx,
-                                 lg_sep_init=self.lg_sep_init, accum_mask=mask)
+                                 lg_sep_init=self.lg_sep_init, accum_mask=mask,
+                                 identity_trans=int(bool(self.identity_trans)))
 
     # host mirrors of the device transforms (float32)
     def to_est(self, theta):
-        return np.asarray(theta, dtype=F32).copy()
+        th = np.asarray(theta, dtype=F32)
+        return th.copy() if self.identity_trans else self._to_est(th)
 
     def from_est(self, theta):
+        th = np.asarray(theta, dtype=F32)
+        return th.copy() if self.identity_trans else self._from_est(th)
+
+    def _to_est(self, theta):
+        return np.asarray(theta, dtype=F32).copy()
+
+    def _from_est(self, theta):
         return np.asarray(theta, dtype=F32).copy()
 
     def theta_array(self, theta):
@@ -89,3 +101,14 @@ class NativeModel:
         if arr.shape[-1] != self.d:
             raise ValueError(f"theta has {arr.shape[-1]} entries, model needs {self.d}")
         return arr
+
+
+class ParTrans:
+    """Identity parameter transform: the reference's ``ParTrans()`` with no
+    transforms (core/par_trans.py:70-111), usable as a panel-level transform."""
+
+    def to_est(self, theta):
+        return np.array(theta, dtype=F32, copy=True)
+
+    def from_est(self, theta):
+        return np.array(theta, dtype=F32, copy=True)

@@ -41,13 +41,13 @@ class LGModel(NativeModel):
     def _diag_mask(self):
         return np.array([len(n) == 3 and n[0] in ("Q", "R") and n[1] == n[2] for n in self.param_names])
 
-    def to_est(self, theta):
+    def _to_est(self, theta):
         th = np.array(theta, dtype=F32, copy=True)
         m = self._diag_mask()
         th[..., m] = np.log(th[..., m])
         return th
 
-    def from_est(self, theta):
+    def _from_est(self, theta):
         th = np.array(theta, dtype=F32, copy=True)
         m = self._diag_mask()
         th[..., m] = np.exp(th[..., m])

@@ -28,14 +28,14 @@ class SEIRModel(NativeModel):
     statenames = ["S", "E", "I", "R", "H"]
     obsnames = ["reports"]
 
-    def to_est(self, theta):
+    def _to_est(self, theta):
         th = np.array(theta, dtype=F32, copy=True)
         th[..., _LOG] = np.log(th[..., _LOG])
         p = th[..., _LOGIT]
         th[..., _LOGIT] = np.log(p / (F32(1) - p))
         return th.astype(F32)
 
-    def from_est(self, theta):
+    def _from_est(self, theta):
         th = np.array(theta, dtype=F32, copy=True)
         th[..., _LOG] = np.exp(th[..., _LOG])
         th[..., _LOGIT] = F32(1) / (F32(1) + np.exp(-th[..., _LOGIT]))

@@ -23,7 +23,7 @@ class SVModel(NativeModel):
     statenames = ["v", "eta"]
     obsnames = ["r"]
 
-    def to_est(self, theta):
+    def _to_est(self, theta):
         th = np.array(theta, dtype=F32, copy=True)
         th[..., 1] = np.arctanh(th[..., 1])
         th[..., 2] = np.log(th[..., 2])
@@ -31,7 +31,7 @@ class SVModel(NativeModel):
         th[..., 5] = np.log(th[..., 5])
         return th
 
-    def from_est(self, theta):
+    def _from_est(self, theta):
         th = np.array(theta, dtype=F32, copy=True)
         th[..., 1] = np.tanh(th[..., 1])
         th[..., 2] = np.exp(th[..., 2])

@@ -1,0 +1,163 @@
+"""GPU parity of IF2 on panels: the reference's panel-mif baseline, the oracle's
+panel restatement (shared parameters, block=False ancestry, uneven SEIR units
+in one launch) and HIP-graph replay of the iteration loop."""
+
+import numpy as np
+import pytest
+
+from oracle import filters as fl
+from oracle import lg as olg, seir as oseir, threefry as tf
+from tests.helpers import assert_close_tol, lg1_data, lg1_theta, workloads
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+LG_NAMES = ["A11", "C11", "Q11", "R11", "X0_1"]
+
+
+def lg_panel_struct(par_trans=True):
+    import pypomp_amd as pp
+    import pypomp_amd.functional as F
+    from pypomp_amd.models import LGModel
+    from pypomp_amd.series import Series
+    units = [Series.build(lg1_data(s), np.arange(1, 5.0), 0.0, nstep=1) for s in (0, 1)]
+    return F.PanelStruct(model=LGModel(1, 1), units=units, shared_param_names=["A11", "C11"],
+                         unit_param_names=["Q11", "R11", "X0_1"], par_trans=pp.ParTrans() if par_trans else None)
+
+
+def lg_panel_inputs(J=12):
+    th = lg1_theta()
+    sh = np.repeat(th[None, None, :2], J, axis=1)                          # (1, J, 2)
+    un = np.repeat(np.stack([th[2:], th[2:]])[None, None], J, axis=1)      # (1, J, 2, 3)
+    return sh, un
+
+
+def test_panel_mif_regression(golden):
+    import pypomp_amd as pp
+    import pypomp_amd.functional as F
+    sh, un = lg_panel_inputs()
+    rw = pp.RWSigma({n: 0.02 for n in LG_NAMES}).geometric_cooling(0.5)
+    st, ut, fs, fu = F.panel_mif(lg_panel_struct(), sh, un, rw, 3, 12, tf.split(tf.key(20260811), 1))
+    g = golden["test_panel_mif_regression"]
+    assert_close_tol(st[:, -1, :].ravel(), g["final_shared"])
+    assert_close_tol(ut[:, -1, :, :].ravel(), g["final_unit"])
+    assert fs.shape == (1, 12, 2) and fu.shape == (1, 12, 2, 3)
+
+
+@pytest.mark.parametrize("block", [True, False])
+def test_panel_mif_matches_oracle(block):
+    import pypomp_amd as pp
+    import pypomp_amd.functional as F
+    sh, un = lg_panel_inputs(J=64)
+    rw = pp.RWSigma({n: 0.05 for n in LG_NAMES}, init_names=["X0_1"]).geometric_cooling(0.5)
+    keys = tf.split(tf.key(31), 2)
+    sh2, un2 = np.repeat(sh, 2, 0), np.repeat(un, 2, 0)
+    st, ut, fs, fu = F.panel_mif(lg_panel_struct(par_trans=False), sh2, un2, rw, 3, 64, keys, block=block)
+    model = olg.LGModel(olg.LGSpec(1, 1))
+    sers = [fl.make_series(lg1_data(s), np.arange(1, 5.0), 0.0, nstep=1) for s in (0, 1)]
+    orw = fl.RWSigma(np.full(5, 0.05), np.array([0, 0, 0, 0, 1.0]), a=0.5)
+    for r in range(2):
+        est, eut, efs, efu = fl.panel_mif(model, sh2[r], un2[r], sers, keys[r], 64, 3, orw, np.arange(5),
+                                          block=block)
+        np.testing.assert_allclose(st[r], est, rtol=1e-5, atol=1e-6)
+        np.testing.assert_allclose(ut[r], eut, rtol=1e-5, atol=1e-6)
+        np.testing.assert_allclose(fs[r], efs, rtol=1e-6, atol=1e-7)
+        np.testing.assert_allclose(fu[r], efu, rtol=1e-6, atol=1e-7)
+
+
+def test_perfilter_ancestry_matches_oracle():
+    import pypomp_amd as pp
+    import pypomp_amd.functional as F
+    struct = lg_panel_struct(par_trans=False)
+    sub = F.PompStruct(model=struct.model, units=[struct.units[0]])
+    J = 200
+    rng = np.random.default_rng(3)
+    th = (lg1_theta()[None] + F32(0.01) * rng.standard_normal((J, 5)).astype(F32)).astype(F32)
+    model = olg.LGModel(olg.LGSpec(1, 1))
+    th_est = model.to_est(th)
+    rw = pp.RWSigma({n: 0.02 for n in LG_NAMES}).geometric_cooling(0.5)
+    k = tf.key(17)
+    out, mean, ll, anc = F.perfilter(sub, th_est[None], k[None], J, rw, 1, ancestry=True)
+    orw = fl.RWSigma(np.full(5, 0.02), np.zeros(5), a=0.5)
+    eout, ell, eanc = fl.perfilter(model, 1, th_est, fl.make_series(lg1_data(0), np.arange(1, 5.0), 0.0, nstep=1),
+                                   k, J, orw, return_ancestry=True)
+    np.testing.assert_array_equal(anc[0], eanc)
+    np.testing.assert_array_equal(out[0], eout)
+    assert abs(float(ll[0]) - float(ell)) <= 1e-5 * abs(float(ell))
+
+
+def seir_panel(U=4, Tmax=24):
+    """The first U units of the seed-5 SEIR panel, each cut to at most Tmax weeks."""
+    import pypomp_amd.functional as F
+    from pypomp_amd.models import SEIRModel
+    from pypomp_amd.series import Series
+    w = workloads()
+    off, beta, ys = w["panel_off"], w["panel_beta"], w["panel_ys"]
+    lens = [min(int(off[u + 1] - off[u]), Tmax - 5 * u) for u in range(U)]  # uneven lengths
+    data = [ys[off[u]:off[u] + lens[u]][:, None] for u in range(U)]
+    model = SEIRModel()
+    units = [Series.build(d, np.arange(1, len(d) + 1.0), 0.0, nstep=7) for d in data]
+    struct = F.PanelStruct(model=model, units=units, shared_param_names=[], unit_param_names=model.param_names)
+    theta = oseir.default_theta()
+    un = np.repeat(theta[None, None, None, :], U, axis=2)
+    un[..., 0] = beta[:U][None, None, :]
+    return struct, un, data
+
+
+def test_panel_mif_independent_seir_units_match_oracle():
+    import pypomp_amd as pp
+    import pypomp_amd.functional as F
+    struct, un1, data = seir_panel()
+    J, M, n = 64, 2, 2
+    names = struct.model.param_names
+    rw = pp.RWSigma({p: 0.02 for p in names}, init_names=["S0f", "E0", "I0"]).geometric_cooling(0.5)
+    un = np.repeat(np.repeat(un1, J, axis=1), n, axis=0)  # (n, J, U, d)
+    keys = tf.split(tf.key(55), n)
+    st, ut, fs, fu = F.panel_mif(struct, np.zeros((n, J, 0), F32), un, rw, M, J, keys)
+    model = oseir.SEIRModel()
+    sers = [fl.make_series(d, np.arange(1, len(d) + 1.0), 0.0, nstep=7) for d in data]
+    mask = np.array([1.0 if p in ("S0f", "E0", "I0") else 0.0 for p in names])
+    orw = fl.RWSigma(np.full(len(names), 0.02), mask, a=0.5)
+    for r in range(n):
+        est, eut, _, efu = fl.panel_mif(model, np.zeros((J, 0), F32), un[r], sers, keys[r], J, M, orw,
+                                        np.arange(len(names)))
+        np.testing.assert_allclose(ut[r][1:, :, 0], eut[1:, :, 0], rtol=1e-5)
+        np.testing.assert_allclose(ut[r][1:, :, 1:], eut[1:, :, 1:], rtol=1e-5)
+        np.testing.assert_allclose(fu[r], efu, rtol=1e-6)
+
+
+def test_graph_replays_match_eager():
+    # the M iterations recorded as one HIP graph, replayed several times with
+    # eager work and synchronisation between replays: every replay equals the
+    # eager run bit for bit
+    import torch
+    import pypomp_amd as pp
+    import pypomp_amd.functional as F
+    from pypomp_amd import random as prandom
+    struct, un1, _ = seir_panel(U=3, Tmax=20)
+    J, M, n = 128, 3, 2
+    U = struct.U
+    names = struct.model.param_names
+    rw = pp.RWSigma({p: 0.02 for p in names}).geometric_cooling(0.5).canonicalize(names)
+    th0 = struct.model.to_est(np.repeat(np.repeat(un1, J, axis=1), n, axis=0))       # (n, J, U, d)
+    th0 = np.ascontiguousarray(np.transpose(th0, (0, 2, 1, 3)).reshape(n * U, J, -1))
+    unit = np.tile(np.arange(U, dtype=np.int32), n)
+    mkeys = np.stack([prandom.split(k, M) for k in tf.split(tf.key(8), n * U)], axis=1)
+    dev = torch.device("cuda", 0)
+    run = F._PerRunner(struct, n * U, J, rw, 0.0, unit)
+    t0 = torch.from_numpy(th0).to(dev)
+    ref = [x.clone() for x in run.iterate(t0.clone(), mkeys, graph=False)]
+    torch.cuda.synchronize()
+    start = t0.clone()
+    fin, means, lls = run.iterate(start, mkeys, graph=True)
+    g = run.graph
+    for rep in range(3):
+        torch.cuda.synchronize()
+        for got, exp in zip((fin, means, lls), ref):
+            assert torch.equal(got, exp), rep
+        # eager work between replays: scribble on the outputs, restore the input
+        fin.fill_(0.0)
+        means.fill_(-1.0)
+        lls.zero_()
+        start.copy_(t0)
+        torch.cuda.synchronize()
+        g.replay()

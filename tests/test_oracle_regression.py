@@ -86,3 +86,65 @@ def test_panel_pfilter_units(golden):
     got = [fl.pfilter(model, lg1_theta(), fl.make_series(lg1_data(s), np.arange(1, 5.0), 0.0, nstep=1),
                       keys[u], 12)["logLik"] for u, s in enumerate((0, 1))]
     assert_close_tol(got, golden["test_panel_pfilter_regression"]["logLik"])
+
+
+def _lg_panel():
+    model = lg.LGModel(lg.LGSpec(1, 1))
+    sers = [fl.make_series(lg1_data(s), np.arange(1, 5.0), 0.0, nstep=1) for s in (0, 1)]
+    th = lg1_theta()
+    J = 12
+    sh0 = np.repeat(th[None, :2], J, 0)
+    un0 = np.repeat(np.stack([th[2:], th[2:]])[None], J, 0)
+    rw = fl.RWSigma(np.full(5, 0.02), np.zeros(5), a=0.5)
+    return model, sers, sh0, un0, rw, J
+
+
+def test_panel_mif(golden):
+    # lg_panel_struct: shared (A11, C11), unit-specific (Q11, R11, X0_1); the
+    # panel's ParTrans() is the identity while the filter applies the LG model's
+    # own transforms (bound when the unit Pomps were built)
+    model, sers, sh0, un0, rw, J = _lg_panel()
+    st, ut, _, _ = fl.panel_mif(model, sh0, un0, sers, tf.split(tf.key(20260811), 1)[0], J, 3, rw,
+                                np.arange(5), par_trans=fl.IdentityTrans(model))
+    g = golden["test_panel_mif_regression"]
+    assert_close_tol(st[-1], g["final_shared"])
+    assert_close_tol(ut[-1].ravel(), g["final_unit"])
+
+
+def test_perfilter_ancestry_maps_final_to_initial_particles():
+    # with every random-walk sigma zero the final swarm is the initial swarm
+    # gathered through the composed resampling ancestry
+    model, sers, _, _, _, J = _lg_panel()
+    rng = np.random.default_rng(0)
+    th = (lg1_theta()[None] + F32(0.01) * rng.standard_normal((J, 5)).astype(F32)).astype(F32)
+    th_est = model.to_est(th)
+    rw0 = fl.RWSigma(np.zeros(5), np.zeros(5), a=0.5)
+    out, _, anc = fl.perfilter(model, 0, th_est, sers[0], tf.key(7), J, rw0, return_ancestry=True)
+    assert anc.shape == (J,) and anc.min() >= 0 and anc.max() < J
+    assert len(np.unique(anc)) < J  # resampling happened
+    np.testing.assert_array_equal(out, th_est[anc])
+
+
+def test_panel_mif_single_unit_block_false_equals_block_true():
+    model, sers, sh0, un0, rw, J = _lg_panel()
+    k = tf.key(99)
+    a = fl.panel_mif(model, sh0, un0[:, :1], sers[:1], k, J, 2, rw, np.arange(5), block=True)
+    b = fl.panel_mif(model, sh0, un0[:, :1], sers[:1], k, J, 2, rw, np.arange(5), block=False)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+
+
+def test_panel_mif_all_unit_specific_equals_per_unit_mif():
+    # with no shared parameters (block=True) unit u's chain is the single-unit
+    # mif of that unit driven by the unit keys split(iter_key, U)[u]
+    model, sers, sh0, un0, rw, J = _lg_panel()
+    full = np.concatenate([np.repeat(sh0[:, None, :], 2, 1), un0], axis=2)  # (J, U, 5)
+    key = tf.key(5)
+    st, ut, _, fu = fl.panel_mif(model, np.zeros((J, 0), F32), full, sers, key, J, 2, rw, np.arange(5))
+    mkeys = tf.split(key, 3)[1:]
+    for u in range(2):
+        th = model.to_est(full[:, u, :])
+        for m in range(2):
+            th, ll = fl.perfilter(model, m, th, sers[u], tf.split(mkeys[m], 2)[u], J, rw)
+            assert_close_tol(ut[m + 1, u, 0], ll)
+        np.testing.assert_array_equal(fu[:, u, :], model.from_est(th))
