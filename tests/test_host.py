@@ -1,0 +1,99 @@
+"""Host-side logic (no GPU): the Euler time grid, replicate aggregation,
+random-walk sigmas, key derivation and argument errors -- cases as the
+reference's own tests (tests/test_maths.py, tests/test_core/test_helpers.py)."""
+
+import warnings
+
+import numpy as np
+import pytest
+
+from oracle import threefry as tf
+from pypomp_amd import RWSigma
+from pypomp_amd.maths import logmeanexp, logmeanexp_se
+from pypomp_amd.series import Series, calc_steps
+
+
+def test_logmeanexp_matches_r_pomp():
+    # R: pomp::logmeanexp(c(100,101,102,103,104), se=TRUE) -> 102.8424765, 0.7510094
+    x = np.array([100.0, 101.0, 102.0, 103.0, 104.0])
+    assert abs(logmeanexp(x) - 102.8424765) < 1e-6
+    assert abs(logmeanexp_se(x) - 0.7510094) < 1e-6
+
+
+@pytest.mark.parametrize("arr, lme_nan, se_nan", [
+    (np.array([100.0]), False, True),
+    (np.array([]), True, True),
+    (np.array([np.nan, np.nan]), True, True),
+    (np.array([100, 101, -np.inf]), False, False),
+    (np.array([1, np.nan, -np.inf]), True, True),
+    (np.array([42.0]), False, True),
+])
+def test_logmeanexp_edge_cases(arr, lme_nan, se_nan):
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        lme, se = logmeanexp(arr), logmeanexp_se(arr)
+    assert np.isnan(lme) == lme_nan and (lme_nan or np.isfinite(lme))
+    assert np.isnan(se) == se_nan and (se_nan or np.isfinite(se))
+
+
+def test_logmeanexp_large_values():
+    arr = np.array([1e10, 1e10 + 1, 1e10 + 2])
+    assert np.isclose(logmeanexp(arr), 1e10 + logmeanexp(np.array([0.0, 1, 2])), atol=1e-5)
+
+
+def test_euler_grid_dt_and_nstep():
+    times0 = np.array([-1.0, 0, 0.5, 1, 1.5, 2, 2.5, 3, 3.5, 4, 4.5, 5])
+    ns, dts = calc_steps(times0, dt=0.5)
+    assert ns.tolist() == [2] + [1] * 10 and np.allclose(dts, [0.5] * 11)
+    ns, dts = calc_steps(np.array([0.0, 1.0, 1.0, 3.0]), dt=0.3)
+    assert ns.tolist() == [4, 0, 7]
+    assert np.allclose(dts, [0.25, 0.0, 2.0 / 7])
+    ns, dts = calc_steps(np.array([0.0, 1.0, 3.0]), nstep=7)
+    assert ns.tolist() == [7, 7] and np.allclose(dts, [1 / 7, 2 / 7])
+    with pytest.raises(ValueError):
+        calc_steps(times0, dt=0.5, nstep=2)
+    with pytest.raises(ValueError):
+        calc_steps(times0)
+    s = Series.build(np.zeros((3, 1)), [1.0, 2.0, 4.0], 0.0, dt=0.5)
+    assert s.nstep.tolist() == [2, 2, 4] and s.dt_ext.shape == (8,)
+    assert np.isclose(s.dt_ext.astype(np.float64).sum(), 4.0)
+
+
+def test_rwsigma_validation_and_order():
+    rw = RWSigma({"a": 0.1, "b": 0.2, "c": 0.3}, init_names=["c"]).geometric_cooling(0.5)
+    c = rw.canonicalize(["c", "a", "b"])
+    assert np.allclose(c.sigmas_array, [0, 0.1, 0.2]) and np.allclose(c.sigmas_init_array, [0.3, 0, 0])
+    assert np.float32(c.factor) == np.float32(0.5 ** (1 / 50))
+    # cooling factor at nt = 0 of iteration m is factor^m
+    assert abs(float(c.cooling_factor(0, 3, 10)) - 0.5 ** (3 / 50)) < 1e-6
+    for bad in ({"a": -1.0}, {"a": True}, {1: 0.1}):
+        with pytest.raises(ValueError):
+            RWSigma(bad)
+    with pytest.raises(ValueError):
+        RWSigma({"a": 0.1}, init_names=["b"])
+    with pytest.raises(ValueError):
+        rw.canonicalize(["a", "b"])
+    with pytest.raises(ValueError):
+        rw.geometric_cooling(1.5)
+
+
+def test_host_key_derivation_matches_oracle():
+    from pypomp_amd import random as prandom
+    k = prandom.key(20260811)
+    np.testing.assert_array_equal(np.asarray(k, np.uint32), tf.key(20260811))
+    np.testing.assert_array_equal(prandom.split(k, 5), tf.split(tf.key(20260811), 5))
+
+
+def test_pomp_argument_errors():
+    import pypomp_amd as pp
+    from pypomp_amd.models import LG
+    P = LG(ys=np.zeros((4, 2), np.float32))
+    with pytest.raises(ValueError):
+        P.pfilter(J=0, key=prandom_key())
+    with pytest.raises(ValueError):
+        P.mif(J=10, M=1, key=prandom_key(), rw_sd=pp.RWSigma({"nope": 0.1}))
+
+
+def prandom_key():
+    from pypomp_amd import random as prandom
+    return prandom.key(1)
