@@ -311,7 +311,7 @@ def cpu_baseline(target_s=12.0):
     cpu.lg_pfilter(theta, 2, ys[:T], J, keys)
     dt = time.perf_counter() - t0
     return {"value": cores * J * T / dt, "unit": "particle-steps/s", "cores": cores, "kind": "port",
-            "sample": f"C restatement (oracle/cpu_pf.c, -O3 -march=native, OpenMP), LG 2-D pfilter, "
+            "sample": f"C restatement (oracle/cpu_pf.c, -O3 -march=x86-64-v3, OpenMP), LG 2-D pfilter, "
                       f"{cores} replicates x J={J} x T={T} ({dt:.1f} s)"}
 
 

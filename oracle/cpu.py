@@ -7,7 +7,7 @@ from pathlib import Path
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
-LIB = HERE / "_ref" / "libcpupf.so"
+LIB = HERE / "_lib" / "libcpupf.so"
 _lib = None
 
 
