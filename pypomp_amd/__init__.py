@@ -5,7 +5,7 @@ filter run as hand-written HIP kernels for gfx950 in ``_lib/libmipomp.so``
 (C ABI: include/mipomp.h), behind the reference's ``Pomp`` / functional API.
 """
 
-from . import functional, models, random
+from . import autotune, functional, models, random
 from .maths import logmeanexp, logmeanexp_se
 from .models import ParTrans
 from .pomp import Pomp
@@ -14,4 +14,4 @@ from .rw_sigma import RWSigma
 functional_api = functional
 F = functional
 
-__all__ = ["Pomp", "ParTrans", "RWSigma", "functional", "F", "models", "random", "logmeanexp", "logmeanexp_se"]
+__all__ = ["Pomp", "ParTrans", "RWSigma", "autotune", "functional", "F", "models", "random", "logmeanexp", "logmeanexp_se"]

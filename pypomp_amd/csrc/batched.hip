@@ -337,8 +337,8 @@ __device__ void run_filter(const BatchArgs& a, const int f, char* buf, double* r
 // A filter's result does not depend on which workgroup runs it.  Global
 // scratch is indexed by the filter's queue position, which every workgroup
 // takes once.
-template <class M, int ALGO>
-__global__ __launch_bounds__(256) void k_batched(const BatchArgs a) {
+template <class M, int ALGO, int NT>
+__global__ __launch_bounds__(NT) void k_batched(const BatchArgs a) {
   extern __shared__ __align__(256) char smem[];
   __shared__ int next;
   double* red = reinterpret_cast<double*>(smem);
@@ -381,9 +381,16 @@ __global__ void k_order(SeriesSet ser, const int32_t* __restrict__ unit, int R, 
 
 // Host launchers ------------------------------------------------------------
 
+// Workgroup sizes of the batched form: 256 when several filters share a CU,
+// up to 768 (12 waves) when one large filter per CU must fill it alone.
 template <class M, int ALGO>
 static hipError_t launch_batched_t(const BatchArgs& a, int block, int grid, size_t lds, hipStream_t st) {
-  hipLaunchKernelGGL((k_batched<M, ALGO>), dim3(grid), dim3(block), lds, st, a);
+  switch (block) {
+    case 256: hipLaunchKernelGGL((k_batched<M, ALGO, 256>), dim3(grid), dim3(256), lds, st, a); break;
+    case 512: hipLaunchKernelGGL((k_batched<M, ALGO, 512>), dim3(grid), dim3(512), lds, st, a); break;
+    case 768: hipLaunchKernelGGL((k_batched<M, ALGO, 768>), dim3(grid), dim3(768), lds, st, a); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

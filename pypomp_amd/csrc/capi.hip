@@ -68,6 +68,7 @@ constexpr int64_t kParamSlot = 512;  // bytes per filter of prepared model param
 // Launch-shape parameters (set by the autotuner through mp_tune_set).
 int g_plain_scan_rows = 8;  // particle rows of 256 per scan tile (1..8)
 int g_plain_prop_rows = 2;  // particle rows of 256 per propagate block (1..16)
+int g_batched_block = 0;    // batched workgroup size: 0 = automatic, 256, 512, 768
 
 // Optional per-kernel timing with HIP events recorded on the launch stream
 // (mp_profile_*).  Kinds: 0 plain propagate, 1 plain scan, 2 batched filter,
@@ -160,8 +161,23 @@ int check_series(const mp_series_desc* s) {
   return MP_OK;
 }
 
+// Batched workgroup size.  Filters of >= 1536 particles take 768 threads (12
+// waves: a CU's full share at the kernels' ~168 VGPRs), which measured best
+// for both one filter per CU (SEIR, J = 20000, 256 filters: 1.7x over 256
+// threads) and several (SEIR panel, J = 2000, 800 filters: 1.2x).  Smaller
+// filters aim at ~12 waves per CU from several workgroups.  mp_tune_set
+// overrides the choice.
+int batched_block(int R, int J) {
+  if (g_batched_block) return g_batched_block;
+  if (J >= 1536) return 768;
+  const int per_cu = (R + num_cus() - 1) / num_cus();
+  int nt = per_cu >= 3 ? 256 : per_cu == 2 ? 512 : 768;
+  while (nt > 256 && nt / 2 >= J) nt -= 256;  // at least two particles per thread
+  return nt;
+}
+
 struct BatchedPlan {
-  int W;
+  int W, block;
   int64_t sched_len, sched_bytes, order_bytes;
   int64_t per_filter;  // global scratch per filter (0 when LDS-resident)
   bool lds_resident;
@@ -178,7 +194,8 @@ BatchedPlan plan_batched(int algo, const mp_model_desc* m, const mp_series_desc*
   p.sched_bytes = align256((int64_t)R * p.sched_len * 8);
   p.order_bytes = align256((int64_t)R * 4) + 256;  // filter order + queue counter
   const FilterScratch L = filter_scratch_layout(p.W, J);
-  const size_t small = batched_small_lds(J, kBatchedBlock);
+  p.block = batched_block(R, J);
+  const size_t small = batched_small_lds(J, p.block);
   // Keep filters LDS-resident when two workgroups still fit per CU.
   p.lds_resident = (int64_t)small + L.total <= (int64_t)max_lds_bytes() / 2;
   p.per_filter = p.lds_resident ? 0 : L.total;
@@ -241,7 +258,7 @@ int run_batched(int algo, const mp_model_desc* m, const mp_series_desc* s, Batch
   MP_HIP(timed(3, st, [&] { return launch_sched(algo, a.keys, a.ser, a.unit, p.sched_len, sched, a.R, st); }));
   const int per_cu = p.lds_resident ? 2 : 1;
   int grid = a.R < num_cus() * per_cu * 4 ? a.R : num_cus() * per_cu * 4;
-  MP_HIP(timed(2, st, [&] { return launch_batched(algo, a, kBatchedBlock, grid, p.lds, st); }));
+  MP_HIP(timed(2, st, [&] { return launch_batched(algo, a, p.block, grid, p.lds, st); }));
   return MP_OK;
 }
 
@@ -567,6 +584,11 @@ int32_t mp_tune_set(int32_t param, int32_t value) {
       if (value < 1 || value > 16) return fail(MP_ERR_ARG, "propagate rows must be 1..16");
       g_plain_prop_rows = value;
       return MP_OK;
+    case MP_TUNE_BATCHED_BLOCK:
+      if (value != 0 && value != 256 && value != 512 && value != 768)
+        return fail(MP_ERR_ARG, "batched workgroup size must be 0 (automatic), 256, 512 or 768");
+      g_batched_block = value;
+      return MP_OK;
   }
   return fail(MP_ERR_ARG, "unknown tuning parameter %d", param);
 }
@@ -575,6 +597,7 @@ int32_t mp_tune_get(int32_t param) {
   switch (param) {
     case MP_TUNE_PLAIN_SCAN_ROWS: return g_plain_scan_rows;
     case MP_TUNE_PLAIN_PROP_ROWS: return g_plain_prop_rows;
+    case MP_TUNE_BATCHED_BLOCK: return g_batched_block;
   }
   return -1;
 }

@@ -69,3 +69,16 @@ def test_argument_errors_are_reported_without_touching_the_device(lib):
     assert rc == 1 and b"series" in lib.mp_last_error()
     rc = lib.mp_resample_systematic(None, None, 1, 0, None, None, 0, None)
     assert rc == 1
+
+
+def test_tuning_settings_round_trip_and_reject_bad_values(lib):
+    from pypomp_amd import autotune
+    for name, (pid, cands) in autotune.PARAMS.items():
+        old = autotune.get(name)
+        for v in cands:
+            autotune.set(name, v)
+            assert autotune.get(name) == v
+        autotune.set(name, old)
+    assert lib.mp_tune_set(3, 300) != 0
+    assert lib.mp_tune_set(1, 0) != 0
+    assert lib.mp_tune_set(99, 1) != 0

@@ -166,3 +166,29 @@ def test_j_equals_one_and_workspace_errors():
     assert np.isfinite(out["logLik64"]).all()
     with pytest.raises(ValueError):
         F.pfilter(struct, theta[None], 0, tf.split(tf.key(3), 1).reshape(1, 1, 2))
+
+
+def test_launch_shapes_do_not_change_results():
+    # every tunable launch shape computes the same bits
+    import pypomp_amd.functional as F
+    from pypomp_amd import autotune
+    struct, theta, ys = lg2_struct(1)
+    keys = tf.split(tf.key(77), 3).reshape(1, 3, 2)
+    saved = {n: autotune.get(n) for n in autotune.PARAMS}
+    try:
+        outs = []
+        for nt in (256, 512, 768):
+            autotune.set("batched_block", nt)
+            outs.append(F.pfilter(struct, theta[None], 5000, keys, form="batched")["logLik64"])
+        for sr in (2, 8):
+            for pr in (1, 4):
+                autotune.set("plain_scan_rows", sr)
+                autotune.set("plain_prop_rows", pr)
+                outs.append(F.pfilter(struct, theta[None], 5000, keys, form="plain")["logLik64"])
+        for o in outs[1:]:
+            np.testing.assert_array_equal(o, outs[0])
+        best = autotune.tune_pfilter(struct, theta, 5000, 3, form="batched")
+        assert best["batched_block"] in (256, 512, 768) and best["ms"] > 0
+    finally:
+        for n, v in saved.items():
+            autotune.set(n, v)
