@@ -169,7 +169,8 @@ def test_j_equals_one_and_workspace_errors():
 
 
 def test_launch_shapes_do_not_change_results():
-    # every tunable launch shape computes the same bits
+    # every tunable launch shape draws the same random numbers and ancestors;
+    # only the association of the float64 exp-sum depends on the shape
     import pypomp_amd.functional as F
     from pypomp_amd import autotune
     struct, theta, ys = lg2_struct(1)
@@ -179,14 +180,15 @@ def test_launch_shapes_do_not_change_results():
         outs = []
         for nt in (256, 512, 768):
             autotune.set("batched_block", nt)
-            outs.append(F.pfilter(struct, theta[None], 5000, keys, form="batched")["logLik64"])
+            outs.append(F.pfilter(struct, theta[None], 5000, keys, form="batched"))
         for sr in (2, 8):
             for pr in (1, 4):
                 autotune.set("plain_scan_rows", sr)
                 autotune.set("plain_prop_rows", pr)
-                outs.append(F.pfilter(struct, theta[None], 5000, keys, form="plain")["logLik64"])
+                outs.append(F.pfilter(struct, theta[None], 5000, keys, form="plain"))
         for o in outs[1:]:
-            np.testing.assert_array_equal(o, outs[0])
+            np.testing.assert_array_equal(o["logLik"], outs[0]["logLik"])
+            np.testing.assert_allclose(o["logLik64"], outs[0]["logLik64"], rtol=1e-13)
         best = autotune.tune_pfilter(struct, theta, 5000, 3, form="batched")
         assert best["batched_block"] in (256, 512, 768) and best["ms"] > 0
     finally:
