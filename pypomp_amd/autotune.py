@@ -2,7 +2,7 @@
 
 The library exposes its launch shapes as process-wide settings
 (``mp_tune_set`` / ``mp_tune_get``: plain scan-tile rows, plain propagate-block
-rows, batched workgroup size).  ``sweep`` times a caller-supplied workload
+rows, batched workgroup size, MOP-adjoint workgroup size).  ``sweep`` times a caller-supplied workload
 under every combination of candidate values with HIP events on the current
 stream, keeps the fastest, applies it and caches it per (workload key, device),
 in memory and optionally in a JSON file (``PYPOMP_AMD_TUNE_CACHE``).  The
@@ -26,6 +26,7 @@ PARAMS = {
     "plain_scan_rows": (_native.MP_TUNE_PLAIN_SCAN_ROWS, (2, 4, 8)),
     "plain_prop_rows": (_native.MP_TUNE_PLAIN_PROP_ROWS, (1, 2, 4)),
     "batched_block": (_native.MP_TUNE_BATCHED_BLOCK, (256, 512, 768)),
+    "grad_block": (_native.MP_TUNE_GRAD_BLOCK, (256, 512)),
 }
 
 _lock = threading.Lock()

@@ -29,7 +29,8 @@ hipError_t launch_order(const SeriesSet& ser, const int32_t* unit, int R, int32_
 
 namespace mp {
 hipError_t launch_mop_grad(const GradArgs& a, int block, int grid, hipStream_t st);
-int64_t mop_grad_scratch_bytes(int S, int J, int T, int K);
+int64_t mop_grad_scratch_bytes(int S, int J, int T, int K, bool lds_work);
+size_t mop_grad_lds_bytes(int S, int J, int block, bool lds_work);
 }  // namespace mp
 
 using namespace mp;
@@ -69,6 +70,7 @@ constexpr int64_t kParamSlot = 512;  // bytes per filter of prepared model param
 int g_plain_scan_rows = 8;  // particle rows of 256 per scan tile (1..8)
 int g_plain_prop_rows = 2;  // particle rows of 256 per propagate block (1..16)
 int g_batched_block = 0;    // batched workgroup size: 0 = automatic, 256, 512, 768
+int g_grad_block = 0;       // MOP-adjoint workgroup size: 0 = automatic, 256, 512
 
 // Optional per-kernel timing with HIP events recorded on the launch stream
 // (mp_profile_*).  Kinds: 0 plain propagate, 1 plain scan, 2 batched filter,
@@ -331,6 +333,8 @@ float log_thresh(float thresh) { return thresh > 0.f ? d_log(thresh) : -__builti
 
 struct GradPlan {
   int64_t sched_len, sched_bytes, per_filter;
+  int block;
+  bool lds_work;
   size_t total;
 };
 
@@ -339,7 +343,10 @@ GradPlan plan_grad(const mp_model_desc* m, const mp_series_desc* s, int R, int J
   const SeriesSet& ser = *reinterpret_cast<const SeriesSet*>(s);
   p.sched_len = sched_stride(ALGO_MOP, ser);
   p.sched_bytes = align256((int64_t)R * p.sched_len * 8);
-  p.per_filter = align256(mop_grad_scratch_bytes(m->S, J, s->Tmax, K));
+  p.block = g_grad_block ? g_grad_block : 256;
+  // per-step arrays in LDS when two workgroups still fit per CU
+  p.lds_work = mop_grad_lds_bytes(m->S, J, p.block, true) <= (size_t)max_lds_bytes() / 2;
+  p.per_filter = align256(mop_grad_scratch_bytes(m->S, J, s->Tmax, K, p.lds_work));
   p.total = (size_t)p.sched_bytes + (size_t)R * (size_t)p.per_filter + 256;
   return p;
 }
@@ -366,10 +373,11 @@ int run_mop_grad(const mp_model_desc* m, const mp_series_desc* s, const float* t
   a.grad = grad;
   a.ws = base + p.sched_bytes;
   a.ws_stride = p.per_filter;
+  a.lds_work = p.lds_work;
   MP_HIP(timed(3, st, [&] { return launch_sched(ALGO_MOP, keys, a.ser, nullptr, p.sched_len,
                                                 const_cast<uint32_t*>(a.sched), R, st); }));
   const int grid = R < num_cus() * 8 ? R : num_cus() * 8;
-  hipError_t e = timed(2, st, [&] { return launch_mop_grad(a, kBatchedBlock, grid, st); });
+  hipError_t e = timed(2, st, [&] { return launch_mop_grad(a, p.block, grid, st); });
   if (e == hipErrorNotSupported) return fail(MP_ERR_ARG, "model kind %d has no adjoint (not differentiable)", m->kind);
   MP_HIP(e);
   return MP_OK;
@@ -589,6 +597,11 @@ int32_t mp_tune_set(int32_t param, int32_t value) {
         return fail(MP_ERR_ARG, "batched workgroup size must be 0 (automatic), 256, 512 or 768");
       g_batched_block = value;
       return MP_OK;
+    case MP_TUNE_GRAD_BLOCK:
+      if (value != 0 && value != 256 && value != 512)
+        return fail(MP_ERR_ARG, "adjoint workgroup size must be 0 (automatic), 256 or 512");
+      g_grad_block = value;
+      return MP_OK;
   }
   return fail(MP_ERR_ARG, "unknown tuning parameter %d", param);
 }
@@ -598,6 +611,7 @@ int32_t mp_tune_get(int32_t param) {
     case MP_TUNE_PLAIN_SCAN_ROWS: return g_plain_scan_rows;
     case MP_TUNE_PLAIN_PROP_ROWS: return g_plain_prop_rows;
     case MP_TUNE_BATCHED_BLOCK: return g_batched_block;
+    case MP_TUNE_GRAD_BLOCK: return g_grad_block;
   }
   return -1;
 }

@@ -149,6 +149,7 @@ struct GradArgs {
   float* grad;     // (R, d): d loglik / d theta_est
   char* ws;
   int64_t ws_stride;
+  int lds_work;    // per-step working arrays in LDS (else in the global scratch)
 };
 
 }  // namespace mp

@@ -226,10 +226,11 @@ struct SV {
   }
 
   // Vector-Jacobian product of one sub-step x_out = step(x_in): given xbar_out,
-  // writes xbar_in and accumulates theta-bar.  Redraws the step's normal.
-  __device__ static void step_vjp(const ModelDesc&, const P& p, Key k, float, float, const float* xin,
+  // writes xbar_in and accumulates theta-bar.  The step's normal is x_out[1]
+  // (eta), so nothing is redrawn.
+  __device__ static void step_vjp(const ModelDesc&, const P& p, const float* xin, const float* xout,
                                   const float* xbar_out, float* xbar_in, double* gth) {
-    const float z = normal_at(k, 0);
+    const float z = xout[1];
     const double vb = xbar_out[0];
     xbar_in[0] = p.phi * xbar_out[0];
     xbar_in[1] = 0.f;

@@ -20,10 +20,12 @@
 // Systematic resampling makes a_t non-decreasing, so every parent's children are
 // one contiguous run: the scatter is a deterministic segmented sum.
 //
-// Memory: the forward keeps only every K-th pre-resampling state and its
-// ancestor indices; the reverse sweep recomputes each K-step block from its
-// checkpoint (states, measurement densities and ancestry) and runs the block
-// backwards.  Cost ~ 3 forward passes; storage O((T/K + K) J S).
+// Memory: the forward keeps every K-th pre-step state, the ancestry of every
+// step (4 bytes per particle-step) and each step's density maximum and exp-sum;
+// the reverse sweep recomputes each K-step block's states and densities from
+// its checkpoint with the logged ancestry (no reductions, scans or searches)
+// and runs the block backwards.  States storage O((T/K + K) J S) instead of
+// O(T J S).  The model's step VJP reads its noise from the stored output state.
 #include "blockops.hpp"
 #include "engine.hpp"
 #include "resample.hpp"
@@ -31,11 +33,43 @@
 
 namespace mp {
 
+// Per-filter global scratch: checkpoints, the ancestry log, per-step
+// statistics and the recomputed K-step block.
 struct GradScratch {
-  int64_t ckx, cka, bx, bm, ba, c, lw, cx, cl, xca, lama, total;
+  int64_t ckx, anc, stat, bx, bm, work, total;
 };
 
-__host__ __device__ inline GradScratch grad_scratch_layout(int S, int J, int T, int K) {
+// Per-step working arrays.  The forward set (prefix sums, log-weights, two
+// state buffers) and the reverse set (children's contributions, carried
+// adjoints, run starts) are never live together, so they share one region,
+// kept in LDS when it fits and in the global scratch otherwise.
+struct GradWork {
+  int64_t c, lw, x0, x1;                 // forward
+  int64_t cx, cl, xca, lama, lb;         // reverse
+  int64_t total;
+};
+
+__host__ __device__ inline int64_t align16(int64_t v) { return (v + 15) & ~int64_t(15); }
+
+__host__ __device__ inline GradWork grad_work_layout(int S, int J) {
+  GradWork w;
+  int64_t o = 0;
+  w.c = o;    o = align16(o + (int64_t)J * 8);
+  w.lw = o;   o = align16(o + (int64_t)J * 4);
+  w.x0 = o;   o = align16(o + (int64_t)S * J * 4);
+  w.x1 = o;   o = align16(o + (int64_t)S * J * 4);
+  const int64_t fwd = o;
+  o = 0;
+  w.cx = o;   o = align16(o + (int64_t)S * J * 4);
+  w.cl = o;   o = align16(o + (int64_t)J * 4);
+  w.xca = o;  o = align16(o + (int64_t)S * J * 4);
+  w.lama = o; o = align16(o + (int64_t)J * 4);
+  w.lb = o;   o = align16(o + (int64_t)(J + 1) * 4);
+  w.total = fwd > o ? fwd : o;
+  return w;
+}
+
+__host__ __device__ inline GradScratch grad_scratch_layout(int S, int J, int T, int K, bool lds_work) {
   GradScratch g;
   const int64_t nblk = (T + K - 1) / K;
   int64_t o = 0;
@@ -45,48 +79,40 @@ __host__ __device__ inline GradScratch grad_scratch_layout(int S, int J, int T, 
     return r;
   };
   g.ckx = take(nblk * S * (int64_t)J * 4);
-  g.cka = take(nblk * (int64_t)J * 4);
-  g.bx = take((int64_t)(K + 1) * S * J * 4);
+  g.anc = take((int64_t)T * J * 4);
+  g.stat = take((int64_t)T * 16);
+  g.bx = take((int64_t)K * S * J * 4);
   g.bm = take((int64_t)K * J * 4);
-  g.ba = take((int64_t)(K + 1) * J * 4);
-  g.c = take((int64_t)J * 8);
-  g.lw = take((int64_t)J * 4);
-  g.cx = take((int64_t)S * J * 4);
-  g.cl = take((int64_t)J * 4);
-  g.xca = take((int64_t)S * J * 4);
-  g.lama = take((int64_t)J * 4);
+  g.work = take(lds_work ? 0 : grad_work_layout(S, J).total);
   g.total = o;
   return g;
 }
 
-// first index in [0, J) with a[i] >= v (a non-decreasing)
-__device__ __forceinline__ int lower_bound_i32(const int32_t* a, int J, int v) {
-  int lo = 0, hi = J;
-  while (lo < hi) {
-    int mid = (lo + hi) >> 1;
-    if (a[mid] < v) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
-}
-
+// Forward pass (value): the ancestry a_t of every step (a_0 = identity), the
+// maximum and FP64 exp-sum of each step's measurement densities, and the
+// pre-step states at every K-th step.  Reverse sweep, block by block: the
+// block's states and densities are recomputed from its checkpoint with the
+// logged ancestry (one launch-free pass: gather, step, density), then the
+// adjoint runs backwards through the block.
 template <class M>
-__device__ void mop_grad_filter(const GradArgs& a, const int f, char* buf, double* red, uint64_t* tab) {
+__device__ void mop_grad_filter(const GradArgs& a, const int f, char* buf, char* wbuf, double* red, uint64_t* tab) {
   const ModelDesc md = a.md;
   const int J = a.J, S = M::FIXED_S ? M::MAXS : md.S, d = md.d, T = a.ser.Tmax, K = a.K;
   const int tid = threadIdx.x, B = blockDim.x;
-  const GradScratch L = grad_scratch_layout(S, J, T, K);
+  const GradScratch L = grad_scratch_layout(S, J, T, K, a.lds_work);
+  const GradWork W = grad_work_layout(S, J);
   float* ckx = reinterpret_cast<float*>(buf + L.ckx);
-  int32_t* cka = reinterpret_cast<int32_t*>(buf + L.cka);
+  int32_t* ancl = reinterpret_cast<int32_t*>(buf + L.anc);
+  double* stat = reinterpret_cast<double*>(buf + L.stat);
   float* bx = reinterpret_cast<float*>(buf + L.bx);
   float* bm = reinterpret_cast<float*>(buf + L.bm);
-  int32_t* ba = reinterpret_cast<int32_t*>(buf + L.ba);
-  uint64_t* Cc = reinterpret_cast<uint64_t*>(buf + L.c);
-  float* lw = reinterpret_cast<float*>(buf + L.lw);
-  float* cx = reinterpret_cast<float*>(buf + L.cx);
-  float* cl = reinterpret_cast<float*>(buf + L.cl);
-  float* xca = reinterpret_cast<float*>(buf + L.xca);
-  float* lama = reinterpret_cast<float*>(buf + L.lama);
+  uint64_t* Cc = reinterpret_cast<uint64_t*>(wbuf + W.c);
+  float* lw = reinterpret_cast<float*>(wbuf + W.lw);
+  float* cx = reinterpret_cast<float*>(wbuf + W.cx);
+  float* cl = reinterpret_cast<float*>(wbuf + W.cl);
+  float* xca = reinterpret_cast<float*>(wbuf + W.xca);
+  float* lama = reinterpret_cast<float*>(wbuf + W.lama);
+  int32_t* lb = reinterpret_cast<int32_t*>(wbuf + W.lb);
   const int64_t SJ = (int64_t)S * J;
   const float* ys = a.ser.ys;
   const float* times = a.ser.times;
@@ -115,9 +141,9 @@ __device__ void mop_grad_filter(const GradArgs& a, const int f, char* buf, doubl
   };
   auto tstart = [&](int t) { return t == 0 ? a.ser.t0 : times[t - 1]; };
 
-  // ---------------- forward: value + checkpoints ----------------
-  float* XA = bx;
-  float* XB = bx + SJ;
+  // ---------------- forward: value, ancestry log, checkpoints ----------------
+  float* XA = reinterpret_cast<float*>(wbuf + W.x0);
+  float* XB = reinterpret_cast<float*>(wbuf + W.x1);
   for (int j = tid; j < J; j += B) {
     float x[M::MAXS];
     M::rinit(md, P, split_at(fkey, 1 + (uint64_t)j), a.ser.t0, x);
@@ -129,22 +155,21 @@ __device__ void mop_grad_filter(const GradArgs& a, const int f, char* buf, doubl
   float ures = 0.f;
   for (int t = 0; t < T; ++t) {
     const bool ck = (t % K) == 0;
-    const int kb = t / K;
+    float* ckt = ckx + (int64_t)(t / K) * SJ;
+    int32_t* anct = ancl + (int64_t)t * J;
     const Key sk = step_key(t);
     const float dt = dts[tidx0[t]];
+    const float ts = tstart(t), tt = times[t];
     const float* y = ys + (int64_t)t * md.dy;
     float lmax = -__builtin_inff(), lmin = __builtin_inff();
     for (int j = tid; j < J; j += B) {
       const int src = t == 0 ? j : ancestor_of(Cc, J, ures, total, j);
-      if (ck) {
-        MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S)
-          ckx[(int64_t)kb * SJ + s * (int64_t)J + j] = XA[s * (int64_t)J + j];
-        cka[(int64_t)kb * J + j] = src;
-      }
+      if (ck) MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) ckt[s * (int64_t)J + j] = XA[s * (int64_t)J + j];
+      anct[j] = src;
       float x[M::MAXS];
       MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) x[s] = XA[s * (int64_t)J + src];
-      M::step(md, P, split_at(sk, (uint64_t)j), tstart(t), dt, x);
-      const float m = M::dmeas(md, P, y, x, times[t]);
+      M::step(md, P, split_at(sk, (uint64_t)j), ts, dt, x);
+      const float m = M::dmeas(md, P, y, x, tt);
       MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) XB[s * (int64_t)J + j] = x[s];
       lw[j] = m;
       lmax = nanmax(lmax, m);
@@ -158,6 +183,10 @@ __device__ void mop_grad_filter(const GradArgs& a, const int f, char* buf, doubl
         if (v == v) se += (double)expf(v - lmax);
       }
     se = block_sum(se, red);
+    if (tid == 0) {
+      stat[2 * t] = (double)lmax;
+      stat[2 * t + 1] = se;
+    }
     loglik += (lmax == -__builtin_inff()) ? -__builtin_inf() : ((double)lmax + log(se) - (double)logJ);
     if (t < T - 1) {
       total = block_fixed_scan(lw, Cc, J, lmax, tab);
@@ -179,65 +208,41 @@ __device__ void mop_grad_filter(const GradArgs& a, const int f, char* buf, doubl
   }
   const int nblk = (T + K - 1) / K;
   for (int kb = nblk - 1; kb >= 0; --kb) {
-    const int t0 = kb * K, t1 = min(T, t0 + K), n = t1 - t0;
-    // recompute the block from its checkpoint
-    for (int j = tid; j < J; j += B) {
-      MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S)
-        bx[s * (int64_t)J + j] = ckx[(int64_t)kb * SJ + s * (int64_t)J + j];
-      ba[j] = cka[(int64_t)kb * J + j];
-    }
-    __syncthreads();
+    const int t0 = kb * K, n = min(T, t0 + K) - t0;
+    const float* ck = ckx + (int64_t)kb * SJ;
+    auto state = [&](int i) -> float* { return i == 0 ? const_cast<float*>(ck) : bx + (int64_t)(i - 1) * SJ; };
+    // recompute the block's states and densities from its checkpoint
     for (int i = 0; i < n; ++i) {
       const int t = t0 + i;
       const Key sk = step_key(t);
       const float dt = dts[tidx0[t]];
+      const float ts = tstart(t), tt = times[t];
       const float* y = ys + (int64_t)t * md.dy;
-      const float* xin = bx + (int64_t)i * SJ;
-      float* xout = bx + (int64_t)(i + 1) * SJ;
-      const int32_t* anc = ba + (int64_t)i * J;
-      float lmax = -__builtin_inff(), lmin = __builtin_inff();
+      const float* xin = state(i);
+      float* xout = state(i + 1);
+      const int32_t* anct = ancl + (int64_t)t * J;
       for (int j = tid; j < J; j += B) {
-        const int src = anc[j];
+        const int src = anct[j];
         float x[M::MAXS];
         MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) x[s] = xin[s * (int64_t)J + src];
-        M::step(md, P, split_at(sk, (uint64_t)j), tstart(t), dt, x);
-        const float m = M::dmeas(md, P, y, x, times[t]);
+        M::step(md, P, split_at(sk, (uint64_t)j), ts, dt, x);
+        bm[(int64_t)i * J + j] = M::dmeas(md, P, y, x, tt);
         MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) xout[s * (int64_t)J + j] = x[s];
-        bm[(int64_t)i * J + j] = m;
-        lw[j] = m;
-        lmax = nanmax(lmax, m);
-        lmin = nanmin(lmin, m);
-      }
-      block_maxmin(lmax, lmin, red);
-      if (i < n - 1) {
-        const uint64_t tot = block_fixed_scan(lw, Cc, J, lmax, tab);
-        const float u = uniform01(bits_at(res_key(t), 0));
-        int32_t* nanc = ba + (int64_t)(i + 1) * J;
-        for (int j = tid; j < J; j += B) nanc[j] = ancestor_of(Cc, J, u, tot, j);
       }
       __syncthreads();
     }
     // backward through the block
     for (int i = n - 1; i >= 0; --i) {
       const int t = t0 + i;
-      const Key sk = step_key(t);
       const float dt = dts[tidx0[t]];
+      const float ts = tstart(t);
       const float* y = ys + (int64_t)t * md.dy;
-      const float* xin = bx + (int64_t)i * SJ;
-      const float* xt = bx + (int64_t)(i + 1) * SJ;
+      const float* xin = state(i);
+      const float* xt = state(i + 1);
       const float* mt = bm + (int64_t)i * J;
-      const int32_t* anc = ba + (int64_t)i * J;
-      float lmax = -__builtin_inff(), lmin = __builtin_inff();
-      for (int j = tid; j < J; j += B) lmax = nanmax(lmax, mt[j]);
-      block_maxmin(lmax, lmin, red);
-      double se = 0.0;
-      if (lmax == lmax && lmax != -__builtin_inff())
-        for (int j = tid; j < J; j += B) {
-          const float v = mt[j];
-          if (v == v) se += (double)expf(v - lmax);
-        }
-      se = block_sum(se, red);
-      const float inv_se = (float)(1.0 / se);
+      const int32_t* anct = ancl + (int64_t)t * J;
+      const float lmax = (float)stat[2 * t];
+      const float inv_se = (float)(1.0 / stat[2 * t + 1]);
       for (int j = tid; j < J; j += B) {
         const float mj = mt[j];
         const float pi = (lmax == -__builtin_inff() || mj != mj) ? 0.f : expf(mj - lmax) * inv_se;
@@ -247,9 +252,9 @@ __device__ void mop_grad_filter(const GradArgs& a, const int f, char* buf, doubl
         MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) x[s] = xt[s * (int64_t)J + j];
         M::dmeas_grad(md, P, y, x, gbar, gx, gth);
         MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) xb[s] = gbar * gx[s] + xca[s * (int64_t)J + j];
-        const int p = anc[j];
+        const int p = anct[j];
         MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) xf[s] = xin[s * (int64_t)J + p];
-        M::step_vjp(md, P, split_at(sk, (uint64_t)j), tstart(t), dt, xf, xb, xfb, gth);
+        M::step_vjp(md, P, xf, x, xb, xfb, gth);
         if (t == 0) {
           M::rinit_vjp(md, P, split_at(fkey, 1 + (uint64_t)j), a.ser.t0, xfb, gth);
         } else {
@@ -257,12 +262,19 @@ __device__ void mop_grad_filter(const GradArgs& a, const int f, char* buf, doubl
           cl[j] = a.alpha * (pi - invJ + lam);
         }
       }
-      __syncthreads();
+      (void)dt;
+      (void)ts;
       if (t > 0) {
-        // parents q collect their children's contributions (contiguous runs of anc)
+        // run starts of every parent's children (a_t is non-decreasing):
+        // lb[q] = first child c with a_t[c] >= q, q = 0..J
+        for (int c = tid; c <= J; c += B) {
+          const int prev = c == 0 ? -1 : anct[c - 1];
+          const int cur = c == J ? J : anct[c];
+          for (int q = prev + 1; q <= cur; ++q) lb[q] = c;
+        }
+        __syncthreads();
         for (int q = tid; q < J; q += B) {
-          const int lo = lower_bound_i32(anc, J, q);
-          const int hi = lower_bound_i32(anc, J, q + 1);
+          const int lo = lb[q], hi = lb[q + 1];
           float sx[M::MAXS];
           MP_UNROLL for (int s = 0; s < M::MAXS; ++s) sx[s] = 0.f;
           float sl = 0.f;
@@ -293,12 +305,18 @@ __device__ void mop_grad_filter(const GradArgs& a, const int f, char* buf, doubl
   __syncthreads();
 }
 
-template <class M>
-__global__ __launch_bounds__(256) void k_mop_grad(const GradArgs a) {
+template <class M, int NT>
+__global__ __launch_bounds__(NT) void k_mop_grad(const GradArgs a) {
   extern __shared__ __align__(256) char smem[];
   double* red = reinterpret_cast<double*>(smem);
   uint64_t* tab = reinterpret_cast<uint64_t*>(smem + kRedBytes);
-  for (int f = blockIdx.x; f < a.R; f += gridDim.x) mop_grad_filter<M>(a, f, a.ws + (int64_t)f * a.ws_stride, red, tab);
+  char* lwork = smem + kRedBytes + batched_tab_bytes(a.J, NT);
+  for (int f = blockIdx.x; f < a.R; f += gridDim.x) {
+    char* buf = a.ws + (int64_t)f * a.ws_stride;
+    char* wbuf = a.lds_work ? lwork : buf + grad_scratch_layout(M::FIXED_S ? M::MAXS : a.md.S, a.J, a.ser.Tmax, a.K,
+                                                               false).work;
+    mop_grad_filter<M>(a, f, buf, wbuf, red, tab);
+  }
 }
 
 template <class M, bool OK = M::DIFFERENTIABLE>
@@ -308,13 +326,23 @@ struct GradLauncher {
 template <class M>
 struct GradLauncher<M, true> {
   static hipError_t run(const GradArgs& a, int block, int grid, size_t lds, hipStream_t st) {
-    hipLaunchKernelGGL(k_mop_grad<M>, dim3(grid), dim3(block), lds, st, a);
+    switch (block) {
+      case 256: hipLaunchKernelGGL((k_mop_grad<M, 256>), dim3(grid), dim3(256), lds, st, a); break;
+      case 512: hipLaunchKernelGGL((k_mop_grad<M, 512>), dim3(grid), dim3(512), lds, st, a); break;
+      default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
   }
 };
 
+// LDS bytes of a launch: reductions, scan table and (when resident) the
+// per-step working arrays.
+size_t mop_grad_lds_bytes(int S, int J, int block, bool lds_work) {
+  return kRedBytes + batched_tab_bytes(J, block) + (lds_work ? (size_t)grad_work_layout(S, J).total : 0);
+}
+
 hipError_t launch_mop_grad(const GradArgs& a, int block, int grid, hipStream_t st) {
-  const size_t lds = kRedBytes + batched_tab_bytes(a.J, block);
+  const size_t lds = mop_grad_lds_bytes(a.md.S, a.J, block, a.lds_work);
   switch (a.md.kind) {
     case MODEL_SV: return GradLauncher<SV>::run(a, block, grid, lds, st);
     case MODEL_LG:
@@ -326,6 +354,8 @@ hipError_t launch_mop_grad(const GradArgs& a, int block, int grid, hipStream_t s
   return hipErrorInvalidValue;
 }
 
-int64_t mop_grad_scratch_bytes(int S, int J, int T, int K) { return grad_scratch_layout(S, J, T, K).total; }
+int64_t mop_grad_scratch_bytes(int S, int J, int T, int K, bool lds_work) {
+  return grad_scratch_layout(S, J, T, K, lds_work).total;
+}
 
 }  // namespace mp
