@@ -245,8 +245,38 @@ static uint64_t comb_threshold(float u, int64_t j, int64_t J, uint64_t total) {
   return (X << (32 - sh)) + (lo32 >> sh);
 }
 
-/* ancestors of J particles from log-weights lw (max m) and uniform u */
-static void resample(const float* lw, float m, int64_t J, float u, uint64_t* C, int32_t* anc) {
+/* log of a positive double: the deterministic sequence of oracle/detmath.py dd_log */
+static double dd_log(double x) {
+  if (!(x > 0.0)) return x == 0.0 ? -INFINITY : NAN;
+  if (x == INFINITY) return x;
+  union {
+    double d;
+    uint64_t u;
+  } c;
+  int e = 0;
+  if (x < 2.2250738585072014e-308) {
+    x = x * 4503599627370496.0;
+    e = -52;
+  }
+  c.d = x;
+  e += (int)((c.u >> 52) & 0x7ffu) - 1023;
+  c.u = (c.u & 0x000fffffffffffffull) | 0x3ff0000000000000ull;
+  double m = c.d;
+  if (m > 1.4142135623730951) {
+    m = m * 0.5;
+    e += 1;
+  }
+  const double f = m - 1.0, s = f / (2.0 + f), z = s * s;
+  static const double den[10] = {21.0, 19.0, 17.0, 15.0, 13.0, 11.0, 9.0, 7.0, 5.0, 3.0};
+  double p = 1.0 / 23.0;
+  for (int i = 0; i < 10; ++i) p = p * z + 1.0 / den[i];
+  p = p * z + 1.0;
+  const double lm = (2.0 * s) * p, de = (double)e;
+  return de * 6.93147180369123816490e-01 + (lm + de * 1.90821492927058770002e-10);
+}
+
+/* inclusive prefix sums C of the fixed-point weights of lw (max m); returns the total */
+static uint64_t fixed_prefix(const float* lw, float m, int64_t J, uint64_t* C) {
   const int sh = shift_for(J);
   uint64_t run = 0;
   for (int64_t i = 0; i < J; ++i) {
@@ -257,7 +287,18 @@ static void resample(const float* lw, float m, int64_t J, float u, uint64_t* C, 
     run += e;
     C[i] = run;
   }
-  const uint64_t total = run;
+  return run;
+}
+
+/* log-likelihood term m + log(total 2^-sh) (common.hpp step_loglik) */
+static double step_loglik(float m, uint64_t total, int64_t J) {
+  if (m == -INFINITY) return -INFINITY;
+  if (!(m < INFINITY)) return NAN;
+  return (double)m + dd_log(ldexp((double)total, -shift_for(J)));
+}
+
+/* ancestors of J particles from prefix sums C (total) and uniform u */
+static void resample(const uint64_t* C, uint64_t total, int64_t J, float u, int32_t* anc) {
   int64_t i = 0;
   for (int64_t j = 0; j < J; ++j) { /* merge: thresholds are non-decreasing in j */
     uint64_t t = comb_threshold(u, j, J, total);
@@ -369,12 +410,11 @@ static double lg_pfilter_one(const float* th, int dx, const float* ys, int T, in
       lw[j] = w;
       if (w > m) m = w;
     }
-    double se = 0.0;
-    for (int64_t j = 0; j < J; ++j) se += (double)expf(lw[j] - m);
-    ll += (double)m + log(se);
+    const uint64_t total = fixed_prefix(lw, m, J, C);
+    ll += step_loglik(m, total, J);
     key_t2 rk = split_at(run, 1);
     run = split_at(run, 0);
-    if (t < T - 1) resample(lw, m, J, uniform01(bits_at(rk, 0)), C, anc);
+    if (t < T - 1) resample(C, total, J, uniform01(bits_at(rk, 0)), anc);
     wconst = 0.0f - logJ;
     float* tmp = X;
     X = Xn;
@@ -417,7 +457,7 @@ static double sv_mop_one(const float* th_est, const float* ys, int T, int64_t J,
   const float omr2 = 1.0f - rho * rho, k = 1.0f / omr2, c0 = -0.918938533f - 0.5f * d_log(omr2);
   key_t2 run = split_at(key, 0);
   for (int64_t j = 0; j < J; ++j) V[j] = V0 + S0 * normal_at(split_at(key, 1 + (uint64_t)j), 0);
-  const double logJ = log((double)J);
+  const double logJ = (double)d_log((float)J);
   double ll = 0.0;
   (void)alpha;
   for (int t = 0; t < T; ++t) {
@@ -436,12 +476,11 @@ static double sv_mop_one(const float* th_est, const float* ys, int T, int64_t J,
       lw[j] = g;
       if (g > m) m = g;
     }
-    double se = 0.0;
-    for (int64_t j = 0; j < J; ++j) se += (double)expf(lw[j] - m);
-    ll += (double)m + log(se) - logJ;
+    const uint64_t total = fixed_prefix(lw, m, J, C);
+    ll += step_loglik(m, total, J) - logJ;
     key_t2 rk = split_at(run, 1);
     run = split_at(run, 0);
-    if (t < T - 1) resample(lw, m, J, uniform01(bits_at(rk, 0)), C, anc);
+    if (t < T - 1) resample(C, total, J, uniform01(bits_at(rk, 0)), anc);
     float* tmp = V;
     V = Vn;
     Vn = tmp;

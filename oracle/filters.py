@@ -18,8 +18,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import threefry as tf
-from .detmath import d_log, d_pow
-from .resample import resample_fixed_fast, resample_reference
+from .detmath import d_log, d_pow, dd_log
+from .resample import fixed_weights, resample_fixed_fast, resample_reference, shift_for
 
 F32 = np.float32
 
@@ -83,13 +83,35 @@ def rprocess_interp(model, X, theta, key, series, t, t_idx, nstep):
     return X, t_idx
 
 
-def normalize_weights(w):
-    """``_normalize_weights`` (core/algorithms/helpers.py:34-57), float32."""
+def step_loglik(m, total, J):
+    """The engine's log-likelihood term of one observation (float64):
+    m + log(total 2^-sh), total = exact integer sum of the resampler's
+    fixed-point weights (the device's step_loglik).  m = -inf gives -inf; +inf or
+    NaN give NaN."""
+    m = F32(m)
+    if m == -np.inf:
+        return -np.inf
+    if not (m < np.inf):
+        return np.nan
+    return float(np.float64(m) + dd_log(np.ldexp(np.float64(int(total)), -shift_for(J))))
+
+
+def normalize_weights(w, engine=True):
+    """``_normalize_weights`` (core/algorithms/helpers.py:34-57).  ``engine``:
+    the log-likelihood term is ``step_loglik`` of the fixed-point weights
+    (float64), the normalised weights ``w - fl32(term)``; otherwise the
+    reference's float32 arithmetic literally."""
     w = np.asarray(w, dtype=F32)
     J = w.shape[-1]
     mw = np.max(w)
     if mw == -np.inf:
-        return np.full(J, -d_log(F32(J)), dtype=F32), F32(-np.inf)
+        return np.full(J, -d_log(F32(J)), dtype=F32), (-np.inf if engine else F32(-np.inf))
+    if engine:
+        total = int(np.sum(fixed_weights(w), dtype=np.uint64))
+        ll = step_loglik(mw, total, J)
+        with np.errstate(invalid="ignore"):
+            nw = (w - F32(ll)).astype(F32)
+        return nw, ll
     with np.errstate(invalid="ignore", over="ignore"):
         ll = F32(mw + np.log(np.nansum(np.exp((w - mw).astype(F32)), dtype=F32)))
         nw = (w - ll).astype(F32)
@@ -106,16 +128,19 @@ def _resample(resampler, nw, w_unnorm, key):
 def pfilter(model, theta, series, key, J, thresh=0.0, resampler="fixed",
             diagnostics=False, should_trans=False, return_trace=False):
     """One bootstrap particle filter: ``_pfilter_internal`` + ``_pfilter_step``
-    (core/algorithms/pfilter.py:15-182).  Returns the log-likelihood (float32
-    sum over t, as the reference) and, with ``diagnostics``, CLL/ESS/filter and
-    prediction means.  ``return_trace`` adds per-step ancestors and weights."""
+    (core/algorithms/pfilter.py:15-182).  Returns ``logLik`` and, with
+    ``diagnostics``, CLL/ESS/filter and prediction means; ``return_trace`` adds
+    per-step ancestors and weights.  resampler "fixed" is the engine's
+    definition (fixed-point resampler, float64 sum over t of ``step_loglik``,
+    also as ``logLik64``); "reference" the reference's float32 arithmetic."""
     theta = np.asarray(theta, dtype=F32)
     th = model.from_est(theta) if should_trans else theta
     K = tf.split(key, J + 1)
     key, pkeys = K[0], K[1:]
     X = model.rinit(th, pkeys, series.t0)
     nw = np.full(J, d_log(F32(1.0) / F32(J)), dtype=F32)
-    loglik = F32(0.0)
+    engine = resampler != "reference"
+    loglik = 0.0 if engine else F32(0.0)
     t, t_idx = F32(series.t0), 0
     out = {"CLL": [], "ESS": [], "filter_mean": [], "prediction_mean": [],
            "ancestors": [], "weights": [], "states": []}
@@ -127,10 +152,10 @@ def pfilter(model, theta, series, key, J, thresh=0.0, resampler="fixed",
         t = series.times[i]
         meas = model.dmeas(series.ys[i], X, th, t).astype(F32)
         w = (nw + meas).astype(F32)
-        nw, ll_t = normalize_weights(w)
-        loglik = F32(loglik + ll_t)
+        nw, ll_t = normalize_weights(w, engine)
+        loglik = loglik + ll_t if engine else F32(loglik + ll_t)
         if diagnostics:
-            out["CLL"].append(ll_t)
+            out["CLL"].append(F32(ll_t))
             out["ESS"].append(F32(1.0) / np.sum(np.exp(F32(2.0) * nw), dtype=F32))
             out["filter_mean"].append((X * np.exp(nw)[:, None]).sum(0, dtype=F32))
             out["prediction_mean"].append(X.mean(0, dtype=F32))
@@ -149,20 +174,27 @@ def pfilter(model, theta, series, key, J, thresh=0.0, resampler="fixed",
             a = np.arange(J, dtype=np.int32)
         if return_trace:
             out["ancestors"].append(a)
-    out["logLik"] = loglik
+    out["logLik"] = F32(loglik)
+    if engine:
+        out["logLik64"] = loglik
     return out
 
 
 def mop(model, theta_est, series, key, J, alpha, resampler="fixed", return_trace=False):
     """``_mop_internal`` + ``_mop_step`` (core/algorithms/mop.py:19-135): theta on the
     estimation scale; returns the MOP log-likelihood (the reference returns its
-    negative).  Value-wise the weights stay constant across particles."""
+    negative).  Value-wise the weights stay constant across particles, so each
+    term LSE(wP + m) - LSE(wP) is max(m) + log sum exp(m - max) - log J: with the
+    "fixed" resampler the engine's float64 ``step_loglik`` minus fl32(log J),
+    summed in float64; with "reference" the reference's float32 LSEs."""
     th = model.from_est(np.asarray(theta_est, dtype=F32))
     K = tf.split(key, J + 1)
     key, pkeys = K[0], K[1:]
     X = model.rinit(th, pkeys, series.t0)
     wF = np.full(J, d_log(F32(1.0) / F32(J)), dtype=F32)
-    loglik = F32(0.0)
+    engine = resampler != "reference"
+    loglik = 0.0 if engine else F32(0.0)
+    logJ = float(d_log(F32(J)))
     t, t_idx = F32(series.t0), 0
     trace = {"ancestors": [], "meas": [], "states": []}
     for i in range(series.T):
@@ -172,8 +204,12 @@ def mop(model, theta_est, series, key, J, alpha, resampler="fixed", return_trace
         X, t_idx = rprocess_interp(model, X, th, sub, series, t, t_idx, series.nstep[i])
         t = series.times[i]
         meas = model.dmeas(series.ys[i], X, th, t).astype(F32)
-        loglik = F32(loglik + lse(wP + meas) - lse(wP))
-        nw, _ = normalize_weights(meas)
+        if engine:
+            mm = np.max(meas)
+            loglik = loglik + (step_loglik(mm, int(np.sum(fixed_weights(meas), dtype=np.uint64)), J) - logJ)
+        else:
+            loglik = F32(loglik + lse(wP + meas) - lse(wP))
+        nw, _ = normalize_weights(meas, engine)
         k2 = tf.split(key)
         key, sub = k2[0], k2[1]
         a = _resample(resampler, nw, meas, sub)
@@ -227,7 +263,8 @@ def perfilter(model, m, thetas, series, key, J, rw, thresh=0.0, resampler="fixed
     d = thetas.shape[1]
     T = series.T
     nw = np.full(J, -d_log(F32(J)), dtype=F32)
-    loglik = F32(0.0)
+    engine = resampler != "reference"
+    loglik = 0.0 if engine else F32(0.0)
     cf = [rw.cooling_factor(i, m, T) for i in range(T)]
     k2 = tf.split(key)
     key, sub = k2[0], k2[1]
@@ -252,8 +289,8 @@ def perfilter(model, m, thetas, series, key, J, rw, thresh=0.0, resampler="fixed
         meas = model.dmeas(series.ys[i], X, th_nat, t).astype(F32)
         meas = np.nan_to_num(meas, nan=nan_sub).astype(F32)
         w = (nw + meas).astype(F32)
-        nw, ll_t = normalize_weights(w)
-        loglik = F32(loglik + ll_t)
+        nw, ll_t = normalize_weights(w, engine)
+        loglik = loglik + ll_t if engine else F32(loglik + ll_t)
         with np.errstate(invalid="ignore"):
             do_res = (np.max(nw) - np.min(nw)) > logthresh
         if do_res:

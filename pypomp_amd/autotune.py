@@ -8,11 +8,9 @@ stream, keeps the fastest, applies it and caches it per (workload key, device),
 in memory and optionally in a JSON file (``PYPOMP_AMD_TUNE_CACHE``).  The
 reference has no counterpart: XLA chooses its own launch shapes.
 
-Tuning does not change the random streams, the resampling indices or the
-particle states: the resampler's fixed-point prefix sums are order-independent
-by construction.  Only the association of float64 reductions (the per-step
-exp-sum of the log-likelihood) follows the launch shape, a difference of a few
-float64 ulps.
+Tuning never changes results: every setting computes the same bits.  The
+resampler's fixed-point prefix sums and the log-likelihood's exp-sum (their
+exact integer total) do not depend on the evaluation order.
 """
 
 import itertools

@@ -235,21 +235,12 @@ __device__ void run_filter(const BatchArgs& a, const int f, char* buf, double* r
     }
     block_maxmin(lmax, lmin, red);
     const bool collapsed = (lmax == -__builtin_inff());
-    double se = 0.0;
-    if (!collapsed && lmax == lmax) {
-      for (int j = tid; j < J; j += B) {
-        float v = lw[j];
-        if (v == v) se += (double)expf(v - lmax);
-      }
-    }
-    se = block_sum(se, red);
-    double llt;
-    if (ALGO == ALGO_MOP) {
-      // LSE(wP + m) - LSE(wP) with wP common: max(m) + log sum exp(m - max) - log J
-      llt = collapsed ? -__builtin_inf() : ((double)lmax + log(se)) - (double)logJ;
-    } else {
-      llt = collapsed ? -__builtin_inf() : (double)lmax + log(se);
-    }
+    // fixed-point weights and their prefix sums: the resampler's input and the
+    // exact total of the log-likelihood term
+    total = block_fixed_scan(lw, Cc, J, lmax, tab);
+    double llt = step_loglik(lmax, total, shift_for(J));
+    // MOP: LSE(wP + m) - LSE(wP) with wP common = max(m) + log sum exp(m - max) - log J
+    if (ALGO == ALGO_MOP) llt = llt - (double)logJ;
     loglik += llt;
     const float llf = (float)llt;
     if (tid == 0 && (a.flags & F_CLL)) a.cll[(int64_t)f * a.ser.Tmax + t] = llf;
@@ -290,8 +281,7 @@ __device__ void run_filter(const BatchArgs& a, const int f, char* buf, double* r
     if (resample && !(last && ALGO != ALGO_PER)) {
       // resampling weights: PF/PER use the normalised weights, MOP the
       // measurement densities alone (mop.py:120); both equal lw up to a
-      // common shift, which the fixed-point weights factor out.
-      total = block_fixed_scan(lw, Cc, J, lmax, tab);
+      // common shift, which the fixed-point weights (C, total) factor out.
       const Key rk = ld_key(sch, base + nst);
       ures = uniform01(bits_at(rk, 0));
       have_anc = true;

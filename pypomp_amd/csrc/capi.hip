@@ -21,7 +21,7 @@ hipError_t launch_sched(int algo, const uint32_t* keys, const SeriesSet& ser, co
 hipError_t plain_kernel(int which, const PlainArgs& a, int ntiles, hipStream_t s);
 hipError_t plain_prepare(const ModelDesc& md, const float* theta, int64_t stride, int R, int flags, char* out,
                          int64_t pstride, hipStream_t s);
-hipError_t plain_reset(uint64_t* lb, uint64_t* psum, int ntiles, PlainState* st, hipStream_t s);
+hipError_t plain_reset(uint64_t* lb, int ntiles, PlainState* st, hipStream_t s);
 size_t batched_small_lds(int J, int block);
 hipError_t launch_order(const SeriesSet& ser, const int32_t* unit, int R, int32_t* order, int32_t* queue,
                         hipStream_t st);
@@ -208,7 +208,7 @@ BatchedPlan plan_batched(int algo, const mp_model_desc* m, const mp_series_desc*
 
 struct PlainPlan {
   int ntiles, rows, nprop, rows_prop;
-  int64_t off_sched, off_params, off_x0, off_x1, off_lw, off_c, off_pmax, off_pmin, off_psum, off_lb, off_st, total;
+  int64_t off_sched, off_params, off_x0, off_x1, off_lw, off_c, off_pmax, off_pmin, off_lb, off_st, total;
 };
 
 PlainPlan plan_plain(const mp_model_desc* m, const mp_series_desc* s, int R, int J) {
@@ -233,7 +233,6 @@ PlainPlan plan_plain(const mp_model_desc* m, const mp_series_desc* s, int R, int
   p.off_c = take((int64_t)J * 8);
   p.off_pmax = take((int64_t)p.nprop * 4);
   p.off_pmin = take((int64_t)p.nprop * 4);
-  p.off_psum = take((int64_t)p.ntiles * 8);
   p.off_lb = take((int64_t)p.ntiles * 16);
   p.off_st = take(sizeof(PlainState));
   p.total = o + 256;
@@ -287,7 +286,6 @@ int run_plain_pf(const mp_model_desc* m, const mp_series_desc* s, const float* t
   a.C = reinterpret_cast<uint64_t*>(b + p.off_c);
   a.pmax = reinterpret_cast<float*>(b + p.off_pmax);
   a.pmin = reinterpret_cast<float*>(b + p.off_pmin);
-  a.psum = reinterpret_cast<uint64_t*>(b + p.off_psum);
   a.lb = reinterpret_cast<uint64_t*>(b + p.off_lb);
   a.ntiles = p.ntiles;
   a.rows_prop = p.rows_prop;
@@ -307,7 +305,7 @@ int run_plain_pf(const mp_model_desc* m, const mp_series_desc* s, const float* t
     a.cll = cll ? cll + (int64_t)f * s->Tmax : nullptr;
     a.sched = sched_all + 2 * (int64_t)f * slen;
     a.params = params + (int64_t)f * kParamSlot;
-    MP_HIP(plain_reset(a.lb, a.psum, p.ntiles, a.st, st));
+    MP_HIP(plain_reset(a.lb, p.ntiles, a.st, st));
     for (int t = 0; t < a.T; ++t) {
       a.t = t;
       MP_HIP(timed(0, st, [&] { return plain_kernel(0, a, p.ntiles, st); }));

@@ -189,6 +189,22 @@ MP_HD uint64_t particle_fixed_weight(float lw, float m, int sh) {
   return fixed_weight((double)lw - (double)m, sh);
 }
 
+// Log-likelihood term of one observation, m + log(sum_j exp(w_j - m)), with the
+// sum taken as the exact integer total of the fixed-point weights times 2^-sh
+// (the weights the resampler uses).  Integer totals do not depend on the
+// summation order, and dd_log is a fixed IEEE sequence, so every launch shape,
+// both execution forms and the oracle get the same bits.  The truncation of
+// each weight to 2^-sh of the largest costs < J 2^-sh relative (J = 10^6:
+// 5e-7), the size of the reference's float32 sum error.  m == -inf: -inf
+// (every weight zero); m == +inf or NaN: NaN, as exp(inf - inf) makes it in the
+// reference (_normalize_weights, core/algorithms/helpers.py:34-57).
+MP_HD double step_loglik(float m, uint64_t total, int sh) {
+#pragma clang fp contract(off)
+  if (m == -__builtin_inff()) return -__builtin_inf();
+  if (!(m < __builtin_inff())) return __builtin_nan("");
+  return (double)m + dd_log(ldexp((double)total, -sh));
+}
+
 // T_j = floor(unif_j * total) with unif_j = fl32(fl32(u + j) / J), exactly.
 MP_HD uint64_t comb_threshold(float u, int64_t j, int64_t J, uint64_t total) {
   float f = (u + (float)j) / (float)J;

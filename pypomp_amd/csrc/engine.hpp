@@ -124,7 +124,6 @@ struct PlainArgs {
   float* pmax;          // (ntiles) per-tile max / min log-weight
   float* pmin;
   uint64_t* lb;         // (2, ntiles) look-back words: state << 62 | prefix
-  uint64_t* psum;       // (ntiles) per-tile exp sums (float64 bits), ~0 when empty
   PlainState* st;
   double* loglik_out;   // (1) written after the last step
   float* cll;           // (T) or null

@@ -121,6 +121,7 @@ __device__ void mop_grad_filter(const GradArgs& a, const int f, char* buf, char*
   const uint32_t* sch = a.sched + 2 * (int64_t)f * a.sched_len;
   const Key fkey{a.keys[2 * f], a.keys[2 * f + 1]};
   const float fJ = (float)J, logJ = d_log(fJ), invJ = 1.0f / fJ;
+  const int sh = shift_for(J);
 
   float th_est[M::MAXP], th[M::MAXP];
   MP_UNROLL for (int p = 0; p < M::MAXP; ++p) if (p < d) {
@@ -176,24 +177,13 @@ __device__ void mop_grad_filter(const GradArgs& a, const int f, char* buf, char*
       lmin = nanmin(lmin, m);
     }
     block_maxmin(lmax, lmin, red);
-    double se = 0.0;
-    if (lmax == lmax && lmax != -__builtin_inff())
-      for (int j = tid; j < J; j += B) {
-        const float v = lw[j];
-        if (v == v) se += (double)expf(v - lmax);
-      }
-    se = block_sum(se, red);
+    total = block_fixed_scan(lw, Cc, J, lmax, tab);
     if (tid == 0) {
       stat[2 * t] = (double)lmax;
-      stat[2 * t + 1] = se;
+      stat[2 * t + 1] = ldexp((double)total, -sh);  // sum of exp(m - lmax)
     }
-    loglik += (lmax == -__builtin_inff()) ? -__builtin_inf() : ((double)lmax + log(se) - (double)logJ);
-    if (t < T - 1) {
-      total = block_fixed_scan(lw, Cc, J, lmax, tab);
-      ures = uniform01(bits_at(res_key(t), 0));
-    } else {
-      __syncthreads();
-    }
+    loglik += step_loglik(lmax, total, sh) - (double)logJ;
+    if (t < T - 1) ures = uniform01(bits_at(res_key(t), 0));
     float* tmp = XA;
     XA = XB;
     XB = tmp;

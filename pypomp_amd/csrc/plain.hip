@@ -6,17 +6,17 @@
 //                   step's prefix sums, staged per tile in LDS) fused with the
 //                   gather, the Euler sub-steps, the measurement density and the
 //                   log-weight; per-tile max/min.
-//  k_plain_scan  -- fixed-point weights, per-tile exp-sums and a one-pass
-//                   inclusive prefix sum across tiles with decoupled look-back;
-//                   the last tile finishes the log-likelihood term (FP64 sum
-//                   over t), the resampling decision and the comb's uniform.
+//  k_plain_scan  -- fixed-point weights and a one-pass inclusive prefix sum
+//                   across tiles with decoupled look-back; the last tile
+//                   finishes the step from the exact total: the log-likelihood
+//                   term (FP64 sum over t), the resampling decision and the
+//                   comb's uniform.
 //
 // Cross-workgroup traffic uses only relaxed agent-scope atomics on single
-// 64-bit words (look-back state and prefix packed together, exp sums as
-// float64 bits with an "empty" sentinel): on a multi-XCD GPU an agent-scope
-// release/acquire fence writes back / invalidates the whole L2, which is what
-// this design avoids.  Everything else a kernel reads was written by an
-// earlier launch.
+// 64-bit words (look-back state and prefix packed together): on a multi-XCD
+// GPU an agent-scope release/acquire fence writes back / invalidates the whole
+// L2, which is what this design avoids.  Everything else a kernel reads was
+// written by an earlier launch.
 #include "blockops.hpp"
 #include "engine.hpp"
 #include "seir.hpp"
@@ -26,7 +26,6 @@ namespace mp {
 constexpr uint64_t kStateShift = 62;
 constexpr uint64_t kAggState = 1ull << kStateShift, kInclState = 2ull << kStateShift;
 constexpr uint64_t kValueMask = (1ull << kStateShift) - 1;
-constexpr uint64_t kEmpty = ~0ull;
 constexpr int kWin = 1024;  // LDS search window (uint64 entries)
 
 __device__ __forceinline__ uint64_t ld_relaxed(uint64_t* p) {
@@ -149,18 +148,12 @@ __global__ __launch_bounds__(256) void k_plain_scan(const PlainArgs a) {
   const int tile = s_tile;
   const int64_t tile0 = (int64_t)tile * a.rows * B;
 
-  // per-thread fixed-point weights and exp sums; wave scans per row
-  double se = 0.0;
-  const bool live = m == m && m != -__builtin_inff();
+  // per-thread fixed-point weights; wave scans per row
   uint64_t rowincl[8];
   for (int r = 0; r < a.rows; ++r) {
     const int64_t j = tile0 + (int64_t)r * B + tid;
     uint64_t e = 0;
-    if (j < J) {
-      const float v = a.lw[j];
-      e = particle_fixed_weight(v, m, sh);
-      if (live && v == v) se += (double)expf(v - m);
-    }
+    if (j < J) e = particle_fixed_weight(a.lw[j], m, sh);
     uint64_t inc = wave_incl_scan_u64(e);
     rowincl[r] = inc;
     if (lane == 63) tab[r * nw + w] = inc;
@@ -214,27 +207,15 @@ __global__ __launch_bounds__(256) void k_plain_scan(const PlainArgs a) {
     const int64_t j = tile0 + (int64_t)r * B + tid;
     if (j < J) a.C[j] = prefix + tab[r * nw + w] + rowincl[r];
   }
-  se = block_sum(se, red);
   if (tid == 0) {
-    st_relaxed(&a.psum[tile], __double_as_longlong(se));
     const unsigned k = ticket_relaxed(&a.st->ticket[par]);
     is_last = (k == (unsigned)ntiles - 1u);
   }
   __syncthreads();
   if (!is_last) return;
-  // Last tile: every other tile has published its sum and its inclusive prefix
-  // (both before taking a ticket); wait for them to arrive, then finish the
-  // step with fixed-order (deterministic) sums.
-  double s = 0.0;
-  for (int i = tid; i < ntiles; i += B) {
-    uint64_t v = ld_relaxed(&a.psum[i]);
-    while (v == kEmpty) {
-      __builtin_amdgcn_s_sleep(1);
-      v = ld_relaxed(&a.psum[i]);
-    }
-    s += __longlong_as_double((long long)v);
-  }
-  s = block_sum(s, red);
+  // Last tile: every other tile has published its look-back word before taking
+  // a ticket; the last tile's inclusive prefix is the exact total of the
+  // fixed-point weights, which finishes the step.
   if (tid == 0) {
     uint64_t lastw = ld_relaxed(&lb[ntiles - 1]);
     while ((lastw >> kStateShift) != 2) {
@@ -244,7 +225,7 @@ __global__ __launch_bounds__(256) void k_plain_scan(const PlainArgs a) {
     PlainState* st = a.st;
     const float fJ = (float)J, logJ = d_log(fJ);
     const bool collapsed = (m == -__builtin_inff());
-    const double llt = collapsed ? -__builtin_inf() : (double)m + log(s);
+    const double llt = step_loglik(m, lastw & kValueMask, sh);
     const float llf = (float)llt;
     st->loglik = (a.t == 0 ? 0.0 : st->loglik) + llt;
     if (a.cll) a.cll[a.t] = llf;
@@ -266,12 +247,9 @@ __global__ __launch_bounds__(256) void k_plain_scan(const PlainArgs a) {
     st->ticket[par ^ 1] = 0;
     if (a.t == a.T - 1) *a.loglik_out = st->loglik;
   }
-  // clear the next step's look-back words and this step's sums
+  // clear the next step's look-back words
   uint64_t* lbn = a.lb + (int64_t)(par ^ 1) * ntiles;
-  for (int i = tid; i < ntiles; i += B) {
-    lbn[i] = 0;
-    a.psum[i] = kEmpty;
-  }
+  for (int i = tid; i < ntiles; i += B) lbn[i] = 0;
 }
 
 // Prepared model parameters of each filter (one thread per filter).
@@ -310,13 +288,12 @@ hipError_t plain_prepare(const ModelDesc& md, const float* theta, int64_t stride
   return hipErrorInvalidValue;
 }
 
-// Start of a filter: clear the look-back words, the sums and the step state.
-__global__ void k_plain_reset(uint64_t* lb, uint64_t* psum, int ntiles, PlainState* st) {
+// Start of a filter: clear the look-back words and the step state.
+__global__ void k_plain_reset(uint64_t* lb, int ntiles, PlainState* st) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < ntiles) {
     lb[i] = 0;
     lb[ntiles + i] = 0;
-    psum[i] = kEmpty;
   }
   if (i == 0) {
     PlainState z{};
@@ -324,8 +301,8 @@ __global__ void k_plain_reset(uint64_t* lb, uint64_t* psum, int ntiles, PlainSta
   }
 }
 
-hipError_t plain_reset(uint64_t* lb, uint64_t* psum, int ntiles, PlainState* st, hipStream_t s) {
-  hipLaunchKernelGGL(k_plain_reset, dim3((ntiles + 255) / 256), dim3(256), 0, s, lb, psum, ntiles, st);
+hipError_t plain_reset(uint64_t* lb, int ntiles, PlainState* st, hipStream_t s) {
+  hipLaunchKernelGGL(k_plain_reset, dim3((ntiles + 255) / 256), dim3(256), 0, s, lb, ntiles, st);
   return hipGetLastError();
 }
 
