@@ -296,7 +296,7 @@ __device__ void mop_grad_filter(const GradArgs& a, const int f, char* buf, char*
 }
 
 template <class M, int NT>
-__global__ __launch_bounds__(NT) void k_mop_grad(const GradArgs a) {
+__device__ __forceinline__ void mop_grad_body(const GradArgs& a) {
   extern __shared__ __align__(256) char smem[];
   double* red = reinterpret_cast<double*>(smem);
   uint64_t* tab = reinterpret_cast<uint64_t*>(smem + kRedBytes);
@@ -309,6 +309,19 @@ __global__ __launch_bounds__(NT) void k_mop_grad(const GradArgs a) {
   }
 }
 
+// 256 threads: the occupancy the registers allow (3 waves per SIMD).
+template <class M>
+__global__ __launch_bounds__(256) void k_mop_grad(const GradArgs a) {
+  mop_grad_body<M, 256>(a);
+}
+
+// 512 threads capped at 128 VGPRs, so two workgroups (16 waves) share a CU:
+// two filters per CU at 4 waves per SIMD.
+template <class M>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_mop_grad_wide(const GradArgs a) {
+  mop_grad_body<M, 512>(a);
+}
+
 template <class M, bool OK = M::DIFFERENTIABLE>
 struct GradLauncher {
   static hipError_t run(const GradArgs&, int, int, size_t, hipStream_t) { return hipErrorNotSupported; }
@@ -317,8 +330,8 @@ template <class M>
 struct GradLauncher<M, true> {
   static hipError_t run(const GradArgs& a, int block, int grid, size_t lds, hipStream_t st) {
     switch (block) {
-      case 256: hipLaunchKernelGGL((k_mop_grad<M, 256>), dim3(grid), dim3(256), lds, st, a); break;
-      case 512: hipLaunchKernelGGL((k_mop_grad<M, 512>), dim3(grid), dim3(512), lds, st, a); break;
+      case 256: hipLaunchKernelGGL((k_mop_grad<M>), dim3(grid), dim3(256), lds, st, a); break;
+      case 512: hipLaunchKernelGGL((k_mop_grad_wide<M>), dim3(grid), dim3(512), lds, st, a); break;
       default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

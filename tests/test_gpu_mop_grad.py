@@ -78,3 +78,23 @@ def test_autograd_function_backward():
     (-ll.sum() / 50).backward()
     _, g = F.mop_grad(struct, np.stack([th, th]), 128, 0.97, keys, K=10)
     np.testing.assert_allclose(t.grad.numpy(), -g / 50, rtol=1e-6)
+
+
+def test_workgroup_sizes_agree():
+    # the value is bit-exact across workgroup sizes; the float64 gradient sums
+    # follow the thread layout, so they agree to rounding
+    import pypomp_amd.functional as F
+    from pypomp_amd import autotune
+    struct, _ = sv_struct(70)
+    th = osv.SVModel().to_est(osv.default_theta())
+    keys = tf.split(tf.key(8), 4)
+    old = autotune.get("grad_block")
+    try:
+        autotune.set("grad_block", 256)
+        a = F.mop_grad(struct, np.stack([th] * 4), 700, 0.97, keys, K=16)
+        autotune.set("grad_block", 512)
+        b = F.mop_grad(struct, np.stack([th] * 4), 700, 0.97, keys, K=16)
+    finally:
+        autotune.set("grad_block", old)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_allclose(a[1], b[1], rtol=1e-5, atol=1e-6 * np.abs(a[1]).max())
