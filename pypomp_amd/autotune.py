@@ -21,7 +21,7 @@ import threading
 from . import _native
 
 PARAMS = {
-    "plain_scan_rows": (_native.MP_TUNE_PLAIN_SCAN_ROWS, (2, 4, 8)),
+    "plain_scan_rows": (_native.MP_TUNE_PLAIN_SCAN_ROWS, (4, 8, 16)),
     "plain_prop_rows": (_native.MP_TUNE_PLAIN_PROP_ROWS, (1, 2, 4)),
     "batched_block": (_native.MP_TUNE_BATCHED_BLOCK, (256, 512, 768)),
     "grad_block": (_native.MP_TUNE_GRAD_BLOCK, (256, 512)),

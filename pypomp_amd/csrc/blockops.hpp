@@ -48,6 +48,29 @@ __device__ __forceinline__ uint64_t wave_incl_scan_u64(uint64_t v) {
   return v;
 }
 
+// First index in [lo, hi) whose entry exceeds t (C non-decreasing), found by one
+// whole wave: each round probes 64 evenly spaced entries and keeps the gap in
+// which C crosses t, so J = 10^6 needs 4 dependent loads instead of 20.  Every
+// lane must call with the same arguments; all get the result.
+__device__ inline int64_t wave_upper_bound_u64(const uint64_t* C, int64_t lo, int64_t hi, uint64_t t) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  while (hi - lo > WAVE) {
+    const int64_t n = hi - lo;
+    const int64_t step = (n + WAVE - 1) / WAVE;
+    const int64_t off = (int64_t)(lane + 1) * step;
+    const int64_t q = lo + (off < n ? off : n) - 1;
+    const int c = __popcll(__ballot(C[q] <= t));  // probes are monotone: a prefix of lanes
+    const int64_t cs = (int64_t)c * step;
+    if (cs >= n) return hi;
+    const int64_t cs1 = cs + step;
+    hi = lo + (cs1 < n ? cs1 : n);
+    lo = lo + cs;
+  }
+  const int64_t p = lo + lane;
+  const bool le = p < hi && C[p] <= t;
+  return lo + __popcll(__ballot(le));
+}
+
 // Workgroup reductions.  `red` is LDS scratch of at least 2*nwaves doubles.
 // All threads of the block must call; result is returned to every thread.
 __device__ __forceinline__ void block_maxmin(float& mx, float& mn, double* red) {

@@ -67,7 +67,7 @@ constexpr int kBatchedBlock = 256;
 constexpr int64_t kParamSlot = 512;  // bytes per filter of prepared model parameters
 
 // Launch-shape parameters (set by the autotuner through mp_tune_set).
-int g_plain_scan_rows = 8;  // particle rows of 256 per scan tile (1..8)
+int g_plain_scan_rows = 16; // particle rows of 256 per scan tile (1..16)
 int g_plain_prop_rows = 2;  // particle rows of 256 per propagate block (1..16)
 int g_batched_block = 0;    // batched workgroup size: 0 = automatic, 256, 512, 768
 int g_grad_block = 0;       // MOP-adjoint workgroup size: 0 = automatic, 256, 512
@@ -585,7 +585,7 @@ int32_t mp_perfilter(const mp_model_desc* h_model, const mp_series_desc* h_serie
 int32_t mp_tune_set(int32_t param, int32_t value) {
   switch (param) {
     case MP_TUNE_PLAIN_SCAN_ROWS:
-      if (value < 1 || value > 8) return fail(MP_ERR_ARG, "scan rows must be 1..8");
+      if (value < 1 || value > 16) return fail(MP_ERR_ARG, "scan rows must be 1..16");
       g_plain_scan_rows = value;
       return MP_OK;
     case MP_TUNE_PLAIN_PROP_ROWS:

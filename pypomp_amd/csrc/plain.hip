@@ -27,6 +27,7 @@ constexpr uint64_t kStateShift = 62;
 constexpr uint64_t kAggState = 1ull << kStateShift, kInclState = 2ull << kStateShift;
 constexpr uint64_t kValueMask = (1ull << kStateShift) - 1;
 constexpr int kWin = 1024;  // LDS search window (uint64 entries)
+constexpr int kMaxScanRows = 16;  // rows of 256 particles per scan tile, at most
 
 __device__ __forceinline__ uint64_t ld_relaxed(uint64_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -73,9 +74,12 @@ __global__ __launch_bounds__(256) void k_plain_prop(const PlainArgs a) {
   // non-decreasing in the comb position, so every j of the tile has
   // A(t_j) in [A(t_tile0), A(t_tile1-1)]; that slice of C is staged in LDS.
   if (have_anc) {
-    if (tid < 2) {
-      const int64_t jj = tid == 0 ? tile0 : tile1 - 1;
-      s_win[tid] = upper_bound_u64(a.C, 0, J, comb_threshold(st.ures, jj, J, st.total));
+    // waves 0 and 1 find the two ends of the window concurrently
+    const int wv = tid >> 6;
+    if (wv < 2) {
+      const int64_t jj = wv == 0 ? tile0 : tile1 - 1;
+      const int64_t v = wave_upper_bound_u64(a.C, 0, J, comb_threshold(st.ures, jj, J, st.total));
+      if ((tid & 63) == 0) s_win[wv] = v;
     }
     __syncthreads();
   }
@@ -127,7 +131,7 @@ __global__ __launch_bounds__(256) void k_plain_prop(const PlainArgs a) {
 template <class M>
 __global__ __launch_bounds__(256) void k_plain_scan(const PlainArgs a) {
   __shared__ double red[32];
-  __shared__ uint64_t tab[8 * 4 + 1];
+  __shared__ uint64_t tab[kMaxScanRows * 4 + 1];
   __shared__ uint64_t s_prefix;
   __shared__ int s_tile, is_last;
   const int J = a.J, tid = threadIdx.x, B = blockDim.x, lane = tid & 63, w = tid >> 6;
@@ -138,25 +142,46 @@ __global__ __launch_bounds__(256) void k_plain_scan(const PlainArgs a) {
   const int sh = shift_for(J);
 
   if (tid == 0) s_tile = (int)ticket_relaxed(&a.st->tile_ctr[par]);
-  // global max / min of the log-weights from the per-block values
+  // global max / min of the log-weights from the per-block values, eight
+  // independent loads in flight per thread
   float m = -__builtin_inff(), mn = __builtin_inff();
-  for (int i = tid; i < a.nprop; i += B) {
-    m = nanmax(m, a.pmax[i]);
-    mn = nanmin(mn, a.pmin[i]);
+  for (int i0 = 0; i0 < a.nprop; i0 += 8 * B) {
+    float vx[8], vn[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int i = i0 + k * B + tid;
+      vx[k] = i < a.nprop ? a.pmax[i] : -__builtin_inff();
+      vn[k] = i < a.nprop ? a.pmin[i] : __builtin_inff();
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      m = nanmax(m, vx[k]);
+      mn = nanmin(mn, vn[k]);
+    }
   }
   block_maxmin(m, mn, red);  // includes barriers: s_tile is visible below
   const int tile = s_tile;
   const int64_t tile0 = (int64_t)tile * a.rows * B;
 
-  // per-thread fixed-point weights; wave scans per row
-  uint64_t rowincl[8];
-  for (int r = 0; r < a.rows; ++r) {
+  // per-thread fixed-point weights (all of the tile's loads issued first);
+  // wave scans per row
+  const int rows = a.rows;
+  float lwv[kMaxScanRows];
+#pragma unroll
+  for (int r = 0; r < kMaxScanRows; ++r) {
     const int64_t j = tile0 + (int64_t)r * B + tid;
-    uint64_t e = 0;
-    if (j < J) e = particle_fixed_weight(a.lw[j], m, sh);
-    uint64_t inc = wave_incl_scan_u64(e);
-    rowincl[r] = inc;
-    if (lane == 63) tab[r * nw + w] = inc;
+    lwv[r] = (r < rows && j < J) ? a.lw[j] : -__builtin_inff();
+  }
+  uint64_t rowincl[kMaxScanRows];
+#pragma unroll
+  for (int r = 0; r < kMaxScanRows; ++r) {
+    if (r < rows) {
+      const int64_t j = tile0 + (int64_t)r * B + tid;
+      const uint64_t e = j < J ? particle_fixed_weight(lwv[r], m, sh) : 0;
+      const uint64_t inc = wave_incl_scan_u64(e);
+      rowincl[r] = inc;
+      if (lane == 63) tab[r * nw + w] = inc;
+    }
   }
   __syncthreads();
   const int n = a.rows * nw;
@@ -203,9 +228,10 @@ __global__ __launch_bounds__(256) void k_plain_scan(const PlainArgs a) {
   }
   __syncthreads();
   const uint64_t prefix = s_prefix;
-  for (int r = 0; r < a.rows; ++r) {
+#pragma unroll
+  for (int r = 0; r < kMaxScanRows; ++r) {
     const int64_t j = tile0 + (int64_t)r * B + tid;
-    if (j < J) a.C[j] = prefix + tab[r * nw + w] + rowincl[r];
+    if (r < rows && j < J) a.C[j] = prefix + tab[r * nw + w] + rowincl[r];
   }
   if (tid == 0) {
     const unsigned k = ticket_relaxed(&a.st->ticket[par]);
