@@ -18,7 +18,7 @@ hipError_t launch_resample_rows(const float* logw, const uint32_t* keys, int64_t
 hipError_t launch_batched(int algo, const BatchArgs& a, int block, int grid, size_t lds, hipStream_t st);
 hipError_t launch_sched(int algo, const uint32_t* keys, const SeriesSet& ser, const int32_t* unit,
                         int64_t sched_len, uint32_t* sched, int R, hipStream_t st);
-hipError_t plain_kernel(int which, const PlainArgs& a, int ntiles, hipStream_t s);
+hipError_t plain_kernel(int which, const PlainArgs& a, int ntiles, int threads, hipStream_t s);
 hipError_t plain_prepare(const ModelDesc& md, const float* theta, int64_t stride, int R, int flags, char* out,
                          int64_t pstride, hipStream_t s);
 hipError_t plain_reset(uint64_t* lb, int ntiles, PlainState* st, hipStream_t s);
@@ -67,8 +67,11 @@ constexpr int kBatchedBlock = 256;
 constexpr int64_t kParamSlot = 512;  // bytes per filter of prepared model parameters
 
 // Launch-shape parameters (set by the autotuner through mp_tune_set).
-int g_plain_scan_rows = 16; // particle rows of 256 per scan tile (1..16)
-int g_plain_prop_rows = 2;  // particle rows of 256 per propagate block (1..16)
+// Defaults from a sweep on LG 2-D, J = 1e6 (2.22e10 particle-steps/s).
+int g_plain_scan_rows = 8;  // particle rows per scan tile (1..16)
+int g_plain_prop_rows = 2;  // particle rows per propagate block (1..16)
+int g_plain_scan_threads = 512;  // scan workgroup size (256, 512); rows * waves <= 64
+int g_plain_prop_threads = 512;  // propagate workgroup size (256, 512)
 int g_batched_block = 0;    // batched workgroup size: 0 = automatic, 256, 512, 768
 int g_grad_block = 0;       // MOP-adjoint workgroup size: 0 = automatic, 256, 512
 
@@ -207,17 +210,21 @@ BatchedPlan plan_batched(int algo, const mp_model_desc* m, const mp_series_desc*
 }
 
 struct PlainPlan {
-  int ntiles, rows, nprop, rows_prop;
+  int ntiles, rows, nprop, rows_prop, scan_threads, prop_threads;
   int64_t off_sched, off_params, off_x0, off_x1, off_lw, off_c, off_pmax, off_pmin, off_lb, off_st, total;
 };
 
 PlainPlan plan_plain(const mp_model_desc* m, const mp_series_desc* s, int R, int J) {
   PlainPlan p{};
+  p.scan_threads = g_plain_scan_threads;
+  p.prop_threads = g_plain_prop_threads;
+  // a scan tile's row/wave totals are scanned by one wave: rows * waves <= 64
   p.rows = g_plain_scan_rows;
-  const int tile = p.rows * 256;
+  while (p.rows * (p.scan_threads / 64) > 64) p.rows /= 2;
+  const int tile = p.rows * p.scan_threads;
   p.ntiles = (J + tile - 1) / tile;
   p.rows_prop = g_plain_prop_rows;
-  p.nprop = (J + p.rows_prop * 256 - 1) / (p.rows_prop * 256);
+  p.nprop = (J + p.rows_prop * p.prop_threads - 1) / (p.rows_prop * p.prop_threads);
   const SeriesSet& ser = *reinterpret_cast<const SeriesSet*>(s);
   int64_t o = 0;
   auto take = [&](int64_t bytes) {
@@ -308,8 +315,8 @@ int run_plain_pf(const mp_model_desc* m, const mp_series_desc* s, const float* t
     MP_HIP(plain_reset(a.lb, p.ntiles, a.st, st));
     for (int t = 0; t < a.T; ++t) {
       a.t = t;
-      MP_HIP(timed(0, st, [&] { return plain_kernel(0, a, p.ntiles, st); }));
-      MP_HIP(timed(1, st, [&] { return plain_kernel(1, a, p.ntiles, st); }));
+      MP_HIP(timed(0, st, [&] { return plain_kernel(0, a, p.ntiles, p.prop_threads, st); }));
+      MP_HIP(timed(1, st, [&] { return plain_kernel(1, a, p.ntiles, p.scan_threads, st); }));
     }
   }
   return MP_OK;
@@ -597,6 +604,11 @@ int32_t mp_tune_set(int32_t param, int32_t value) {
         return fail(MP_ERR_ARG, "batched workgroup size must be 0 (automatic), 256, 512 or 768");
       g_batched_block = value;
       return MP_OK;
+    case MP_TUNE_PLAIN_SCAN_THREADS:
+    case MP_TUNE_PLAIN_PROP_THREADS:
+      if (value != 256 && value != 512) return fail(MP_ERR_ARG, "plain workgroup size must be 256 or 512");
+      (param == MP_TUNE_PLAIN_SCAN_THREADS ? g_plain_scan_threads : g_plain_prop_threads) = value;
+      return MP_OK;
     case MP_TUNE_GRAD_BLOCK:
       if (value != 0 && value != 256 && value != 512)
         return fail(MP_ERR_ARG, "adjoint workgroup size must be 0 (automatic), 256 or 512");
@@ -612,6 +624,8 @@ int32_t mp_tune_get(int32_t param) {
     case MP_TUNE_PLAIN_PROP_ROWS: return g_plain_prop_rows;
     case MP_TUNE_BATCHED_BLOCK: return g_batched_block;
     case MP_TUNE_GRAD_BLOCK: return g_grad_block;
+    case MP_TUNE_PLAIN_SCAN_THREADS: return g_plain_scan_threads;
+    case MP_TUNE_PLAIN_PROP_THREADS: return g_plain_prop_threads;
   }
   return -1;
 }

@@ -26,7 +26,7 @@ namespace mp {
 constexpr uint64_t kStateShift = 62;
 constexpr uint64_t kAggState = 1ull << kStateShift, kInclState = 2ull << kStateShift;
 constexpr uint64_t kValueMask = (1ull << kStateShift) - 1;
-constexpr int kWin = 1024;  // LDS search window (uint64 entries)
+constexpr int kWin = 2048;  // LDS search window (uint64 entries)
 constexpr int kMaxScanRows = 16;  // rows of 256 particles per scan tile, at most
 
 __device__ __forceinline__ uint64_t ld_relaxed(uint64_t* p) {
@@ -40,7 +40,7 @@ __device__ __forceinline__ unsigned ticket_relaxed(unsigned int* p) {
 }
 
 template <class M>
-__global__ __launch_bounds__(256) void k_plain_prop(const PlainArgs a) {
+__global__ __launch_bounds__(512) void k_plain_prop(const PlainArgs a) {
   __shared__ double red[32];
   __shared__ int64_t s_win[2];
   __shared__ uint64_t s_C[kWin];
@@ -129,9 +129,9 @@ __global__ __launch_bounds__(256) void k_plain_prop(const PlainArgs a) {
 }
 
 template <class M>
-__global__ __launch_bounds__(256) void k_plain_scan(const PlainArgs a) {
+__global__ __launch_bounds__(512) void k_plain_scan(const PlainArgs a) {
   __shared__ double red[32];
-  __shared__ uint64_t tab[kMaxScanRows * 4 + 1];
+  __shared__ uint64_t tab[65];  // rows * waves <= 64 row/wave totals + the tile total
   __shared__ uint64_t s_prefix;
   __shared__ int s_tile, is_last;
   const int J = a.J, tid = threadIdx.x, B = blockDim.x, lane = tid & 63, w = tid >> 6;
@@ -184,24 +184,21 @@ __global__ __launch_bounds__(256) void k_plain_scan(const PlainArgs a) {
     }
   }
   __syncthreads();
-  const int n = a.rows * nw;
-  uint64_t aggv = 0;
-  if (tid == 0) {
-    uint64_t run = 0;
-    for (int i = 0; i < n; ++i) {
-      uint64_t v = tab[i];
-      tab[i] = run;
-      run += v;
-    }
-    tab[n] = run;
-    aggv = run;
-    st_relaxed(&lb[tile], (tile == 0 ? kInclState : kAggState) | aggv);
-  }
-  // Decoupled look-back, 64 predecessors per probe: lane l inspects tile
-  // base - l; the nearest predecessor with an inclusive prefix ends the walk,
-  // the aggregates in between are summed with a wave reduction.
+  // wave 0: exclusive scan of the <= 64 row/wave totals, then publish the
+  // tile's aggregate
+  const int n = rows * nw;
   if (w == 0) {
-    aggv = __shfl(aggv, 0, WAVE);
+    const uint64_t v = lane < n ? tab[lane] : 0;
+    const uint64_t inc = wave_incl_scan_u64(v);
+    if (lane < n) tab[lane] = inc - v;
+    const uint64_t aggv = __shfl(inc, WAVE - 1, WAVE);
+    if (lane == 0) {
+      tab[n] = aggv;
+      st_relaxed(&lb[tile], (tile == 0 ? kInclState : kAggState) | aggv);
+    }
+    // Decoupled look-back, 64 predecessors per probe: lane l inspects tile
+    // base - l; the nearest predecessor with an inclusive prefix ends the
+    // walk, the aggregates in between are summed with a wave reduction.
     uint64_t prefix = 0;
     if (tile > 0) {
       int bse = tile - 1;
@@ -334,20 +331,20 @@ hipError_t plain_reset(uint64_t* lb, int ntiles, PlainState* st, hipStream_t s) 
 
 // which = 0: propagate kernel, 1: scan kernel
 template <class M>
-static hipError_t plain_kernel_t(int which, const PlainArgs& a, int ntiles, hipStream_t s) {
-  if (which == 0) hipLaunchKernelGGL(k_plain_prop<M>, dim3(a.nprop), dim3(256), 0, s, a);
-  else hipLaunchKernelGGL(k_plain_scan<M>, dim3(ntiles), dim3(256), 0, s, a);
+static hipError_t plain_kernel_t(int which, const PlainArgs& a, int ntiles, int threads, hipStream_t s) {
+  if (which == 0) hipLaunchKernelGGL(k_plain_prop<M>, dim3(a.nprop), dim3(threads), 0, s, a);
+  else hipLaunchKernelGGL(k_plain_scan<M>, dim3(ntiles), dim3(threads), 0, s, a);
   return hipGetLastError();
 }
 
-hipError_t plain_kernel(int which, const PlainArgs& a, int ntiles, hipStream_t s) {
+hipError_t plain_kernel(int which, const PlainArgs& a, int ntiles, int threads, hipStream_t s) {
   switch (a.md.kind) {
     case MODEL_LG:
-      if (a.md.dx == 1 && a.md.dy == 1) return plain_kernel_t<LG1>(which, a, ntiles, s);
-      if (a.md.dx == 2 && a.md.dy == 2) return plain_kernel_t<LG2>(which, a, ntiles, s);
-      return plain_kernel_t<LG4>(which, a, ntiles, s);
-    case MODEL_SV: return plain_kernel_t<SV>(which, a, ntiles, s);
-    case MODEL_SEIR: return plain_kernel_t<SEIR>(which, a, ntiles, s);
+      if (a.md.dx == 1 && a.md.dy == 1) return plain_kernel_t<LG1>(which, a, ntiles, threads, s);
+      if (a.md.dx == 2 && a.md.dy == 2) return plain_kernel_t<LG2>(which, a, ntiles, threads, s);
+      return plain_kernel_t<LG4>(which, a, ntiles, threads, s);
+    case MODEL_SV: return plain_kernel_t<SV>(which, a, ntiles, threads, s);
+    case MODEL_SEIR: return plain_kernel_t<SEIR>(which, a, ntiles, threads, s);
   }
   return hipErrorInvalidValue;
 }

@@ -181,11 +181,12 @@ def test_launch_shapes_do_not_change_results():
         for nt in (256, 512, 768):
             autotune.set("batched_block", nt)
             outs.append(F.pfilter(struct, theta[None], 5000, keys, form="batched"))
-        for sr in (2, 8):
-            for pr in (1, 4):
-                autotune.set("plain_scan_rows", sr)
-                autotune.set("plain_prop_rows", pr)
-                outs.append(F.pfilter(struct, theta[None], 5000, keys, form="plain"))
+        for sr, pr, nt in ((2, 1, 256), (8, 4, 256), (16, 2, 256), (4, 2, 512), (16, 1, 512)):
+            autotune.set("plain_scan_rows", sr)
+            autotune.set("plain_prop_rows", pr)
+            autotune.set("plain_scan_threads", nt)
+            autotune.set("plain_prop_threads", nt)
+            outs.append(F.pfilter(struct, theta[None], 5000, keys, form="plain"))
         for o in outs[1:]:
             np.testing.assert_array_equal(o["logLik64"], outs[0]["logLik64"])
         best = autotune.tune_pfilter(struct, theta, 5000, 3, form="batched")
