@@ -79,8 +79,10 @@ MP_HD float uniform01(uint32_t bits) {
 // Giles' single-precision erfinv with his w = -log((1-x)(1+x)) (deterministic
 // log, see detmath.hpp).  The tail branch (|x| > 0.9966, w >= 5) is entered only
 // by waves that hold such a lane.
+// |x| < 1 gives (1-x)(1+x) >= 2^-24 (x = nextafter(-1, 0) at the extreme),
+// a positive normal number; |x| == 1 is overridden below.
 __device__ __forceinline__ float erfinv_f32(float x) {
-  const float w = -d_log((1.0f - x) * (1.0f + x));
+  const float w = -d_log_pos((1.0f - x) * (1.0f + x));
   const float wc = w - 2.5f;
   float p = 2.81022636e-08f;
   p = 3.43273939e-07f + p * wc;

@@ -205,18 +205,26 @@ struct SV {
     x[1] = z;
   }
 
-  __device__ static float dmeas(const ModelDesc&, const P& p, const float* y, const float* x, float) {
-    float u = y[0] * d_exp(-0.5f * x[0]) - p.rho * x[1];
+  // exp(-v/2): the one transcendental of the density, kept by the adjoint's
+  // recompute pass for the backward sweep
+  __device__ static float dmeas_aux(const P&, const float* x) { return d_exp(-0.5f * x[0]); }
+
+  __device__ static float dmeas_a(const ModelDesc&, const P& p, const float* y, const float* x, float e) {
+    float u = y[0] * e - p.rho * x[1];
     return p.c0 - 0.5f * x[0] - 0.5f * u * u * p.inv1mr2;
+  }
+
+  __device__ static float dmeas(const ModelDesc& md, const P& p, const float* y, const float* x, float) {
+    return dmeas_a(md, p, y, x, dmeas_aux(p, x));
   }
 
   // ---- derivatives for the MOP adjoint (natural-scale parameters) ----
   static constexpr bool DIFFERENTIABLE = true;
 
-  // d(dmeas)/dx into gx[S], accumulated d(dmeas)/dtheta into gth[d] scaled by w.
-  __device__ static void dmeas_grad(const ModelDesc&, const P& p, const float* y, const float* x, float w,
+  // d(dmeas)/dx into gx[S], accumulated d(dmeas)/dtheta into gth[d] scaled by
+  // w; e = dmeas_aux(x).
+  __device__ static void dmeas_grad(const ModelDesc&, const P& p, const float* y, const float* x, float e, float w,
                                     float* gx, double* gth) {
-    const float e = d_exp(-0.5f * x[0]);
     const float ry = y[0] * e;
     const float u = ry - p.rho * x[1];
     const float k = p.inv1mr2;

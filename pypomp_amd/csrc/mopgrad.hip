@@ -211,12 +211,13 @@ __device__ void mop_grad_filter(const GradArgs& a, const int f, char* buf, char*
       const float* xin = state(i);
       float* xout = state(i + 1);
       const int32_t* anct = ancl + (int64_t)t * J;
+      float* bmi = bm + (int64_t)i * J;
       for (int j = tid; j < J; j += B) {
         const int src = anct[j];
         float x[M::MAXS];
         MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) x[s] = xin[s * (int64_t)J + src];
         M::step(md, P, split_at(sk, (uint64_t)j), ts, dt, x);
-        bm[(int64_t)i * J + j] = M::dmeas(md, P, y, x, tt);
+        bmi[j] = M::dmeas(md, P, y, x, tt);
         MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) xout[s * (int64_t)J + j] = x[s];
       }
       __syncthreads();
@@ -240,7 +241,7 @@ __device__ void mop_grad_filter(const GradArgs& a, const int f, char* buf, char*
         const float gbar = pi + lam;
         float x[M::MAXS], gx[M::MAXS], xb[M::MAXS], xf[M::MAXS], xfb[M::MAXS];
         MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) x[s] = xt[s * (int64_t)J + j];
-        M::dmeas_grad(md, P, y, x, gbar, gx, gth);
+        M::dmeas_grad(md, P, y, x, M::dmeas_aux(P, x), gbar, gx, gth);
         MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) xb[s] = gbar * gx[s] + xca[s * (int64_t)J + j];
         const int p = anct[j];
         MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) xf[s] = xin[s * (int64_t)J + p];

@@ -23,9 +23,10 @@ namespace mp {
 
 // Normal quantile for u in (0, 1): sqrt(2) * erfinv(2u - 1) with Giles'
 // single-precision erfinv, its argument w = -log((1-x)(1+x)) formed directly
-// as -log(4 u (1 - u)) so no precision is lost near the tails.
+// as -log(4 u (1 - u)) so no precision is lost near the tails.  The caller
+// clips u to [eps, 1 - eps], so the log's argument is a positive normal number.
 __device__ __forceinline__ float ndtri_f32(float u) {
-  float w = -d_log(4.0f * u * (1.0f - u));
+  float w = -d_log_pos(4.0f * u * (1.0f - u));
   float x = 2.0f * u - 1.0f;
   float p;
   if (w < 5.0f) {
