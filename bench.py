@@ -27,7 +27,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM-side bytes per launch at J = 1e6 from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # passes (profiles/plain_lg2_J1e6_traffic.md): FETCH_SIZE x 2 + WRITE_SIZE x 1, the
 # factors that make a 1 GiB device copy read 1 GiB and write 1 GiB.
-MEASURED_TRAFFIC_J1E6 = {"plain_prop": 30.84e6, "plain_scan": 12.54e6}
+MEASURED_TRAFFIC_J1E6 = {"plain_prop": 33.78e6, "plain_scan": 12.45e6}
 METRIC = "particle-steps/s (pfilter & MOP-grad, batched vs plain GPU); loglik rel err"
 
 
