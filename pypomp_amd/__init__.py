@@ -8,10 +8,11 @@ filter run as hand-written HIP kernels for gfx950 in ``_lib/libmipomp.so``
 from . import autotune, functional, models, random
 from .maths import logmeanexp, logmeanexp_se
 from .models import ParTrans
+from .panel import PanelParameters, PanelPomp
 from .pomp import Pomp
 from .rw_sigma import RWSigma
 
 functional_api = functional
 F = functional
 
-__all__ = ["Pomp", "ParTrans", "RWSigma", "autotune", "functional", "F", "models", "random", "logmeanexp", "logmeanexp_se"]
+__all__ = ["Pomp", "PanelPomp", "PanelParameters", "ParTrans", "RWSigma", "autotune", "functional", "F", "models", "random", "logmeanexp", "logmeanexp_se"]

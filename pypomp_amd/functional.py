@@ -62,8 +62,9 @@ def _workspace(algo, md, ser, R, J, form, dev):
 
 
 def pfilter(struct, thetas_array, J, keys, thresh=0.0, CLL=False, ESS=False, filter_mean=False,
-            prediction_mean=False, form="batched", should_trans=False):
-    """Bootstrap particle filter.  thetas (n, d) natural scale, keys (n, reps, 2).
+            prediction_mean=False, form="batched", should_trans=False, unit=None):
+    """Bootstrap particle filter.  thetas (n, d) natural scale, keys (n, reps, 2);
+    ``unit`` (n,) selects each parameter set's unit of a multi-unit struct.
     Returns a dict with ``logLik`` (n, reps) and the requested diagnostics."""
     import torch
     dev = _native.require_device()
@@ -84,6 +85,11 @@ def pfilter(struct, thetas_array, J, keys, thresh=0.0, CLL=False, ESS=False, fil
     md = struct.model.desc()
     th = torch.from_numpy(np.repeat(thetas, reps, axis=0)).to(dev)
     kt = _keys_tensor(keys.reshape(R, 2), dev)
+    ut = None
+    if unit is not None:
+        if form not in ("batched", _native.MP_FORM_BATCHED):
+            raise ValueError("multi-unit filters run in the batched form")
+        ut = torch.from_numpy(np.repeat(np.asarray(unit, dtype=np.int32).reshape(n), reps)).to(dev)
     T, S = ser.Tmax, struct.model.S
     ll = torch.empty(R, dtype=torch.float64, device=dev)
     cll = torch.empty((R, T), dtype=torch.float32, device=dev) if CLL else None
@@ -97,7 +103,7 @@ def pfilter(struct, thetas_array, J, keys, thresh=0.0, CLL=False, ESS=False, fil
     fid = _form_id(form)
     ws, nws = _workspace(_native.MP_ALGO_PF, md, ser, R, J, fid, dev)
     _native.check(_native.lib().mp_pfilter(
-        md, ser.desc, _native.ptr(th), struct.model.d, _native.ptr(kt), None, R, J, float(thresh), flags,
+        md, ser.desc, _native.ptr(th), struct.model.d, _native.ptr(kt), _native.ptr(ut), R, J, float(thresh), flags,
         fid, _native.ptr(ll), _native.ptr(cll), _native.ptr(ess), _native.ptr(fm), _native.ptr(pm),
         _native.ptr(ws), nws, _native.stream_ptr()))
     torch.cuda.current_stream().synchronize()
@@ -300,6 +306,28 @@ class PanelStruct:
 
     def device_series(self):
         return PompStruct.device_series(self)
+
+
+def panel_pfilter(struct, thetas_panel, J, keys, thresh=0.0, chunk_size=1, CLL=False, ESS=False,
+                  filter_mean=False, prediction_mean=False):
+    """Panel bootstrap filter (functional/pfilter.py panel_pfilter; pfilter.py:248-304):
+    thetas_panel (n, U, n_shared + n_spec) natural scale in the panel order,
+    keys (n, U, 2).  Every (set, unit) filter runs in one launch (``chunk_size``
+    is accepted for signature compatibility).  Returns ``logLik`` (n, U) and the
+    requested diagnostics (n, U, T_max, ...)."""
+    th = np.asarray(thetas_panel, dtype=F32)
+    n, U = th.shape[0], th.shape[1]
+    if U != struct.U:
+        raise ValueError("thetas_panel has a different number of units than the struct")
+    keys = np.asarray(keys, dtype=np.uint32).reshape(n, U, 2)
+    flat = np.ascontiguousarray(th[..., struct.perm].reshape(n * U, -1))
+    unit = np.tile(np.arange(U, dtype=np.int32), n)
+    out = pfilter(struct, flat, J, keys.reshape(n * U, 1, 2), thresh=thresh, CLL=CLL, ESS=ESS,
+                  filter_mean=filter_mean, prediction_mean=prediction_mean, unit=unit)
+    res = {}
+    for k, v in out.items():
+        res[k] = v.reshape((n, U) + v.shape[2:])
+    return res
 
 
 def panel_mif(struct, shared_array, unit_array, rw_sd, M, J, keys, thresh=0.0, n_monitors=0, block=True,

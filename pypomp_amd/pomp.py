@@ -62,6 +62,9 @@ class Pomp:
         self.theta = self._theta_list(theta)
         self.fresh_key = key
         self.results_history = []
+        # estimation-scale transforms a panel applies around IF2 (panel/panel.py:146);
+        # None: the model's own (the filters always apply the model's)
+        self.par_trans = None
 
     # -- parameters -------------------------------------------------------
     @property

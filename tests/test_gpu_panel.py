@@ -161,3 +161,56 @@ def test_graph_replays_match_eager():
         start.copy_(t0)
         torch.cuda.synchronize()
         g.replay()
+
+
+def test_panel_pfilter_functional_regression(golden):
+    # F.panel_pfilter with keys split(key, n * U) reshaped (n, U): the reference's
+    # tests/test_regression/test_panel_pfilter_regression.py
+    import pypomp_amd.functional as F
+    th = lg1_theta()
+    panel = np.stack([th, th])[None]  # (1, U, 5), panel order = model order here
+    keys = tf.split(tf.key(20260811), 2).reshape(1, 2, 2)
+    out = F.panel_pfilter(lg_panel_struct(par_trans=False), panel, 12, keys)
+    assert_close_tol(out["logLik"].ravel(), golden["test_panel_pfilter_regression"]["logLik"])
+
+
+def _panel_pomp():
+    import pandas as pd
+    import pypomp_amd as pp
+    from pypomp_amd.models import LG
+    units = {f"unit{u + 1}": LG(A=np.array([[0.9]]), ys=lg1_data(s)) for u, s in enumerate((0, 1))}
+    th = lg1_theta()
+    theta = pp.PanelParameters([{
+        "shared": pd.DataFrame({"shared": th[:2]}, index=["A11", "C11"]),
+        "unit_specific": pd.DataFrame({"unit1": th[2:], "unit2": th[2:]}, index=["Q11", "R11", "X0_1"]),
+    }])
+    return pp.PanelPomp(units, theta)
+
+
+def test_panel_pomp_methods_follow_the_functional_paths():
+    import pypomp_amd as pp
+    import pypomp_amd.functional as F
+    from pypomp_amd import random as prandom
+    P = _panel_pomp()
+    k = tf.key(42)
+    P.pfilter(J=64, key=k, reps=3)
+    new = prandom.split(k, 2)[1]
+    th = lg1_theta()
+    panel = np.repeat(np.stack([th, th])[None], 3, axis=0)
+    exp = F.panel_pfilter(P.to_struct(), panel, 64, prandom.split(new, 6).reshape(3, 2, 2))["logLik64"]
+    np.testing.assert_array_equal(P.results_history[-1]["logLik"][0], exp)
+    res = P.results()
+    assert list(res["unit"]) == ["shared", "unit1", "unit2"]
+    assert np.isclose(res["logLik"].iloc[0], res["logLik"].iloc[1:].sum())
+    rw = pp.RWSigma({n: 0.02 for n in LG_NAMES}).geometric_cooling(0.5)
+    k2 = tf.key(43)
+    P.mif(J=32, M=2, rw_sd=rw, key=k2)
+    sh = np.repeat(th[None, None, :2], 32, axis=1)
+    un = np.repeat(np.stack([th[2:], th[2:]])[None, None], 32, axis=1)
+    st, ut, _, _ = F.panel_mif(P.to_struct(), sh, un, rw, 2, 32, prandom.split(prandom.split(k2, 2)[1], 1))
+    tr = P.traces()
+    assert len(tr) == 3 * 3  # (shared + 2 units) x (M + 1) iterations
+    last = tr[tr["iteration"] == 2]
+    np.testing.assert_allclose(last[last["unit"] == "shared"][["A11", "C11"]].to_numpy()[0], st[0, -1, 1:])
+    np.testing.assert_allclose(last[last["unit"] == "unit2"]["logLik"].to_numpy()[0], ut[0, -1, 1, 0])
+    assert P.theta.num_replicates() == 1
