@@ -141,7 +141,9 @@ int32_t mp_mop(const mp_model_desc* h_model, const mp_series_desc* h_series, con
  * The forward keeps a checkpoint every K observations; the reverse sweep
  * recomputes each K-step block from it.  One Euler step per observation,
  * U = 1.  loglik (R) float64, grad (R, d) float32.  form MP_FORM_PLAIN runs the
- * filters one launch at a time. */
+ * filters one launch at a time.  Models with an adjoint: SV, and LG with
+ * dx = dy = 1 or dx = dy = 2; others return MP_ERR_ARG.  The train loop
+ * (core/algorithms/train.py:38-180) calls this once per gradient step. */
 size_t mp_mop_grad_workspace_size(const mp_model_desc* h_model, const mp_series_desc* h_series, int32_t R,
                                   int32_t J, int32_t K);
 int32_t mp_mop_grad(const mp_model_desc* h_model, const mp_series_desc* h_series, const float* theta,

@@ -116,3 +116,138 @@ def frozen_fd_grad(theta_est, ys, trace, alpha, h=1e-3):
                                                    trace["anc"], alpha, frozen_m=trace["frozen_m"])))
         g[p] = (vals[0] - vals[1]) / (2 * h)
     return g
+
+
+# ---- linear Gaussian ------------------------------------------------------
+
+def lg_noise(series, key, J, dx):
+    """rinit normals (J, dx) and step normals (T, J, dx) of the filter key tree."""
+    K = tf.split(key, J + 1)
+    run, pk = K[0], K[1:]
+    z0 = tf.normal_many(pk, dx)
+    zs = []
+    for t in range(series.T):
+        k2 = tf.split(run)
+        run, sub = k2[0], k2[1]
+        sk = tf.split(sub)[1]
+        zs.append(tf.normal_many(tf.split(sk, J), dx))
+        run = tf.split(run)[0]
+    return z0, np.stack(zs)
+
+
+def lg_unpack_torch(th, spec):
+    """theta (d,) float64 torch, natural scale -> A, C, LQ, LR, X0, L0."""
+    dx, dy = spec.dx, spec.dy
+    o = 0
+    A = th[o:o + dx * dx].reshape(dx, dx); o += dx * dx
+    C = th[o:o + dy * dx].reshape(dy, dx); o += dy * dx
+
+    def tril(o, d):
+        rows = []
+        for i in range(d):
+            row = [th[o + i * (i + 1) // 2 + j] if j <= i else th.new_zeros(()) for j in range(d)]
+            rows.append(torch.stack(row))
+        return torch.stack(rows), o + d * (d + 1) // 2
+
+    LQ, o = tril(o, dx)
+    LR, o = tril(o, dy)
+    X0 = th[o:o + dx]; o += dx
+    L0 = tril(o, dx)[0] if spec.separate_init else LQ
+    return A, C, LQ, LR, X0, L0
+
+
+def lg_from_est_torch(th_est, spec, identity=False):
+    if identity:
+        return th_est
+    dx, dy = spec.dx, spec.dy
+    out = th_est.clone()
+    base = dx * dx + dy * dx
+    for i in range(dx):
+        o = base + i * (i + 1) // 2 + i
+        out = out.index_put((torch.tensor([o]),), torch.exp(th_est[o:o + 1]))
+    for i in range(dy):
+        o = base + dx * (dx + 1) // 2 + i * (i + 1) // 2 + i
+        out = out.index_put((torch.tensor([o]),), torch.exp(th_est[o:o + 1]))
+    return out
+
+
+def lg_mop_objective(theta_est, ys, z0, zs, anc, alpha, spec, identity=False):
+    """MOP log-likelihood of the LG model (float64 torch), ancestry and noise fixed."""
+    th = lg_from_est_torch(theta_est, spec, identity)
+    A, C, LQ, LR, X0, L0 = lg_unpack_torch(th, spec)
+    J = z0.shape[0]
+    x = X0 + torch.as_tensor(z0, dtype=torch.float64) @ L0.T
+    S = LR @ LR.T
+    Lc = torch.linalg.cholesky(S)
+    logdet = torch.log(torch.diagonal(Lc)).sum()
+    wF = torch.full((J,), float(np.log(1.0 / J)), dtype=torch.float64)
+    ll = torch.zeros((), dtype=torch.float64)
+    dy = spec.dy
+    for t in range(zs.shape[0]):
+        wP = alpha * wF
+        if t > 0:
+            x = x[torch.as_tensor(anc[t - 1], dtype=torch.long)]
+        x = x @ A.T + torch.as_tensor(zs[t], dtype=torch.float64) @ LQ.T
+        r = torch.as_tensor(ys[t], dtype=torch.float64)[None, :] - x @ C.T
+        s = torch.linalg.solve_triangular(Lc, r.T, upper=False).T
+        m = -0.5 * (s * s).sum(1) - 0.5 * dy * np.log(2 * np.pi) - logdet
+        ll = ll + torch.logsumexp(wP + m, 0) - torch.logsumexp(wP, 0)
+        if t < zs.shape[0] - 1:
+            a = torch.as_tensor(anc[t], dtype=torch.long)
+            wF = (wP + m - m.detach())[a]
+    return ll
+
+
+def lg_mop_value_and_grad(model, theta_est, ys, key, J, alpha, identity=False):
+    """MOP value (NumPy oracle), float64 re-evaluation and autograd gradient for
+    an oracle LG model (oracle/lg.py); ``identity`` treats theta as natural."""
+    ser = fl.make_series(ys, np.arange(1, len(ys) + 1.0), 0.0, nstep=1)
+    m = fl.IdentityTrans(model) if identity else model
+    val, tr = fl.mop(m, theta_est, ser, key, J, alpha, return_trace=True)
+    z0, zs = lg_noise(ser, key, J, model.spec.dx)
+    anc = np.stack(tr["ancestors"])
+    th = torch.tensor(np.asarray(theta_est, np.float64), requires_grad=True)
+    ll = lg_mop_objective(th, ys, z0, zs, anc, alpha, model.spec, identity)
+    ll.backward()
+    return float(val), float(ll.detach()), th.grad.numpy().copy()
+
+
+# ---- train (core/algorithms/train.py:38-180) ----------------------------
+
+def cosine_cooling(i, M, c):
+    """``_cosine_cooling`` (core/algorithms/helpers.py:302-314), float32."""
+    F32 = np.float32
+    return F32(F32(c) + (F32(1) - F32(c)) * F32(0.5) * (F32(1) + np.cos(F32(np.pi) * F32(i) / F32(M), dtype=F32)))
+
+
+def train(value_and_grad, theta_est0, key, T, M, eta, alpha, alpha_cooling=1.0, beta1=0.9, beta2=0.999,
+          eps=1e-8, clip_norm=None):
+    """``_train_internal`` / ``_train_scan_step`` with the Adam optimizer
+    (core/optimizer.py:119-153) and one monitor (n_monitors = 1: the MOP value
+    of the same draw is the reported objective).  ``value_and_grad(theta, key,
+    alpha)`` returns (loglik, d loglik / d theta).  eta (M, d).  Returns
+    (neg_logliks (M+1,), NaN first, traces (M+1, d)), float32 like the
+    reference."""
+    F32 = np.float32
+    th = np.asarray(theta_est0, F32).copy()
+    m1 = np.zeros_like(th)
+    v1 = np.zeros_like(th)
+    nll, traces = [np.nan], [th.copy()]
+    for m in range(M):
+        a_m = F32(F32(1) - (F32(1) - F32(alpha)) * cosine_cooling(m, M, alpha_cooling))
+        k2 = tf.split(key)
+        key, sub = k2[0], k2[1]
+        ll, g = value_and_grad(th, sub, float(a_m))
+        grad = (-np.asarray(g, np.float64) / T).astype(F32)  # gradient of the mean objective
+        nll.append(F32(-ll))
+        if clip_norm is not None:
+            grad = np.clip(grad, -clip_norm, clip_norm).astype(F32)
+        key = tf.split(key)[0]  # the Hessian subkey (unused by Adam)
+        m1 = (F32(beta1) * m1 + F32(1 - beta1) * grad).astype(F32)
+        v1 = (F32(beta2) * v1 + F32(1 - beta2) * grad * grad).astype(F32)
+        mh = (m1 / (F32(1) - F32(beta1) ** F32(m + 1))).astype(F32)
+        vh = (v1 / (F32(1) - F32(beta2) ** F32(m + 1))).astype(F32)
+        direction = (-mh / (np.sqrt(vh) + F32(eps))).astype(F32)
+        th = (th + np.asarray(eta[m], F32) * direction).astype(F32)
+        traces.append(th.copy())
+    return np.array(nll, dtype=F32), np.stack(traces)

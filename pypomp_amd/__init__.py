@@ -8,6 +8,7 @@ filter run as hand-written HIP kernels for gfx950 in ``_lib/libmipomp.so``
 from . import autotune, functional, models, random
 from .maths import logmeanexp, logmeanexp_se
 from .models import ParTrans
+from .optim import SGD, Adam, LearningRate, Optimizer
 from .panel import PanelParameters, PanelPomp
 from .pomp import Pomp
 from .rw_sigma import RWSigma
@@ -15,4 +16,4 @@ from .rw_sigma import RWSigma
 functional_api = functional
 F = functional
 
-__all__ = ["Pomp", "PanelPomp", "PanelParameters", "ParTrans", "RWSigma", "autotune", "functional", "F", "models", "random", "logmeanexp", "logmeanexp_se"]
+__all__ = ["Pomp", "PanelPomp", "PanelParameters", "ParTrans", "RWSigma", "Adam", "SGD", "Optimizer", "LearningRate", "autotune", "functional", "F", "models", "random", "logmeanexp", "logmeanexp_se"]

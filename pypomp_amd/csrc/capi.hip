@@ -30,6 +30,7 @@ hipError_t launch_order(const SeriesSet& ser, const int32_t* unit, int R, int32_
 namespace mp {
 hipError_t launch_mop_grad(const GradArgs& a, int block, int grid, hipStream_t st);
 int64_t mop_grad_scratch_bytes(int S, int J, int T, int K, bool lds_work);
+bool mop_grad_supported(const ModelDesc& md);
 size_t mop_grad_lds_bytes(int S, int J, int block, bool lds_work);
 }  // namespace mp
 
@@ -551,6 +552,8 @@ int32_t mp_mop_grad(const mp_model_desc* h_model, const mp_series_desc* h_series
   if (!theta || !loglik || !grad || !workspace) return fail(MP_ERR_ARG, "NULL theta/loglik/grad/workspace");
   if (K < 1) return fail(MP_ERR_ARG, "checkpoint interval K must be >= 1 (K = %d)", K);
   if (h_series->U != 1) return fail(MP_ERR_ARG, "mp_mop_grad runs single-series filters (U = 1)");
+  if (!mop_grad_supported(*reinterpret_cast<const ModelDesc*>(h_model)))
+    return fail(MP_ERR_ARG, "model kind %d (dx %d, dy %d) has no adjoint", h_model->kind, h_model->dx, h_model->dy);
   if (h_series->max_euler != h_series->Tmax)
     return fail(MP_ERR_ARG, "mp_mop_grad needs exactly one Euler step per observation interval");
   hipStream_t st = (hipStream_t)stream;

@@ -44,9 +44,11 @@ struct LGT {
   static constexpr int MAXS = MD;
   static constexpr bool FIXED_S = EXACT;  // S == MAXS for every instance
   static constexpr int MAXP = 2 * MD * MD + 3 * (MD * (MD + 1) / 2) + MD;
-  static constexpr bool DIFFERENTIABLE = false;
+  // the MOP adjoint covers dx, dy <= 2 (the gradient accumulators of larger
+  // shapes would not fit in registers)
+  static constexpr bool DIFFERENTIABLE = MD <= 2;
   struct P {
-    float A[MD][MD], C[MD][MD], LQ[MD][MD], L0[MD][MD], LC[MD][MD];
+    float A[MD][MD], C[MD][MD], LQ[MD][MD], L0[MD][MD], LC[MD][MD], LR[MD][MD];
     float X0[MD];
     float logdet, half_n_log2pi;
     int dx, dy;
@@ -80,6 +82,7 @@ struct LGT {
       if (i < dy && j < dx) p.C[i][j] = th[oC + i * dx + j];
       p.LQ[i][j] = (i < dx && j <= i) ? th[oQ + i * (i + 1) / 2 + j] : 0.f;
       LR[i][j] = (i < dy && j <= i) ? th[oR + i * (i + 1) / 2 + j] : 0.f;
+      p.LR[i][j] = LR[i][j];
       if (md.lg_sep_init) p.L0[i][j] = (i < dx && j <= i) ? th[oP + i * (i + 1) / 2 + j] : 0.f;
       else p.L0[i][j] = p.LQ[i][j];
     }
@@ -146,6 +149,111 @@ struct LGT {
       quad += s[i] * s[i];
     }
     return -0.5f * quad - p.half_n_log2pi - p.logdet;
+  }
+
+  // ---- derivatives for the MOP adjoint (natural-scale parameters) ----
+  // theta offsets: A, C, LQ (lower triangle), LR (lower triangle), X0, [L0]
+  __device__ static int oC(const P& p) { return p.dx * p.dx; }
+  __device__ static int oQ(const P& p) { return oC(p) + p.dy * p.dx; }
+  __device__ static int oR(const P& p) { return oQ(p) + p.dx * (p.dx + 1) / 2; }
+  __device__ static int oX(const P& p) { return oR(p) + p.dy * (p.dy + 1) / 2; }
+  __device__ static int oP(const P& p) { return oX(p) + p.dx; }
+
+  __device__ static float dmeas_aux(const P&, const float*) { return 0.f; }
+  __device__ static float dmeas_a(const ModelDesc& md, const P& p, const float* y, const float* x, float) {
+    return dmeas(md, p, y, x, 0.f);
+  }
+
+  // log N(y; C x, S), S = LR LR^T = LC LC^T:  with w = S^-1 (y - C x),
+  //   d/dx = C^T w,  d/dC = w x^T,  d/dS = (w w^T - S^-1) / 2,  d/dLR = 2 (d/dS) LR.
+  __device__ static void dmeas_grad(const ModelDesc&, const P& p, const float* y, const float* x, float,
+                                    float wgt, float* gx, double* gth) {
+    float r[MD], s[MD], w[MD], Li[MD][MD], Si[MD][MD];
+    MP_UNROLL for (int i = 0; i < MD; ++i) {
+      float m = 0.f;
+      MP_UNROLL for (int j = 0; j < MD; ++j) if (j < p.dx) m += p.C[i][j] * x[j];
+      r[i] = i < p.dy ? y[i] - m : 0.f;
+    }
+    // s = LC^-1 r, w = LC^-T s; Li = LC^-1 (lower), Si = Li^T Li
+    MP_UNROLL for (int i = 0; i < MD; ++i) {
+      float a = r[i];
+      MP_UNROLL for (int k = 0; k < MD; ++k) if (k < i) a -= p.LC[i][k] * s[k];
+      s[i] = i < p.dy ? a / p.LC[i][i] : 0.f;
+    }
+    MP_UNROLL for (int ii = MD - 1; ii >= 0; --ii) {
+      float a = s[ii];
+      MP_UNROLL for (int k = 0; k < MD; ++k) if (k > ii && k < p.dy) a -= p.LC[k][ii] * w[k];
+      w[ii] = ii < p.dy ? a / p.LC[ii][ii] : 0.f;
+    }
+    MP_UNROLL for (int c = 0; c < MD; ++c) MP_UNROLL for (int i = 0; i < MD; ++i) {
+      float a = (i == c) ? 1.f : 0.f;
+      MP_UNROLL for (int k = 0; k < MD; ++k) if (k < i) a -= p.LC[i][k] * Li[k][c];
+      Li[i][c] = (i < p.dy && c < p.dy) ? a / p.LC[i][i] : 0.f;
+    }
+    MP_UNROLL for (int i = 0; i < MD; ++i) MP_UNROLL for (int j = 0; j < MD; ++j) {
+      float a = 0.f;
+      MP_UNROLL for (int k = 0; k < MD; ++k) a += Li[k][i] * Li[k][j];
+      Si[i][j] = a;
+    }
+    MP_UNROLL for (int j = 0; j < MD; ++j) {
+      float a = 0.f;
+      MP_UNROLL for (int i = 0; i < MD; ++i) if (i < p.dy) a += p.C[i][j] * w[i];
+      gx[j] = a;
+    }
+    const double wd = (double)wgt;
+    const int oc = oC(p), orr = oR(p);
+    MP_UNROLL for (int i = 0; i < MD; ++i) if (i < p.dy)
+      MP_UNROLL for (int j = 0; j < MD; ++j) if (j < p.dx) gth[oc + i * p.dx + j] += wd * (double)(w[i] * x[j]);
+    // d/dLR_ij = sum_k (w_i w_k - Si_ik) LR_kj, j <= i
+    MP_UNROLL for (int i = 0; i < MD; ++i) if (i < p.dy) MP_UNROLL for (int j = 0; j <= i; ++j) {
+      float a = 0.f;
+      MP_UNROLL for (int k = 0; k < MD; ++k) if (k < p.dy) a += (w[i] * w[k] - Si[i][k]) * p.LR[k][j];
+      gth[orr + i * (i + 1) / 2 + j] += wd * (double)a;
+    }
+  }
+
+  // x_out = A x_in + LQ z, z redrawn from the particle key
+  __device__ static void step_vjp(const ModelDesc&, const P& p, Key k, const float* xin, const float*,
+                                  const float* xbar_out, float* xbar_in, double* gth) {
+    float z[MD];
+    MP_UNROLL for (int i = 0; i < MD; ++i) z[i] = i < p.dx ? normal_at(k, i) : 0.f;
+    MP_UNROLL for (int j = 0; j < MD; ++j) {
+      float a = 0.f;
+      MP_UNROLL for (int i = 0; i < MD; ++i) if (i < p.dx) a += p.A[i][j] * xbar_out[i];
+      xbar_in[j] = a;
+    }
+    const int oq = oQ(p);
+    MP_UNROLL for (int i = 0; i < MD; ++i) if (i < p.dx) {
+      const double b = (double)xbar_out[i];
+      MP_UNROLL for (int j = 0; j < MD; ++j) if (j < p.dx) gth[i * p.dx + j] += b * (double)xin[j];
+      MP_UNROLL for (int j = 0; j <= i; ++j) gth[oq + i * (i + 1) / 2 + j] += b * (double)z[j];
+    }
+  }
+
+  // x_0 = X0 + L0 z (L0 = LQ unless a separate initial factor is given)
+  __device__ static void rinit_vjp(const ModelDesc& md, const P& p, Key k, float, const float* xbar, double* gth) {
+    float z[MD];
+    MP_UNROLL for (int i = 0; i < MD; ++i) z[i] = i < p.dx ? normal_at(k, i) : 0.f;
+    const int ox = oX(p), ol = md.lg_sep_init ? oP(p) : oQ(p);
+    MP_UNROLL for (int i = 0; i < MD; ++i) if (i < p.dx) {
+      const double b = (double)xbar[i];
+      gth[ox + i] += b;
+      MP_UNROLL for (int j = 0; j <= i; ++j) gth[ol + i * (i + 1) / 2 + j] += b * (double)z[j];
+    }
+  }
+
+  // d(natural)/d(estimation): exp on the diagonals of the LQ and LR triangles
+  __device__ static void from_est_jac(const ModelDesc& md, const float* est, float* jac) {
+    const int dx = EXACT ? MD : md.dx, dy = EXACT ? MD : md.dy;
+    MP_UNROLL for (int q = 0; q < MAXP; ++q) jac[q] = 1.f;
+    MP_UNROLL for (int i = 0; i < MD; ++i) if (i < dx) {
+      const int o = dx * dx + dy * dx + i * (i + 1) / 2 + i;
+      jac[o] = d_exp(est[o]);
+    }
+    MP_UNROLL for (int i = 0; i < MD; ++i) if (i < dy) {
+      const int o = dx * dx + dy * dx + dx * (dx + 1) / 2 + i * (i + 1) / 2 + i;
+      jac[o] = d_exp(est[o]);
+    }
   }
 };
 
@@ -236,7 +344,7 @@ struct SV {
   // Vector-Jacobian product of one sub-step x_out = step(x_in): given xbar_out,
   // writes xbar_in and accumulates theta-bar.  The step's normal is x_out[1]
   // (eta), so nothing is redrawn.
-  __device__ static void step_vjp(const ModelDesc&, const P& p, const float* xin, const float* xout,
+  __device__ static void step_vjp(const ModelDesc&, const P& p, Key, const float* xin, const float* xout,
                                   const float* xbar_out, float* xbar_in, double* gth) {
     const float z = xout[1];
     const double vb = xbar_out[0];

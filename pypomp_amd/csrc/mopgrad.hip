@@ -225,6 +225,7 @@ __device__ void mop_grad_filter(const GradArgs& a, const int f, char* buf, char*
     // backward through the block
     for (int i = n - 1; i >= 0; --i) {
       const int t = t0 + i;
+      const Key sk = step_key(t);
       const float dt = dts[tidx0[t]];
       const float ts = tstart(t);
       const float* y = ys + (int64_t)t * md.dy;
@@ -245,7 +246,7 @@ __device__ void mop_grad_filter(const GradArgs& a, const int f, char* buf, char*
         MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) xb[s] = gbar * gx[s] + xca[s * (int64_t)J + j];
         const int p = anct[j];
         MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) xf[s] = xin[s * (int64_t)J + p];
-        M::step_vjp(md, P, xf, x, xb, xfb, gth);
+        M::step_vjp(md, P, split_at(sk, (uint64_t)j), xf, x, xb, xfb, gth);
         if (t == 0) {
           M::rinit_vjp(md, P, split_at(fkey, 1 + (uint64_t)j), a.ser.t0, xfb, gth);
         } else {
@@ -356,6 +357,18 @@ hipError_t launch_mop_grad(const GradArgs& a, int block, int grid, hipStream_t s
     case MODEL_SEIR: return GradLauncher<SEIR>::run(a, block, grid, lds, st);
   }
   return hipErrorInvalidValue;
+}
+
+bool mop_grad_supported(const ModelDesc& md) {
+  switch (md.kind) {
+    case MODEL_SV: return SV::DIFFERENTIABLE;
+    case MODEL_LG:
+      if (md.dx == 1 && md.dy == 1) return LG1::DIFFERENTIABLE;
+      if (md.dx == 2 && md.dy == 2) return LG2::DIFFERENTIABLE;
+      return false;
+    case MODEL_SEIR: return SEIR::DIFFERENTIABLE;
+  }
+  return false;
 }
 
 int64_t mop_grad_scratch_bytes(int S, int J, int T, int K, bool lds_work) {

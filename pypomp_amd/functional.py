@@ -475,6 +475,93 @@ def mop_grad(struct, thetas_array, J, alpha, keys, K=50, form="batched"):
     return ll.cpu().numpy(), g.cpu().numpy()
 
 
+def _cosine_cooling(i, M, c):
+    F = np.float32
+    return F(F(c) + (F(1) - F(c)) * F(0.5) * (F(1) + np.cos(F(np.pi) * F(i) / F(M), dtype=F)))
+
+
+def train(struct, thetas_array, J, optimizer, M, eta, alpha, keys, alpha_cooling=1.0, thresh=0.0, n_monitors=1,
+          K=50):
+    """Gradient ascent on the MOP log-likelihood (functional/train.py:11-113;
+    core/algorithms/train.py:38-180): per iteration m, alpha_m = 1 - (1 -
+    alpha) c_m with cosine cooling c_m, the MOP value and hand-adjoint gradient
+    (one launch over all n parameter sets), the optimizer's direction and
+    theta += eta[m] direction.  thetas (n, d) are used as the estimation-scale
+    start, as the reference passes them through.  Keys per iteration follow the
+    reference: (key, sub) = split(key) for the MOP draw, n_monitors further
+    keys when n_monitors > 1, then one more split.  The parameters and the
+    optimizer state stay on the device for all M iterations.  Returns
+    (neg_logliks (n, M+1), NaN first, traces (n, M+1, d)), float32."""
+    import torch
+    from .optim import Adam, Optimizer, SGD
+    if not isinstance(optimizer, (Adam, SGD)):
+        raise ValueError(f"Optimizer '{type(optimizer).__name__}' not supported")
+    if optimizer.ls:
+        raise ValueError("line search is not supported by the native train loop")
+    dev = _native.require_device()
+    th0 = np.ascontiguousarray(np.asarray(thetas_array, dtype=F32))
+    if th0.ndim == 1:
+        th0 = th0[None]
+    keys = np.asarray(keys, dtype=np.uint32).reshape(-1, 2)
+    n, d = th0.shape
+    if keys.shape[0] != n:
+        raise ValueError("thetas and keys disagree on the number of rows")
+    for u in struct.units:
+        if not np.all(u.nstep == u.nstep[0]):
+            raise ValueError("train needs the same number of Euler steps in every observation interval")
+    T = struct.units[0].T
+    eta_arr = torch.from_numpy(eta.to_array(struct.param_names, M)).to(dev)
+    # host key schedule: MOP keys (M, n, 2) and the running keys
+    mop_keys = np.zeros((M, n, 2), np.uint32)
+    mon_keys = []
+    for r in range(n):
+        k = keys[r]
+        row_mon = []
+        for m in range(M):
+            k2 = prandom.split(k, 2)
+            k, mop_keys[m, r] = k2[0], k2[1]
+            if n_monitors > 1:
+                ks = prandom.split(k, n_monitors + 1)
+                k, sub = ks[0], ks[1:]
+                row_mon.append(sub)
+            k = prandom.split(k, 2)[0]
+        mon_keys.append(row_mon)
+    ser = struct.device_series()
+    md = struct.model.desc()
+    fid = _native.MP_FORM_BATCHED
+    nws = _native.lib().mp_mop_grad_workspace_size(md, ser.desc, n, J, K)
+    ws = torch.empty(int(nws), dtype=torch.uint8, device=dev)
+    kt = torch.from_numpy(mop_keys.view(np.int32)).to(dev)
+    th = torch.from_numpy(th0).to(dev)
+    state = optimizer.init_state(th)
+    lls = torch.empty((M, n), dtype=torch.float64, device=dev)
+    g = torch.empty((n, d), dtype=torch.float32, device=dev)
+    traces = [th.clone()]
+    nll = np.full((n, M + 1), np.nan, dtype=F32)
+    for m in range(M):
+        a_m = float(F32(F32(1) - (F32(1) - F32(alpha)) * _cosine_cooling(m, M, alpha_cooling)))
+        _native.check(_native.lib().mp_mop_grad(md, ser.desc, _native.ptr(th), d, _native.ptr(kt[m]), n, J, a_m,
+                                                int(K), fid, _native.ptr(lls[m]), _native.ptr(g), _native.ptr(ws),
+                                                int(nws), _native.stream_ptr()))
+        grad = -g / np.float32(T)  # gradient of the mean objective, -loglik / T
+        if n_monitors > 1:
+            means = th.cpu().numpy()
+            res = pfilter(struct, means, J, np.stack([mon_keys[r][m] for r in range(n)]), thresh=thresh,
+                          should_trans=True)
+            nll[:, m + 1] = -res["logLik"].mean(axis=1, dtype=F32)
+        if optimizer.clip_norm is not None:
+            grad = torch.clamp(grad, -optimizer.clip_norm, optimizer.clip_norm)
+        direction, state = optimizer.step(grad, state, m)
+        if optimizer.scale:
+            direction = direction / torch.linalg.norm(direction, dim=-1, keepdim=True)
+        th = th + eta_arr[m][None, :] * direction
+        traces.append(th.clone())
+    torch.cuda.current_stream().synchronize()
+    if n_monitors == 1:
+        nll[:, 1:] = (-lls.cpu().numpy().T).astype(F32)
+    return nll, torch.stack(traces, dim=1).cpu().numpy()
+
+
 class _MopFn:
     """Builds the torch.autograd.Function lazily (torch is only needed on use)."""
     fn = None

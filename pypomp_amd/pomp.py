@@ -149,6 +149,35 @@ class Pomp:
             "key": old, "theta": theta_used, "traces": np.concatenate([lls_nan[:, :, None], traces], axis=-1),
             "J": J, "M": M, "rw_sd": rw_sd, "thresh": thresh, "n_monitors": n_monitors})
 
+    def train(self, J, M, eta, key=None, theta=None, optimizer=None, alpha=0.97, thresh=0.0, alpha_cooling=1.0,
+              n_monitors=1, track_time=True):
+        """Gradient ascent on the MOP objective with the native adjoint
+        (estimation_mixin.py:446-634): theta to the estimation scale, F.train,
+        traces back to the natural scale."""
+        from .optim import Adam, LearningRate
+        start = time.time()
+        optimizer = optimizer or Adam()
+        thresh = float(max(0.0, thresh))
+        if J < 1:
+            raise ValueError("J should be greater than 0")
+        if not isinstance(eta, LearningRate):
+            raise TypeError("eta must be a LearningRate object")
+        thetas = self._theta_array(theta)
+        theta_used = self.theta if theta is None else self._theta_list(theta)
+        new, old = self._update_fresh_key(key)
+        n = thetas.shape[0]
+        keys = prandom.split(new, n)
+        nll, tr_est = F.train(self.to_struct(), self.model.to_est(thetas), J, optimizer, M, eta, alpha, keys,
+                              alpha_cooling, thresh, n_monitors)
+        tr = self.model.from_est(tr_est.reshape(n * (M + 1), -1)).reshape(n, M + 1, -1)
+        names = self.canonical_param_names
+        self.theta = [dict(zip(names, map(float, tr[i, M]))) for i in range(n)]
+        self.results_history.append({
+            "method": "train", "execution_time": time.time() - start if track_time else None,
+            "key": old, "theta": theta_used, "traces": np.concatenate([-nll[:, :, None], tr], axis=-1),
+            "optimizer": optimizer, "J": J, "M": M, "eta": eta, "alpha": alpha, "thresh": thresh,
+            "alpha_cooling": alpha_cooling})
+
     def mop(self, J, alpha, key=None, theta=None):
         """MOP(alpha) log-likelihood estimate at natural-scale theta (value only)."""
         thetas = self._theta_array(theta)

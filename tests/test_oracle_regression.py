@@ -148,3 +148,22 @@ def test_panel_mif_all_unit_specific_equals_per_unit_mif():
             th, ll = fl.perfilter(model, m, th, sers[u], tf.split(mkeys[m], 2)[u], J, rw)
             assert_close_tol(ut[m + 1, u, 0], ll)
         np.testing.assert_array_equal(fu[:, u, :], model.from_est(th))
+
+
+def test_train(golden, lg1):
+    # the reference's F.train on the lg_struct fixture: 3 Adam steps,
+    # eta 0.01, alpha 0, one monitor; theta0 (natural values) used as the
+    # estimation-scale start, as the reference passes it straight through
+    from oracle import mop_grad as mg
+    model, ser = lg1
+    ys = lg1_data(0)
+
+    def vg(th, k, alpha):
+        val, _, g = mg.lg_mop_value_and_grad(model, th, ys, k, 12, alpha)
+        return val, g
+
+    eta = np.full((3, 5), 0.01, np.float32)
+    nll, tr = mg.train(vg, lg1_theta(), tf.split(tf.key(20260811), 1)[0], 4, 3, eta, 0.0)
+    g = golden["test_train_regression"]
+    assert_close_tol(nll[-1], g["final_neg_loglik"])
+    assert_close_tol(tr[-1], g["final_theta"])
