@@ -150,6 +150,19 @@ int32_t mp_mop_grad(const mp_model_desc* h_model, const mp_series_desc* h_series
                     int64_t theta_stride, const uint32_t* keys, int32_t R, int32_t J, float alpha, int32_t K,
                     int32_t form, double* loglik, float* grad, void* workspace, size_t ws_bytes, void* stream);
 
+/* Simulation of states and observations: _simulate_internal /
+ * _simulate_helper (core/algorithms/simulate.py:15-142) vmapped over R
+ * parameter rows (simulate.py:71-74; functional/simulate.py:11-78), with the
+ * reference's key tree and its time bookkeeping (the process of interval i
+ * starts at t0 for i <= 1 and at times[i-2] after; rmeas sees times0[i]).
+ * theta (R, d) natural scale unless flags has MP_SHOULD_TRANS; keys (R, 2);
+ * times (T) or NULL for the series' own; U = 1.  X (R, T+1, S, nsim) and
+ * Y (R, T, dy, nsim) float32: the reference's internal layout before its
+ * transpose to (R, nsim, T+1, S).  No workspace. */
+int32_t mp_simulate(const mp_model_desc* h_model, const mp_series_desc* h_series, const float* theta,
+                    int64_t theta_stride, const uint32_t* keys, const float* times, int32_t R, int32_t nsim,
+                    int32_t flags, float* X, float* Y, void* stream);
+
 /* One IF2 iteration m: _perfilter_internal (core/algorithms/mif.py:88-258) for R
  * swarms.  thetas_in/thetas_out (R, J, d) on the estimation scale; sigmas and
  * sigmas_init (d) as RWSigma.sigmas_array / sigmas_init_array

@@ -161,6 +161,29 @@ def dd_log(x):
     return r
 
 
+def dd_exp(x):
+    """float64 exp: Cody-Waite reduction by a split ln 2, Taylor polynomial to
+    r^13/13!, scaling by 2^n (two factors below 2^-1000)."""
+    x = np.asarray(x, dtype=np.float64)
+    with np.errstate(all="ignore"):
+        n = np.rint(x * 1.4426950408889634)
+        r = (x - n * 6.93147180369123816490e-01) - n * 1.90821492927058770002e-10
+        p = np.full(x.shape, 1.0 / 6227020800.0)
+        for den in (479001600.0, 39916800.0, 3628800.0, 362880.0, 40320.0, 5040.0, 720.0, 120.0, 24.0, 6.0):
+            p = p * r + 1.0 / den
+        p = p * r + 0.5
+        p = p * r + 1.0
+        p = p * r + 1.0
+        k = np.nan_to_num(n, nan=0.0, posinf=0.0, neginf=0.0).astype(np.int64)
+        low = k < -1000
+        out = np.where(low, np.ldexp(np.ldexp(p, np.where(low, k + 600, 0)), np.where(low, -600, 0)),
+                       np.ldexp(p, np.where(low, 0, k)))
+        out = np.where(x > 709.782712893384, np.inf, out)
+        out = np.where(x < -745.1332191019412, 0.0, out)
+        out = np.where(x != x, x, out)
+    return out
+
+
 def dd_lgamma(x):
     """log Gamma for x > 0 (scalar loop per element, as the device)."""
     x = np.atleast_1d(np.asarray(x, dtype=np.float64))

@@ -191,14 +191,47 @@ class SEIRModel:
         return v.astype(F32)
 
     def rmeas(self, X, theta, keys, t):
-        """Data generation only: NegBin(mean rho H, size k) by inversion of one uniform."""
+        """NegBin(mean rho H, size k) by inversion of one uniform u: the smallest
+        y >= 0 with F(y) >= u (scipy's ppf), and 0 at u = 0."""
         J = X.shape[0]
         beta, amp, sig, gam, rho, k, N, S0f, E0, I0 = self._cols(theta, J)
         mu = np.maximum((X[:, 4] * rho).astype(np.float64), 1e-10)
         u = tf.uniform_from_bits(tf.random_bits_many(keys, 1)[:, 0]).astype(np.float64)
         kk = k.astype(np.float64)
-        y = stats.nbinom.ppf(u, kk, kk / (kk + mu))
+        y = np.where(u > 0, stats.nbinom.ppf(u, kk, kk / (kk + mu)), 0.0)
         return y.astype(F32)[:, None]
+
+
+def negbin_inverse_sum(u, k, mu):
+    """Smallest y >= 0 with F(y) >= u for NegBin(size k, mean mu) by upward
+    summation in float64 (vectorised over draws): pmf(0) = p^k, pmf(y+1) =
+    pmf(y) (y + k)/(y + 1) q with p = k/(k+mu), q = mu/(k+mu); F and pmf are
+    carried in units of pmf(0) 2^(600 r) (r rescalings), so p^k may underflow.
+    The device sampler uses this construction; it agrees with scipy's
+    ``nbinom.ppf`` except where u lies within rounding of a CDF step."""
+    from .detmath import dd_exp, dd_log
+    u, k, mu = (np.asarray(a, np.float64) for a in np.broadcast_arrays(u, k, mu))
+    p, q = k / (k + mu), mu / (k + mu)
+    with np.errstate(divide="ignore"):
+        lt = dd_log(u) - k * dd_log(p)
+    term = np.ones_like(u)
+    cdf = np.ones_like(u)
+    y = np.zeros_like(u)
+    big, lbig = 2.0 ** 600, 600 * 0.6931471805599453
+    target = np.where(lt > 700.0, np.inf, dd_exp(np.minimum(lt, 700.0)))
+    active = (u > 0) & (cdf < target)
+    while np.any(active):
+        ratio = (y + k) / (y + 1.0)
+        term = np.where(active, (term * ratio) * q, term)
+        cdf = np.where(active, cdf + term, cdf)
+        y = np.where(active, y + 1.0, y)
+        resc = active & (lt > 700.0) & (cdf > big)
+        term = np.where(resc, term * 2.0 ** -600, term)
+        cdf = np.where(resc, cdf * 2.0 ** -600, cdf)
+        lt = np.where(resc, lt - lbig, lt)
+        target = np.where(resc & (lt <= 700.0), dd_exp(np.minimum(lt, 700.0)), target)
+        active = active & (cdf < target) & ~((term == 0) & (y > mu))
+    return np.where(u > 0, y, 0.0)
 
 
 def default_theta():

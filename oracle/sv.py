@@ -80,7 +80,9 @@ class SVModel:
         J = X.shape[0]
         mu, phi, sig, rho, V0, S0 = self._cols(theta, J)
         e = tf.normal_many(keys, 1)[:, 0]
-        r = np.exp(F32(0.5) * X[:, 0]) * (rho * X[:, 1] + np.sqrt(F32(1) - rho * rho) * e)
+        sd = np.sqrt((F32(1) - (rho * rho).astype(F32)).astype(F32))
+        r = (d_exp((F32(0.5) * X[:, 0]).astype(F32))
+             * ((rho * X[:, 1]).astype(F32) + (sd * e).astype(F32)).astype(F32))
         return r.astype(F32)[:, None]
 
 

@@ -15,7 +15,7 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 OUT_DIR = HERE / "_lib"
 LIB = OUT_DIR / "libmipomp.so"
-SOURCES = ["capi.hip", "prims.hip", "batched.hip", "plain.hip", "mopgrad.hip"]
+SOURCES = ["capi.hip", "prims.hip", "batched.hip", "plain.hip", "mopgrad.hip", "simulate.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

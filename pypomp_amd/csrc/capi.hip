@@ -31,6 +31,7 @@ namespace mp {
 hipError_t launch_mop_grad(const GradArgs& a, int block, int grid, hipStream_t st);
 int64_t mop_grad_scratch_bytes(int S, int J, int T, int K, bool lds_work);
 bool mop_grad_supported(const ModelDesc& md);
+hipError_t launch_simulate(const SimArgs& a, hipStream_t st);
 size_t mop_grad_lds_bytes(int S, int J, int block, bool lds_work);
 }  // namespace mp
 
@@ -396,7 +397,7 @@ int run_mop_grad(const mp_model_desc* m, const mp_series_desc* s, const float* t
 extern "C" {
 
 const char* mp_last_error(void) { return g_err.c_str(); }
-int32_t mp_abi_version(void) { return 2; }
+int32_t mp_abi_version(void) { return 3; }
 
 int32_t mp_host_threefry2x32(const uint32_t* h_keys, const uint32_t* h_ctr, uint32_t* h_out, int64_t n) {
   if (n < 0 || (n > 0 && (!h_keys || !h_ctr || !h_out))) return fail(MP_ERR_ARG, "bad threefry arguments");
@@ -567,6 +568,36 @@ int32_t mp_mop_grad(const mp_model_desc* h_model, const mp_series_desc* h_series
   }
   return run_mop_grad(h_model, h_series, theta, theta_stride, keys, R, J, alpha, K, loglik, grad, workspace,
                       ws_bytes, st);
+}
+
+int32_t mp_simulate(const mp_model_desc* h_model, const mp_series_desc* h_series, const float* theta,
+                    int64_t theta_stride, const uint32_t* keys, const float* times, int32_t R, int32_t nsim,
+                    int32_t flags, float* X, float* Y, void* stream) {
+  int rc = check_common(h_model, h_series, R, 1, keys);
+  if (rc) return rc;
+  if (!theta || !X || !Y) return fail(MP_ERR_ARG, "NULL theta/X/Y");
+  if (nsim < 1) return fail(MP_ERR_ARG, "nsim must be >= 1 (nsim = %d)", nsim);
+  if (h_series->U != 1) return fail(MP_ERR_ARG, "mp_simulate runs single-series models (U = 1)");
+  if (h_model->dy > kSimMaxDy) return fail(MP_ERR_ARG, "dy = %d exceeds %d", h_model->dy, kSimMaxDy);
+  SimArgs a{};
+  a.md = *reinterpret_cast<const ModelDesc*>(h_model);
+  a.ser = *reinterpret_cast<const SeriesSet*>(h_series);
+  a.theta_stride = theta_stride;
+  a.times = times ? times : a.ser.times;
+  a.nsim = nsim;
+  a.flags = flags;
+  const int64_t S = h_model->S, T = h_series->Tmax, dy = h_model->dy;
+  hipStream_t st = (hipStream_t)stream;
+  // grid.y carries the rows: at most 65535 per launch
+  for (int32_t r0 = 0; r0 < R; r0 += 65535) {
+    a.R = R - r0 < 65535 ? R - r0 : 65535;
+    a.theta = theta + r0 * theta_stride;
+    a.keys = keys + 2 * (int64_t)r0;
+    a.X = X + (int64_t)r0 * (T + 1) * S * nsim;
+    a.Y = Y + (int64_t)r0 * T * dy * nsim;
+    MP_HIP(launch_simulate(a, st));
+  }
+  return MP_OK;
 }
 
 int32_t mp_perfilter(const mp_model_desc* h_model, const mp_series_desc* h_series, const float* thetas_in,

@@ -15,6 +15,7 @@
 //   tanh:  -t / (t + 2), t = expm1(-2|x|);
 //   cos:   Cody-Waite reduction by pi/2 (three-part constant), Taylor sin/cos
 //          polynomials by quadrant;
+//   exp (double): the same construction in float64 with a degree-13 polynomial;
 //   lgamma (double): upward shift to x >= 12 and Stirling's series with
 //          Bernoulli terms to 1/x^9 (Abramowitz & Stegun 6.1.40).
 #pragma once
@@ -219,6 +220,35 @@ MP_DM double dd_log(double x) {
   const double lm = (2.0 * s) * p;
   const double de = (double)e;
   return de * 6.93147180369123816490e-01 + (lm + de * 1.90821492927058770002e-10);
+}
+
+// exp(x) in float64: Cody-Waite reduction x = n ln2 + r (split ln2, |r| <=
+// ln2/2), Taylor polynomial to r^13/13! (truncation < 2^-55 relative), exact
+// scaling by 2^n in two factors so that subnormal results round once.
+MP_DM double dd_exp(double x) {
+#pragma clang fp contract(off)
+  if (x != x) return x;
+  if (x > 709.782712893384) return __builtin_inf();
+  if (x < -745.1332191019412) return 0.0;
+  const double n = __builtin_rint(x * 1.4426950408889634);
+  const double r = (x - n * 6.93147180369123816490e-01) - n * 1.90821492927058770002e-10;
+  double p = 1.0 / 6227020800.0;
+  p = p * r + 1.0 / 479001600.0;
+  p = p * r + 1.0 / 39916800.0;
+  p = p * r + 1.0 / 3628800.0;
+  p = p * r + 1.0 / 362880.0;
+  p = p * r + 1.0 / 40320.0;
+  p = p * r + 1.0 / 5040.0;
+  p = p * r + 1.0 / 720.0;
+  p = p * r + 1.0 / 120.0;
+  p = p * r + 1.0 / 24.0;
+  p = p * r + 1.0 / 6.0;
+  p = p * r + 0.5;
+  p = p * r + 1.0;
+  p = p * r + 1.0;
+  const int k = (int)n;
+  if (k < -1000) return __builtin_ldexp(__builtin_ldexp(p, k + 600), -600);
+  return __builtin_ldexp(p, k);
 }
 
 // log Gamma(x), x > 0.

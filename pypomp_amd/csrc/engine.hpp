@@ -151,4 +151,19 @@ struct GradArgs {
   int lds_work;    // per-step working arrays in LDS (else in the global scratch)
 };
 
+// Simulation launch (one thread per (row, simulation)); U = 1.
+constexpr int kSimMaxDy = 4;
+struct SimArgs {
+  ModelDesc md;
+  SeriesSet ser;
+  const float* theta;   // (R, d), natural scale unless F_SHOULD_TRANS
+  int64_t theta_stride;
+  const uint32_t* keys; // (R, 2)
+  const float* times;   // (T) observation times (the series' own or the caller's)
+  int R, nsim;
+  int flags;
+  float* X;             // (R, T+1, S, nsim)
+  float* Y;             // (R, T, dy, nsim)
+};
+
 }  // namespace mp

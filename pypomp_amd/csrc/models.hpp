@@ -151,6 +151,19 @@ struct LGT {
     return -0.5f * quad - p.half_n_log2pi - p.logdet;
   }
 
+  // y = C x + LR z, z = normal(key, (dy,)) (linear_gaussian.py:181-191)
+  __device__ static void rmeas(const ModelDesc&, const P& p, Key k, const float* x, float, float* y) {
+    float z[MD];
+    MP_UNROLL for (int i = 0; i < MD; ++i) z[i] = i < p.dy ? normal_at(k, i) : 0.f;
+    MP_UNROLL for (int i = 0; i < MD; ++i) if (i < p.dy) {
+      float m = p.C[i][0] * x[0];
+      MP_UNROLL for (int j = 1; j < MD; ++j) if (j < p.dx) m += p.C[i][j] * x[j];
+      float s = p.LR[i][0] * z[0];
+      MP_UNROLL for (int j = 1; j < MD; ++j) if (j < p.dy) s += p.LR[i][j] * z[j];
+      y[i] = m + s;
+    }
+  }
+
   // ---- derivatives for the MOP adjoint (natural-scale parameters) ----
   // theta offsets: A, C, LQ (lower triangle), LR (lower triangle), X0, [L0]
   __device__ static int oC(const P& p) { return p.dx * p.dx; }
@@ -324,6 +337,13 @@ struct SV {
 
   __device__ static float dmeas(const ModelDesc& md, const P& p, const float* y, const float* x, float) {
     return dmeas_a(md, p, y, x, dmeas_aux(p, x));
+  }
+
+  // r = exp(v/2) (rho eta + sqrt(1 - rho^2) e), e = normal(key, (1,))
+  __device__ static void rmeas(const ModelDesc&, const P& p, Key k, const float* x, float, float* y) {
+    const float e = normal_at(k, 0);
+    const float sd = sqrtf(1.0f - p.rho * p.rho);
+    y[0] = d_exp(0.5f * x[0]) * (p.rho * x[1] + sd * e);
   }
 
   // ---- derivatives for the MOP adjoint (natural-scale parameters) ----

@@ -122,6 +122,37 @@ __device__ __forceinline__ int fast_binomial_i(Key k, int n, float p) {
   return r != r ? -1 : (int)r;
 }
 
+// NegBin(size k, mean mu) by inversion of u: the smallest y >= 0 with
+// F(y) >= u, summing the pmf upward in float64 from pmf(0) = p^k with
+// pmf(y+1) = pmf(y) (y + k)/(y + 1) q, p = k/(k+mu), q = mu/(k+mu).  pmf and F
+// are carried in units of pmf(0) 2^(600 r) (r rescalings), so p^k may
+// underflow.  Equals scipy's nbinom.ppf except where u lies within rounding of
+// a CDF step; 0 at u = 0; NaN for k <= 0 or mu not finite.  O(y) steps, for
+// data generation only.
+__device__ inline double negbin_inverse(double u, double k, double mu) {
+#pragma clang fp contract(off)
+  if (!(k > 0.0) || !(mu >= 0.0) || mu == __builtin_inf()) return __builtin_nan("");
+  if (!(u > 0.0)) return 0.0;
+  const double p = k / (k + mu), q = mu / (k + mu);
+  double lt = dd_log(u) - k * dd_log(p);
+  double term = 1.0, cdf = 1.0, y = 0.0;
+  double target = lt > 700.0 ? __builtin_inf() : dd_exp(lt);
+  while (cdf < target) {
+    const double ratio = (y + k) / (y + 1.0);
+    term = (term * ratio) * q;
+    cdf = cdf + term;
+    y = y + 1.0;
+    if (lt > 700.0 && cdf > 0x1p600) {
+      term = term * 0x1p-600;
+      cdf = cdf * 0x1p-600;
+      lt = lt - 600 * 0.6931471805599453;
+      if (lt <= 700.0) target = dd_exp(lt);
+    }
+    if ((term == 0.0 && y > mu) || y >= 1e9) break;
+  }
+  return y;
+}
+
 struct SEIR {
   static constexpr int MAXS = 5;
   static constexpr bool FIXED_S = true;
@@ -203,6 +234,13 @@ struct SEIR {
     const double v = dd_lgamma(yy + k) - p.lgk - dd_lgamma(yy + 1.0) + k * dd_log(k / (k + mu)) +
                      yy * dd_log(mu / (k + mu));
     return (float)v;
+  }
+
+  // reports y ~ NegBin(mean max(rho H, 1e-10), size k) from one uniform
+  __device__ static void rmeas(const ModelDesc&, const P& p, Key key, const float* x, float, float* y) {
+    const double mu = fmax((double)(x[4] * p.rho), 1e-10);
+    const double u = (double)uniform01(bits_at(key, 0));
+    y[0] = (float)negbin_inverse(u, (double)p.k, mu);
   }
 };
 

@@ -120,6 +120,46 @@ def pfilter(struct, thetas_array, J, keys, thresh=0.0, CLL=False, ESS=False, fil
     return out
 
 
+def simulate(struct, thetas_array, nsim, keys, times=None, should_trans=False):
+    """Simulate states and observations (functional/simulate.py:11-78;
+    core/algorithms/simulate.py:15-142).  thetas (n, d) natural scale, keys
+    (n, 2); ``times`` (T,) replaces the observation times (the Euler grid stays
+    the struct's).  Returns X (n, nsim, T+1, S) and Y (n, nsim, T, dy), float32."""
+    import torch
+    dev = _native.require_device()
+    thetas = np.ascontiguousarray(np.asarray(thetas_array, dtype=F32))
+    if thetas.ndim == 1:
+        thetas = thetas[None]
+    keys = np.asarray(keys, dtype=np.uint32).reshape(-1, 2)
+    n = keys.shape[0]
+    if thetas.shape[0] != n:
+        raise ValueError("thetas and keys disagree on the number of parameter sets")
+    if len(struct.units) != 1:
+        raise ValueError("simulate runs single-series models")
+    if int(nsim) < 1:
+        raise ValueError("nsim must be at least 1")
+    ser = struct.device_series()
+    md = struct.model.desc()
+    T, S, dy = ser.Tmax, struct.model.S, struct.units[0].ys.shape[1]
+    tt = None
+    if times is not None:
+        times = np.asarray(times, dtype=F32).reshape(-1)
+        if times.shape[0] != T:
+            raise ValueError(f"times must have the series' length {T} (the Euler grid is the struct's)")
+        tt = torch.from_numpy(times).to(dev)
+    th = torch.from_numpy(thetas).to(dev)
+    kt = _keys_tensor(keys, dev)
+    X = torch.empty((n, T + 1, S, int(nsim)), dtype=torch.float32, device=dev)
+    Y = torch.empty((n, T, dy, int(nsim)), dtype=torch.float32, device=dev)
+    _native.check(_native.lib().mp_simulate(md, ser.desc, _native.ptr(th), struct.model.d, _native.ptr(kt),
+                                            _native.ptr(tt), n, int(nsim),
+                                            _native.MP_SHOULD_TRANS if should_trans else 0,
+                                            _native.ptr(X), _native.ptr(Y), _native.stream_ptr()))
+    Xo = X.permute(0, 3, 1, 2).contiguous().cpu().numpy()
+    Yo = Y.permute(0, 3, 1, 2).contiguous().cpu().numpy()
+    return Xo, Yo
+
+
 def mop(struct, thetas_array, J, alpha, keys, form="batched", return_float64=False):
     """MOP(alpha) objective: thetas (n, d) on the estimation scale, keys (n, 2).
     Returns the NEGATIVE log-likelihood estimates (n,), as the reference."""

@@ -116,15 +116,17 @@ def lg_theta(A=None, C=None, Q=None, R=None, X0=None, P0=None, dx=None, dy=None)
 def LG(T=4, A=None, C=None, Q=None, R=None, X0=None, P0=None, ys=None, times=None, key=None):
     """A ``Pomp`` for the linear-Gaussian model (reference constructor:
     linear_gaussian.py:337-451).  Observations: ``ys`` (T, dy) if given;
-    otherwise they are simulated with the reference's key tree from ``key``
-    (default ``key(1)``) by the device simulator."""
+    otherwise they are simulated on the device with the reference's key tree
+    from ``key`` (default ``key(1)``): ``simulate(key, nsim=1, as_pomp=True)``
+    of a model with zero observations (linear_gaussian.py:392, 425-449)."""
+    from .. import random as prandom
     from ..pomp import Pomp
     theta, dx, dy = lg_theta(A=A, C=C, Q=Q, R=R, X0=X0, P0=P0)
     model = LGModel(dx, dy, separate_init=P0 is not None)
     if times is None:
         times = np.arange(1, T + 1, dtype=float) if ys is None else np.arange(1, len(ys) + 1, dtype=float)
-    if ys is None:
-        raise NotImplementedError(
-            "LG(...) without ys needs the device simulator; pass ys= (observations) instead")
-    return Pomp(ys=np.asarray(ys, dtype=np.float32).reshape(len(times), dy), theta=theta, t0=0.0,
-                times=times, nstep=1, model=model)
+    if ys is not None:
+        return Pomp(ys=np.asarray(ys, dtype=np.float32).reshape(len(times), dy), theta=theta, t0=0.0,
+                    times=times, nstep=1, model=model)
+    tmp = Pomp(ys=np.zeros((len(times), dy), np.float32), theta=theta, t0=0.0, times=times, nstep=1, model=model)
+    return tmp.simulate(key=prandom.key(1) if key is None else key, nsim=1, as_pomp=True)

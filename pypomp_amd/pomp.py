@@ -9,7 +9,9 @@ rejected at construction.
 """
 
 import copy
+import dataclasses
 import time
+import warnings
 
 import numpy as np
 import pandas as pd
@@ -177,6 +179,38 @@ class Pomp:
             "key": old, "theta": theta_used, "traces": np.concatenate([-nll[:, :, None], tr], axis=-1),
             "optimizer": optimizer, "J": J, "M": M, "eta": eta, "alpha": alpha, "thresh": thresh,
             "alpha_cooling": alpha_cooling})
+
+    def simulate(self, key=None, theta=None, times=None, nsim=1, as_pomp=False):
+        """States and observations from the model (estimation_mixin.py:1081-1223):
+        long frames (theta_idx, sim, time, variables), or with ``as_pomp`` a copy
+        whose ys are the first parameter set's simulation."""
+        if as_pomp:
+            if nsim > 1:
+                warnings.warn("as_pomp is True, but nsim > 1. Only 1 simulation will be performed as_pomp "
+                              "overrides nsim.", UserWarning, stacklevel=2)
+            nsim = 1
+        theta_used = self.theta if theta is None else self._theta_list(theta)
+        thetas = self._theta_array(theta)
+        new, _ = self._update_fresh_key(key)
+        keys = prandom.split(new, thetas.shape[0])
+        times_arr = self.series.times.astype(float) if times is None else np.asarray(times, dtype=float)
+        X, Y = F.simulate(self.to_struct(), thetas, nsim, keys, times=times)
+
+        def long(arr, tvec, names):
+            nt, ns, nT, nf = arr.shape
+            df = pd.DataFrame(arr.reshape(nt * ns * nT, nf), columns=list(names))
+            df.insert(0, "time", np.tile(np.asarray(tvec, dtype=float), nt * ns))
+            df.insert(0, "sim", np.tile(np.repeat(np.arange(ns), nT), nt))
+            df.insert(0, "theta_idx", np.repeat(np.arange(nt), ns * nT))
+            return df
+
+        times0 = np.concatenate([[self.t0], times_arr])
+        if as_pomp:
+            out = copy.deepcopy(self)
+            out.series = dataclasses.replace(self.series, ys=np.ascontiguousarray(Y[0, 0]))
+            out.theta = [dict(theta_used[0])]
+            return out
+        return long(X, times0, self.statenames), long(Y, times_arr, self.model.obsnames)
 
     def mop(self, J, alpha, key=None, theta=None):
         """MOP(alpha) log-likelihood estimate at natural-scale theta (value only)."""

@@ -10,6 +10,7 @@
   judged outputs.
 
 Usage: python tests/golden/make_fixtures.py [--reference /root/reference]
+       python tests/golden/make_fixtures.py --only sv[,lg,seir,panel]   (workloads only)
 """
 
 import argparse
@@ -76,35 +77,39 @@ def lg_data(dx, T, seed, P0=None, Q=None, R=None):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--reference", default="/root/reference")
-    ap.add_argument("--only-lg", action="store_true", help="refresh the LG series only")
+    ap.add_argument("--only", default="", help="comma list of workloads to refresh (lg, seir, sv, panel); "
+                    "keeps the others and the regression values")
     args = ap.parse_args()
-    (HERE / "reference_regression.json").write_text(json.dumps(regression(args.reference), indent=1))
-
+    only = set(filter(None, args.only.split(",")))
     arrays = {}
-    if args.only_lg and (HERE / "workloads.npz").exists():
+    if only:
         old = np.load(HERE / "workloads.npz")
         arrays = {k: old[k] for k in old.files}
-    # LG 2-D, Q = R = 0.01 I, x0 ~ N(0, I): seed 1 T=50 (correctness workload),
-    # seed 2 T=200 (throughput workload)
-    for seed, T in ((1, 50), (2, 200)):
-        ys, theta = lg_data(2, T, seed, P0=np.eye(2), Q=0.01 * np.eye(2), R=0.01 * np.eye(2))
-        arrays[f"lg2_seed{seed}_ys"] = ys
-        arrays[f"lg2_seed{seed}_theta"] = theta
-    if args.only_lg:
-        np.savez_compressed(HERE / "workloads.npz", **arrays)
-        return
-    # SEIR, seed 3, T = 260 weeks, 7 Euler steps per week
-    ys, theta = seir.simulate_data(seed=3, T=260)
-    arrays["seir_seed3_ys"], arrays["seir_seed3_theta"] = ys, theta
-    # SV, seed 4, T = 1000
-    ys, theta = sv.simulate_data(seed=4, T=1000)
-    arrays["sv_seed4_ys"], arrays["sv_seed4_theta"] = ys, theta
-    # SEIR panel, seed 5: 100 units, beta_u ~ LogNormal(log 1.5, 0.2), T_u ~ U{52..520}
-    panel = seir.simulate_panel(seed=5, U=100)
-    for k, v in panel.items():
-        arrays[f"panel_{k}"] = v
+    else:
+        (HERE / "reference_regression.json").write_text(json.dumps(regression(args.reference), indent=1))
+    want = (lambda name: not only or name in only)
+    if want("lg"):
+        # LG 2-D, Q = R = 0.01 I, x0 ~ N(0, I): seed 1 T=50 (correctness
+        # workload), seed 2 T=200 (throughput workload)
+        for seed, T in ((1, 50), (2, 200)):
+            ys, theta = lg_data(2, T, seed, P0=np.eye(2), Q=0.01 * np.eye(2), R=0.01 * np.eye(2))
+            arrays[f"lg2_seed{seed}_ys"] = ys
+            arrays[f"lg2_seed{seed}_theta"] = theta
+    if want("seir"):
+        # SEIR, seed 3, T = 260 weeks, 7 Euler steps per week
+        ys, theta = seir.simulate_data(seed=3, T=260)
+        arrays["seir_seed3_ys"], arrays["seir_seed3_theta"] = ys, theta
+    if want("sv"):
+        # SV, seed 4, T = 1000
+        ys, theta = sv.simulate_data(seed=4, T=1000)
+        arrays["sv_seed4_ys"], arrays["sv_seed4_theta"] = ys, theta
+    if want("panel"):
+        # SEIR panel, seed 5: 100 units, beta_u ~ LogNormal(log 1.5, 0.2), T_u ~ U{52..520}
+        panel = seir.simulate_panel(seed=5, U=100)
+        for k, v in panel.items():
+            arrays[f"panel_{k}"] = v
     np.savez_compressed(HERE / "workloads.npz", **arrays)
-    print("wrote", HERE / "reference_regression.json", HERE / "workloads.npz")
+    print("wrote", HERE / "workloads.npz")
 
 
 if __name__ == "__main__":

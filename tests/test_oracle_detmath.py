@@ -73,3 +73,17 @@ def test_double_log_and_lgamma():
     # factorials are exact
     for n in range(1, 25):
         assert abs(float(dm.dd_lgamma(np.array([n + 1.0]))[0]) - math.lgamma(n + 1)) <= 1e-13 * max(1, math.lgamma(n + 1))
+
+
+def test_double_exp():
+    rng = np.random.default_rng(6)
+    x = np.concatenate([rng.uniform(-745, 709.7, 40000), rng.uniform(-1, 1, 10000), np.arange(-50, 51.0)])
+    got = dm.dd_exp(x)
+    ref = np.exp(x)
+    normal = ref > 2.2250738585072014e-308
+    assert np.max(np.abs(got[normal] / ref[normal] - 1)) <= 4 * np.finfo(np.float64).eps
+    assert np.max(np.abs(got[~normal] - ref[~normal])) <= 4 * 5e-324
+    assert dm.dd_exp(np.array([0.0]))[0] == 1.0
+    assert abs(dm.dd_exp(np.array([1.0]))[0] - math.e) <= 2 * np.spacing(math.e)
+    edge = dm.dd_exp(np.array([800.0, -800.0, np.inf, -np.inf, np.nan]))
+    assert edge[0] == np.inf and edge[1] == 0.0 and edge[2] == np.inf and edge[3] == 0.0 and np.isnan(edge[4])

@@ -104,3 +104,15 @@ def test_first_substep_transition_means():
         p = 1 - np.exp(-rate * dt)
         se = np.sqrt(n * p * (1 - p) / d.size)
         assert abs(d.astype(np.float64).mean() - n * p) < 5 * se, (n, p, d.mean())
+
+
+def test_negbin_inverse_summation_matches_scipy_ppf():
+    # the device's NegBin sampler (upward CDF summation with rescaling) against
+    # scipy's nbinom.ppf, including p^k below the float64 range (k = 200, mu = 1e4)
+    rng = np.random.default_rng(12)
+    for k, mu in ((10.0, 0.3), (10.0, 37.5), (10.0, 4200.0), (0.7, 15.0), (200.0, 1e4), (1e3, 2.0), (10.0, 1e-10)):
+        u = np.concatenate([rng.uniform(0, 1, 3000).astype(np.float32).astype(np.float64), [1 - 2.0 ** -24, 2.0 ** -24]])
+        got = seir.negbin_inverse_sum(u, k, mu)
+        ref = stats.nbinom.ppf(u, k, k / (k + mu))
+        assert np.array_equal(got, ref), (k, mu, np.flatnonzero(got != ref)[:5])
+    assert seir.negbin_inverse_sum(np.array([0.0]), 10.0, 5.0)[0] == 0.0
