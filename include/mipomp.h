@@ -150,6 +150,14 @@ int32_t mp_mop_grad(const mp_model_desc* h_model, const mp_series_desc* h_series
                     int64_t theta_stride, const uint32_t* keys, int32_t R, int32_t J, float alpha, int32_t K,
                     int32_t form, double* loglik, float* grad, void* workspace, size_t ws_bytes, void* stream);
 
+/* The model's estimation-to-natural transform (from_est) applied on the host
+ * to n rows of d parameters in place, with the device's arithmetic (the same
+ * deterministic exp/tanh/logistic), so host-side traces and priors see the
+ * values the kernels see.  The reference applies ParTrans.from_est to
+ * parameter arrays on the host (core/par_trans.py:70-111).  No-op for
+ * identity_trans. */
+int32_t mp_host_from_est(const mp_model_desc* h_model, float* h_theta, int64_t n);
+
 /* Simulation of states and observations: _simulate_internal /
  * _simulate_helper (core/algorithms/simulate.py:15-142) vmapped over R
  * parameter rows (simulate.py:71-74; functional/simulate.py:11-78), with the

@@ -148,6 +148,10 @@ class LGModel:
         self.S = spec.dx
         self.accumvars = ()
 
+    @property
+    def param_names(self):
+        return self.spec.param_names
+
     def from_est(self, theta):
         return self.spec.from_est(theta)
 

@@ -5,10 +5,11 @@ filter run as hand-written HIP kernels for gfx950 in ``_lib/libmipomp.so``
 (C ABI: include/mipomp.h), behind the reference's ``Pomp`` / functional API.
 """
 
-from . import autotune, functional, models, random
+from . import autotune, functional, models, proposals, random
 from .maths import logmeanexp, logmeanexp_se
 from .models import ParTrans
 from .optim import SGD, Adam, LearningRate, Optimizer
+from .proposals import MVNDiagRW, MVNRWAdaptive, MVNRWFull
 from .panel import PanelParameters, PanelPomp
 from .pomp import Pomp
 from .rw_sigma import RWSigma
@@ -16,4 +17,4 @@ from .rw_sigma import RWSigma
 functional_api = functional
 F = functional
 
-__all__ = ["Pomp", "PanelPomp", "PanelParameters", "ParTrans", "RWSigma", "Adam", "SGD", "Optimizer", "LearningRate", "autotune", "functional", "F", "models", "random", "logmeanexp", "logmeanexp_se"]
+__all__ = ["Pomp", "PanelPomp", "PanelParameters", "ParTrans", "RWSigma", "Adam", "SGD", "Optimizer", "LearningRate", "MVNDiagRW", "MVNRWFull", "MVNRWAdaptive", "proposals", "autotune", "functional", "F", "models", "random", "logmeanexp", "logmeanexp_se"]

@@ -10,6 +10,7 @@
 
 #include "../../include/mipomp.h"
 #include "engine.hpp"
+#include "seir.hpp"
 
 namespace mp {
 hipError_t launch_bits(const uint32_t* keys, int64_t nkeys, int64_t n, void* out, int mode, hipStream_t s);
@@ -392,6 +393,18 @@ int run_mop_grad(const mp_model_desc* m, const mp_series_desc* s, const float* t
   return MP_OK;
 }
 
+// the model's from_est run on the host CPU (detmath is host-and-device code)
+template <class M>
+void host_from_est_rows(const ModelDesc& md, float* theta, int64_t n) {
+  for (int64_t r = 0; r < n; ++r) {
+    float th[M::MAXP];
+    float* row = theta + r * md.d;
+    for (int p = 0; p < md.d; ++p) th[p] = row[p];
+    M::from_est(md, th);
+    for (int p = 0; p < md.d; ++p) row[p] = th[p];
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -568,6 +581,20 @@ int32_t mp_mop_grad(const mp_model_desc* h_model, const mp_series_desc* h_series
   }
   return run_mop_grad(h_model, h_series, theta, theta_stride, keys, R, J, alpha, K, loglik, grad, workspace,
                       ws_bytes, st);
+}
+
+int32_t mp_host_from_est(const mp_model_desc* h_model, float* h_theta, int64_t n) {
+  int rc = check_model(h_model);
+  if (rc) return rc;
+  if (n < 0 || (n > 0 && !h_theta)) return fail(MP_ERR_ARG, "bad theta rows (n = %lld)", (long long)n);
+  const ModelDesc md = *reinterpret_cast<const ModelDesc*>(h_model);
+  if (md.identity_trans) return MP_OK;
+  switch (md.kind) {
+    case MODEL_LG: host_from_est_rows<LG4>(md, h_theta, n); break;
+    case MODEL_SV: host_from_est_rows<SV>(md, h_theta, n); break;
+    case MODEL_SEIR: host_from_est_rows<SEIR>(md, h_theta, n); break;
+  }
+  return MP_OK;
 }
 
 int32_t mp_simulate(const mp_model_desc* h_model, const mp_series_desc* h_series, const float* theta,

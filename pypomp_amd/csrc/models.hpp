@@ -54,7 +54,7 @@ struct LGT {
     int dx, dy;
   };
 
-  __device__ static void from_est(const ModelDesc& md, float* th) {
+  __host__ __device__ static void from_est(const ModelDesc& md, float* th) {
     const int dx = EXACT ? MD : md.dx, dy = EXACT ? MD : md.dy;
     // diagonal entries of the Q and R triangles
     MP_UNROLL for (int i = 0; i < MD; ++i) if (i < dx) {
@@ -295,7 +295,7 @@ struct SV {
     float inv1mr2;  // 1 / (1 - rho^2)
   };
 
-  __device__ static void from_est(const ModelDesc&, float* th) {
+  __host__ __device__ static void from_est(const ModelDesc&, float* th) {
     th[1] = d_tanh(th[1]);
     th[2] = d_exp(th[2]);
     th[3] = d_tanh(th[3]);

@@ -164,9 +164,9 @@ struct SEIR {
     double lgk;
   };
 
-  __device__ static float logit_inv(float x) { return 1.0f / (1.0f + d_exp(-x)); }
+  __host__ __device__ static float logit_inv(float x) { return 1.0f / (1.0f + d_exp(-x)); }
 
-  __device__ static void from_est(const ModelDesc&, float* th) {
+  __host__ __device__ static void from_est(const ModelDesc&, float* th) {
     th[0] = d_exp(th[0]);
     th[1] = logit_inv(th[1]);
     th[2] = d_exp(th[2]);

@@ -160,6 +160,105 @@ def simulate(struct, thetas_array, nsim, keys, times=None, should_trans=False):
     return Xo, Yo
 
 
+def _est_prior(model, dprior):
+    """(estimation-scale log prior with the change-of-variables term, natural
+    log prior) for a user prior ``dprior(theta: dict) -> float`` on the natural
+    scale, or a flat prior (core/model_mechanics.py:799-867;
+    functional/structs.py:271-295)."""
+    names = list(model.param_names)
+
+    def natural(th_nat):
+        if dprior is None:
+            return F32(0.0)
+        return F32(dprior({n: F32(v) for n, v in zip(names, th_nat)}))
+
+    def est(th_est):
+        return F32(natural(model.from_est(th_est)) + model.log_jac_from_est(th_est))
+
+    return est, natural
+
+
+def pmcmc(struct, thetas_array, proposal, M, J, keys, thresh=0.0, dprior=None):
+    """Particle marginal Metropolis-Hastings (functional/pmcmc.py:15-112;
+    core/algorithms/pmcmc.py:30-163): one chain per row of thetas (n, d,
+    natural scale), keys (n, 2).  The chain runs on the estimation scale; each
+    step draws (key, prop, pf, accept) = split(key, 4), proposes, evaluates the
+    prior and a bootstrap particle filter at the proposal and accepts when
+    log(u) < ll' + lp' - ll - lp with both finite.  The n filters of a step run
+    in one launch; the proposal normals and accept uniforms of all steps are
+    drawn on the device up front.  Returns (loglik (n, M+1), log_prior (n,
+    M+1) natural scale, theta (n, M+1, d) natural scale, accepts (n,))."""
+    import torch
+    dev = _native.require_device()
+    model = struct.model
+    if J < 1:
+        raise ValueError("J should be greater than 0.")
+    th_nat0 = np.ascontiguousarray(np.asarray(thetas_array, dtype=F32))
+    if th_nat0.ndim == 1:
+        th_nat0 = th_nat0[None]
+    keys = np.asarray(keys, dtype=np.uint32).reshape(-1, 2)
+    n, d = th_nat0.shape
+    if keys.shape[0] != n:
+        raise ValueError("thetas and keys disagree on the number of chains")
+    M = int(M)
+    proposal = proposal.canonicalize(model.param_names)
+    lp_est_fn, lp_nat_fn = _est_prior(model, dprior)
+    # key tree of every chain: (key, init_pf) = split(key); per step split(key, 4)
+    init_pf = np.zeros((n, 2), np.uint32)
+    step_keys = np.zeros((M, 3, n, 2), np.uint32)  # prop, pf, accept
+    for c in range(n):
+        k2 = prandom.split(keys[c], 2)
+        k, init_pf[c] = k2[0], k2[1]
+        for m in range(M):
+            ks = prandom.split(k, 4)
+            k = ks[0]
+            step_keys[m, :, c] = ks[1:]
+    if M > 0:
+        z = prandom.normal(step_keys[:, 0].reshape(-1, 2), d).cpu().numpy().reshape(M, n, d)
+        u = prandom.uniform(step_keys[:, 2].reshape(-1, 2), 1).cpu().numpy().reshape(M, n)
+    ser = struct.device_series()
+    md = model.desc()
+    ws, nws = _workspace(_native.MP_ALGO_PF, md, ser, n, J, _native.MP_FORM_BATCHED, dev)
+    th_dev = torch.empty((n, d), dtype=torch.float32, device=dev)
+    pf_keys = torch.from_numpy(np.concatenate([init_pf[None], step_keys[:, 1]]).view(np.int32)).to(dev)
+    ll_dev = torch.empty(n, dtype=torch.float64, device=dev)
+    thresh = float(max(0.0, thresh))
+
+    def loglik(theta_est, m):
+        th_dev.copy_(torch.from_numpy(np.ascontiguousarray(theta_est, dtype=F32)))
+        _native.check(_native.lib().mp_pfilter(
+            md, ser.desc, _native.ptr(th_dev), d, _native.ptr(pf_keys[m]), None, n, J, thresh,
+            _native.MP_SHOULD_TRANS, _native.MP_FORM_BATCHED, _native.ptr(ll_dev), None, None, None, None,
+            _native.ptr(ws), nws, _native.stream_ptr()))
+        return ll_dev.cpu().numpy().astype(F32)
+
+    cur = model.to_est(th_nat0).astype(F32)
+    ll_cur = loglik(cur, 0)
+    lp_cur = np.array([lp_est_fn(cur[c]) for c in range(n)], dtype=F32)
+    states = [proposal.init_state(cur[c]) for c in range(n)]
+    accepts = np.zeros(n, np.int32)
+    ll_tr = np.empty((n, M + 1), F32)
+    est_tr = np.empty((n, M + 1, d), F32)
+    ll_tr[:, 0], est_tr[:, 0] = ll_cur, cur
+    for m in range(M):
+        prop = np.empty_like(cur)
+        for c in range(n):
+            prop[c], states[c] = proposal.step(states[c], cur[c], z[m, c], m + 1, int(accepts[c]))
+        lp_prop = np.array([lp_est_fn(prop[c]) for c in range(n)], dtype=F32)
+        ll_prop = loglik(prop, m + 1)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            log_alpha = (((ll_prop + lp_prop).astype(F32) - ll_cur).astype(F32) - lp_cur).astype(F32)
+            acc = np.isfinite(log_alpha) & np.isfinite(lp_prop) & (np.log(u[m]) < log_alpha)
+        cur = np.where(acc[:, None], prop, cur)
+        ll_cur = np.where(acc, ll_prop, ll_cur)
+        lp_cur = np.where(acc, lp_prop, lp_cur)
+        accepts += acc.astype(np.int32)
+        ll_tr[:, m + 1], est_tr[:, m + 1] = ll_cur, cur
+    nat_tr = model.from_est(est_tr.reshape(-1, d)).reshape(n, M + 1, d)
+    lp_tr = np.array([[lp_nat_fn(nat_tr[c, m]) for m in range(M + 1)] for c in range(n)], dtype=F32)
+    return ll_tr, lp_tr, nat_tr, accepts
+
+
 def mop(struct, thetas_array, J, alpha, keys, form="batched", return_float64=False):
     """MOP(alpha) objective: thetas (n, d) on the estimation scale, keys (n, 2).
     Returns the NEGATIVE log-likelihood estimates (n,), as the reference."""

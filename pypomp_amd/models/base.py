@@ -74,14 +74,43 @@ class NativeModel:
         return th.copy() if self.identity_trans else self._to_est(th)
 
     def from_est(self, theta):
-        th = np.asarray(theta, dtype=F32)
-        return th.copy() if self.identity_trans else self._from_est(th)
+        """Estimation -> natural scale with the device's own arithmetic
+        (``mp_host_from_est``), bit-identical to the kernels' transform."""
+        th = np.array(theta, dtype=F32, copy=True, order="C")
+        if self.identity_trans or th.size == 0:
+            return th
+        import ctypes as C
+        flat = th.reshape(-1, self.d)
+        _native.check(_native.lib().mp_host_from_est(self.desc(), flat.ctypes.data_as(C.c_void_p), flat.shape[0]))
+        return flat.reshape(th.shape)
 
     def _to_est(self, theta):
         return np.asarray(theta, dtype=F32).copy()
 
-    def _from_est(self, theta):
-        return np.asarray(theta, dtype=F32).copy()
+    def log_jac_from_est(self, theta_est):
+        """log |det d from_est / d theta_est| over the last axis, float32: the
+        change-of-variables term the reference adds to an estimation-scale
+        prior (core/model_mechanics.py:841-862).  The transforms act
+        elementwise, so the Jacobian is diagonal."""
+        th = np.asarray(theta_est, dtype=np.float64)
+        if self.identity_trans:
+            return np.zeros(th.shape[:-1], dtype=F32)
+        return self._log_jac(th).astype(F32)
+
+    def _log_jac(self, th):
+        return np.zeros(th.shape[:-1])
+
+    @staticmethod
+    def _log_dtanh(x):
+        # log(1 - tanh(x)^2) = 2 (log 2 - |x| - log1p(exp(-2|x|)))
+        a = np.abs(x)
+        return 2.0 * (np.log(2.0) - a - np.log1p(np.exp(-2.0 * a)))
+
+    @staticmethod
+    def _log_dlogistic(x):
+        # log(s (1 - s)), s = 1 / (1 + exp(-x))
+        a = np.abs(x)
+        return -a - 2.0 * np.log1p(np.exp(-a))
 
     def theta_array(self, theta):
         """dict (or sequence of dicts) of natural-scale values -> (n, d) float32."""

@@ -47,11 +47,8 @@ class LGModel(NativeModel):
         th[..., m] = np.log(th[..., m])
         return th
 
-    def _from_est(self, theta):
-        th = np.array(theta, dtype=F32, copy=True)
-        m = self._diag_mask()
-        th[..., m] = np.exp(th[..., m])
-        return th
+    def _log_jac(self, th):
+        return th[..., self._diag_mask()].sum(axis=-1)
 
 
 def _default_A(dx):

@@ -35,8 +35,5 @@ class SEIRModel(NativeModel):
         th[..., _LOGIT] = np.log(p / (F32(1) - p))
         return th.astype(F32)
 
-    def _from_est(self, theta):
-        th = np.array(theta, dtype=F32, copy=True)
-        th[..., _LOG] = np.exp(th[..., _LOG])
-        th[..., _LOGIT] = F32(1) / (F32(1) + np.exp(-th[..., _LOGIT]))
-        return th.astype(F32)
+    def _log_jac(self, th):
+        return th[..., _LOG].sum(axis=-1) + self._log_dlogistic(th[..., _LOGIT]).sum(axis=-1)

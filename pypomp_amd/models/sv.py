@@ -31,10 +31,5 @@ class SVModel(NativeModel):
         th[..., 5] = np.log(th[..., 5])
         return th
 
-    def _from_est(self, theta):
-        th = np.array(theta, dtype=F32, copy=True)
-        th[..., 1] = np.tanh(th[..., 1])
-        th[..., 2] = np.exp(th[..., 2])
-        th[..., 3] = np.tanh(th[..., 3])
-        th[..., 5] = np.exp(th[..., 5])
-        return th
+    def _log_jac(self, th):
+        return self._log_dtanh(th[..., 1]) + th[..., 2] + self._log_dtanh(th[..., 3]) + th[..., 5]

@@ -212,6 +212,28 @@ class Pomp:
             return out
         return long(X, times0, self.statenames), long(Y, times_arr, self.model.obsnames)
 
+    def pmcmc(self, J, M, proposal, dprior=None, key=None, theta=None, thresh=0.0, track_time=True):
+        """Particle marginal Metropolis-Hastings, one chain per parameter set
+        (estimation_mixin.py:823-943).  ``dprior(theta: dict) -> float`` is the
+        log prior on the natural scale (flat when None)."""
+        start = time.time()
+        if J < 1:
+            raise ValueError("J should be greater than 0.")
+        thetas = self._theta_array(theta)
+        theta_used = self.theta if theta is None else self._theta_list(theta)
+        new, old = self._update_fresh_key(key)
+        n = thetas.shape[0]
+        keys = prandom.split(new, n)
+        ll, lp, th, accepts = F.pmcmc(self.to_struct(), thetas, proposal, M, J, keys, thresh, dprior)
+        names = self.canonical_param_names
+        self.theta = [dict(zip(names, map(float, th[i, -1]))) for i in range(n)]
+        self.results_history.append({
+            "method": "pmcmc", "execution_time": time.time() - start if track_time else None,
+            "key": old, "theta": theta_used,
+            "traces": np.concatenate([ll[:, :, None], lp[:, :, None], th], axis=-1),
+            "trace_vars": ["logLik", "log_prior"] + names, "accepts": accepts,
+            "acceptance_rate": accepts / max(M, 1), "J": J, "M": M, "thresh": thresh})
+
     def mop(self, J, alpha, key=None, theta=None):
         """MOP(alpha) log-likelihood estimate at natural-scale theta (value only)."""
         thetas = self._theta_array(theta)
@@ -230,6 +252,16 @@ class Pomp:
                   else np.full_like(ll, np.nan))
             df = pd.DataFrame({"theta_idx": np.arange(len(ll)), "logLik": ll, "se": se})
             return pd.concat([df, pd.DataFrame(r["theta"])[names]], axis=1)
+        if r["method"] == "pmcmc":
+            # the whole chain (render.py:421-431)
+            tr = r["traces"]
+            n, M1, _ = tr.shape
+            df = pd.DataFrame(tr.reshape(n * M1, -1), columns=r["trace_vars"])
+            df.insert(0, "iteration", np.tile(np.arange(M1), n))
+            df.insert(0, "theta_idx", np.repeat(np.arange(n), M1))
+            if ignore_nan:
+                df = df.dropna()
+            return df
         tr = r["traces"][:, -1, :]
         df = pd.DataFrame(tr, columns=["logLik"] + names)
         df.insert(0, "theta_idx", np.arange(tr.shape[0]))
@@ -239,11 +271,16 @@ class Pomp:
     def traces(self):
         frames = []
         names = self.canonical_param_names
-        for r in self.results_history:
+        for i, r in enumerate(self.results_history):
             if r["method"] == "pfilter":
-                df = self.results(self.results_history.index(r))
+                df = self.results(i)
                 df.insert(1, "iteration", 0)
                 df.insert(2, "method", "pfilter")
+            elif r["method"] == "pmcmc":
+                # render.py:434-443
+                df = self.results(i)
+                df.insert(2, "method", "pmcmc")
+                df.insert(4, "se", np.nan)
             else:
                 tr = r["traces"]
                 n, M1, _ = tr.shape
