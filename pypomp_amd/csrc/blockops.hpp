@@ -37,15 +37,36 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
   return v;
 }
 
-// Inclusive prefix sum across the 64 lanes of a wave.
+// v from the lane K places lower in the same 16-lane row, 0 for the first K
+// lanes of the row (DPP row_shr:K; no LDS traffic).
+template <int K>
+__device__ __forceinline__ uint64_t row_shr_u64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_update_dpp(0u, (uint32_t)v, 0x110 + K, 0xf, 0xf, false);
+  const uint32_t hi = __builtin_amdgcn_update_dpp(0u, (uint32_t)(v >> 32), 0x110 + K, 0xf, 0xf, false);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t lane_u64(uint64_t v, int lane) {
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, lane);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), lane);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Inclusive prefix sum across the 64 lanes of a wave: Hillis-Steele within
+// each 16-lane row by DPP row shifts, then the row totals (read from lanes 15,
+// 31, 47) added to the later rows.  Integer sums: exact in any order.  All 64
+// lanes must be active.
 __device__ __forceinline__ uint64_t wave_incl_scan_u64(uint64_t v) {
   const int lane = threadIdx.x & (WAVE - 1);
-#pragma unroll
-  for (int o = 1; o < WAVE; o <<= 1) {
-    uint64_t t = __shfl_up(v, o, WAVE);
-    if (lane >= o) v += t;
-  }
-  return v;
+  v += row_shr_u64<1>(v);
+  v += row_shr_u64<2>(v);
+  v += row_shr_u64<4>(v);
+  v += row_shr_u64<8>(v);
+  const uint64_t r0 = lane_u64(v, 15);
+  const uint64_t r01 = r0 + lane_u64(v, 31);
+  const uint64_t r012 = r01 + lane_u64(v, 47);
+  const int row = lane >> 4;
+  return v + (row == 0 ? 0ull : row == 1 ? r0 : row == 2 ? r01 : r012);
 }
 
 // First index in [lo, hi) whose entry exceeds t (C non-decreasing), found by one

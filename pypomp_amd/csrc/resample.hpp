@@ -56,10 +56,23 @@ __device__ inline uint64_t block_fixed_scan(const float* lw, uint64_t* C, int J,
   return total;
 }
 
+// #{i < n : C[i] <= t} for non-decreasing C and n >= 1: a branch-free
+// search whose trip count depends on n only, so every lane of a wave runs the
+// same ceil(log2 n) probes and the loop counter stays scalar.
+__device__ __forceinline__ int upper_bound_fixed(const uint64_t* C, int n, uint64_t t) {
+  int base = 0;
+  while (n > 1) {
+    const int half = n >> 1;
+    base = C[base + half] <= t ? base + half : base;
+    n -= half;
+  }
+  return base + (C[base] <= t ? 1 : 0);
+}
+
 __device__ __forceinline__ int ancestor_of(const uint64_t* C, int J, float u, uint64_t total, int j) {
-  uint64_t tj = comb_threshold(u, j, J, total);
-  int64_t a = upper_bound_u64(C, 0, J, tj);
-  return a < J ? (int)a : J - 1;
+  const uint64_t tj = comb_threshold(u, j, J, total);
+  const int a = upper_bound_fixed(C, J, tj);
+  return a < J ? a : J - 1;
 }
 
 }  // namespace mp

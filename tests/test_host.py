@@ -111,3 +111,20 @@ def test_host_from_est_equals_oracle_transform():
         x = rng.normal(scale=2.0, size=(64, hm.d)).astype(np.float32)
         np.testing.assert_array_equal(hm.from_est(x), om.from_est(x))
         assert hm.from_est(x[0]).shape == (hm.d,)
+
+
+def test_pomp_pickles_with_its_key_and_history():
+    # the reference pickles a Pomp with its key and results (core/pomp.py:615-826);
+    # native models need no function bytes
+    import pickle
+    from pypomp_amd import random as prandom
+    from pypomp_amd.models.lg import LG
+    P = LG(A=np.array([[0.9]]), ys=np.arange(4, dtype=np.float32)[:, None] / 4)
+    P.fresh_key = prandom.key(9)
+    P.results_history.append({"method": "pfilter", "logLiks": np.zeros((1, 2)), "theta": P.theta})
+    Q = pickle.loads(pickle.dumps(P))
+    np.testing.assert_array_equal(Q.ys.to_numpy(), P.ys.to_numpy())
+    np.testing.assert_array_equal(Q.fresh_key, P.fresh_key)
+    assert Q.theta == P.theta and Q.canonical_param_names == P.canonical_param_names
+    assert Q.results_history[0]["method"] == "pfilter"
+    assert bytes(Q.model.desc()) == bytes(P.model.desc())
