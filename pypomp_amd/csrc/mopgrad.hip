@@ -22,10 +22,11 @@
 //
 // Memory: the forward keeps every K-th pre-step state, the ancestry of every
 // step (4 bytes per particle-step) and each step's density maximum and exp-sum;
-// the reverse sweep recomputes each K-step block's states and densities from
-// its checkpoint with the logged ancestry (no reductions, scans or searches)
-// and runs the block backwards.  States storage O((T/K + K) J S) instead of
-// O(T J S).  The model's step VJP reads its noise from the stored output state.
+// the reverse sweep recomputes each K-step block's states from its checkpoint
+// with the logged ancestry (no reductions, scans or searches) and runs the
+// block backwards, evaluating each density where its adjoint needs it.  States
+// storage O((T/K + K) J S) instead of O(T J S).  The model's step VJP reads its
+// noise from the stored output state.
 #include "blockops.hpp"
 #include "engine.hpp"
 #include "resample.hpp"
@@ -36,16 +37,16 @@ namespace mp {
 // Per-filter global scratch: checkpoints, the ancestry log, per-step
 // statistics and the recomputed K-step block.
 struct GradScratch {
-  int64_t ckx, anc, stat, bx, bm, work, total;
+  int64_t ckx, anc, stat, bx, work, total;
 };
 
 // Per-step working arrays.  The forward set (prefix sums, log-weights, two
 // state buffers) and the reverse set (children's contributions, carried
-// adjoints, run starts) are never live together, so they share one region,
+// adjoints, the step's ancestry) are never live together, so they share one region,
 // kept in LDS when it fits and in the global scratch otherwise.
 struct GradWork {
   int64_t c, lw, x0, x1;                 // forward
-  int64_t cx, cl, xca, lama, lb;         // reverse
+  int64_t cx, cl, xca, lama, ap;         // reverse
   int64_t total;
 };
 
@@ -64,7 +65,7 @@ __host__ __device__ inline GradWork grad_work_layout(int S, int J) {
   w.cl = o;   o = align16(o + (int64_t)J * 4);
   w.xca = o;  o = align16(o + (int64_t)S * J * 4);
   w.lama = o; o = align16(o + (int64_t)J * 4);
-  w.lb = o;   o = align16(o + (int64_t)(J + 1) * 4);
+  w.ap = o;   o = align16(o + (int64_t)J * 4);
   w.total = fwd > o ? fwd : o;
   return w;
 }
@@ -82,7 +83,6 @@ __host__ __device__ inline GradScratch grad_scratch_layout(int S, int J, int T, 
   g.anc = take((int64_t)T * J * 4);
   g.stat = take((int64_t)T * 16);
   g.bx = take((int64_t)K * S * J * 4);
-  g.bm = take((int64_t)K * J * 4);
   g.work = take(lds_work ? 0 : grad_work_layout(S, J).total);
   g.total = o;
   return g;
@@ -91,9 +91,8 @@ __host__ __device__ inline GradScratch grad_scratch_layout(int S, int J, int T, 
 // Forward pass (value): the ancestry a_t of every step (a_0 = identity), the
 // maximum and FP64 exp-sum of each step's measurement densities, and the
 // pre-step states at every K-th step.  Reverse sweep, block by block: the
-// block's states and densities are recomputed from its checkpoint with the
-// logged ancestry (one launch-free pass: gather, step, density), then the
-// adjoint runs backwards through the block.
+// block's states are recomputed from its checkpoint with the logged ancestry
+// (gather and step), then the adjoint runs backwards through the block.
 template <class M>
 __device__ void mop_grad_filter(const GradArgs& a, const int f, char* buf, char* wbuf, double* red, uint64_t* tab) {
   const ModelDesc md = a.md;
@@ -105,14 +104,13 @@ __device__ void mop_grad_filter(const GradArgs& a, const int f, char* buf, char*
   int32_t* ancl = reinterpret_cast<int32_t*>(buf + L.anc);
   double* stat = reinterpret_cast<double*>(buf + L.stat);
   float* bx = reinterpret_cast<float*>(buf + L.bx);
-  float* bm = reinterpret_cast<float*>(buf + L.bm);
   uint64_t* Cc = reinterpret_cast<uint64_t*>(wbuf + W.c);
   float* lw = reinterpret_cast<float*>(wbuf + W.lw);
   float* cx = reinterpret_cast<float*>(wbuf + W.cx);
   float* cl = reinterpret_cast<float*>(wbuf + W.cl);
   float* xca = reinterpret_cast<float*>(wbuf + W.xca);
   float* lama = reinterpret_cast<float*>(wbuf + W.lama);
-  int32_t* lb = reinterpret_cast<int32_t*>(wbuf + W.lb);
+  int32_t* ap = reinterpret_cast<int32_t*>(wbuf + W.ap);  // this step's ancestry
   const int64_t SJ = (int64_t)S * J;
   const float* ys = a.ser.ys;
   const float* times = a.ser.times;
@@ -201,23 +199,21 @@ __device__ void mop_grad_filter(const GradArgs& a, const int f, char* buf, char*
     const int t0 = kb * K, n = min(T, t0 + K) - t0;
     const float* ck = ckx + (int64_t)kb * SJ;
     auto state = [&](int i) -> float* { return i == 0 ? const_cast<float*>(ck) : bx + (int64_t)(i - 1) * SJ; };
-    // recompute the block's states and densities from its checkpoint
+    // recompute the block's states from its checkpoint (the densities are
+    // evaluated where the backward step needs them)
     for (int i = 0; i < n; ++i) {
       const int t = t0 + i;
       const Key sk = step_key(t);
       const float dt = dts[tidx0[t]];
-      const float ts = tstart(t), tt = times[t];
-      const float* y = ys + (int64_t)t * md.dy;
+      const float ts = tstart(t);
       const float* xin = state(i);
       float* xout = state(i + 1);
       const int32_t* anct = ancl + (int64_t)t * J;
-      float* bmi = bm + (int64_t)i * J;
       for (int j = tid; j < J; j += B) {
         const int src = anct[j];
         float x[M::MAXS];
         MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) x[s] = xin[s * (int64_t)J + src];
         M::step(md, P, split_at(sk, (uint64_t)j), ts, dt, x);
-        bmi[j] = M::dmeas(md, P, y, x, tt);
         MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) xout[s * (int64_t)J + j] = x[s];
       }
       __syncthreads();
@@ -231,19 +227,28 @@ __device__ void mop_grad_filter(const GradArgs& a, const int f, char* buf, char*
       const float* y = ys + (int64_t)t * md.dy;
       const float* xin = state(i);
       const float* xt = state(i + 1);
-      const float* mt = bm + (int64_t)i * J;
       const int32_t* anct = ancl + (int64_t)t * J;
       const float lmax = (float)stat[2 * t];
       const float inv_se = (float)(1.0 / stat[2 * t + 1]);
       for (int j = tid; j < J; j += B) {
-        const float mj = mt[j];
-        const float pi = (lmax == -__builtin_inff() || mj != mj) ? 0.f : expf(mj - lmax) * inv_se;
-        const float lam = lama[j];
-        const float gbar = pi + lam;
         float x[M::MAXS], gx[M::MAXS], xb[M::MAXS], xf[M::MAXS], xfb[M::MAXS];
         MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) x[s] = xt[s * (int64_t)J + j];
-        M::dmeas_grad(md, P, y, x, M::dmeas_aux(P, x), gbar, gx, gth);
-        MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) xb[s] = gbar * gx[s] + xca[s * (int64_t)J + j];
+        // the forward's density, evaluated by the same operations
+        const float e = M::dmeas_aux(P, x);
+        const float mj = M::dmeas_a(md, P, y, x, e);
+        const float pi = (lmax == -__builtin_inff() || mj != mj) ? 0.f : expf(mj - lmax) * inv_se;
+        // this particle's carried adjoints are consumed here; as a parent it
+        // starts the scatter below at zero
+        const float lam = lama[j];
+        float xc[M::MAXS];
+        MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) {
+          xc[s] = xca[s * (int64_t)J + j];
+          xca[s * (int64_t)J + j] = 0.f;
+        }
+        lama[j] = 0.f;
+        const float gbar = pi + lam;
+        M::dmeas_grad(md, P, y, x, e, gbar, gx, gth);
+        MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) xb[s] = gbar * gx[s] + xc[s];
         const int p = anct[j];
         MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) xf[s] = xin[s * (int64_t)J + p];
         M::step_vjp(md, P, split_at(sk, (uint64_t)j), xf, x, xb, xfb, gth);
@@ -252,28 +257,28 @@ __device__ void mop_grad_filter(const GradArgs& a, const int f, char* buf, char*
         } else {
           MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) cx[s * (int64_t)J + j] = xfb[s];
           cl[j] = a.alpha * (pi - invJ + lam);
+          ap[j] = p;
         }
       }
       (void)dt;
       (void)ts;
       if (t > 0) {
-        // run starts of every parent's children (a_t is non-decreasing):
-        // lb[q] = first child c with a_t[c] >= q, q = 0..J
-        for (int c = tid; c <= J; c += B) {
-          const int prev = c == 0 ? -1 : anct[c - 1];
-          const int cur = c == J ? J : anct[c];
-          for (int q = prev + 1; q <= cur; ++q) lb[q] = c;
-        }
+        // a_t is non-decreasing, so every parent's children are one run: the
+        // first child of each run sums it in order and writes the parent's
+        // adjoints (parents without children stay zero)
         __syncthreads();
-        for (int q = tid; q < J; q += B) {
-          const int lo = lb[q], hi = lb[q + 1];
+        for (int c = tid; c < J; c += B) {
+          const int q = ap[c];
+          if (c > 0 && ap[c - 1] == q) continue;
           float sx[M::MAXS];
           MP_UNROLL for (int s = 0; s < M::MAXS; ++s) sx[s] = 0.f;
           float sl = 0.f;
-          for (int c = lo; c < hi; ++c) {
-            MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) sx[s] += cx[s * (int64_t)J + c];
-            sl += cl[c];
-          }
+          int k = c;
+          do {
+            MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) sx[s] += cx[s * (int64_t)J + k];
+            sl += cl[k];
+            ++k;
+          } while (k < J && ap[k] == q);
           MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) xca[s * (int64_t)J + q] = sx[s];
           lama[q] = sl;
         }
