@@ -191,7 +191,7 @@ __device__ void run_filter(const BatchArgs& a, const int f, char* buf, double* r
     float* Xn = X[cur ^ 1];
 
     for (int j = tid; j < J; j += B) {
-      int src = have_anc ? ancestor_of(Cc, J, ures, total, j) : j;
+      int src = have_anc ? ancestor_seg(Cc, tab, J, ures, total, j) : j;
       float x[WM];
       MP_UNROLL for (int s = 0; s < WM; ++s) if (s < W) x[s] = Xc[(int64_t)s * J + src];
       const float wprev = uniform_w ? wconst : lw[src];
@@ -237,7 +237,7 @@ __device__ void run_filter(const BatchArgs& a, const int f, char* buf, double* r
     const bool collapsed = (lmax == -__builtin_inff());
     // fixed-point weights and their prefix sums: the resampler's input and the
     // exact total of the log-likelihood term
-    total = block_fixed_scan(lw, Cc, J, lmax, tab);
+    total = block_fixed_scan_seg(lw, Cc, J, lmax, tab);
     double llt = step_loglik(lmax, total, shift_for(J));
     // MOP: LSE(wP + m) - LSE(wP) with wP common = max(m) + log sum exp(m - max) - log J
     if (ALGO == ALGO_MOP) llt = llt - (double)logJ;
@@ -302,7 +302,7 @@ __device__ void run_filter(const BatchArgs& a, const int f, char* buf, double* r
     double mean[M::MAXP];
     for (int p = 0; p < M::MAXP; ++p) mean[p] = 0.0;
     for (int j = tid; j < J; j += B) {
-      int src = have_anc ? ancestor_of(Cc, J, ures, total, j) : j;
+      int src = have_anc ? ancestor_seg(Cc, tab, J, ures, total, j) : j;
       MP_UNROLL for (int p = 0; p < M::MAXP; ++p) if (p < d) {
         float v = X[cur][(int64_t)(S + p) * J + src];
         a.theta_out[((int64_t)f * J + j) * d + p] = v;
@@ -311,7 +311,7 @@ __device__ void run_filter(const BatchArgs& a, const int f, char* buf, double* r
     }
     if (track)
       for (int j = tid; j < J; j += B) {
-        int src = have_anc ? ancestor_of(Cc, J, ures, total, j) : j;
+        int src = have_anc ? ancestor_seg(Cc, tab, J, ures, total, j) : j;
         a.ancestry[(int64_t)f * J + j] = __float_as_int(X[cur][(int64_t)(S + d) * J + src]);
       }
     block_sum_n<M::MAXP>(mean, red);

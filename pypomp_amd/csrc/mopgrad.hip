@@ -94,7 +94,8 @@ __host__ __device__ inline GradScratch grad_scratch_layout(int S, int J, int T, 
 // block's states are recomputed from its checkpoint with the logged ancestry
 // (gather and step), then the adjoint runs backwards through the block.
 template <class M>
-__device__ void mop_grad_filter(const GradArgs& a, const int f, char* buf, char* wbuf, double* red, uint64_t* tab) {
+__device__ __forceinline__ void mop_grad_filter(const GradArgs& a, const int f, char* buf, char* wbuf, double* red,
+                                                uint64_t* tab) {
   const ModelDesc md = a.md;
   const int J = a.J, S = M::FIXED_S ? M::MAXS : md.S, d = md.d, T = a.ser.Tmax, K = a.K;
   const int tid = threadIdx.x, B = blockDim.x;
@@ -162,7 +163,7 @@ __device__ void mop_grad_filter(const GradArgs& a, const int f, char* buf, char*
     const float* y = ys + (int64_t)t * md.dy;
     float lmax = -__builtin_inff(), lmin = __builtin_inff();
     for (int j = tid; j < J; j += B) {
-      const int src = t == 0 ? j : ancestor_of(Cc, J, ures, total, j);
+      const int src = t == 0 ? j : ancestor_seg(Cc, tab, J, ures, total, j);
       if (ck) MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) ckt[s * (int64_t)J + j] = XA[s * (int64_t)J + j];
       anct[j] = src;
       float x[M::MAXS];
@@ -175,7 +176,7 @@ __device__ void mop_grad_filter(const GradArgs& a, const int f, char* buf, char*
       lmin = nanmin(lmin, m);
     }
     block_maxmin(lmax, lmin, red);
-    total = block_fixed_scan(lw, Cc, J, lmax, tab);
+    total = block_fixed_scan_seg(lw, Cc, J, lmax, tab);
     if (tid == 0) {
       stat[2 * t] = (double)lmax;
       stat[2 * t + 1] = ldexp((double)total, -sh);  // sum of exp(m - lmax)
@@ -302,7 +303,10 @@ __device__ void mop_grad_filter(const GradArgs& a, const int f, char* buf, char*
   __syncthreads();
 }
 
-template <class M, int NT>
+// LDSW: the per-step working arrays live in LDS.  A compile-time choice, so
+// their accesses compile to LDS instructions (a run-time choice between LDS
+// and global memory would make every access a generic flat one).
+template <class M, int NT, bool LDSW>
 __device__ __forceinline__ void mop_grad_body(const GradArgs& a) {
   extern __shared__ __align__(256) char smem[];
   double* red = reinterpret_cast<double*>(smem);
@@ -310,23 +314,26 @@ __device__ __forceinline__ void mop_grad_body(const GradArgs& a) {
   char* lwork = smem + kRedBytes + batched_tab_bytes(a.J, NT);
   for (int f = blockIdx.x; f < a.R; f += gridDim.x) {
     char* buf = a.ws + (int64_t)f * a.ws_stride;
-    char* wbuf = a.lds_work ? lwork : buf + grad_scratch_layout(M::FIXED_S ? M::MAXS : a.md.S, a.J, a.ser.Tmax, a.K,
-                                                               false).work;
-    mop_grad_filter<M>(a, f, buf, wbuf, red, tab);
+    if constexpr (LDSW) {
+      mop_grad_filter<M>(a, f, buf, lwork, red, tab);
+    } else {
+      const int S = M::FIXED_S ? M::MAXS : a.md.S;
+      mop_grad_filter<M>(a, f, buf, buf + grad_scratch_layout(S, a.J, a.ser.Tmax, a.K, false).work, red, tab);
+    }
   }
 }
 
 // 256 threads: the occupancy the registers allow (3 waves per SIMD).
-template <class M>
+template <class M, bool LDSW>
 __global__ __launch_bounds__(256) void k_mop_grad(const GradArgs a) {
-  mop_grad_body<M, 256>(a);
+  mop_grad_body<M, 256, LDSW>(a);
 }
 
 // 512 threads capped at 128 VGPRs, so two workgroups (16 waves) share a CU:
 // two filters per CU at 4 waves per SIMD.
-template <class M>
+template <class M, bool LDSW>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_mop_grad_wide(const GradArgs a) {
-  mop_grad_body<M, 512>(a);
+  mop_grad_body<M, 512, LDSW>(a);
 }
 
 template <class M, bool OK = M::DIFFERENTIABLE>
@@ -337,8 +344,14 @@ template <class M>
 struct GradLauncher<M, true> {
   static hipError_t run(const GradArgs& a, int block, int grid, size_t lds, hipStream_t st) {
     switch (block) {
-      case 256: hipLaunchKernelGGL((k_mop_grad<M>), dim3(grid), dim3(256), lds, st, a); break;
-      case 512: hipLaunchKernelGGL((k_mop_grad_wide<M>), dim3(grid), dim3(512), lds, st, a); break;
+      case 256:
+        if (a.lds_work) hipLaunchKernelGGL((k_mop_grad<M, true>), dim3(grid), dim3(256), lds, st, a);
+        else hipLaunchKernelGGL((k_mop_grad<M, false>), dim3(grid), dim3(256), lds, st, a);
+        break;
+      case 512:
+        if (a.lds_work) hipLaunchKernelGGL((k_mop_grad_wide<M, true>), dim3(grid), dim3(512), lds, st, a);
+        else hipLaunchKernelGGL((k_mop_grad_wide<M, false>), dim3(grid), dim3(512), lds, st, a);
+        break;
       default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

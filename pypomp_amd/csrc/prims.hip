@@ -47,10 +47,10 @@ __global__ __launch_bounds__(1024) void k_resample_rows(const float* __restrict_
       if (v == v) mx = fmaxf(mx, v);  // NaN weights carry no mass
     }
     block_maxmin(mx, mn, red);
-    const uint64_t total = block_fixed_scan(lw, C, J, mx, tab);
+    const uint64_t total = block_fixed_scan_seg(lw, C, J, mx, tab);
     const Key k{keys[2 * r], keys[2 * r + 1]};
     const float u = uniform01(bits_at(k, 0));
-    for (int j = threadIdx.x; j < J; j += blockDim.x) anc[r * J + j] = ancestor_of(C, J, u, total, j);
+    for (int j = threadIdx.x; j < J; j += blockDim.x) anc[r * J + j] = ancestor_seg(C, tab, J, u, total, j);
     __syncthreads();
   }
 }

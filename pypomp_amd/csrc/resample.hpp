@@ -15,10 +15,14 @@ __host__ __device__ inline int64_t batched_tab_bytes(int J, int block) {
   return align256((rows * nw + 1) * 8);
 }
 
-// Inclusive prefix sums C[j] of fixed-point weights over a workgroup.  Rows of
-// blockDim particles: wave scans in registers, one table of wave totals in LDS,
-// one wave scans the table.  Returns the total.
-__device__ inline uint64_t block_fixed_scan(const float* lw, uint64_t* C, int J, float m, uint64_t* tab) {
+// Prefix sums of fixed-point weights over a workgroup in segmented form.
+// Particle j = r * blockDim + tid belongs to segment k = j / 64 (the wave's 64
+// lanes of row r; blockDim is a multiple of 64).  On return C[j] holds the
+// inclusive sum within its segment, tab[k] the segment's exclusive offset and
+// tab[n] the total, n = rows * waves: the global inclusive prefix sum of j is
+// tab[j / 64] + C[j].  Wave scans in registers, one wave scans the table of
+// segment totals; no pass over the particles after the table scan.
+__device__ inline uint64_t block_fixed_scan_seg(const float* lw, uint64_t* C, int J, float m, uint64_t* tab) {
   const int B = blockDim.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int nw = (B + 63) >> 6;
   const int rows = (J + B - 1) / B;
@@ -47,13 +51,7 @@ __device__ inline uint64_t block_fixed_scan(const float* lw, uint64_t* C, int J,
     if (lane == 63) tab[n] = incl;
   }
   __syncthreads();
-  for (int r = 0; r < rows; ++r) {
-    const int j = r * B + tid;
-    if (j < J) C[j] += tab[r * nw + w];
-  }
-  uint64_t total = tab[n];
-  __syncthreads();
-  return total;
+  return tab[n];
 }
 
 // #{i < n : C[i] <= t} for non-decreasing C and n >= 1: a branch-free
@@ -69,9 +67,22 @@ __device__ __forceinline__ int upper_bound_fixed(const uint64_t* C, int n, uint6
   return base + (C[base] <= t ? 1 : 0);
 }
 
-__device__ __forceinline__ int ancestor_of(const uint64_t* C, int J, float u, uint64_t total, int j) {
+// Ancestor of particle j from the segmented sums of block_fixed_scan_seg:
+// the last segment whose offset is <= T_j (offsets are non-decreasing and
+// tab[0] = 0), then the count within it.  Equals upper_bound over the global
+// prefix sums, clamped to J-1 like the reference's gather.
+__device__ __forceinline__ int ancestor_seg(const uint64_t* C, const uint64_t* tab, int J, float u, uint64_t total,
+                                            int j) {
   const uint64_t tj = comb_threshold(u, j, J, total);
-  const int a = upper_bound_fixed(C, J, tj);
+  const int B = blockDim.x;
+  const int n = ((J + B - 1) / B) * (B >> 6);
+  const int k = upper_bound_fixed(tab, n, tj) - 1;
+  const int start = k * 64;
+  int a = start;
+  if (start < J) {
+    const int len = J - start < 64 ? J - start : 64;
+    a += upper_bound_fixed(C + start, len, tj - tab[k]);
+  }
   return a < J ? a : J - 1;
 }
 
