@@ -92,7 +92,7 @@ struct WMax {
 };
 
 template <class M, int ALGO>
-__device__ void run_filter(const BatchArgs& a, const int f, char* buf, double* red, uint64_t* tab) {
+__device__ __forceinline__ void run_filter(const BatchArgs& a, const int f, char* buf, double* red, uint64_t* tab) {
   constexpr int WM = (ALGO == ALGO_PER) ? WMax<M>::v : M::MAXS;
   const ModelDesc md = a.md;
   const int J = a.J, S = M::FIXED_S ? M::MAXS : md.S, d = md.d, dy = md.dy;
@@ -327,7 +327,9 @@ __device__ void run_filter(const BatchArgs& a, const int f, char* buf, double* r
 // A filter's result does not depend on which workgroup runs it.  Global
 // scratch is indexed by the filter's queue position, which every workgroup
 // takes once.
-template <class M, int ALGO, int NT>
+// LDSR: the filter's buffers live in LDS.  A compile-time choice, so their
+// accesses compile to LDS (or global) instructions rather than generic flat ones.
+template <class M, int ALGO, int NT, bool LDSR>
 __global__ __launch_bounds__(NT) void k_batched(const BatchArgs a) {
   extern __shared__ __align__(256) char smem[];
   __shared__ int next;
@@ -341,8 +343,8 @@ __global__ __launch_bounds__(NT) void k_batched(const BatchArgs a) {
     __syncthreads();
     if (q >= a.R) break;
     const int f = a.order[q];
-    char* buf = a.lds_resident ? ldsbuf : a.ws + (int64_t)q * a.ws_stride;
-    run_filter<M, ALGO>(a, f, buf, red, tab);
+    if constexpr (LDSR) run_filter<M, ALGO>(a, f, ldsbuf, red, tab);
+    else run_filter<M, ALGO>(a, f, a.ws + (int64_t)q * a.ws_stride, red, tab);
   }
 }
 
@@ -376,9 +378,18 @@ __global__ void k_order(SeriesSet ser, const int32_t* __restrict__ unit, int R, 
 template <class M, int ALGO>
 static hipError_t launch_batched_t(const BatchArgs& a, int block, int grid, size_t lds, hipStream_t st) {
   switch (block) {
-    case 256: hipLaunchKernelGGL((k_batched<M, ALGO, 256>), dim3(grid), dim3(256), lds, st, a); break;
-    case 512: hipLaunchKernelGGL((k_batched<M, ALGO, 512>), dim3(grid), dim3(512), lds, st, a); break;
-    case 768: hipLaunchKernelGGL((k_batched<M, ALGO, 768>), dim3(grid), dim3(768), lds, st, a); break;
+    case 256:
+      if (a.lds_resident) hipLaunchKernelGGL((k_batched<M, ALGO, 256, true>), dim3(grid), dim3(256), lds, st, a);
+      else hipLaunchKernelGGL((k_batched<M, ALGO, 256, false>), dim3(grid), dim3(256), lds, st, a);
+      break;
+    case 512:
+      if (a.lds_resident) hipLaunchKernelGGL((k_batched<M, ALGO, 512, true>), dim3(grid), dim3(512), lds, st, a);
+      else hipLaunchKernelGGL((k_batched<M, ALGO, 512, false>), dim3(grid), dim3(512), lds, st, a);
+      break;
+    case 768:
+      if (a.lds_resident) hipLaunchKernelGGL((k_batched<M, ALGO, 768, true>), dim3(grid), dim3(768), lds, st, a);
+      else hipLaunchKernelGGL((k_batched<M, ALGO, 768, false>), dim3(grid), dim3(768), lds, st, a);
+      break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
