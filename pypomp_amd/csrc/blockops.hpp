@@ -6,6 +6,8 @@
 namespace mp {
 
 constexpr int WAVE = 64;
+// Bytes of the LDS reduction scratch (`red`) the block reductions may use.
+constexpr int kRedUsable = 4088;
 
 // max that propagates NaN, as jnp.max does.
 __device__ __forceinline__ float nanmax(float a, float b) { return (a > b || a != a) ? a : b; }
@@ -92,6 +94,26 @@ __device__ inline int64_t wave_upper_bound_u64(const uint64_t* C, int64_t lo, in
   return lo + __popcll(__ballot(le));
 }
 
+// A key every lane holds the same value of, moved to scalar registers, so the
+// arithmetic on it (a uniform threefry) runs on the scalar unit.
+__device__ __forceinline__ Key uniform_key(uint32_t k0, uint32_t k1) {
+  return Key{(uint32_t)__builtin_amdgcn_readfirstlane(k0), (uint32_t)__builtin_amdgcn_readfirstlane(k1)};
+}
+
+// Workgroup max (NaN-propagating) to every thread.  `red` as below.
+__device__ __forceinline__ float block_nanmax(float mx, double* red) {
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  const int nw = (blockDim.x + WAVE - 1) / WAVE;
+  mx = wave_nanmax(mx);
+  __syncthreads();
+  float* r = reinterpret_cast<float*>(red);
+  if (lane == 0) r[w] = mx;
+  __syncthreads();
+  mx = r[0];
+  for (int i = 1; i < nw; ++i) mx = nanmax(mx, r[i]);
+  return mx;
+}
+
 // Workgroup reductions.  `red` is LDS scratch of at least 2*nwaves doubles.
 // All threads of the block must call; result is returned to every thread.
 __device__ __forceinline__ void block_maxmin(float& mx, float& mn, double* red) {
@@ -120,23 +142,29 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   return s;
 }
 
-// Sum of `n` doubles per thread-group (n <= 8) in one pass.
+// Workgroup sums of N doubles per thread, in passes of as many as fit in the
+// reduction scratch; each sum is taken over the waves in a fixed order.
 template <int N>
 __device__ __forceinline__ void block_sum_n(double (&v)[N], double* red) {
   const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
   const int nw = (blockDim.x + WAVE - 1) / WAVE;
+  const int cap = (kRedUsable / 8) / nw;
 #pragma unroll
   for (int k = 0; k < N; ++k) v[k] = wave_sum(v[k]);
-  __syncthreads();
-  if (lane == 0)
+  for (int k0 = 0; k0 < N; k0 += cap) {
+    __syncthreads();
+    if (lane == 0)
 #pragma unroll
-    for (int k = 0; k < N; ++k) red[k * nw + w] = v[k];
-  __syncthreads();
+      for (int k = 0; k < N; ++k)
+        if (k >= k0 && k < k0 + cap) red[(k - k0) * nw + w] = v[k];
+    __syncthreads();
 #pragma unroll
-  for (int k = 0; k < N; ++k) {
-    double s = 0.0;
-    for (int i = 0; i < nw; ++i) s += red[k * nw + i];
-    v[k] = s;
+    for (int k = 0; k < N; ++k) {
+      if (k < k0 || k >= k0 + cap) continue;
+      double s = 0.0;
+      for (int i = 0; i < nw; ++i) s += red[(k - k0) * nw + i];
+      v[k] = s;
+    }
   }
 }
 

@@ -118,7 +118,7 @@ __device__ __forceinline__ void mop_grad_filter(const GradArgs& a, const int f, 
   const int32_t* tidx0 = a.ser.tidx0;
   const float* dts = a.ser.dt_ext;
   const uint32_t* sch = a.sched + 2 * (int64_t)f * a.sched_len;
-  const Key fkey{a.keys[2 * f], a.keys[2 * f + 1]};
+  const Key fkey = uniform_key(a.keys[2 * f], a.keys[2 * f + 1]);
   const float fJ = (float)J, logJ = d_log(fJ), invJ = 1.0f / fJ;
   const int sh = shift_for(J);
 
@@ -133,11 +133,11 @@ __device__ __forceinline__ void mop_grad_filter(const GradArgs& a, const int f, 
 
   auto step_key = [&](int t) -> Key {
     const int64_t base = tidx0[t] + (int64_t)t;
-    return Key{sch[2 * base], sch[2 * base + 1]};
+    return uniform_key(sch[2 * base], sch[2 * base + 1]);
   };
   auto res_key = [&](int t) -> Key {
     const int64_t base = tidx0[t] + (int64_t)t + 1;  // one sub-step per observation
-    return Key{sch[2 * base], sch[2 * base + 1]};
+    return uniform_key(sch[2 * base], sch[2 * base + 1]);
   };
   auto tstart = [&](int t) { return t == 0 ? a.ser.t0 : times[t - 1]; };
 
@@ -153,7 +153,15 @@ __device__ __forceinline__ void mop_grad_filter(const GradArgs& a, const int f, 
   double loglik = 0.0;
   uint64_t total = 0;
   float ures = 0.f;
+  // the comb's uniform of step t, drawn by the first wave while the others
+  // propagate and read after the step's barriers (two slots: step t writes
+  // slot t & 1 while step t-1's value is still being read from the other)
+  float* ures_slot = reinterpret_cast<float*>(reinterpret_cast<char*>(red) + kUresOffset);
   for (int t = 0; t < T; ++t) {
+    if (tid < WAVE && t < T - 1) {
+      const float u = uniform01(bits_at(res_key(t), 0));
+      if (tid == 0) ures_slot[t & 1] = u;
+    }
     const bool ck = (t % K) == 0;
     float* ckt = ckx + (int64_t)(t / K) * SJ;
     int32_t* anct = ancl + (int64_t)t * J;
@@ -161,7 +169,7 @@ __device__ __forceinline__ void mop_grad_filter(const GradArgs& a, const int f, 
     const float dt = dts[tidx0[t]];
     const float ts = tstart(t), tt = times[t];
     const float* y = ys + (int64_t)t * md.dy;
-    float lmax = -__builtin_inff(), lmin = __builtin_inff();
+    float lmax = -__builtin_inff();
     for (int j = tid; j < J; j += B) {
       const int src = t == 0 ? j : ancestor_seg(Cc, tab, J, ures, total, j);
       if (ck) MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) ckt[s * (int64_t)J + j] = XA[s * (int64_t)J + j];
@@ -173,16 +181,15 @@ __device__ __forceinline__ void mop_grad_filter(const GradArgs& a, const int f, 
       MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) XB[s * (int64_t)J + j] = x[s];
       lw[j] = m;
       lmax = nanmax(lmax, m);
-      lmin = nanmin(lmin, m);
     }
-    block_maxmin(lmax, lmin, red);
+    lmax = block_nanmax(lmax, red);
     total = block_fixed_scan_seg(lw, Cc, J, lmax, tab);
     if (tid == 0) {
       stat[2 * t] = (double)lmax;
       stat[2 * t + 1] = ldexp((double)total, -sh);  // sum of exp(m - lmax)
     }
-    loglik += step_loglik(lmax, total, sh) - (double)logJ;
-    if (t < T - 1) ures = uniform01(bits_at(res_key(t), 0));
+    if (tid < WAVE) loglik += step_loglik(lmax, total, sh) - (double)logJ;  // read by thread 0 only
+    if (t < T - 1) ures = ures_slot[t & 1];
     float* tmp = XA;
     XA = XB;
     XB = tmp;

@@ -7,7 +7,8 @@
 
 namespace mp {
 
-constexpr int kRedBytes = 4096;  // up to 128 doubles x 4 waves of reductions
+constexpr int kRedBytes = 4096;  // reductions (kRedUsable bytes) and two uniforms
+constexpr int kUresOffset = kRedUsable;
 
 // LDS bytes of the scan table of block_fixed_scan: rows * waves + 1 entries.
 __host__ __device__ inline int64_t batched_tab_bytes(int J, int block) {
