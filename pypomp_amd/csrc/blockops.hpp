@@ -71,6 +71,19 @@ __device__ __forceinline__ uint64_t wave_incl_scan_u64(uint64_t v) {
   return v + (row == 0 ? 0ull : row == 1 ? r0 : row == 2 ? r01 : r012);
 }
 
+// #{i < n : C[i] <= t} for non-decreasing C and n >= 1: a branch-free
+// search whose trip count depends on n only, so every lane of a wave runs the
+// same ceil(log2 n) probes and the loop counter stays scalar.
+__device__ __forceinline__ int upper_bound_fixed(const uint64_t* C, int n, uint64_t t) {
+  int base = 0;
+  while (n > 1) {
+    const int half = n >> 1;
+    base = C[base + half] <= t ? base + half : base;
+    n -= half;
+  }
+  return base + (C[base] <= t ? 1 : 0);
+}
+
 // First index in [lo, hi) whose entry exceeds t (C non-decreasing), found by one
 // whole wave: each round probes 64 evenly spaced entries and keeps the gap in
 // which C crosses t, so J = 10^6 needs 4 dependent loads instead of 20.  Every

@@ -40,7 +40,7 @@ __device__ __forceinline__ unsigned ticket_relaxed(unsigned int* p) {
 }
 
 template <class M>
-__global__ __launch_bounds__(512) void k_plain_prop(const PlainArgs a) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k_plain_prop(const PlainArgs a) {
   __shared__ double red[32];
   __shared__ int64_t s_win[2];
   __shared__ uint64_t s_C[kWin];
@@ -99,7 +99,7 @@ __global__ __launch_bounds__(512) void k_plain_prop(const PlainArgs a) {
     } else {
       if (have_anc) {
         const uint64_t tj = comb_threshold(st.ures, j, J, st.total);
-        const int64_t an = in_lds ? wlo + upper_bound_u64(s_C, 0, whi - wlo, tj)
+        const int64_t an = in_lds ? wlo + (whi > wlo ? upper_bound_fixed(s_C, (int)(whi - wlo), tj) : 0)
                                   : upper_bound_u64(a.C, wlo, whi, tj);
         src = an < J ? an : J - 1;
       }
@@ -110,7 +110,7 @@ __global__ __launch_bounds__(512) void k_plain_prop(const PlainArgs a) {
       if ((md.accum_mask >> s) & 1u) x[s] = 0.f;
     float tt = tstart;
     for (int k = 0; k < nst; ++k) {
-      const Key sk{a.sched[2 * (base + k)], a.sched[2 * (base + k) + 1]};
+      const Key sk = uniform_key(a.sched[2 * (base + k)], a.sched[2 * (base + k) + 1]);
       const float dt = dts[ti0 + k];
       M::step(md, P, split_at(sk, (uint64_t)j), tt, dt, x);
       tt = tt + dt;

@@ -55,19 +55,6 @@ __device__ inline uint64_t block_fixed_scan_seg(const float* lw, uint64_t* C, in
   return tab[n];
 }
 
-// #{i < n : C[i] <= t} for non-decreasing C and n >= 1: a branch-free
-// search whose trip count depends on n only, so every lane of a wave runs the
-// same ceil(log2 n) probes and the loop counter stays scalar.
-__device__ __forceinline__ int upper_bound_fixed(const uint64_t* C, int n, uint64_t t) {
-  int base = 0;
-  while (n > 1) {
-    const int half = n >> 1;
-    base = C[base + half] <= t ? base + half : base;
-    n -= half;
-  }
-  return base + (C[base] <= t ? 1 : 0);
-}
-
 // Ancestor of particle j from the segmented sums of block_fixed_scan_seg:
 // the last segment whose offset is <= T_j (offsets are non-decreasing and
 // tab[0] = 0), then the count within it.  Equals upper_bound over the global
