@@ -27,7 +27,10 @@ HBM_PEAK_GBS = 8000.0
 # HBM-side bytes per launch at J = 1e6 from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # passes (profiles/plain_lg2_J1e6_traffic.md): FETCH_SIZE x 2 + WRITE_SIZE x 1, the
 # factors that make a 1 GiB device copy read 1 GiB and write 1 GiB.
-MEASURED_TRAFFIC_J1E6 = {"plain_prop": 33.78e6, "plain_scan": 12.45e6}
+MEASURED_TRAFFIC_J1E6 = {"plain_prop": 33.81e6, "plain_scan": 12.49e6}
+# HBM bytes per launch of the SV MOP-gradient kernel at J=1000, T=1000, K=50, 512
+# filters (FETCH_SIZE x 2 + WRITE_SIZE, copy-calibrated)
+MEASURED_TRAFFIC_MOP = 14.77e9
 METRIC = "particle-steps/s (pfilter & MOP-grad, batched vs plain GPU); loglik rel err"
 
 
@@ -182,6 +185,11 @@ class MopGradSV:
     def bytes_per_particle_step(self):
         return 3 * 16 * (2 + 1 + 1)  # forward, recompute, reverse; S = v, eta, weightsF
 
+    @property
+    def measured_traffic(self):
+        full = (self.form == "batched" and self.R == 512 and self.J == 1000 and self.T == 1000 and self.K == 50)
+        return MEASURED_TRAFFIC_MOP if full else None
+
 
 class PanelIF2:
     """Config: IF2 on a panel of 100 SEIR units of uneven length (T_u in
@@ -266,6 +274,7 @@ def time_workload(wl, steps, warmup):
             "kernel_avg_ms": avg, "launches_per_step": n,
             "roofline": {"bound": "hbm", "achieved": gbps, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": (gbps / HBM_PEAK_GBS) if gbps else None,
+                         "traffic": getattr(wl, "measured_traffic", None),
                          "bytes_per_particle_step": wl.bytes_per_particle_step()}}
 
 
