@@ -109,7 +109,17 @@ def main():
         for k, v in panel.items():
             arrays[f"panel_{k}"] = v
     np.savez_compressed(HERE / "workloads.npz", **arrays)
-    print("wrote", HERE / "workloads.npz")
+    # exact FP64 Kalman log-likelihoods of the LG series (the accuracy yardstick
+    # the benchmark reports against, without running the oracle itself)
+    from oracle import kalman
+    kal = {}
+    for seed in (1, 2):
+        spec = lg.LGSpec(2, 2, separate_init=True)
+        A, C, LQ, LR, X0, L0 = (np.asarray(m, np.float64) for m in spec.unpack(arrays[f"lg2_seed{seed}_theta"]))
+        kal[f"lg2_seed{seed}"] = kalman.kalman_loglik(arrays[f"lg2_seed{seed}_ys"], A, C, LQ @ LQ.T, LR @ LR.T, X0,
+                                                      L0 @ L0.T)
+    (HERE / "kalman_loglik.json").write_text(json.dumps(kal, indent=1))
+    print("wrote", HERE / "workloads.npz", HERE / "kalman_loglik.json")
 
 
 if __name__ == "__main__":

@@ -35,3 +35,18 @@ def test_one_observation_closed_form():
     var = (0.9 * 1.0 * 0.9 + 0.1) + 0.1
     exact = stats.norm.logpdf(0.7, loc=0.9 * 0.3, scale=np.sqrt(var))
     assert abs(kalman.kalman_loglik(y, A, C, Q, R, m, P0) - exact) <= 1e-12
+
+
+def test_kalman_fixture_values():
+    # the benchmark's accuracy yardstick equals the oracle's Kalman filter on the fixtures
+    import json
+    from pathlib import Path
+    from oracle import lg as olg
+    from tests.helpers import workloads
+    w = workloads()
+    kal = json.loads((Path(__file__).resolve().parent / "golden" / "kalman_loglik.json").read_text())
+    for seed in (1, 2):
+        spec = olg.LGSpec(2, 2, separate_init=True)
+        A, C, LQ, LR, X0, L0 = (np.asarray(m, np.float64) for m in spec.unpack(w[f"lg2_seed{seed}_theta"]))
+        v = kalman.kalman_loglik(w[f"lg2_seed{seed}_ys"], A, C, LQ @ LQ.T, LR @ LR.T, X0, L0 @ L0.T)
+        assert v == kal[f"lg2_seed{seed}"]
