@@ -34,6 +34,25 @@ MEASURED_TRAFFIC_J1E6 = {"plain_prop": 33.81e6, "plain_scan": 12.49e6}
 # HBM bytes per launch of the SV MOP-gradient kernel at J=1000, T=1000, K=50, 512
 # filters (FETCH_SIZE x 2 + WRITE_SIZE, copy-calibrated)
 MEASURED_TRAFFIC_MOP = 14.77e9
+# VALU instructions one lane executes per particle-step (SQ_INSTS_VALU x 64 /
+# particle-steps per dispatch, rocprofv3 --pmc; profiles/valu_plain_lg2_seir_mopgrad.md).
+# The kernels are VALU- or latency-bound rather than HBM-bound, so each roofline
+# object also carries the VALU issue rate implied by the live kernel time.
+VALU_INSTR_PER_PSTEP = {"plain_prop": 709.0, "plain_scan": 151.0, "seir": 12551.0, "mop_grad": 1144.0}
+# one wave64 VALU instruction per CU per cycle: 256 CUs x 2.4 GHz
+VALU_PEAK_GINSTR = 256 * 2.4
+
+
+def valu_rate(instr_per_pstep, psteps, seconds):
+    """VALU issue rate (wave64 Ginstr/s) of a kernel that ran `psteps` particle-steps
+    in `seconds`, from its measured instruction count per particle-step."""
+    if not seconds or instr_per_pstep is None:
+        return None
+    g = instr_per_pstep * psteps / 64.0 / seconds / 1e9
+    return {"achieved": g, "peak": VALU_PEAK_GINSTR, "unit": "Ginstr/s (wave64)", "frac": g / VALU_PEAK_GINSTR,
+            "instr_per_particle_step": instr_per_pstep}
+
+
 METRIC = "particle-steps/s (pfilter & MOP-grad, batched vs plain GPU); loglik rel err"
 
 
@@ -140,6 +159,8 @@ class BatchedSEIR:
     def bytes_per_particle_step(self):
         return 16 * (5 + 1)
 
+    valu_instr_per_pstep = VALU_INSTR_PER_PSTEP["seir"]
+
 
 class MopGradSV:
     """Config: SV MOP(0.97) value + adjoint gradient (6 parameters), J=1000, T=1000,
@@ -187,6 +208,10 @@ class MopGradSV:
 
     def bytes_per_particle_step(self):
         return 3 * 16 * (2 + 1 + 1)  # forward, recompute, reverse; S = v, eta, weightsF
+
+    @property
+    def valu_instr_per_pstep(self):
+        return VALU_INSTR_PER_PSTEP["mop_grad"]
 
     @property
     def measured_traffic(self):
@@ -278,7 +303,9 @@ def time_workload(wl, steps, warmup):
             "roofline": {"bound": "hbm", "achieved": gbps, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": (gbps / HBM_PEAK_GBS) if gbps else None,
                          "traffic": getattr(wl, "measured_traffic", None),
-                         "bytes_per_particle_step": wl.bytes_per_particle_step()}}
+                         "bytes_per_particle_step": wl.bytes_per_particle_step(),
+                         "valu": valu_rate(getattr(wl, "valu_instr_per_pstep", None), wl.particle_steps,
+                                           ms * 1e-3)}}
 
 
 def kalman_loglik(seed):
@@ -405,13 +432,15 @@ def main():
         ms, n = prof[k]
         avg = ms / max(n, 1)
         kernels[k] = {"avg_us": avg * 1e3, "launches": n, "bytes": kb[k],
-                      "GBps": kb[k] / (avg * 1e-3) / 1e9 if avg > 0 else None}
+                      "GBps": kb[k] / (avg * 1e-3) / 1e9 if avg > 0 else None,
+                      "valu": valu_rate(VALU_INSTR_PER_PSTEP[k], wl.J, avg * 1e-3)}
     dom = max(kernels, key=lambda k: prof[k][0])
     d = kernels[dom]
     roofline = {"bound": "hbm", "achieved": d["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": d["GBps"] / HBM_PEAK_GBS,
                 "traffic": MEASURED_TRAFFIC_J1E6[dom] if J == 1_000_000 else None, "kernel": dom,
-                "bytes_per_launch": d["bytes"], "avg_launch_us": d["avg_us"]}
+                "bytes_per_launch": d["bytes"], "avg_launch_us": d["avg_us"],
+                "valu": valu_rate(VALU_INSTR_PER_PSTEP[dom], J, d["avg_us"] * 1e-6)}
     log(f"kernels {json.dumps(kernels)}")
 
     exact = kalman_loglik(2)
