@@ -75,6 +75,9 @@ int g_plain_scan_rows = 8;  // particle rows per scan tile (1..16)
 int g_plain_prop_rows = 2;  // particle rows per propagate block (1..16)
 int g_plain_scan_threads = 512;  // scan workgroup size (256, 512); rows * waves <= 64
 int g_plain_prop_threads = 512;  // propagate workgroup size (256, 512)
+// look-back probes a scan tile spends on a late predecessor before computing its
+// aggregate itself (0: at once, which forces the fallback path)
+int g_plain_scan_spin = 256;
 int g_batched_block = 0;    // batched workgroup size: 0 = automatic, 256, 512, 768
 int g_grad_block = 0;       // MOP-adjoint workgroup size: 0 = automatic, 256, 512
 
@@ -302,6 +305,7 @@ int run_plain_pf(const mp_model_desc* m, const mp_series_desc* s, const float* t
   a.nprop = p.nprop;
   a.st = reinterpret_cast<PlainState*>(b + p.off_st);
   a.rows = p.rows;
+  a.scan_spin = g_plain_scan_spin;
   const int64_t slen = sched_stride(ALGO_PF, a.ser);
   uint32_t* sched_all = reinterpret_cast<uint32_t*>(b + p.off_sched);
   MP_HIP(timed(3, st, [&] { return launch_sched(ALGO_PF, keys, a.ser, nullptr, slen, sched_all, R, st); }));
@@ -670,6 +674,10 @@ int32_t mp_tune_set(int32_t param, int32_t value) {
       if (value != 256 && value != 512) return fail(MP_ERR_ARG, "plain workgroup size must be 256 or 512");
       (param == MP_TUNE_PLAIN_SCAN_THREADS ? g_plain_scan_threads : g_plain_prop_threads) = value;
       return MP_OK;
+    case MP_TUNE_PLAIN_SCAN_SPIN:
+      if (value < 0 || value > (1 << 20)) return fail(MP_ERR_ARG, "scan spin limit must be 0..2^20");
+      g_plain_scan_spin = value;
+      return MP_OK;
     case MP_TUNE_GRAD_BLOCK:
       if (value != 0 && value != 256 && value != 512)
         return fail(MP_ERR_ARG, "adjoint workgroup size must be 0 (automatic), 256 or 512");
@@ -687,6 +695,7 @@ int32_t mp_tune_get(int32_t param) {
     case MP_TUNE_GRAD_BLOCK: return g_grad_block;
     case MP_TUNE_PLAIN_SCAN_THREADS: return g_plain_scan_threads;
     case MP_TUNE_PLAIN_PROP_THREADS: return g_plain_prop_threads;
+    case MP_TUNE_PLAIN_SCAN_SPIN: return g_plain_scan_spin;
   }
   return -1;
 }

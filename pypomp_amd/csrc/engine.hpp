@@ -104,7 +104,6 @@ struct PlainState {
   int uniform_w;       // weights are the constant wconst
   float wconst;
   int collapsed;       // previous step had all weights -inf
-  unsigned int tile_ctr[2], ticket[2];  // per step parity
 };
 
 struct PlainArgs {
@@ -131,6 +130,7 @@ struct PlainArgs {
   int ntiles;           // scan tiles
   int rows_prop;        // particle rows per propagate block
   int nprop;            // propagate blocks
+  int scan_spin;        // look-back probes before a tile computes a late predecessor's aggregate
 };
 
 // MOP value + adjoint gradient launch (one workgroup per filter).

@@ -7,10 +7,11 @@
 //                   gather, the Euler sub-steps, the measurement density and the
 //                   log-weight; per-tile max/min.
 //  k_plain_scan  -- fixed-point weights and a one-pass inclusive prefix sum
-//                   across tiles with decoupled look-back; the last tile
-//                   finishes the step from the exact total: the log-likelihood
-//                   term (FP64 sum over t), the resampling decision and the
-//                   comb's uniform.
+//                   across tiles with decoupled look-back (bounded waits: a
+//                   tile computes a late predecessor's aggregate itself); the
+//                   tile with the last index finishes the step from the exact
+//                   total: the log-likelihood term (FP64 sum over t), the
+//                   resampling decision and the comb's uniform.
 //
 // Cross-workgroup traffic uses only relaxed agent-scope atomics on single
 // 64-bit words (look-back state and prefix packed together): on a multi-XCD
@@ -28,15 +29,13 @@ constexpr uint64_t kAggState = 1ull << kStateShift, kInclState = 2ull << kStateS
 constexpr uint64_t kValueMask = (1ull << kStateShift) - 1;
 constexpr int kWin = 2048;  // LDS search window (uint64 entries)
 constexpr int kMaxScanRows = 16;  // rows of 256 particles per scan tile, at most
+constexpr int kLookback = 4;      // look-back loads per lane and probe
 
 __device__ __forceinline__ uint64_t ld_relaxed(uint64_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void st_relaxed(uint64_t* p, uint64_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ unsigned ticket_relaxed(unsigned int* p) {
-  return __hip_atomic_fetch_add(p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <class M>
@@ -128,20 +127,41 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
   }
 }
 
+// Aggregate of scan tile `idx` (the sum of its particles' fixed-point weights),
+// computed by one wave: what that tile publishes, computed here instead when it
+// has not published in time (a tile whose workgroup is not running yet).
+__device__ uint64_t wave_tile_aggregate(const float* lw, int64_t idx, int64_t tile_len, int64_t J, float m,
+                                        int sh) {
+  const int lane = threadIdx.x & 63;
+  const int64_t j0 = idx * tile_len, j1 = min(J, j0 + tile_len);
+  uint64_t v = 0;
+  for (int64_t j = j0 + lane; j < j1; j += 64) v += particle_fixed_weight(lw[j], m, sh);
+  return wave_sum_u64(v);
+}
+
 template <class M>
 __global__ __launch_bounds__(512) void k_plain_scan(const PlainArgs a) {
   __shared__ double red[32];
   __shared__ uint64_t tab[65];  // rows * waves <= 64 row/wave totals + the tile total
   __shared__ uint64_t s_prefix;
-  __shared__ int s_tile, is_last;
   const int J = a.J, tid = threadIdx.x, B = blockDim.x, lane = tid & 63, w = tid >> 6;
   const int nw = (B + 63) >> 6;
   const int ntiles = a.ntiles;
   const int par = a.t & 1;
   uint64_t* lb = a.lb + (int64_t)par * ntiles;
   const int sh = shift_for(J);
+  const int tile = blockIdx.x;
+  const int rows = a.rows;
+  const int64_t tile0 = (int64_t)tile * rows * B;
 
-  if (tid == 0) s_tile = (int)ticket_relaxed(&a.st->tile_ctr[par]);
+  // the tile's log-weights, loads issued first so that they are in flight
+  // together with the per-block maxima below
+  float lwv[kMaxScanRows];
+#pragma unroll
+  for (int r = 0; r < kMaxScanRows; ++r) {
+    const int64_t j = tile0 + (int64_t)r * B + tid;
+    lwv[r] = (r < rows && j < J) ? a.lw[j] : -__builtin_inff();
+  }
   // global max / min of the log-weights from the per-block values, eight
   // independent loads in flight per thread
   float m = -__builtin_inff(), mn = __builtin_inff();
@@ -159,19 +179,9 @@ __global__ __launch_bounds__(512) void k_plain_scan(const PlainArgs a) {
       mn = nanmin(mn, vn[k]);
     }
   }
-  block_maxmin(m, mn, red);  // includes barriers: s_tile is visible below
-  const int tile = s_tile;
-  const int64_t tile0 = (int64_t)tile * a.rows * B;
+  block_maxmin(m, mn, red);
 
-  // per-thread fixed-point weights (all of the tile's loads issued first);
-  // wave scans per row
-  const int rows = a.rows;
-  float lwv[kMaxScanRows];
-#pragma unroll
-  for (int r = 0; r < kMaxScanRows; ++r) {
-    const int64_t j = tile0 + (int64_t)r * B + tid;
-    lwv[r] = (r < rows && j < J) ? a.lw[j] : -__builtin_inff();
-  }
+  // per-thread fixed-point weights; wave scans per row
   uint64_t rowincl[kMaxScanRows];
 #pragma unroll
   for (int r = 0; r < kMaxScanRows; ++r) {
@@ -196,28 +206,65 @@ __global__ __launch_bounds__(512) void k_plain_scan(const PlainArgs a) {
       tab[n] = aggv;
       st_relaxed(&lb[tile], (tile == 0 ? kInclState : kAggState) | aggv);
     }
-    // Decoupled look-back, 64 predecessors per probe: lane l inspects tile
-    // base - l; the nearest predecessor with an inclusive prefix ends the
-    // walk, the aggregates in between are summed with a wave reduction.
+    // Decoupled look-back, kLookback * 64 predecessors per probe with all of
+    // their loads in flight together: lane l inspects tiles base - l - 64 k.
+    // The nearest predecessor with an inclusive prefix ends the walk; the
+    // aggregates in between are summed with a wave reduction.  Up to
+    // kLookback * 64 tiles this is one round trip once the predecessors have
+    // published.  Tiles are block indices, so a predecessor may not be running
+    // yet (its workgroup not dispatched while this one waits): after
+    // a.scan_spin probes without progress the wave computes the missing
+    // aggregates itself from the log-weights.  Every wait is therefore bounded
+    // and no dispatch order is assumed.
     uint64_t prefix = 0;
     if (tile > 0) {
       int bse = tile - 1;
+      int spins = 0;
       while (true) {
-        const int idx = bse - lane;
-        const uint64_t word = idx >= 0 ? ld_relaxed(&lb[idx]) : kInclState;
-        const uint64_t state = word >> kStateShift;
-        const uint64_t incl_mask = __ballot(state == 2);
-        const uint64_t notready = __ballot(state == 0);
-        const int first = incl_mask ? __ffsll((unsigned long long)incl_mask) - 1 : 64;
-        const uint64_t need = first >= 63 ? ~0ull : ((1ull << (first + 1)) - 1ull);
-        if (notready & need) {
-          __builtin_amdgcn_s_sleep(1);
-          continue;
+        uint64_t word[kLookback];
+#pragma unroll
+        for (int k = 0; k < kLookback; ++k) {
+          const int idx = bse - lane - 64 * k;
+          word[k] = idx >= 0 ? ld_relaxed(&lb[idx]) : kInclState;
         }
-        const uint64_t v = lane <= first ? (word & kValueMask) : 0;
+        // offset (lane + 64 k) of the nearest inclusive predecessor
+        int first = 64 * kLookback;
+#pragma unroll
+        for (int k = kLookback - 1; k >= 0; --k) {
+          const uint64_t im = __ballot((word[k] >> kStateShift) == 2);
+          if (im) first = 64 * k + __ffsll((unsigned long long)im) - 1;
+        }
+        uint64_t nr[kLookback];
+        bool ready = true;
+#pragma unroll
+        for (int k = 0; k < kLookback; ++k) {
+          nr[k] = __ballot((word[k] >> kStateShift) == 0 && 64 * k + lane <= first);
+          ready = ready && nr[k] == 0;
+        }
+        if (!ready) {
+          if (spins < a.scan_spin) {
+            ++spins;
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+          }
+#pragma unroll
+          for (int k = 0; k < kLookback; ++k) {
+            uint64_t msk = nr[k];
+            while (msk) {
+              const int l = __ffsll((unsigned long long)msk) - 1;
+              msk &= msk - 1;
+              const uint64_t agg = wave_tile_aggregate(a.lw, bse - l - 64 * k, (int64_t)rows * B, J, m, sh);
+              if (lane == l) word[k] = kAggState | agg;
+            }
+          }
+        }
+        spins = 0;
+        uint64_t v = 0;
+#pragma unroll
+        for (int k = 0; k < kLookback; ++k) v += 64 * k + lane <= first ? (word[k] & kValueMask) : 0;
         prefix += wave_sum_u64(v);
-        if (first < 64) break;
-        bse -= 64;
+        if (first < 64 * kLookback) break;
+        bse -= 64 * kLookback;
       }
       if (lane == 0) st_relaxed(&lb[tile], kInclState | (prefix + aggv));
     }
@@ -230,25 +277,16 @@ __global__ __launch_bounds__(512) void k_plain_scan(const PlainArgs a) {
     const int64_t j = tile0 + (int64_t)r * B + tid;
     if (r < rows && j < J) a.C[j] = prefix + tab[r * nw + w] + rowincl[r];
   }
+  // The tile holding the last index finishes the step: its inclusive prefix is
+  // the exact total of the fixed-point weights.  It touches only the step
+  // state and the other parity's words, which no tile of this launch reads.
+  if (tile != ntiles - 1) return;
   if (tid == 0) {
-    const unsigned k = ticket_relaxed(&a.st->ticket[par]);
-    is_last = (k == (unsigned)ntiles - 1u);
-  }
-  __syncthreads();
-  if (!is_last) return;
-  // Last tile: every other tile has published its look-back word before taking
-  // a ticket; the last tile's inclusive prefix is the exact total of the
-  // fixed-point weights, which finishes the step.
-  if (tid == 0) {
-    uint64_t lastw = ld_relaxed(&lb[ntiles - 1]);
-    while ((lastw >> kStateShift) != 2) {
-      __builtin_amdgcn_s_sleep(1);
-      lastw = ld_relaxed(&lb[ntiles - 1]);
-    }
+    const uint64_t total = prefix + tab[n];
     PlainState* st = a.st;
     const float fJ = (float)J, logJ = d_log(fJ);
     const bool collapsed = (m == -__builtin_inff());
-    const double llt = step_loglik(m, lastw & kValueMask, sh);
+    const double llt = step_loglik(m, total, sh);
     const float llf = (float)llt;
     st->loglik = (a.t == 0 ? 0.0 : st->loglik) + llt;
     if (a.cll) a.cll[a.t] = llf;
@@ -260,14 +298,12 @@ __global__ __launch_bounds__(512) void k_plain_scan(const PlainArgs a) {
     st->have_anc = res;
     st->uniform_w = res;
     st->wconst = 0.0f - logJ;
-    st->total = lastw & kValueMask;
+    st->total = total;
     const int64_t o0 = a.ser.obs_off[a.unit];
     const int64_t base = a.ser.tidx0[o0 + a.t] + (int64_t)a.t;
     const int nst = a.ser.nstep[o0 + a.t];
     const Key rk{a.sched[2 * (base + nst)], a.sched[2 * (base + nst) + 1]};
     st->ures = uniform01(bits_at(rk, 0));
-    st->tile_ctr[par ^ 1] = 0;
-    st->ticket[par ^ 1] = 0;
     if (a.t == a.T - 1) *a.loglik_out = st->loglik;
   }
   // clear the next step's look-back words

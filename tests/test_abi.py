@@ -82,3 +82,7 @@ def test_tuning_settings_round_trip_and_reject_bad_values(lib):
     assert lib.mp_tune_set(3, 300) != 0
     assert lib.mp_tune_set(1, 0) != 0
     assert lib.mp_tune_set(99, 1) != 0
+    assert lib.mp_tune_set(7, -1) != 0
+    old = lib.mp_tune_get(7)
+    assert lib.mp_tune_set(7, 0) == 0 and lib.mp_tune_get(7) == 0
+    assert lib.mp_tune_set(7, old) == 0

@@ -194,3 +194,36 @@ def test_launch_shapes_do_not_change_results():
     finally:
         for n, v in saved.items():
             autotune.set(n, v)
+
+
+def test_plain_scan_fallback_and_long_look_back():
+    # Spin limit 0: a scan tile that finds a predecessor's look-back word
+    # unpublished computes that aggregate itself from the log-weights.  With
+    # 256-particle tiles (782 tiles at J = 200000) the look-back also spans
+    # several 256-tile probes.  Both give the bits of the default launch.
+    import pypomp_amd.functional as F
+    from pypomp_amd import _native, autotune
+    lib = _native.lib()
+    struct, theta, _ = lg2_struct(1)
+    keys = tf.split(tf.key(91), 2).reshape(1, 2, 2)
+    J = 200_000
+    ref = F.pfilter(struct, theta[None], J, keys, form="plain")
+    saved = {n: autotune.get(n) for n in ("plain_scan_rows", "plain_scan_threads")}
+    spin = lib.mp_tune_get(_native.MP_TUNE_PLAIN_SCAN_SPIN)
+    try:
+        autotune.set("plain_scan_rows", 1)
+        autotune.set("plain_scan_threads", 256)
+        long_lb = F.pfilter(struct, theta[None], J, keys, form="plain")
+        _native.check(lib.mp_tune_set(_native.MP_TUNE_PLAIN_SCAN_SPIN, 0))
+        fallback = F.pfilter(struct, theta[None], J, keys, form="plain")
+        for n, v in saved.items():
+            autotune.set(n, v)
+        fallback_default_tiles = F.pfilter(struct, theta[None], J, keys, form="plain")
+    finally:
+        for n, v in saved.items():
+            autotune.set(n, v)
+        _native.check(lib.mp_tune_set(_native.MP_TUNE_PLAIN_SCAN_SPIN, spin))
+    for o in (long_lb, fallback, fallback_default_tiles):
+        np.testing.assert_array_equal(o["logLik64"], ref["logLik64"])
+    batched = F.pfilter(struct, theta[None], J, keys, form="batched")
+    np.testing.assert_array_equal(batched["logLik64"], ref["logLik64"])
