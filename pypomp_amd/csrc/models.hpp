@@ -121,9 +121,21 @@ struct LGT {
     }
   }
 
-  __device__ static void step(const ModelDesc&, const P& p, Key k, float, float, float* x) {
-    float z[MD], xn[MD];
+  // The step's noise depends on the particle's key only, not on its state:
+  // kernels may draw it before the ancestor is known (draw + step_z == step).
+  static constexpr int NOISE = MD;
+  __device__ static void draw(const P& p, Key k, float* z) {
     MP_UNROLL for (int i = 0; i < MD; ++i) z[i] = i < p.dx ? normal_at(k, i) : 0.f;
+  }
+
+  __device__ static void step(const ModelDesc& md, const P& p, Key k, float t, float dt, float* x) {
+    float z[MD];
+    draw(p, k, z);
+    step_z(md, p, z, t, dt, x);
+  }
+
+  __device__ static void step_z(const ModelDesc&, const P& p, const float* z, float, float, float* x) {
+    float xn[MD];
     MP_UNROLL for (int i = 0; i < MD; ++i) {
       float a = p.A[i][0] * x[0];
       MP_UNROLL for (int j = 1; j < MD; ++j) if (j < p.dx) a += p.A[i][j] * x[j];
@@ -320,10 +332,20 @@ struct SV {
     x[1] = 0.0f;
   }
 
-  __device__ static void step(const ModelDesc&, const P& p, Key k, float, float, float* x) {
-    float z = normal_at(k, 0);
+  // the step's one normal depends on the particle's key only (draw + step_z == step)
+  static constexpr int NOISE = 1;
+  __device__ static void draw(const P&, Key k, float* z) { z[0] = normal_at(k, 0); }
+
+  __device__ static void step_z(const ModelDesc&, const P& p, const float* zp, float, float, float* x) {
+    const float z = zp[0];
     x[0] = p.mu + p.phi * (x[0] - p.mu) + p.sigma * z;
     x[1] = z;
+  }
+
+  __device__ static void step(const ModelDesc& md, const P& p, Key k, float t, float dt, float* x) {
+    float z[1];
+    draw(p, k, z);
+    step_z(md, p, z, t, dt, x);
   }
 
   // exp(-v/2): the one transcendental of the density, kept by the adjoint's

@@ -72,25 +72,37 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
   // Ancestor search window of this tile.  A(t) = #{i : C_i <= t} is
   // non-decreasing in the comb position, so every j of the tile has
   // A(t_j) in [A(t_tile0), A(t_tile1-1)]; that slice of C is staged in LDS.
-  if (have_anc) {
-    // waves 0 and 1 find the two ends of the window concurrently
-    const int wv = tid >> 6;
-    if (wv < 2) {
-      const int64_t jj = wv == 0 ? tile0 : tile1 - 1;
-      const int64_t v = wave_upper_bound_u64(a.C, 0, J, comb_threshold(st.ures, jj, J, st.total));
-      if ((tid & 63) == 0) s_win[wv] = v;
-    }
-    __syncthreads();
+  // Waves 0 and 1 find the two ends of the window concurrently.
+  const int wv = tid >> 6;
+  if (have_anc && wv < 2) {
+    const int64_t jj = wv == 0 ? tile0 : tile1 - 1;
+    const int64_t v = wave_upper_bound_u64(a.C, 0, J, comb_threshold(st.ures, jj, J, st.total));
+    if ((tid & 63) == 0) s_win[wv] = v;
   }
+  // Models whose step noise depends on the particle's key alone draw it for
+  // the first kPreRows rows here: the other waves do this VALU work while
+  // waves 0 and 1 wait on the search's dependent loads.
+  constexpr int kPreRows = 2;
+  constexpr int NZ = M::NOISE > 0 ? M::NOISE : 1;
+  const bool pre = M::NOISE > 0 && nst == 1;
+  float zpre[kPreRows][NZ];
+  if (pre) {
+    const Key sk = uniform_key(a.sched[2 * base], a.sched[2 * base + 1]);
+    MP_UNROLL for (int r = 0; r < kPreRows; ++r) {
+      const int64_t j = tile0 + (int64_t)r * B + tid;
+      if (r < a.rows_prop && j < J) M::draw(P, split_at(sk, (uint64_t)j), zpre[r]);
+    }
+  }
+  if (have_anc) __syncthreads();
   const int64_t wlo = have_anc ? s_win[0] : 0, whi = have_anc ? s_win[1] : 0;
   const bool in_lds = have_anc && (whi - wlo) <= kWin;
   if (in_lds) {
     for (int64_t i = tid; i < whi - wlo; i += B) s_C[i] = a.C[wlo + i];
     __syncthreads();
   }
-  for (int r = 0; r < a.rows_prop; ++r) {
+  auto row = [&](int r, bool use_z, const float (&zp)[NZ]) {
     const int64_t j = tile0 + (int64_t)r * B + tid;
-    if (j >= J) break;
+    if (j >= J) return;
     float x[M::MAXS];
     int64_t src = j;
     if (t == 0) {
@@ -108,18 +120,25 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
     MP_UNROLL for (int s = 0; s < M::MAXS; ++s)
       if ((md.accum_mask >> s) & 1u) x[s] = 0.f;
     float tt = tstart;
-    for (int k = 0; k < nst; ++k) {
-      const Key sk = uniform_key(a.sched[2 * (base + k)], a.sched[2 * (base + k) + 1]);
-      const float dt = dts[ti0 + k];
-      M::step(md, P, split_at(sk, (uint64_t)j), tt, dt, x);
-      tt = tt + dt;
+    if (use_z) {
+      M::step_z(md, P, zp, tt, dts[ti0], x);
+    } else {
+      for (int k = 0; k < nst; ++k) {
+        const Key sk = uniform_key(a.sched[2 * (base + k)], a.sched[2 * (base + k) + 1]);
+        const float dt = dts[ti0 + k];
+        M::step(md, P, split_at(sk, (uint64_t)j), tt, dt, x);
+        tt = tt + dt;
+      }
     }
     const float w = wprev + M::dmeas(md, P, y, x, ty);
     MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) Xn[(int64_t)s * J + j] = x[s];
     a.lw[j] = w;
     lmax = nanmax(lmax, w);
     lmin = nanmin(lmin, w);
-  }
+  };
+  MP_UNROLL for (int r = 0; r < kPreRows; ++r)
+    if (r < a.rows_prop) row(r, pre, zpre[r]);
+  for (int r = kPreRows; r < a.rows_prop; ++r) row(r, false, zpre[0]);
   block_maxmin(lmax, lmin, red);
   if (tid == 0) {
     a.pmax[blockIdx.x] = lmax;

@@ -158,6 +158,8 @@ struct SEIR {
   static constexpr bool FIXED_S = true;
   static constexpr int MAXP = 10;
   static constexpr bool DIFFERENTIABLE = false;  // integer-valued transitions
+  // the binomial draws depend on the state: no noise can be drawn ahead of it
+  static constexpr int NOISE = 0;
   struct P {
     float beta, amp, sigma, gamma, rho, k, N, S0f, E0, I0;
     float invN;
@@ -202,6 +204,9 @@ struct SEIR {
     x[3] = p.N - S - E - I;
     x[4] = 0.f;
   }
+
+  __device__ static void draw(const P&, Key, float*) {}
+  __device__ static void step_z(const ModelDesc&, const P&, const float*, float, float, float*) {}
 
   __device__ static void step(const ModelDesc&, const P& p, Key key, float t, float dt, float* x) {
     int S = (int)x[0], E = (int)x[1], I = (int)x[2], R = (int)x[3], H = (int)x[4];
