@@ -216,7 +216,7 @@ BatchedPlan plan_batched(int algo, const mp_model_desc* m, const mp_series_desc*
 }
 
 struct PlainPlan {
-  int ntiles, rows, nprop, rows_prop, scan_threads, prop_threads;
+  int ntiles, rows, nprop, rows_prop, prop_len, scan_threads, prop_threads;
   int64_t off_sched, off_params, off_x0, off_x1, off_lw, off_c, off_pmax, off_pmin, off_lb, off_st, total;
 };
 
@@ -231,6 +231,13 @@ PlainPlan plan_plain(const mp_model_desc* m, const mp_series_desc* s, int R, int
   p.ntiles = (J + tile - 1) / tile;
   p.rows_prop = g_plain_prop_rows;
   p.nprop = (J + p.rows_prop * p.prop_threads - 1) / (p.rows_prop * p.prop_threads);
+  // Whole rounds of resident workgroups (8 waves per SIMD: 2048 threads per
+  // CU): past half a round the block count is rounded up to a multiple of a
+  // round and the particles split evenly, so no CU runs one workgroup more
+  // than another.  A trailing block may be left without particles.
+  const int round = num_cus() * (2048 / p.prop_threads);
+  if (p.nprop > round / 2) p.nprop = (p.nprop + round - 1) / round * round;
+  p.prop_len = (int)(((int64_t)J + p.nprop - 1) / p.nprop);
   const SeriesSet& ser = *reinterpret_cast<const SeriesSet*>(s);
   int64_t o = 0;
   auto take = [&](int64_t bytes) {
@@ -302,6 +309,7 @@ int run_plain_pf(const mp_model_desc* m, const mp_series_desc* s, const float* t
   a.lb = reinterpret_cast<uint64_t*>(b + p.off_lb);
   a.ntiles = p.ntiles;
   a.rows_prop = p.rows_prop;
+  a.prop_len = p.prop_len;
   a.nprop = p.nprop;
   a.st = reinterpret_cast<PlainState*>(b + p.off_st);
   a.rows = p.rows;

@@ -129,6 +129,7 @@ struct PlainArgs {
   int rows;             // particle rows per scan tile (tile = rows * blockDim)
   int ntiles;           // scan tiles
   int rows_prop;        // particle rows per propagate block
+  int prop_len;         // particles per propagate block (<= rows_prop * blockDim)
   int nprop;            // propagate blocks
   int scan_spin;        // look-back probes before a tile computes a late predecessor's aggregate
 };

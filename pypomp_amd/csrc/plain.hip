@@ -67,8 +67,15 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
   const bool have_anc = t == 0 ? false : st.have_anc;
 
   float lmax = -__builtin_inff(), lmin = __builtin_inff();
-  const int64_t tile0 = (int64_t)blockIdx.x * a.rows_prop * B;
-  const int64_t tile1 = min((int64_t)J, tile0 + (int64_t)a.rows_prop * B);
+  const int64_t tile0 = (int64_t)blockIdx.x * a.prop_len;
+  const int64_t tile1 = min((int64_t)J, tile0 + (int64_t)a.prop_len);
+  if (tile0 >= J) {  // a trailing block without particles
+    if (tid == 0) {
+      a.pmax[blockIdx.x] = lmax;
+      a.pmin[blockIdx.x] = lmin;
+    }
+    return;
+  }
   // Ancestor search window of this tile.  A(t) = #{i : C_i <= t} is
   // non-decreasing in the comb position, so every j of the tile has
   // A(t_j) in [A(t_tile0), A(t_tile1-1)]; that slice of C is staged in LDS.
@@ -90,7 +97,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
     const Key sk = uniform_key(a.sched[2 * base], a.sched[2 * base + 1]);
     MP_UNROLL for (int r = 0; r < kPreRows; ++r) {
       const int64_t j = tile0 + (int64_t)r * B + tid;
-      if (r < a.rows_prop && j < J) M::draw(P, split_at(sk, (uint64_t)j), zpre[r]);
+      if (r < a.rows_prop && j < tile1) M::draw(P, split_at(sk, (uint64_t)j), zpre[r]);
     }
   }
   if (have_anc) __syncthreads();
@@ -102,7 +109,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
   }
   auto row = [&](int r, bool use_z, const float (&zp)[NZ]) {
     const int64_t j = tile0 + (int64_t)r * B + tid;
-    if (j >= J) return;
+    if (j >= tile1) return;
     float x[M::MAXS];
     int64_t src = j;
     if (t == 0) {
