@@ -198,7 +198,7 @@ int32_t mp_perfilter(const mp_model_desc* h_model, const mp_series_desc* h_serie
 #define MP_TUNE_PLAIN_PROP_ROWS 2 /* rows of 256 particles per propagate block, 1..16 */
 #define MP_TUNE_BATCHED_BLOCK 3   /* batched workgroup size: 0 automatic, 256, 512, 768 */
 #define MP_TUNE_GRAD_BLOCK 4      /* MOP-adjoint workgroup size: 0 automatic, 256, 512 */
-#define MP_TUNE_PLAIN_SCAN_THREADS 5 /* plain scan workgroup size: 256, 512 */
+#define MP_TUNE_PLAIN_SCAN_THREADS 5 /* plain scan workgroup size: 256, 512, 1024 */
 #define MP_TUNE_PLAIN_PROP_THREADS 6 /* plain propagate workgroup size: 256, 512 */
 #define MP_TUNE_PLAIN_SCAN_SPIN 7    /* look-back probes before a scan tile computes a late
                                         predecessor's aggregate itself, 0..2^20 (default 256) */

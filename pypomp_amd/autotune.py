@@ -25,7 +25,7 @@ PARAMS = {
     "plain_prop_rows": (_native.MP_TUNE_PLAIN_PROP_ROWS, (1, 2, 4)),
     "batched_block": (_native.MP_TUNE_BATCHED_BLOCK, (256, 512, 768)),
     "grad_block": (_native.MP_TUNE_GRAD_BLOCK, (256, 512)),
-    "plain_scan_threads": (_native.MP_TUNE_PLAIN_SCAN_THREADS, (256, 512)),
+    "plain_scan_threads": (_native.MP_TUNE_PLAIN_SCAN_THREADS, (256, 512, 1024)),
     "plain_prop_threads": (_native.MP_TUNE_PLAIN_PROP_THREADS, (256, 512)),
 }
 

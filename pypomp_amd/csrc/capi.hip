@@ -73,7 +73,7 @@ constexpr int64_t kParamSlot = 512;  // bytes per filter of prepared model param
 // Defaults from a sweep on LG 2-D, J = 1e6 (2.22e10 particle-steps/s).
 int g_plain_scan_rows = 8;  // particle rows per scan tile (1..16)
 int g_plain_prop_rows = 2;  // particle rows per propagate block (1..16)
-int g_plain_scan_threads = 512;  // scan workgroup size (256, 512); rows * waves <= 64
+int g_plain_scan_threads = 512;  // scan workgroup size (256, 512, 1024); rows * waves <= 64
 int g_plain_prop_threads = 512;  // propagate workgroup size (256, 512)
 // look-back probes a scan tile spends on a late predecessor before computing its
 // aggregate itself (0: at once, which forces the fallback path)
@@ -678,9 +678,13 @@ int32_t mp_tune_set(int32_t param, int32_t value) {
       g_batched_block = value;
       return MP_OK;
     case MP_TUNE_PLAIN_SCAN_THREADS:
+      if (value != 256 && value != 512 && value != 1024)
+        return fail(MP_ERR_ARG, "plain scan workgroup size must be 256, 512 or 1024");
+      g_plain_scan_threads = value;
+      return MP_OK;
     case MP_TUNE_PLAIN_PROP_THREADS:
       if (value != 256 && value != 512) return fail(MP_ERR_ARG, "plain workgroup size must be 256 or 512");
-      (param == MP_TUNE_PLAIN_SCAN_THREADS ? g_plain_scan_threads : g_plain_prop_threads) = value;
+      g_plain_prop_threads = value;
       return MP_OK;
     case MP_TUNE_PLAIN_SCAN_SPIN:
       if (value < 0 || value > (1 << 20)) return fail(MP_ERR_ARG, "scan spin limit must be 0..2^20");

@@ -166,7 +166,7 @@ __device__ uint64_t wave_tile_aggregate(const float* lw, int64_t idx, int64_t ti
 }
 
 template <class M>
-__global__ __launch_bounds__(512) void k_plain_scan(const PlainArgs a) {
+__global__ __launch_bounds__(1024) void k_plain_scan(const PlainArgs a) {
   __shared__ double red[32];
   __shared__ uint64_t tab[65];  // rows * waves <= 64 row/wave totals + the tile total
   __shared__ uint64_t s_prefix;
