@@ -91,6 +91,20 @@ struct WMax {
   static constexpr int v = M::MAXS + M::MAXP + 1;
 };
 
+// Sub-steps per observation whose particle-independent values fit the LDS table.
+constexpr int kMaxSub = 64;
+
+template <class M, bool ON>
+struct HoistTables {  // models without particle-independent terms: nothing
+  struct Sub {};
+  struct Obs {};
+};
+template <class M>
+struct HoistTables<M, true> {
+  using Sub = typename M::Sub;
+  using Obs = typename M::Obs;
+};
+
 template <class M, int ALGO>
 __device__ __forceinline__ void run_filter(const BatchArgs& a, const int f, char* buf, double* red, uint64_t* tab) {
   constexpr int WM = (ALGO == ALGO_PER) ? WMax<M>::v : M::MAXS;
@@ -189,6 +203,27 @@ __device__ __forceinline__ void run_filter(const BatchArgs& a, const int f, char
     for (int s = 0; s < M::MAXS; ++s) pm[s] = 0.0;
     float* Xc = X[cur];
     float* Xn = X[cur ^ 1];
+    // With one parameter set for all particles, the model's particle-
+    // independent sub-step values (thread k: sub-step k, its start time summed
+    // in the order the step loop adds) and observation term (one thread of
+    // another wave) are computed once here.  The previous observation's
+    // reductions end with a barrier, so the tables are free to overwrite.
+    constexpr bool kHoist = M::HOIST && ALGO != ALGO_PER;
+    using HT = HoistTables<M, kHoist>;
+    __shared__ typename HT::Sub s_sub[kHoist ? kMaxSub : 1];
+    __shared__ typename HT::Obs s_obs;
+    const bool hoisted = kHoist && nst <= kMaxSub;
+    if constexpr (kHoist) {
+      if (hoisted) {
+        if (tid < nst) {
+          float tk = tstart;
+          for (int k = 0; k < tid; ++k) tk = tk + dts[ti0 + k];
+          s_sub[tid] = M::substep(P, tk, dts[ti0 + tid]);
+        }
+        if (tid == WAVE) s_obs = M::obs(P, y);
+        __syncthreads();
+      }
+    }
 
     for (int j = tid; j < J; j += B) {
       int src = have_anc ? ancestor_seg(Cc, tab, J, ures, total, j) : j;
@@ -211,13 +246,25 @@ __device__ __forceinline__ void run_filter(const BatchArgs& a, const int f, char
         Pj = P;
       }
       float tt = tstart;
-      for (int k = 0; k < nst; ++k) {
-        const Key sk = ld_key(sch, base + k);
-        const float dt = dts[ti0 + k];
-        M::step(md, Pj, split_at(sk, (uint64_t)j), tt, dt, x);
-        tt = tt + dt;
+      float meas;
+      if constexpr (kHoist) {
+        if (hoisted) {
+          for (int k = 0; k < nst; ++k) {
+            const Key sk = ld_key(sch, base + k);
+            M::step_sub(Pj, s_sub[k], split_at(sk, (uint64_t)j), dts[ti0 + k], x);
+          }
+          meas = M::dmeas_obs(Pj, s_obs, y, x);
+        }
       }
-      float meas = M::dmeas(md, Pj, y, x, ty);
+      if (!hoisted) {
+        for (int k = 0; k < nst; ++k) {
+          const Key sk = ld_key(sch, base + k);
+          const float dt = dts[ti0 + k];
+          M::step(md, Pj, split_at(sk, (uint64_t)j), tt, dt, x);
+          tt = tt + dt;
+        }
+        meas = M::dmeas(md, Pj, y, x, ty);
+      }
       if (ALGO == ALGO_PER) {
         if (meas != meas) meas = nan_sub;
         else if (meas == __builtin_inff()) meas = 3.40282347e+38f;

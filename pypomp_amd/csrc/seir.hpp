@@ -208,16 +208,33 @@ struct SEIR {
   __device__ static void draw(const P&, Key, float*) {}
   __device__ static void step_z(const ModelDesc&, const P&, const float*, float, float, float*) {}
 
-  __device__ static void step(const ModelDesc&, const P& p, Key key, float t, float dt, float* x) {
+  // Per-sub-step and per-observation values that do not depend on the
+  // particle: a filter whose particles share one parameter set computes them
+  // once per sub-step / observation (step_sub, dmeas_obs) with the same
+  // operations as step / dmeas.
+  static constexpr bool HOIST = true;
+  struct Sub {
+    float bt, pEI, pIR;
+  };
+  struct Obs {
+    double c;  // lgamma(y + k) - lgamma(k) - lgamma(y + 1)
+  };
+
+  __device__ static Sub substep(const P& p, float t, float dt) {
+    Sub sb;
+    sb.bt = p.beta * (1.0f + p.amp * d_cos(6.28318531f * t / 52.0f));
+    sb.pEI = -d_expm1(-p.sigma * dt);
+    sb.pIR = -d_expm1(-p.gamma * dt);
+    return sb;
+  }
+
+  __device__ static void step_sub(const P& p, const Sub& sb, Key key, float dt, float* x) {
     int S = (int)x[0], E = (int)x[1], I = (int)x[2], R = (int)x[3], H = (int)x[4];
-    const float bt = p.beta * (1.0f + p.amp * d_cos(6.28318531f * t / 52.0f));
-    const float foi = bt * (float)I * p.invN;
+    const float foi = sb.bt * (float)I * p.invN;
     const float pSE = -d_expm1(-foi * dt);
-    const float pEI = -d_expm1(-p.sigma * dt);
-    const float pIR = -d_expm1(-p.gamma * dt);
     const int dSE = fast_binomial_i(split_at(key, 0), S, pSE);
-    const int dEI = fast_binomial_i(split_at(key, 1), E, pEI);
-    const int dIR = fast_binomial_i(split_at(key, 2), I, pIR);
+    const int dEI = fast_binomial_i(split_at(key, 1), E, sb.pEI);
+    const int dIR = fast_binomial_i(split_at(key, 2), I, sb.pIR);
     S -= dSE;
     E += dSE - dEI;
     I += dEI - dIR;
@@ -230,15 +247,28 @@ struct SEIR {
     x[4] = (float)H;
   }
 
+  __device__ static void step(const ModelDesc&, const P& p, Key key, float t, float dt, float* x) {
+    step_sub(p, substep(p, t, dt), key, dt, x);
+  }
+
+  __device__ static Obs obs(const P& p, const float* y) {
+    const double yy = (double)rintf(y[0]);
+    const double k = (double)p.k;
+    return Obs{dd_lgamma(yy + k) - p.lgk - dd_lgamma(yy + 1.0)};
+  }
+
   // NegBin log pmf evaluated in float64 (its three log-gamma terms cancel to
   // far below float32 resolution at counts ~1e4-1e5).
-  __device__ static float dmeas(const ModelDesc&, const P& p, const float* y, const float* x, float) {
+  __device__ static float dmeas_obs(const P& p, const Obs& o, const float* y, const float* x) {
     const double mu = (double)fmaxf(x[4] * p.rho, 1e-10f);
     const double yy = (double)rintf(y[0]);
     const double k = (double)p.k;
-    const double v = dd_lgamma(yy + k) - p.lgk - dd_lgamma(yy + 1.0) + k * dd_log(k / (k + mu)) +
-                     yy * dd_log(mu / (k + mu));
+    const double v = o.c + k * dd_log(k / (k + mu)) + yy * dd_log(mu / (k + mu));
     return (float)v;
+  }
+
+  __device__ static float dmeas(const ModelDesc&, const P& p, const float* y, const float* x, float) {
+    return dmeas_obs(p, obs(p, y), y, x);
   }
 
   // reports y ~ NegBin(mean max(rho H, 1e-10), size k) from one uniform
