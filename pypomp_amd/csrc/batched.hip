@@ -24,13 +24,25 @@ __device__ __forceinline__ void st_key(uint32_t* p, int64_t i, Key k) {
   p[2 * i + 1] = k.k1;
 }
 
+__device__ __forceinline__ Key shfl_key(Key k, int lane) {
+  return Key{(uint32_t)__shfl((int)k.k0, lane, 64), (uint32_t)__shfl((int)k.k1, lane, 64)};
+}
+
 // Key schedule of a bootstrap / MOP filter (pfilter.py:23-25,105,160 and the
 // sub-step chain of model_mechanics.py:904): per observation t, at
 // tidx0[t] + t, the nstep[t] sub-step keys followed by the resampling key.
+// Per observation the reference's tree is (key1, sub) = split(key), the
+// sub-step chain (out, sub) <- split(sub), then (key, out) <- split(key1).
+// Four lanes per filter evaluate sibling splits together (one threefry
+// instruction stream, a different counter per lane): lanes 0 and 1 the
+// sub-step chain, lanes 2 and 3 the split of key1 alongside its first link, so
+// an observation costs 1 + nstep dependent hashes instead of 2 nstep + 4.
 __global__ void k_sched_pf(const uint32_t* __restrict__ keys, SeriesSet ser, const int32_t* __restrict__ unit,
                            int64_t sched_len, uint32_t* __restrict__ sched, int R) {
-  int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= R) return;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = g >> 2, c = g & 1, hi = (g >> 1) & 1;
+  if (f >= R) return;  // whole quads: all four lanes share f
+  const int qb = (threadIdx.x & 63) & ~3;  // the quad's first lane in its wave
   const int u = unit ? unit[f] : 0;
   const int64_t o0 = ser.obs_off[u];
   const int T = (int)(ser.obs_off[u + 1] - o0);
@@ -38,15 +50,20 @@ __global__ void k_sched_pf(const uint32_t* __restrict__ keys, SeriesSet ser, con
   Key key = split_at(ld_key(keys, f), 0);  // split(key, J+1)[0]
   int64_t pos = 0;
   for (int t = 0; t < T; ++t) {
-    Key sub = split_at(key, 1);
-    key = split_at(key, 0);
+    const Key a = split_at(key, (uint64_t)c);  // lane c: split(key)[c]
+    const Key key1 = shfl_key(a, qb), sub0 = shfl_key(a, qb + 1);
     const int n = ser.nstep[o0 + t];
-    for (int k = 0; k < n; ++k) {
-      st_key(out, pos++, split_at(sub, 1));
-      sub = split_at(sub, 0);
+    Key sub = sub0, next = key1;
+    for (int k = 0; k < (n > 0 ? n : 1); ++k) {
+      const Key r = split_at(hi ? key1 : sub, (uint64_t)c);
+      if (!hi && c == 1 && k < n) st_key(out, pos + k, r);  // sub-step key k
+      if (hi && c == 1 && k == 0) st_key(out, pos + n, r);  // resampling key
+      const Key r0 = shfl_key(r, qb), r2 = shfl_key(r, qb + 2);
+      sub = r0;
+      if (k == 0) next = r2;
     }
-    st_key(out, pos++, split_at(key, 1));
-    key = split_at(key, 0);
+    pos += n + 1;
+    key = next;
   }
 }
 
@@ -476,7 +493,7 @@ hipError_t launch_sched(int algo, const uint32_t* keys, const SeriesSet& ser, co
     dim3 grid((ser.Tmax + 1 + 127) / 128, R);
     hipLaunchKernelGGL(k_sched_per, grid, dim3(128), 0, st, keys, ser, unit, sched_len, sched, R);
   } else {
-    hipLaunchKernelGGL(k_sched_pf, dim3((R + 63) / 64), dim3(64), 0, st, keys, ser, unit, sched_len, sched, R);
+    hipLaunchKernelGGL(k_sched_pf, dim3((4 * R + 63) / 64), dim3(64), 0, st, keys, ser, unit, sched_len, sched, R);
   }
   return hipGetLastError();
 }
