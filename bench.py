@@ -33,7 +33,7 @@ HBM_PEAK_GBS = 8000.0
 MEASURED_TRAFFIC_J1E6 = {"plain_prop": 34.24e6, "plain_scan": 12.51e6}
 # HBM bytes per launch of the SV MOP-gradient kernel at J=1000, T=1000, K=50, 512
 # filters (FETCH_SIZE x 2 + WRITE_SIZE, copy-calibrated)
-MEASURED_TRAFFIC_MOP = 14.77e9
+MEASURED_TRAFFIC_MOP = 19.03e9
 # HBM bytes per launch of the batched SEIR filter at J=20000, T=260, 256 filters
 # (profiles/seir_batched_J20000_T260_R256_kernel_trace.md)
 MEASURED_TRAFFIC_SEIR = 86.33e9
@@ -41,7 +41,7 @@ MEASURED_TRAFFIC_SEIR = 86.33e9
 # particle-steps per dispatch, rocprofv3 --pmc; profiles/valu_plain_lg2_seir_mopgrad.md).
 # The kernels are VALU- or latency-bound rather than HBM-bound, so each roofline
 # object also carries the VALU issue rate implied by the live kernel time.
-VALU_INSTR_PER_PSTEP = {"plain_prop": 751.0, "plain_scan": 148.0, "seir": 10578.0, "mop_grad": 1144.0}
+VALU_INSTR_PER_PSTEP = {"plain_prop": 751.0, "plain_scan": 148.0, "seir": 10578.0, "mop_grad": 773.0}
 # one wave64 VALU instruction per CU per cycle: 256 CUs x 2.4 GHz
 VALU_PEAK_GINSTR = 256 * 2.4
 

@@ -30,7 +30,8 @@ hipError_t launch_order(const SeriesSet& ser, const int32_t* unit, int R, int32_
 
 namespace mp {
 hipError_t launch_mop_grad(const GradArgs& a, int block, int grid, hipStream_t st);
-int64_t mop_grad_scratch_bytes(int S, int J, int T, int K, bool lds_work);
+int64_t mop_grad_scratch_bytes(int S, int nz, int J, int T, int K, bool lds_work);
+int mop_grad_noise_words(const ModelDesc& md);
 bool mop_grad_supported(const ModelDesc& md);
 hipError_t launch_simulate(const SimArgs& a, hipStream_t st);
 size_t mop_grad_lds_bytes(int S, int J, int block, bool lds_work);
@@ -368,7 +369,8 @@ GradPlan plan_grad(const mp_model_desc* m, const mp_series_desc* s, int R, int J
   p.block = g_grad_block ? g_grad_block : (J >= 512 ? 512 : 256);
   // per-step arrays in LDS when two workgroups still fit per CU
   p.lds_work = mop_grad_lds_bytes(m->S, J, p.block, true) <= (size_t)max_lds_bytes() / 2;
-  p.per_filter = align256(mop_grad_scratch_bytes(m->S, J, s->Tmax, K, p.lds_work));
+  p.per_filter = align256(mop_grad_scratch_bytes(
+      m->S, mop_grad_noise_words(*reinterpret_cast<const ModelDesc*>(m)), J, s->Tmax, K, p.lds_work));
   p.total = (size_t)p.sched_bytes + (size_t)R * (size_t)p.per_filter + 256;
   return p;
 }

@@ -34,10 +34,10 @@
 
 namespace mp {
 
-// Per-filter global scratch: checkpoints, the ancestry log, per-step
-// statistics and the recomputed K-step block.
+// Per-filter global scratch: checkpoints, the ancestry log, the step-noise
+// log, per-step statistics and the recomputed K-step block.
 struct GradScratch {
-  int64_t ckx, anc, stat, bx, work, total;
+  int64_t ckx, anc, noise, stat, bx, work, total;
 };
 
 // Per-step working arrays.  The forward set (prefix sums, log-weights, two
@@ -70,7 +70,8 @@ __host__ __device__ inline GradWork grad_work_layout(int S, int J) {
   return w;
 }
 
-__host__ __device__ inline GradScratch grad_scratch_layout(int S, int J, int T, int K, bool lds_work) {
+// nz: noise words per particle-step (the model's NOISE)
+__host__ __device__ inline GradScratch grad_scratch_layout(int S, int nz, int J, int T, int K, bool lds_work) {
   GradScratch g;
   const int64_t nblk = (T + K - 1) / K;
   int64_t o = 0;
@@ -81,6 +82,7 @@ __host__ __device__ inline GradScratch grad_scratch_layout(int S, int J, int T, 
   };
   g.ckx = take(nblk * S * (int64_t)J * 4);
   g.anc = take((int64_t)T * J * 4);
+  g.noise = take((int64_t)T * nz * J * 4);
   g.stat = take((int64_t)T * 16);
   g.bx = take((int64_t)K * S * J * 4);
   g.work = take(lds_work ? 0 : grad_work_layout(S, J).total);
@@ -89,20 +91,25 @@ __host__ __device__ inline GradScratch grad_scratch_layout(int S, int J, int T, 
 }
 
 // Forward pass (value): the ancestry a_t of every step (a_0 = identity), the
-// maximum and FP64 exp-sum of each step's measurement densities, and the
-// pre-step states at every K-th step.  Reverse sweep, block by block: the
-// block's states are recomputed from its checkpoint with the logged ancestry
-// (gather and step), then the adjoint runs backwards through the block.
+// step noise of every particle-step (models whose noise depends on the
+// particle's key alone: draw + step_z == step), the maximum and FP64 exp-sum
+// of each step's measurement densities, and the pre-step states at every K-th
+// step.  Reverse sweep, block by block: the block's states are recomputed from
+// its checkpoint with the logged ancestry and noise (gather and step_z, no
+// random numbers drawn again), then the adjoint runs backwards through the
+// block.
 template <class M>
 __device__ __forceinline__ void mop_grad_filter(const GradArgs& a, const int f, char* buf, char* wbuf, double* red,
                                                 uint64_t* tab) {
   const ModelDesc md = a.md;
   const int J = a.J, S = M::FIXED_S ? M::MAXS : md.S, d = md.d, T = a.ser.Tmax, K = a.K;
   const int tid = threadIdx.x, B = blockDim.x;
-  const GradScratch L = grad_scratch_layout(S, J, T, K, a.lds_work);
+  constexpr int NZ = M::NOISE > 0 ? M::NOISE : 1;
+  const GradScratch L = grad_scratch_layout(S, M::NOISE, J, T, K, a.lds_work);
   const GradWork W = grad_work_layout(S, J);
   float* ckx = reinterpret_cast<float*>(buf + L.ckx);
   int32_t* ancl = reinterpret_cast<int32_t*>(buf + L.anc);
+  float* zlog = reinterpret_cast<float*>(buf + L.noise);  // (T, NOISE, J)
   double* stat = reinterpret_cast<double*>(buf + L.stat);
   float* bx = reinterpret_cast<float*>(buf + L.bx);
   uint64_t* Cc = reinterpret_cast<uint64_t*>(wbuf + W.c);
@@ -174,9 +181,11 @@ __device__ __forceinline__ void mop_grad_filter(const GradArgs& a, const int f, 
       const int src = t == 0 ? j : ancestor_seg(Cc, tab, J, ures, total, j);
       if (ck) MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) ckt[s * (int64_t)J + j] = XA[s * (int64_t)J + j];
       anct[j] = src;
-      float x[M::MAXS];
+      float x[M::MAXS], z[NZ];
       MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) x[s] = XA[s * (int64_t)J + src];
-      M::step(md, P, split_at(sk, (uint64_t)j), ts, dt, x);
+      M::draw(P, split_at(sk, (uint64_t)j), z);
+      MP_UNROLL for (int i = 0; i < M::NOISE; ++i) zlog[((int64_t)t * M::NOISE + i) * J + j] = z[i];
+      M::step_z(md, P, z, ts, dt, x);
       const float m = M::dmeas(md, P, y, x, tt);
       MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) XB[s * (int64_t)J + j] = x[s];
       lw[j] = m;
@@ -211,7 +220,6 @@ __device__ __forceinline__ void mop_grad_filter(const GradArgs& a, const int f, 
     // evaluated where the backward step needs them)
     for (int i = 0; i < n; ++i) {
       const int t = t0 + i;
-      const Key sk = step_key(t);
       const float dt = dts[tidx0[t]];
       const float ts = tstart(t);
       const float* xin = state(i);
@@ -219,9 +227,10 @@ __device__ __forceinline__ void mop_grad_filter(const GradArgs& a, const int f, 
       const int32_t* anct = ancl + (int64_t)t * J;
       for (int j = tid; j < J; j += B) {
         const int src = anct[j];
-        float x[M::MAXS];
+        float x[M::MAXS], z[NZ];
         MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) x[s] = xin[s * (int64_t)J + src];
-        M::step(md, P, split_at(sk, (uint64_t)j), ts, dt, x);
+        MP_UNROLL for (int q = 0; q < M::NOISE; ++q) z[q] = zlog[((int64_t)t * M::NOISE + q) * J + j];
+        M::step_z(md, P, z, ts, dt, x);
         MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) xout[s * (int64_t)J + j] = x[s];
       }
       __syncthreads();
@@ -325,7 +334,8 @@ __device__ __forceinline__ void mop_grad_body(const GradArgs& a) {
       mop_grad_filter<M>(a, f, buf, lwork, red, tab);
     } else {
       const int S = M::FIXED_S ? M::MAXS : a.md.S;
-      mop_grad_filter<M>(a, f, buf, buf + grad_scratch_layout(S, a.J, a.ser.Tmax, a.K, false).work, red, tab);
+      mop_grad_filter<M>(a, f, buf, buf + grad_scratch_layout(S, M::NOISE, a.J, a.ser.Tmax, a.K, false).work, red,
+                         tab);
     }
   }
 }
@@ -396,8 +406,20 @@ bool mop_grad_supported(const ModelDesc& md) {
   return false;
 }
 
-int64_t mop_grad_scratch_bytes(int S, int J, int T, int K, bool lds_work) {
-  return grad_scratch_layout(S, J, T, K, lds_work).total;
+int64_t mop_grad_scratch_bytes(int S, int nz, int J, int T, int K, bool lds_work) {
+  return grad_scratch_layout(S, nz, J, T, K, lds_work).total;
+}
+
+// Noise words per particle-step the gradient logs for this model.
+int mop_grad_noise_words(const ModelDesc& md) {
+  switch (md.kind) {
+    case MODEL_SV: return SV::NOISE;
+    case MODEL_LG:
+      if (md.dx == 1 && md.dy == 1) return LG1::NOISE;
+      if (md.dx == 2 && md.dy == 2) return LG2::NOISE;
+      return 0;
+  }
+  return 0;
 }
 
 }  // namespace mp
