@@ -111,15 +111,23 @@ struct WMax {
 // Sub-steps per observation whose particle-independent values fit the LDS table.
 constexpr int kMaxSub = 64;
 
-template <class M, bool ON>
-struct HoistTables {  // models without particle-independent terms: nothing
+// LDS table entries: MODE 0 none, 1 the particle-independent sub-step and
+// observation values (one parameter set for all particles), 2 their parts
+// that depend on the time or the data only (per-particle parameters).
+template <class M, int MODE>
+struct HoistTables {
   struct Sub {};
   struct Obs {};
 };
 template <class M>
-struct HoistTables<M, true> {
+struct HoistTables<M, 1> {
   using Sub = typename M::Sub;
   using Obs = typename M::Obs;
+};
+template <class M>
+struct HoistTables<M, 2> {
+  using Sub = typename M::Tab;
+  using Obs = typename M::TabObs;
 };
 
 template <class M, int ALGO>
@@ -220,24 +228,30 @@ __device__ __forceinline__ void run_filter(const BatchArgs& a, const int f, char
     for (int s = 0; s < M::MAXS; ++s) pm[s] = 0.0;
     float* Xc = X[cur];
     float* Xn = X[cur ^ 1];
-    // With one parameter set for all particles, the model's particle-
-    // independent sub-step values (thread k: sub-step k, its start time summed
-    // in the order the step loop adds) and observation term (one thread of
-    // another wave) are computed once here.  The previous observation's
-    // reductions end with a barrier, so the tables are free to overwrite.
-    constexpr bool kHoist = M::HOIST && ALGO != ALGO_PER;
+    // The model's particle-independent sub-step values (thread k: sub-step k,
+    // its start time summed in the order the step loop adds) and observation
+    // term (one thread of another wave) are computed once here: all of them
+    // with one parameter set for all particles (mode 1), their time- and
+    // data-only parts with per-particle parameters (mode 2, IF2).  The previous
+    // observation's reductions end with a barrier, so the tables are free to
+    // overwrite.
+    constexpr int kHoist = !M::HOIST ? 0 : (ALGO == ALGO_PER ? 2 : 1);
     using HT = HoistTables<M, kHoist>;
     __shared__ typename HT::Sub s_sub[kHoist ? kMaxSub : 1];
     __shared__ typename HT::Obs s_obs;
     const bool hoisted = kHoist && nst <= kMaxSub;
-    if constexpr (kHoist) {
+    if constexpr (kHoist != 0) {
       if (hoisted) {
         if (tid < nst) {
           float tk = tstart;
           for (int k = 0; k < tid; ++k) tk = tk + dts[ti0 + k];
-          s_sub[tid] = M::substep(P, tk, dts[ti0 + tid]);
+          if constexpr (kHoist == 1) s_sub[tid] = M::substep(P, tk, dts[ti0 + tid]);
+          else s_sub[tid] = M::tab_sub(tk);
         }
-        if (tid == WAVE) s_obs = M::obs(P, y);
+        if (tid == WAVE) {
+          if constexpr (kHoist == 1) s_obs = M::obs(P, y);
+          else s_obs = M::tab_obs(y);
+        }
         __syncthreads();
       }
     }
@@ -264,13 +278,16 @@ __device__ __forceinline__ void run_filter(const BatchArgs& a, const int f, char
       }
       float tt = tstart;
       float meas;
-      if constexpr (kHoist) {
+      if constexpr (kHoist != 0) {
         if (hoisted) {
           for (int k = 0; k < nst; ++k) {
             const Key sk = ld_key(sch, base + k);
-            M::step_sub(Pj, s_sub[k], split_at(sk, (uint64_t)j), dts[ti0 + k], x);
+            const float dtk = dts[ti0 + k];
+            if constexpr (kHoist == 1) M::step_sub(Pj, s_sub[k], split_at(sk, (uint64_t)j), dtk, x);
+            else M::step_sub(Pj, M::substep_tab(Pj, s_sub[k], dtk), split_at(sk, (uint64_t)j), dtk, x);
           }
-          meas = M::dmeas_obs(Pj, s_obs, y, x);
+          if constexpr (kHoist == 1) meas = M::dmeas_obs(Pj, s_obs, y, x);
+          else meas = M::dmeas_obs(Pj, M::obs_tab(Pj, s_obs, y), y, x);
         }
       }
       if (!hoisted) {

@@ -211,8 +211,16 @@ struct SEIR {
   // Per-sub-step and per-observation values that do not depend on the
   // particle: a filter whose particles share one parameter set computes them
   // once per sub-step / observation (step_sub, dmeas_obs) with the same
-  // operations as step / dmeas.
+  // operations as step / dmeas.  Tab and TabObs are the parts that depend on
+  // the time or the data only, shared even when every particle carries its own
+  // parameters (IF2).
   static constexpr bool HOIST = true;
+  struct Tab {
+    float cs;  // seasonal cosine at the sub-step's start time
+  };
+  struct TabObs {
+    double lgy1;  // lgamma(y + 1)
+  };
   struct Sub {
     float bt, pEI, pIR;
   };
@@ -220,13 +228,17 @@ struct SEIR {
     double c;  // lgamma(y + k) - lgamma(k) - lgamma(y + 1)
   };
 
-  __device__ static Sub substep(const P& p, float t, float dt) {
+  __device__ static Tab tab_sub(float t) { return Tab{d_cos(6.28318531f * t / 52.0f)}; }
+
+  __device__ static Sub substep_tab(const P& p, const Tab& tb, float dt) {
     Sub sb;
-    sb.bt = p.beta * (1.0f + p.amp * d_cos(6.28318531f * t / 52.0f));
+    sb.bt = p.beta * (1.0f + p.amp * tb.cs);
     sb.pEI = -d_expm1(-p.sigma * dt);
     sb.pIR = -d_expm1(-p.gamma * dt);
     return sb;
   }
+
+  __device__ static Sub substep(const P& p, float t, float dt) { return substep_tab(p, tab_sub(t), dt); }
 
   __device__ static void step_sub(const P& p, const Sub& sb, Key key, float dt, float* x) {
     int S = (int)x[0], E = (int)x[1], I = (int)x[2], R = (int)x[3], H = (int)x[4];
@@ -251,11 +263,15 @@ struct SEIR {
     step_sub(p, substep(p, t, dt), key, dt, x);
   }
 
-  __device__ static Obs obs(const P& p, const float* y) {
+  __device__ static TabObs tab_obs(const float* y) { return TabObs{dd_lgamma((double)rintf(y[0]) + 1.0)}; }
+
+  __device__ static Obs obs_tab(const P& p, const TabObs& to, const float* y) {
     const double yy = (double)rintf(y[0]);
     const double k = (double)p.k;
-    return Obs{dd_lgamma(yy + k) - p.lgk - dd_lgamma(yy + 1.0)};
+    return Obs{dd_lgamma(yy + k) - p.lgk - to.lgy1};
   }
+
+  __device__ static Obs obs(const P& p, const float* y) { return obs_tab(p, tab_obs(y), y); }
 
   // NegBin log pmf evaluated in float64 (its three log-gamma terms cancel to
   // far below float32 resolution at counts ~1e4-1e5).
