@@ -220,18 +220,73 @@ def cosine_cooling(i, M, c):
     return F32(F32(c) + (F32(1) - F32(c)) * F32(0.5) * (F32(1) + np.cos(F32(np.pi) * F32(i) / F32(M), dtype=F32)))
 
 
+def fmadam_direction(grad, m1, v1, step, beta1=0.9, beta2=0.999, eps=1e-4):
+    """Full-matrix Adam (core/optimizer.py:157-213) on one parameter vector:
+    returns (direction, m, F)."""
+    F32 = np.float32
+    m1 = (F32(beta1) * m1 + F32(1 - beta1) * grad).astype(F32)
+    mh = (m1 / (F32(1) - F32(beta1) ** F32(step + 1))).astype(F32)
+    v1 = (F32(beta2) * v1 + F32(1 - beta2) * np.outer(grad, grad)).astype(F32)
+    fh = (v1 / (F32(1) - F32(beta2) ** F32(step + 1))).astype(F32)
+    ev, vec = np.linalg.eigh(fh)
+    inv = (F32(1) / np.sqrt(np.maximum(ev, F32(0)) + F32(eps))).astype(F32)
+    fis = ((vec * inv[None, :]) @ vec.T).astype(F32)
+    return (-(fis @ mh)).astype(F32), m1, v1
+
+
+def bfgs_direction(grad, hess, prev, step, eta_i):
+    """BFGS on the inverse Hessian (core/optimizer.py:216-267) on one parameter
+    vector, the previous direction taken as -prev as the reference does:
+    returns (direction, H, grad)."""
+    F32 = np.float32
+    if step <= 1:
+        return (-grad).astype(F32), hess, grad
+    s = (F32(np.mean(np.asarray(eta_i, F32), dtype=F32)) * (-prev)).astype(F32)
+    y = (grad - prev).astype(F32)
+    rho = F32(1) / F32(np.dot(y, s))
+    hy = (hess @ y).astype(F32)
+    yhy = F32(np.dot(y, hy))
+    new = (hess - rho * np.outer(s, hy) - rho * np.outer(hy, s)
+           + (rho * (rho * yhy + F32(1))) * np.outer(s, s)).astype(F32)
+    if not np.isfinite(rho):
+        new = hess
+    return (-(new @ grad)).astype(F32), new, grad
+
+
+def armijo(obj, curr, pt, grad, direction, eta, c=0.1, max_itn=10, frac=0.5):
+    """Step size of the reference's backtracking line search
+    (core/algorithms/train.py:442-499, stoch=False)."""
+    F32 = np.float32
+    eta = F32(eta)
+    slope = F32(np.sum(grad * direction, dtype=F32))
+    itn = 0
+    while True:
+        nxt = F32(obj((pt + eta * direction).astype(F32)))
+        cont = (nxt > curr + (eta * F32(c)) * slope) or np.isnan(nxt)
+        if not (cont and itn < max_itn):
+            return eta
+        eta = F32(eta * F32(frac))
+        itn += 1
+
+
 def train(value_and_grad, theta_est0, key, T, M, eta, alpha, alpha_cooling=1.0, beta1=0.9, beta2=0.999,
-          eps=1e-8, clip_norm=None):
-    """``_train_internal`` / ``_train_scan_step`` with the Adam optimizer
-    (core/optimizer.py:119-153) and one monitor (n_monitors = 1: the MOP value
-    of the same draw is the reported objective).  ``value_and_grad(theta, key,
-    alpha)`` returns (loglik, d loglik / d theta).  eta (M, d).  Returns
-    (neg_logliks (M+1,), NaN first, traces (M+1, d)), float32 like the
+          eps=1e-8, clip_norm=None, optimizer="adam", ls=None, c=0.1, max_ls_itn=10):
+    """``_train_internal`` / ``_train_scan_step`` (core/algorithms/train.py:38-180)
+    with one monitor (n_monitors = 1: the MOP value of the same draw is the
+    reported objective) and the optimizer ``optimizer``: "adam"
+    (core/optimizer.py:119-153), "sgd", "fmadam" (full-matrix Adam, epsilon
+    1e-4 unless ``eps`` is given) or "bfgs".  ``value_and_grad(theta, key,
+    alpha)`` returns (loglik, d loglik / d theta).  eta (M, d).  ``ls``: None,
+    or ``neg_loglik_pf(theta, key)``, the bootstrap filter's negative
+    log-likelihood with the iteration's MOP key, for the Armijo line search.
+    Returns (neg_logliks (M+1,), NaN first, traces (M+1, d)), float32 like the
     reference."""
     F32 = np.float32
     th = np.asarray(theta_est0, F32).copy()
+    d = th.shape[0]
     m1 = np.zeros_like(th)
-    v1 = np.zeros_like(th)
+    v1 = np.zeros_like(th) if optimizer != "fmadam" else np.zeros((d, d), F32)
+    hess, prev = np.eye(d, dtype=F32), np.zeros_like(th)
     nll, traces = [np.nan], [th.copy()]
     for m in range(M):
         a_m = F32(F32(1) - (F32(1) - F32(alpha)) * cosine_cooling(m, M, alpha_cooling))
@@ -242,12 +297,26 @@ def train(value_and_grad, theta_est0, key, T, M, eta, alpha, alpha_cooling=1.0, 
         nll.append(F32(-ll))
         if clip_norm is not None:
             grad = np.clip(grad, -clip_norm, clip_norm).astype(F32)
-        key = tf.split(key)[0]  # the Hessian subkey (unused by Adam)
-        m1 = (F32(beta1) * m1 + F32(1 - beta1) * grad).astype(F32)
-        v1 = (F32(beta2) * v1 + F32(1 - beta2) * grad * grad).astype(F32)
-        mh = (m1 / (F32(1) - F32(beta1) ** F32(m + 1))).astype(F32)
-        vh = (v1 / (F32(1) - F32(beta2) ** F32(m + 1))).astype(F32)
-        direction = (-mh / (np.sqrt(vh) + F32(eps))).astype(F32)
-        th = (th + np.asarray(eta[m], F32) * direction).astype(F32)
+        key = tf.split(key)[0]  # the Hessian subkey (unused by these optimizers)
+        if optimizer == "adam":
+            m1 = (F32(beta1) * m1 + F32(1 - beta1) * grad).astype(F32)
+            v1 = (F32(beta2) * v1 + F32(1 - beta2) * grad * grad).astype(F32)
+            mh = (m1 / (F32(1) - F32(beta1) ** F32(m + 1))).astype(F32)
+            vh = (v1 / (F32(1) - F32(beta2) ** F32(m + 1))).astype(F32)
+            direction = (-mh / (np.sqrt(vh) + F32(eps))).astype(F32)
+        elif optimizer == "sgd":
+            direction = -grad
+        elif optimizer == "fmadam":
+            direction, m1, v1 = fmadam_direction(grad, m1, v1, m, beta1, beta2, 1e-4 if eps == 1e-8 else eps)
+        elif optimizer == "bfgs":
+            direction, hess, prev = bfgs_direction(grad, hess, prev, m, eta[m])
+        else:
+            raise ValueError(optimizer)
+        if ls is not None:
+            step = armijo(lambda x: ls(x, sub), F32(-ll), th, grad, direction,
+                          F32(np.mean(np.asarray(eta[m], F32), dtype=F32)), c, max_ls_itn)
+            th = (th + step * direction).astype(F32)
+        else:
+            th = (th + np.asarray(eta[m], F32) * direction).astype(F32)
         traces.append(th.copy())
     return np.array(nll, dtype=F32), np.stack(traces)

@@ -8,7 +8,7 @@ filter run as hand-written HIP kernels for gfx950 in ``_lib/libmipomp.so``
 from . import autotune, functional, models, proposals, random
 from .maths import logmeanexp, logmeanexp_se
 from .models import ParTrans
-from .optim import SGD, Adam, LearningRate, Optimizer
+from .optim import BFGS, SGD, Adam, FullMatrixAdam, LearningRate, Newton, Optimizer, WeightedNewton
 from .proposals import MVNDiagRW, MVNRWAdaptive, MVNRWFull
 from .panel import PanelParameters, PanelPomp
 from .pomp import Pomp
@@ -17,4 +17,4 @@ from .rw_sigma import RWSigma
 functional_api = functional
 F = functional
 
-__all__ = ["Pomp", "PanelPomp", "PanelParameters", "ParTrans", "RWSigma", "Adam", "SGD", "Optimizer", "LearningRate", "MVNDiagRW", "MVNRWFull", "MVNRWAdaptive", "proposals", "autotune", "functional", "F", "models", "random", "logmeanexp", "logmeanexp_se"]
+__all__ = ["Pomp", "PanelPomp", "PanelParameters", "ParTrans", "RWSigma", "Adam", "SGD", "FullMatrixAdam", "BFGS", "Newton", "WeightedNewton", "Optimizer", "LearningRate", "MVNDiagRW", "MVNRWFull", "MVNRWAdaptive", "proposals", "autotune", "functional", "F", "models", "random", "logmeanexp", "logmeanexp_se"]

@@ -619,6 +619,36 @@ def _cosine_cooling(i, M, c):
     return F(F(c) + (F(1) - F(c)) * F(0.5) * (F(1) + np.cos(F(np.pi) * F(i) / F(M), dtype=F)))
 
 
+def _armijo_step(struct, th, grad, direction, curr, J, keys, thresh, optimizer, eta0):
+    """Per-row step sizes of the reference's Armijo backtracking
+    (core/algorithms/train.py:442-499, stoch=False, frac=0.5): the loop of a
+    row stops at the first trial point whose bootstrap-filter negative
+    log-likelihood is not NaN and at most curr + eta c (grad . direction);
+    after max_ls_itn halvings it stops at the current size.  All rows still
+    searching share one filter launch per round.  float32 like the
+    reference."""
+    n = th.shape[0]
+    etas = np.full(n, F32(eta0), dtype=F32)
+    slope = (grad * direction).sum(dim=1).cpu().numpy().astype(F32)
+    active = np.ones(n, dtype=bool)
+    itn = 0
+    thn = th.cpu().numpy()
+    dirn = direction.cpu().numpy()
+    c = F32(optimizer.c)
+    while active.any():
+        rows = np.nonzero(active)[0]
+        pts = (thn[rows] + etas[rows, None] * dirn[rows]).astype(F32)
+        res = pfilter(struct, pts, J, np.asarray(keys, np.uint32)[rows].reshape(-1, 1, 2), thresh=thresh,
+                      should_trans=True)
+        nxt = (-res["logLik"][:, 0]).astype(F32)
+        cont = (nxt > curr[rows] + (etas[rows] * c) * slope[rows]) | np.isnan(nxt)
+        go = cont & (itn < optimizer.max_ls_itn)
+        etas[rows[go]] = etas[rows[go]] * F32(0.5)
+        active[rows[~go]] = False
+        itn += 1
+    return etas
+
+
 def train(struct, thetas_array, J, optimizer, M, eta, alpha, keys, alpha_cooling=1.0, thresh=0.0, n_monitors=1,
           K=50):
     """Gradient ascent on the MOP log-likelihood (functional/train.py:11-113;
@@ -628,15 +658,25 @@ def train(struct, thetas_array, J, optimizer, M, eta, alpha, keys, alpha_cooling
     theta += eta[m] direction.  thetas (n, d) are used as the estimation-scale
     start, as the reference passes them through.  Keys per iteration follow the
     reference: (key, sub) = split(key) for the MOP draw, n_monitors further
-    keys when n_monitors > 1, then one more split.  The parameters and the
-    optimizer state stay on the device for all M iterations.  Returns
-    (neg_logliks (n, M+1), NaN first, traces (n, M+1, d)), float32."""
+    keys when n_monitors > 1, then one more split.  With ``optimizer.ls`` the
+    step size comes from the reference's Armijo backtracking
+    (core/algorithms/train.py:139-162, 442-499): starting from the mean
+    learning rate, halve while the bootstrap filter's negative log-likelihood
+    at the trial point, drawn with the iteration's MOP key, exceeds the current
+    objective plus eta c (grad . direction) or is NaN, at most max_ls_itn
+    times.  The parameters and the optimizer state stay on the device for all M
+    iterations (line search and FullMatrixAdam's eigendecompositions read back
+    every iteration).  Returns (neg_logliks (n, M+1), NaN first, traces (n,
+    M+1, d)), float32."""
     import torch
-    from .optim import Adam, Optimizer, SGD
-    if not isinstance(optimizer, (Adam, SGD)):
+    from .optim import BFGS, SGD, Adam, FullMatrixAdam, Newton, WeightedNewton
+    if isinstance(optimizer, (Newton, WeightedNewton)):
+        raise ValueError(f"Optimizer '{type(optimizer).__name__}' needs the MOP Hessian, which this build does "
+                         "not compute; use SGD, Adam, FullMatrixAdam or BFGS")
+    if not isinstance(optimizer, (Adam, SGD, FullMatrixAdam, BFGS)):
         raise ValueError(f"Optimizer '{type(optimizer).__name__}' not supported")
-    if optimizer.ls:
-        raise ValueError("line search is not supported by the native train loop")
+    if n_monitors < 1 and optimizer.ls:
+        raise ValueError("Line search requires at least one monitor")
     dev = _native.require_device()
     th0 = np.ascontiguousarray(np.asarray(thetas_array, dtype=F32))
     if th0.ndim == 1:
@@ -690,10 +730,16 @@ def train(struct, thetas_array, J, optimizer, M, eta, alpha, keys, alpha_cooling
             nll[:, m + 1] = -res["logLik"].mean(axis=1, dtype=F32)
         if optimizer.clip_norm is not None:
             grad = torch.clamp(grad, -optimizer.clip_norm, optimizer.clip_norm)
-        direction, state = optimizer.step(grad, state, m)
+        direction, state = optimizer.step(grad, state, m, eta_i=eta_arr[m])
         if optimizer.scale:
             direction = direction / torch.linalg.norm(direction, dim=-1, keepdim=True)
-        th = th + eta_arr[m][None, :] * direction
+        if optimizer.ls:
+            curr = (-lls[m].cpu().numpy()).astype(F32) if n_monitors == 1 else nll[:, m + 1]
+            step = _armijo_step(struct, th, grad, direction, curr, J, mop_keys[m], thresh, optimizer,
+                                float(eta_arr[m].mean()))
+            th = th + torch.from_numpy(step).to(dev)[:, None] * direction
+        else:
+            th = th + eta_arr[m][None, :] * direction
         traces.append(th.clone())
     torch.cuda.current_stream().synchronize()
     if n_monitors == 1:

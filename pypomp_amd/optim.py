@@ -18,7 +18,10 @@ class Optimizer:
     def init_state(self, theta):
         raise NotImplementedError
 
-    def step(self, grad, state, step_num):
+    def step(self, grad, state, step_num, eta_i=None):
+        """(direction, new state) for gradients ``grad`` (n, d) of the mean
+        negative log-likelihood; ``eta_i`` (d,) is the iteration's learning
+        rates (BFGS uses their mean)."""
         raise NotImplementedError
 
 
@@ -29,7 +32,7 @@ class SGD(Optimizer):
     def init_state(self, theta):
         return ()
 
-    def step(self, grad, state, step_num):
+    def step(self, grad, state, step_num, eta_i=None):
         return -grad, state
 
 
@@ -44,7 +47,7 @@ class Adam(Optimizer):
         import torch
         return torch.zeros_like(theta), torch.zeros_like(theta)
 
-    def step(self, grad, state, step_num):
+    def step(self, grad, state, step_num, eta_i=None):
         # float32 arithmetic; the constants 1 - beta are formed in double and
         # rounded once, as JAX does with Python-float operands
         import torch
@@ -57,6 +60,90 @@ class Adam(Optimizer):
         mh = m / float(c1)
         vh = v / float(c2)
         return -mh / (torch.sqrt(vh) + float(F(self.epsilon))), (m, v)
+
+
+@dataclass(frozen=True)
+class FullMatrixAdam(Optimizer):
+    """Full-matrix Adam (core/optimizer.py:157-213): first moment m and the
+    running second-moment matrix F of the gradients, bias-corrected, and the
+    direction -(F_hat)^(-1/2) m_hat with the inverse square root taken through
+    an eigendecomposition, eigenvalues floored at 0 and epsilon added.  The
+    d x d eigendecompositions run on the host in float32."""
+    beta1: float = 0.9
+    beta2: float = 0.999
+    epsilon: float = 1e-4
+
+    def init_state(self, theta):
+        import torch
+        n, d = theta.shape
+        return torch.zeros_like(theta), torch.zeros((n, d, d), dtype=theta.dtype, device=theta.device)
+
+    def step(self, grad, state, step_num, eta_i=None):
+        import torch
+        F = np.float32
+        m, v = state
+        m = float(F(self.beta1)) * m + float(F(1 - self.beta1)) * grad
+        mh = m / float(F(1) - np.power(F(self.beta1), F(step_num + 1), dtype=F))
+        v = float(F(self.beta2)) * v + float(F(1 - self.beta2)) * (grad[:, :, None] * grad[:, None, :])
+        vh = (v / float(F(1) - np.power(F(self.beta2), F(step_num + 1), dtype=F))).cpu()
+        evals, evecs = torch.linalg.eigh(vh)
+        inv = 1.0 / torch.sqrt(torch.clamp(evals, min=0.0) + float(F(self.epsilon)))
+        fis = (evecs * inv[:, None, :]) @ evecs.transpose(1, 2)
+        direction = -(fis.to(grad.device) @ mh[:, :, None])[:, :, 0]
+        return direction, (m, v)
+
+
+@dataclass(frozen=True)
+class BFGS(Optimizer):
+    """Quasi-Newton BFGS on the inverse Hessian (core/optimizer.py:216-267).
+    From the third iteration (step_num > 1): s = mean(eta_i) (-g_prev), y = g -
+    g_prev, rho = 1 / (y . s), H <- H - rho s (Hy)^T - rho (Hy) s^T + rho (rho
+    y^T H y + 1) s s^T (kept when rho is not finite), direction -H g; before,
+    direction -g with H unchanged.  The previous direction is taken as
+    -g_prev, as the reference does."""
+
+    def init_state(self, theta):
+        import torch
+        n, d = theta.shape
+        eye = torch.eye(d, dtype=theta.dtype, device=theta.device)
+        return eye.expand(n, d, d).clone(), torch.zeros_like(theta)
+
+    def step(self, grad, state, step_num, eta_i=None):
+        import torch
+        if eta_i is None:
+            raise ValueError("BFGS optimizer requires eta_i")
+        hess, prev = state
+        if step_num <= 1:
+            return -grad, (hess, grad)
+        s = eta_i.mean() * (-prev)                             # (n, d)
+        y = grad - prev
+        rho = 1.0 / (y * s).sum(dim=1)                         # (n,)
+        hy = (hess @ y[:, :, None])[:, :, 0]
+        yhy = (y * hy).sum(dim=1)
+        r = rho[:, None, None]
+        new = (hess - r * (s[:, :, None] * hy[:, None, :]) - r * (hy[:, :, None] * s[:, None, :])
+               + (r * (rho * yhy + 1.0)[:, None, None]) * (s[:, :, None] * s[:, None, :]))
+        new = torch.where(torch.isfinite(rho)[:, None, None], new, hess)
+        return -(new @ grad[:, :, None])[:, :, 0], (new, grad)
+
+
+@dataclass(frozen=True)
+class Newton(Optimizer):
+    """Newton step on the MOP Hessian (core/optimizer.py:270-289).  The Hessian
+    of the MOP estimator is not computed by this build: ``train`` raises."""
+
+    def init_state(self, theta):
+        return ()
+
+
+@dataclass(frozen=True)
+class WeightedNewton(Optimizer):
+    """Weighted Newton step on the MOP Hessian (core/optimizer.py:292-330).  The
+    Hessian of the MOP estimator is not computed by this build: ``train``
+    raises."""
+
+    def init_state(self, theta):
+        return ()
 
 
 class LearningRate:

@@ -153,3 +153,73 @@ def test_pomp_train_object_api():
     df = P.traces()
     assert len(df) == 10 and list(df.columns[:5]) == ["theta_idx", "iteration", "method", "logLik", "se"]
     assert P.results().shape[0] == 2
+
+
+def _lg1_train_case(M=8, J=100):
+    import pypomp_amd as pp
+    from pypomp_amd.models import LGModel
+    ys = lg1_long(40, 3)
+    struct = struct_of(LGModel(1, 1), ys)
+    omodel = olg.LGModel(olg.LGSpec(1, 1))
+    th0 = omodel.to_est(lg1_theta())
+    rates = np.linspace(0.02, 0.005, M)
+    eta = pp.LearningRate({n: rates for n in struct.param_names})
+
+    def vg(th, k, alpha):
+        val, _, gr = mg.lg_mop_value_and_grad(omodel, th, ys, k, J, alpha)
+        return val, gr
+
+    return struct, omodel, ys, th0, eta, vg
+
+
+@pytest.mark.parametrize("name", ["fmadam", "bfgs"])
+def test_train_full_matrix_adam_and_bfgs_match_oracle(name):
+    # FullMatrixAdam (core/optimizer.py:157-213) and BFGS (:216-267) against the
+    # oracle's restatement driven by its own value and gradient
+    import pypomp_amd as pp
+    import pypomp_amd.functional as F
+    M, J = 8, 100
+    struct, omodel, ys, th0, eta, vg = _lg1_train_case(M, J)
+    opt = pp.FullMatrixAdam() if name == "fmadam" else pp.BFGS()
+    key = tf.split(tf.key(41), 1)
+    nll, tr = F.train(struct, th0[None], J, opt, M, eta, 0.8, key, alpha_cooling=0.5)
+    onll, otr = mg.train(vg, th0, key[0], 40, M, eta.to_array(struct.param_names, M), 0.8, alpha_cooling=0.5,
+                         optimizer=name)
+    np.testing.assert_allclose(nll[0, 1:], onll[1:], rtol=1e-4)
+    np.testing.assert_allclose(tr[0], otr, rtol=1e-4, atol=1e-5)
+    assert not np.allclose(tr[0, -1], tr[0, 0])
+
+
+def test_train_line_search_matches_oracle():
+    # Armijo backtracking (core/algorithms/train.py:139-162, 442-499): trial
+    # points scored by the bootstrap filter with the iteration's MOP key
+    import pypomp_amd as pp
+    import pypomp_amd.functional as F
+    M, J = 5, 100
+    struct, omodel, ys, th0, eta, vg = _lg1_train_case(M, J)
+    eta = pp.LearningRate({n: 0.5 for n in struct.param_names})   # large: forces backtracking
+    ser = fl.make_series(ys, np.arange(1, len(ys) + 1.0), 0.0, nstep=1)
+
+    def neg_pf(th, k):
+        return -fl.pfilter(omodel, th, ser, k, J, should_trans=True)["logLik"]
+
+    key = tf.split(tf.key(43), 1)
+    nll, tr = F.train(struct, th0[None], J, pp.SGD(ls=True), M, eta, 0.9, key)
+    onll, otr = mg.train(vg, th0, key[0], 40, M, eta.to_array(struct.param_names, M), 0.9, optimizer="sgd",
+                         ls=neg_pf)
+    np.testing.assert_allclose(nll[0, 1:], onll[1:], rtol=1e-4)
+    np.testing.assert_allclose(tr[0], otr, rtol=1e-4, atol=1e-5)
+    # the search backtracked: fixed steps of the full rate give another path
+    _, tr_fixed = F.train(struct, th0[None], J, pp.SGD(), M, eta, 0.9, key)
+    assert not np.allclose(tr[0, 1:], tr_fixed[0, 1:])
+
+
+def test_train_newton_needs_the_hessian():
+    import pypomp_amd as pp
+    import pypomp_amd.functional as F
+    struct, _, _, th0, eta, _ = _lg1_train_case(2, 50)
+    for opt in (pp.Newton(), pp.WeightedNewton()):
+        with pytest.raises(ValueError, match="Hessian"):
+            F.train(struct, th0[None], 50, opt, 2, eta, 0.9, tf.split(tf.key(1), 1))
+    with pytest.raises(ValueError, match="monitor"):
+        F.train(struct, th0[None], 50, pp.SGD(ls=True), 2, eta, 0.9, tf.split(tf.key(1), 1), n_monitors=0)

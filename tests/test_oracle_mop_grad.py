@@ -37,3 +37,67 @@ def test_alpha_zero_drops_weight_history(sv_case):
     _, _, g0, _ = mg.sv_mop_reference_grad(th, ys, tf.split(tf.key(7), 1)[0], 64, 0.0)
     _, _, g1, _ = mg.sv_mop_reference_grad(th, ys, tf.split(tf.key(7), 1)[0], 64, 1.0)
     assert not np.allclose(g0, g1)
+
+
+# ---- optimizers of the train loop (core/optimizer.py, core/algorithms/train.py) ----
+
+def test_bfgs_update_satisfies_the_secant_equation():
+    rng = np.random.default_rng(5)
+    d = 4
+    a = rng.normal(size=(d, d)).astype(np.float32)
+    H = (a @ a.T + d * np.eye(d)).astype(np.float32)      # symmetric positive definite
+    prev = rng.normal(size=d).astype(np.float32)
+    g = rng.normal(size=d).astype(np.float32)
+    eta = np.full(d, 0.05, np.float32)
+    # the first two iterations take -grad and keep H
+    for step in (0, 1):
+        dr, Hn, pg = mg.bfgs_direction(g, H, prev, step, eta)
+        np.testing.assert_array_equal(dr, -g)
+        np.testing.assert_array_equal(Hn, H)
+        np.testing.assert_array_equal(pg, g)
+    dr, Hn, _ = mg.bfgs_direction(g, H, prev, 2, eta)
+    s = np.float32(0.05) * (-prev)
+    y = g - prev
+    np.testing.assert_allclose(Hn @ y, s, rtol=2e-3, atol=1e-6)   # secant: H_new y = s
+    np.testing.assert_allclose(Hn, Hn.T, rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(dr, -(Hn @ g), rtol=1e-5, atol=1e-6)
+    # a non-finite curvature (y . s = 0) keeps H
+    _, Hk, _ = mg.bfgs_direction(prev, H, prev, 3, eta)
+    np.testing.assert_array_equal(Hk, H)
+
+
+def test_full_matrix_adam_is_the_inverse_square_root():
+    rng = np.random.default_rng(6)
+    d = 3
+    m1 = np.zeros(d, np.float32)
+    v1 = np.zeros((d, d), np.float32)
+    for step in range(4):
+        g = rng.normal(size=d).astype(np.float32)
+        dr, m_new, v_new = mg.fmadam_direction(g, m1, v1, step)
+        np.testing.assert_allclose(v_new, 0.999 * v1 + 0.001 * np.outer(g, g), rtol=1e-5, atol=1e-9)
+        mh = m_new.astype(np.float64) / (1 - 0.9 ** (step + 1))
+        A = v_new.astype(np.float64) / (1 - 0.999 ** (step + 1)) + 1e-4 * np.eye(d)
+        # d = -A^(-1/2) mh  <=>  d' A d = |mh|^2  and  |d|^2 = mh' A^-1 mh
+        dd = dr.astype(np.float64)
+        np.testing.assert_allclose(dd @ A @ dd, mh @ mh, rtol=1e-3)
+        np.testing.assert_allclose(dd @ dd, mh @ np.linalg.solve(A, mh), rtol=1e-3)
+        assert dd @ mh < 0
+        m1, v1 = m_new, v_new
+    # one parameter: the scalar closed form
+    g = np.array([0.3], np.float32)
+    dr, _, _ = mg.fmadam_direction(g, np.zeros(1, np.float32), np.zeros((1, 1), np.float32), 0)
+    np.testing.assert_allclose(dr, [-0.3 / np.sqrt(0.09 + 1e-4)], rtol=1e-5)
+
+
+def test_armijo_backtracking():
+    q = lambda x: float(np.sum(x.astype(np.float64) ** 2))   # noqa: E731
+    x = np.array([1.0, -2.0], np.float32)
+    g = (2 * x).astype(np.float32)
+    dr = -g
+    eta = mg.armijo(q, np.float32(q(x)), x, g, dr, 4.0, c=0.1)
+    # halved from 4 until sufficient decrease: 4, 2, 1 fail, 0.5 passes (q(x - 0.5 g) = 0)
+    assert eta == np.float32(0.5)
+    assert q(x + eta * dr) <= q(x) + eta * 0.1 * float(g @ dr)
+    # an objective that never decreases (or is NaN) stops after max_itn halvings
+    eta = mg.armijo(lambda _: np.nan, np.float32(1.0), x, g, dr, 1.0, max_itn=10)
+    assert eta == np.float32(0.5 ** 10)
