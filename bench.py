@@ -37,11 +37,15 @@ MEASURED_TRAFFIC_MOP = 19.03e9
 # HBM bytes per launch of the batched SEIR filter at J=20000, T=260, 256 filters
 # (profiles/seir_batched_J20000_T260_R256_kernel_trace.md)
 MEASURED_TRAFFIC_SEIR = 86.33e9
+# HBM bytes per launch (one IF2 iteration over 800 swarms) of the panel IF2 kernel
+# (profiles/panel_if2_seir_U100_J2000_M50_kernel_trace.md)
+MEASURED_TRAFFIC_PANEL = 61.02e9
 # VALU instructions one lane executes per particle-step (SQ_INSTS_VALU x 64 /
 # particle-steps per dispatch, rocprofv3 --pmc; profiles/valu_plain_lg2_seir_mopgrad.md).
 # The kernels are VALU- or latency-bound rather than HBM-bound, so each roofline
 # object also carries the VALU issue rate implied by the live kernel time.
-VALU_INSTR_PER_PSTEP = {"plain_prop": 751.0, "plain_scan": 148.0, "seir": 10578.0, "mop_grad": 773.0}
+VALU_INSTR_PER_PSTEP = {"plain_prop": 751.0, "plain_scan": 148.0, "seir": 10578.0, "mop_grad": 773.0,
+                        "panel_if2": 14809.0}
 # one wave64 VALU instruction per CU per cycle: 256 CUs x 2.4 GHz
 VALU_PEAK_GINSTR = 256 * 2.4
 
@@ -233,7 +237,7 @@ class PanelIF2:
     perturbed at t0 only), geometric cooling 0.5.  One step = the whole M=50
     run: 50 launches of 800 swarms, recorded as one HIP graph and replayed."""
 
-    def __init__(self, J=2000, chains=8, M=50):
+    def __init__(self, J=2000, chains=8, M=50, graph=True):
         import torch
         import pypomp_amd as pp
         import pypomp_amd.functional as F
@@ -267,13 +271,17 @@ class PanelIF2:
         self.J, self.R, self.M, self.U = J, R, M, U
         self.sumT = int(off[-1])
         self.graph = None
+        self.use_graph = graph
 
     @property
     def particle_steps(self):
         return self.J * self.M * (self.R // self.U) * self.sumT
 
     def step(self):
-        if self.graph is None:
+        if not self.use_graph:  # eager launches (for counter collection)
+            self.start.copy_(self.t0)
+            self.out = self.run.iterate(self.start, self.mkeys, graph=False)
+        elif self.graph is None:
             self.out = self.run.iterate(self.start, self.mkeys, graph=True)
             self.graph = self.run.graph
         else:
@@ -282,6 +290,12 @@ class PanelIF2:
 
     def bytes_per_particle_step(self):
         return 16 * (5 + 10 + 1)
+
+    valu_instr_per_pstep = VALU_INSTR_PER_PSTEP["panel_if2"]
+
+    @property
+    def measured_traffic(self):
+        return MEASURED_TRAFFIC_PANEL if (self.J == 2000 and self.R == 800) else None
 
 
 def time_workload(wl, steps, warmup):
@@ -376,7 +390,8 @@ def run_only(args):
         work, unit = src.numel() * 4, "bytes copied"
     else:
         wl = {"plain": lambda: PlainLG(reps=args.reps), "seir": BatchedSEIR, "mop_batched": MopGradSV,
-              "mop_plain": lambda: MopGradSV("plain", reps=16), "panel": PanelIF2}[args.only]()
+              "mop_plain": lambda: MopGradSV("plain", reps=16), "panel": PanelIF2,
+              "panel_eager": lambda: PanelIF2(M=args.reps if args.reps < 50 else 50, graph=False)}[args.only]()
         step, work, unit = wl.step, wl.particle_steps, "particle-steps"
     for _ in range(args.warmup):
         step()
@@ -398,7 +413,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--reps", type=int, default=64)
     ap.add_argument("--no-extra", action="store_true", help="skip the batched workloads")
-    ap.add_argument("--only", choices=["plain", "seir", "mop_batched", "mop_plain", "panel", "copy"],
+    ap.add_argument("--only", choices=["plain", "seir", "mop_batched", "mop_plain", "panel", "panel_eager", "copy"],
                     help="run one workload --steps times and print a short JSON line (for profilers)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
