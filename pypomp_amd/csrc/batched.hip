@@ -460,16 +460,16 @@ template <class M, int ALGO>
 static hipError_t launch_batched_t(const BatchArgs& a, int block, int grid, size_t lds, hipStream_t st) {
   switch (block) {
     case 256:
-      if (a.lds_resident) hipLaunchKernelGGL((k_batched<M, ALGO, 256, true>), dim3(grid), dim3(256), lds, st, a);
-      else hipLaunchKernelGGL((k_batched<M, ALGO, 256, false>), dim3(grid), dim3(256), lds, st, a);
+      if (a.lds_resident) mp_launch(k_batched<M, ALGO, 256, true>, dim3(grid), dim3(256), lds, st, a);
+      else mp_launch(k_batched<M, ALGO, 256, false>, dim3(grid), dim3(256), lds, st, a);
       break;
     case 512:
-      if (a.lds_resident) hipLaunchKernelGGL((k_batched<M, ALGO, 512, true>), dim3(grid), dim3(512), lds, st, a);
-      else hipLaunchKernelGGL((k_batched<M, ALGO, 512, false>), dim3(grid), dim3(512), lds, st, a);
+      if (a.lds_resident) mp_launch(k_batched<M, ALGO, 512, true>, dim3(grid), dim3(512), lds, st, a);
+      else mp_launch(k_batched<M, ALGO, 512, false>, dim3(grid), dim3(512), lds, st, a);
       break;
     case 768:
-      if (a.lds_resident) hipLaunchKernelGGL((k_batched<M, ALGO, 768, true>), dim3(grid), dim3(768), lds, st, a);
-      else hipLaunchKernelGGL((k_batched<M, ALGO, 768, false>), dim3(grid), dim3(768), lds, st, a);
+      if (a.lds_resident) mp_launch(k_batched<M, ALGO, 768, true>, dim3(grid), dim3(768), lds, st, a);
+      else mp_launch(k_batched<M, ALGO, 768, false>, dim3(grid), dim3(768), lds, st, a);
       break;
     default: return hipErrorInvalidValue;
   }
@@ -500,7 +500,7 @@ hipError_t launch_batched(int algo, const BatchArgs& a, int block, int grid, siz
 
 hipError_t launch_order(const SeriesSet& ser, const int32_t* unit, int R, int32_t* order, int32_t* queue,
                         hipStream_t st) {
-  hipLaunchKernelGGL(k_order, dim3((R + 255) / 256), dim3(256), 0, st, ser, unit, R, order, queue);
+  mp_launch(k_order, dim3((R + 255) / 256), dim3(256), 0, st, ser, unit, R, order, queue);
   return hipGetLastError();
 }
 
@@ -508,9 +508,9 @@ hipError_t launch_sched(int algo, const uint32_t* keys, const SeriesSet& ser, co
                         int64_t sched_len, uint32_t* sched, int R, hipStream_t st) {
   if (algo == ALGO_PER) {
     dim3 grid((ser.Tmax + 1 + 127) / 128, R);
-    hipLaunchKernelGGL(k_sched_per, grid, dim3(128), 0, st, keys, ser, unit, sched_len, sched, R);
+    mp_launch(k_sched_per, grid, dim3(128), 0, st, keys, ser, unit, sched_len, sched, R);
   } else {
-    hipLaunchKernelGGL(k_sched_pf, dim3((4 * R + 63) / 64), dim3(64), 0, st, keys, ser, unit, sched_len, sched, R);
+    mp_launch(k_sched_pf, dim3((4 * R + 63) / 64), dim3(64), 0, st, keys, ser, unit, sched_len, sched, R);
   }
   return hipGetLastError();
 }

@@ -39,6 +39,9 @@ size_t mop_grad_lds_bytes(int S, int J, int block, bool lds_work);
 
 using namespace mp;
 
+// the timing events the next mp_launch attaches to its dispatch (engine.hpp)
+thread_local LaunchTiming mp::g_launch_timing;
+
 static_assert(sizeof(mp_model_desc) == sizeof(ModelDesc), "model desc layout");
 static_assert(sizeof(mp_series_desc) == sizeof(SeriesSet), "series desc layout");
 
@@ -82,9 +85,10 @@ int g_plain_scan_spin = 256;
 int g_batched_block = 0;    // batched workgroup size: 0 = automatic, 256, 512, 768
 int g_grad_block = 0;       // MOP-adjoint workgroup size: 0 = automatic, 256, 512
 
-// Optional per-kernel timing with HIP events recorded on the launch stream
-// (mp_profile_*).  Kinds: 0 plain propagate, 1 plain scan, 2 batched filter,
-// 3 key schedule.
+// Optional per-kernel timing (mp_profile_*): the events are attached to the
+// kernel's own dispatch packet (hipExtLaunchKernel through mp_launch), so they
+// time that kernel alone.  Kinds: 0 plain propagate, 1 plain scan, 2 batched
+// filter, 3 key schedule.
 struct ProfRec {
   int kind;
   hipEvent_t a, b;
@@ -105,19 +109,24 @@ hipEvent_t take_event() {
   return e;
 }
 
-// Runs `launch` between two recorded events when profiling is on.
+// Runs `launch` with the timing events handed to its kernel launch when
+// profiling is on; a launch that does not take them is not timed.
 template <class F>
-hipError_t timed(int kind, hipStream_t st, F&& launch) {
+hipError_t timed(int kind, hipStream_t, F&& launch) {
   if (!g_prof_on) return launch();
   std::lock_guard<std::mutex> lk(g_prof_mu);
   ProfRec r{kind, take_event(), take_event()};
   if (!r.a || !r.b) return launch();
-  hipError_t e = hipEventRecord(r.a, st);
-  if (e != hipSuccess) return e;
-  e = launch();
-  if (e != hipSuccess) return e;
-  e = hipEventRecord(r.b, st);
-  g_prof.push_back(r);
+  g_launch_timing = LaunchTiming{r.a, r.b};
+  const hipError_t e = launch();
+  const bool used = g_launch_timing.start == nullptr;
+  g_launch_timing = LaunchTiming{};
+  if (e == hipSuccess && used) {
+    g_prof.push_back(r);
+  } else {
+    g_event_pool.push_back(r.a);
+    g_event_pool.push_back(r.b);
+  }
   return e;
 }
 

@@ -3,8 +3,30 @@
 #pragma once
 
 #include "models.hpp"
+#include <hip/hip_ext.h>
 
 namespace mp {
+
+// Kernel timing for mp_profile_enable: the launch wrapper of the C ABI sets
+// these events, and the next launch attaches them to its own dispatch packet
+// (start and end timestamps of that kernel alone, with no event packets of
+// their own in the stream), then clears them.
+struct LaunchTiming {
+  hipEvent_t start = nullptr, stop = nullptr;
+};
+extern thread_local LaunchTiming g_launch_timing;
+
+template <typename K, typename... Args>
+inline void mp_launch(K kernel, dim3 grid, dim3 block, uint32_t lds, hipStream_t s, Args... args) {
+  if (g_launch_timing.start) {
+    const LaunchTiming lt = g_launch_timing;
+    g_launch_timing = LaunchTiming{};
+    hipExtLaunchKernelGGL(kernel, grid, block, lds, s, lt.start, lt.stop, 0u, args...);
+  } else {
+    hipLaunchKernelGGL(kernel, grid, block, lds, s, args...);
+  }
+}
+
 
 enum Algo : int { ALGO_PF = 0, ALGO_MOP = 1, ALGO_PER = 2 };
 

@@ -362,12 +362,12 @@ struct GradLauncher<M, true> {
   static hipError_t run(const GradArgs& a, int block, int grid, size_t lds, hipStream_t st) {
     switch (block) {
       case 256:
-        if (a.lds_work) hipLaunchKernelGGL((k_mop_grad<M, true>), dim3(grid), dim3(256), lds, st, a);
-        else hipLaunchKernelGGL((k_mop_grad<M, false>), dim3(grid), dim3(256), lds, st, a);
+        if (a.lds_work) mp_launch(k_mop_grad<M, true>, dim3(grid), dim3(256), lds, st, a);
+        else mp_launch(k_mop_grad<M, false>, dim3(grid), dim3(256), lds, st, a);
         break;
       case 512:
-        if (a.lds_work) hipLaunchKernelGGL((k_mop_grad_wide<M, true>), dim3(grid), dim3(512), lds, st, a);
-        else hipLaunchKernelGGL((k_mop_grad_wide<M, false>), dim3(grid), dim3(512), lds, st, a);
+        if (a.lds_work) mp_launch(k_mop_grad_wide<M, true>, dim3(grid), dim3(512), lds, st, a);
+        else mp_launch(k_mop_grad_wide<M, false>, dim3(grid), dim3(512), lds, st, a);
         break;
       default: return hipErrorInvalidValue;
     }

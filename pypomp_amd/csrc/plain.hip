@@ -355,7 +355,7 @@ template <class M>
 static hipError_t plain_prepare_t(const ModelDesc& md, const float* theta, int64_t stride, int R, int flags,
                                   char* out, int64_t pstride, hipStream_t s) {
   static_assert(sizeof(typename M::P) <= 512, "prepared parameters exceed their slot");
-  hipLaunchKernelGGL(k_plain_prepare<M>, dim3((R + 63) / 64), dim3(64), 0, s, md, theta, stride, R, flags, out,
+  mp_launch(k_plain_prepare<M>, dim3((R + 63) / 64), dim3(64), 0, s, md, theta, stride, R, flags, out,
                      pstride);
   return hipGetLastError();
 }
@@ -387,15 +387,15 @@ __global__ void k_plain_reset(uint64_t* lb, int ntiles, PlainState* st) {
 }
 
 hipError_t plain_reset(uint64_t* lb, int ntiles, PlainState* st, hipStream_t s) {
-  hipLaunchKernelGGL(k_plain_reset, dim3((ntiles + 255) / 256), dim3(256), 0, s, lb, ntiles, st);
+  mp_launch(k_plain_reset, dim3((ntiles + 255) / 256), dim3(256), 0, s, lb, ntiles, st);
   return hipGetLastError();
 }
 
 // which = 0: propagate kernel, 1: scan kernel
 template <class M>
 static hipError_t plain_kernel_t(int which, const PlainArgs& a, int ntiles, int threads, hipStream_t s) {
-  if (which == 0) hipLaunchKernelGGL(k_plain_prop<M>, dim3(a.nprop), dim3(threads), 0, s, a);
-  else hipLaunchKernelGGL(k_plain_scan<M>, dim3(ntiles), dim3(threads), 0, s, a);
+  if (which == 0) mp_launch(k_plain_prop<M>, dim3(a.nprop), dim3(threads), 0, s, a);
+  else mp_launch(k_plain_scan<M>, dim3(ntiles), dim3(threads), 0, s, a);
   return hipGetLastError();
 }
 
