@@ -74,7 +74,8 @@ __device__ __forceinline__ float binominv_f32(float u, float n, float p) {
   const float npq = np_ * q;
   const float sq = sqrtf(fmaxf(npq, tiny));
   const float pq = ps * q;
-  float q0 = np_ + sq * w + (2.0f + 2.0f * ps + (q - ps) * w2) / 6.0f;
+  // the numerator is at least 2 (q >= ps): div_by is exact
+  float q0 = np_ + sq * w + div_by(2.0f + 2.0f * ps + (q - ps) * w2, 6.0f, 1.0f / 6.0f);
   float w3 = w2 * w;
   float q1 = q0 + ((-2.0f + 14.0f * pq) * w + (-1.0f - 2.0f * pq) * w3) / fmaxf(72.0f * sq, tiny);
   float w4 = w3 * w;
