@@ -223,3 +223,26 @@ def test_train_newton_needs_the_hessian():
             F.train(struct, th0[None], 50, opt, 2, eta, 0.9, tf.split(tf.key(1), 1))
     with pytest.raises(ValueError, match="monitor"):
         F.train(struct, th0[None], 50, pp.SGD(ls=True), 2, eta, 0.9, tf.split(tf.key(1), 1), n_monitors=0)
+
+
+def test_train_rows_are_independent_bfgs_line_search():
+    # two parameter rows in one run: each row's BFGS state, line-search rounds
+    # and trace equal the oracle's single-row run with that row's key
+    import pypomp_amd as pp
+    import pypomp_amd.functional as F
+    M, J = 5, 100
+    struct, omodel, ys, th0, _, vg = _lg1_train_case(M, J)
+    eta = pp.LearningRate({n: 0.3 for n in struct.param_names})
+    ser = fl.make_series(ys, np.arange(1, len(ys) + 1.0), 0.0, nstep=1)
+
+    def neg_pf(th, k):
+        return -fl.pfilter(omodel, th, ser, k, J, should_trans=True)["logLik"]
+
+    rows = np.stack([th0, th0 + np.float32(0.05)])
+    keys = tf.split(tf.key(47), 2)
+    nll, tr = F.train(struct, rows, J, pp.BFGS(ls=True), M, eta, 0.9, keys)
+    for r in range(2):
+        onll, otr = mg.train(vg, rows[r], keys[r], 40, M, eta.to_array(struct.param_names, M), 0.9,
+                             optimizer="bfgs", ls=neg_pf)
+        np.testing.assert_allclose(nll[r, 1:], onll[1:], rtol=1e-4)
+        np.testing.assert_allclose(tr[r], otr, rtol=1e-4, atol=1e-5)
