@@ -56,8 +56,11 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
   const float fJ = (float)J;
   const float logJ = d_log(fJ);
 
-  // parameters prepared once per filter (k_plain_prepare); uniform loads
-  const typename M::P P = *reinterpret_cast<const typename M::P*>(a.params);
+  // parameters prepared once per filter (k_plain_prepare), staged in LDS
+  __shared__ typename M::P s_P;
+  if (tid == 0) s_P = *reinterpret_cast<const typename M::P*>(a.params);
+  __syncthreads();
+  const typename M::P& P = s_P;
 
   const PlainState st = *a.st;
   const float* Xc = (t & 1) ? a.X1 : a.X0;
