@@ -44,7 +44,7 @@ MEASURED_TRAFFIC_PANEL = 61.02e9
 # particle-steps per dispatch, rocprofv3 --pmc; profiles/valu_plain_lg2_seir_mopgrad.md).
 # The kernels are VALU- or latency-bound rather than HBM-bound, so each roofline
 # object also carries the VALU issue rate implied by the live kernel time.
-VALU_INSTR_PER_PSTEP = {"plain_prop": 739.0, "plain_scan": 148.0, "seir": 10289.0, "mop_grad": 762.0,
+VALU_INSTR_PER_PSTEP = {"plain_prop": 695.0, "plain_scan": 148.0, "seir": 10289.0, "mop_grad": 762.0,
                         "panel_if2": 14436.0}
 # one wave64 VALU instruction per CU per cycle: 256 CUs x 2.4 GHz
 VALU_PEAK_GINSTR = 256 * 2.4
