@@ -4,8 +4,8 @@ Follows pypomp/models/linear_gaussian.py:139-191 (components), :221-280
 (defaults), :337-451 (constructor and parameter naming).  State x has dimension
 dx, observation y dimension dy:
 
-    x0 = X0 + L0 z      (reference: L0 = L_Q; the build also allows a separate
-                         initial Cholesky factor P, which configs with
+    x0 = X0 + L0 z      (reference: L0 = L_Q; a separate initial Cholesky
+                         factor P is also allowed, which workloads with
                          x0 ~ N(0, I) need -- see ``LGSpec.init_chol``)
     x' = A x + L_Q z
     y  ~ N(C x, L_R L_R^T)
