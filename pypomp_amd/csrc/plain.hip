@@ -171,7 +171,7 @@ __device__ uint64_t wave_tile_aggregate(const float* lw, int64_t idx, int64_t ti
 template <class M>
 __global__ __launch_bounds__(1024) void k_plain_scan(const PlainArgs a) {
   __shared__ double red[32];
-  __shared__ uint64_t tab[65];  // rows * waves <= 64 row/wave totals + the tile total
+  __shared__ uint64_t tab[65];  // wave totals (<= 16) and the tile total
   __shared__ uint64_t s_prefix;
   const int J = a.J, tid = threadIdx.x, B = blockDim.x, lane = tid & 63, w = tid >> 6;
   const int nw = (B + 63) >> 6;
@@ -182,14 +182,31 @@ __global__ __launch_bounds__(1024) void k_plain_scan(const PlainArgs a) {
   const int tile = blockIdx.x;
   const int rows = a.rows;
   const int64_t tile0 = (int64_t)tile * rows * B;
+  // Each thread owns `rows` consecutive particles j0 .. j0 + rows - 1: it
+  // loads them as float4 vectors, scans them serially and takes one wave scan
+  // of its total (rather than one wave scan per row).
+  const int64_t j0 = tile0 + (int64_t)tid * rows;
+  const bool full = j0 + rows <= J;  // all of the thread's particles exist
+  const bool vec = full && (rows & 3) == 0;
 
-  // the tile's log-weights, loads issued first so that they are in flight
+  // the thread's log-weights, loads issued first so that they are in flight
   // together with the per-block maxima below
   float lwv[kMaxScanRows];
+  if (vec) {
+    const float4* p4 = reinterpret_cast<const float4*>(a.lw + j0);
 #pragma unroll
-  for (int r = 0; r < kMaxScanRows; ++r) {
-    const int64_t j = tile0 + (int64_t)r * B + tid;
-    lwv[r] = (r < rows && j < J) ? a.lw[j] : -__builtin_inff();
+    for (int q = 0; q < kMaxScanRows / 4; ++q) {
+      if (4 * q < rows) {
+        const float4 v = p4[q];
+        lwv[4 * q] = v.x;
+        lwv[4 * q + 1] = v.y;
+        lwv[4 * q + 2] = v.z;
+        lwv[4 * q + 3] = v.w;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < kMaxScanRows; ++r) lwv[r] = (r < rows && j0 + r < J) ? a.lw[j0 + r] : -__builtin_inff();
   }
   // global max / min of the log-weights from the per-block values, eight
   // independent loads in flight per thread
@@ -210,22 +227,24 @@ __global__ __launch_bounds__(1024) void k_plain_scan(const PlainArgs a) {
   }
   block_maxmin(m, mn, red);
 
-  // per-thread fixed-point weights; wave scans per row
-  uint64_t rowincl[kMaxScanRows];
+  // fixed-point weights, their running sum over the thread's particles, and
+  // one wave scan of the thread totals
+  uint64_t loc[kMaxScanRows];
+  uint64_t run = 0;
 #pragma unroll
   for (int r = 0; r < kMaxScanRows; ++r) {
     if (r < rows) {
-      const int64_t j = tile0 + (int64_t)r * B + tid;
-      const uint64_t e = j < J ? particle_fixed_weight(lwv[r], m, sh) : 0;
-      const uint64_t inc = wave_incl_scan_u64(e);
-      rowincl[r] = inc;
-      if (lane == 63) tab[r * nw + w] = inc;
+      run += (j0 + r < J) ? particle_fixed_weight(lwv[r], m, sh) : 0;
+      loc[r] = run;
     }
   }
+  const uint64_t winc = wave_incl_scan_u64(run);
+  const uint64_t texcl = winc - run;
+  if (lane == 63) tab[w] = winc;
   __syncthreads();
-  // wave 0: exclusive scan of the <= 64 row/wave totals, then publish the
-  // tile's aggregate
-  const int n = rows * nw;
+  // wave 0: exclusive scan of the wave totals, then publish the tile's
+  // aggregate
+  const int n = nw;
   if (w == 0) {
     const uint64_t v = lane < n ? tab[lane] : 0;
     const uint64_t inc = wave_incl_scan_u64(v);
@@ -301,10 +320,16 @@ __global__ __launch_bounds__(1024) void k_plain_scan(const PlainArgs a) {
   }
   __syncthreads();
   const uint64_t prefix = s_prefix;
+  const uint64_t base = prefix + tab[w] + texcl;
+  if (full && (rows & 1) == 0) {
+    ulonglong2* p2 = reinterpret_cast<ulonglong2*>(a.C + j0);
 #pragma unroll
-  for (int r = 0; r < kMaxScanRows; ++r) {
-    const int64_t j = tile0 + (int64_t)r * B + tid;
-    if (r < rows && j < J) a.C[j] = prefix + tab[r * nw + w] + rowincl[r];
+    for (int q = 0; q < kMaxScanRows / 2; ++q)
+      if (2 * q < rows) p2[q] = ulonglong2{base + loc[2 * q], base + loc[2 * q + 1]};
+  } else {
+#pragma unroll
+    for (int r = 0; r < kMaxScanRows; ++r)
+      if (r < rows && j0 + r < J) a.C[j0 + r] = base + loc[r];
   }
   // The tile holding the last index finishes the step: its inclusive prefix is
   // the exact total of the fixed-point weights.  It touches only the step
