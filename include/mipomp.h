@@ -194,7 +194,7 @@ int32_t mp_perfilter(const mp_model_desc* h_model, const mp_series_desc* h_serie
  * query the size after setting).  The Python autotuner
  * (pypomp_amd/autotune.py) sweeps them on the caller's workload and caches the
  * best values. */
-#define MP_TUNE_PLAIN_SCAN_ROWS 1 /* rows of 256 particles per scan tile, 1..16 */
+#define MP_TUNE_PLAIN_SCAN_ROWS 1 /* consecutive particles per scan thread, 1..16 */
 #define MP_TUNE_PLAIN_PROP_ROWS 2 /* rows of 256 particles per propagate block, 1..16 */
 #define MP_TUNE_BATCHED_BLOCK 3   /* batched workgroup size: 0 automatic, 256, 512, 768 */
 #define MP_TUNE_GRAD_BLOCK 4      /* MOP-adjoint workgroup size: 0 automatic, 256, 512 */

@@ -74,10 +74,10 @@ constexpr int kBatchedBlock = 256;
 constexpr int64_t kParamSlot = 512;  // bytes per filter of prepared model parameters
 
 // Launch-shape parameters (set by the autotuner through mp_tune_set).
-// Defaults from a sweep on LG 2-D, J = 1e6 (2.22e10 particle-steps/s).
-int g_plain_scan_rows = 8;  // particle rows per scan tile (1..16)
+// Defaults from a sweep on LG 2-D, J = 1e6 (2.86e10 particle-steps/s).
+int g_plain_scan_rows = 4;  // particles per scan thread (1..16)
 int g_plain_prop_rows = 2;  // particle rows per propagate block (1..16)
-int g_plain_scan_threads = 512;  // scan workgroup size (256, 512, 1024); rows * waves <= 64
+int g_plain_scan_threads = 1024;  // scan workgroup size (256, 512, 1024)
 int g_plain_prop_threads = 512;  // propagate workgroup size (256, 512)
 // look-back probes a scan tile spends on a late predecessor before computing its
 // aggregate itself (0: at once, which forces the fallback path)
@@ -234,9 +234,7 @@ PlainPlan plan_plain(const mp_model_desc* m, const mp_series_desc* s, int R, int
   PlainPlan p{};
   p.scan_threads = g_plain_scan_threads;
   p.prop_threads = g_plain_prop_threads;
-  // a scan tile's row/wave totals are scanned by one wave: rows * waves <= 64
   p.rows = g_plain_scan_rows;
-  while (p.rows * (p.scan_threads / 64) > 64) p.rows /= 2;
   const int tile = p.rows * p.scan_threads;
   p.ntiles = (J + tile - 1) / tile;
   p.rows_prop = g_plain_prop_rows;
