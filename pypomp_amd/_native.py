@@ -59,6 +59,10 @@ def lib():
         raise NativeError(
             f"{LIB_PATH} is missing: build it with `python -m pypomp_amd.build` "
             "(there is no CPU fallback for the particle-filter engine)")
+    # torch first: its bundled HIP runtime then satisfies the library's
+    # libamdhip64.so.7 dependency.  Loaded the other way round, the process
+    # would map two HIP/HSA runtimes and the library's would see no device.
+    import torch  # noqa: F401
     L = C.CDLL(str(LIB_PATH))
     vp, i32, i64, u32p, f32p, f64p, sz = (C.c_void_p, C.c_int32, C.c_int64, C.c_void_p,
                                            C.c_void_p, C.c_void_p, C.c_size_t)

@@ -24,6 +24,12 @@ class Component:
     def __repr__(self):
         return f"<{type(self.model).__name__}.{self.role}>"
 
+    def __eq__(self, other):
+        return isinstance(other, Component) and self.role == other.role and self.model == other.model
+
+    def __hash__(self):
+        return hash((self.role, self.model))
+
 
 class NativeModel:
     kind = 0
@@ -43,6 +49,13 @@ class NativeModel:
     @property
     def d(self):
         return len(self.param_names)
+
+    def __eq__(self, other):
+        # one device model and the same sizes, names and transform choice
+        return type(self) is type(other) and self.__dict__ == other.__dict__
+
+    def __hash__(self):
+        return hash((type(self).__name__, tuple(self.param_names)))
 
     @property
     def rinit(self):

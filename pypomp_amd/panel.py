@@ -9,7 +9,9 @@ panel IF2 (functional.panel_mif).  Keys follow the reference:
 and into n_sets chain keys for mif.
 """
 
+import copy
 import time
+import warnings
 
 import numpy as np
 import pandas as pd
@@ -17,6 +19,7 @@ import pandas as pd
 from . import functional as F
 from . import random as prandom
 from .maths import logmeanexp, logmeanexp_se
+from .pomp import _same, _same_history
 
 
 class PanelParameters:
@@ -48,6 +51,31 @@ class PanelParameters:
     def num_replicates(self):
         return len(self.sets)
 
+    def __eq__(self, other):
+        return (isinstance(other, PanelParameters) and self.shared_names == other.shared_names
+                and self.unit_param_names == other.unit_param_names and self.unit_names == other.unit_names
+                and _same(self.sets, other.sets) and _same(self.logLik_unit, other.logLik_unit))
+
+    __hash__ = None
+
+    @classmethod
+    def merge(cls, *param_objs):
+        """The replicate sets of several parameter objects, in order
+        (core/parameters/base.py:205-240)."""
+        if len(param_objs) == 0:
+            raise ValueError("At least one PanelParameters object must be provided.")
+        first = param_objs[0]
+        for p in param_objs:
+            if not isinstance(p, cls):
+                raise TypeError("All merged objects must be of type PanelParameters.")
+            if (p.shared_names, p.unit_param_names, p.unit_names) != \
+                    (first.shared_names, first.unit_param_names, first.unit_names):
+                raise ValueError("All PanelParameters objects must have the same parameter and unit names.")
+        out = cls([dict(s) for p in param_objs for s in p.sets])
+        if all(p.logLik_unit is not None for p in param_objs):
+            out.logLik_unit = np.concatenate([np.asarray(p.logLik_unit) for p in param_objs], axis=0)
+        return out
+
     def arrays(self, unit_names):
         """shared (n, n_shared), unit (n, U, n_spec) float32."""
         n, U = len(self.sets), len(unit_names)
@@ -69,6 +97,32 @@ class PanelParameters:
                  if unit_param_names else None}
             sets.append(s)
         return cls(sets)
+
+
+def _merge_panel_results(entries):
+    """Same-index panel results entries joined along the parameter-set axis;
+    the configuration must agree (core/results/result.py:178-213)."""
+    first = entries[0]
+    per_set = ("theta", "logLik", "out", "shared_traces", "unit_traces")
+    for e in entries:
+        if e["method"] != first["method"]:
+            raise ValueError("All merged results must share method/kind/panel.")
+        if e.keys() != first.keys():
+            raise ValueError("All merged results must have the same fields.")
+        for k in first:
+            if k not in per_set and k not in ("key", "time") and not _same(e[k], first[k]):
+                raise ValueError(f"All merged results must have the same {k}.")
+    out = dict(first)
+    for k in first:
+        if k == "theta":
+            out[k] = PanelParameters.merge(*[e[k] for e in entries])
+        elif k == "out":
+            out[k] = {n: np.concatenate([np.asarray(e[k][n]) for e in entries], axis=0) for n in first[k]}
+        elif k in per_set:
+            out[k] = np.concatenate([np.asarray(e[k]) for e in entries], axis=0)
+    if "time" in first:
+        out["time"] = max(e["time"] for e in entries)
+    return out
 
 
 class PanelPomp:
@@ -96,6 +150,75 @@ class PanelPomp:
 
     def get_unit_names(self):
         return list(self.unit_objects)
+
+    # -- equality, merge and pickling (panel/panel.py:171-300) --------------
+    def __eq__(self, other):
+        if not isinstance(other, type(self)):
+            return False
+        if (self.canonical_param_names, self.canonical_shared_param_names, self.canonical_unit_param_names) != \
+                (other.canonical_param_names, other.canonical_shared_param_names, other.canonical_unit_param_names):
+            return False
+        if self.theta != other.theta or self.get_unit_names() != other.get_unit_names():
+            return False
+        if any(self.unit_objects[u] != other.unit_objects[u] for u in self.unit_objects):
+            return False
+        if not _same_history(self.results_history, other.results_history):
+            return False
+        if (self.fresh_key is None) != (other.fresh_key is None):
+            return False
+        return self.fresh_key is None or _same(np.asarray(self.fresh_key, np.uint32),
+                                               np.asarray(other.fresh_key, np.uint32))
+
+    __hash__ = None
+
+    @staticmethod
+    def merge(*panel_pomp_objs):
+        """One panel holding the parameter sets and results of several panels
+        with the same units and parameter names (panel/panel.py:230-268)."""
+        if len(panel_pomp_objs) == 0:
+            raise ValueError("At least one PanelPomp object must be provided.")
+        first = panel_pomp_objs[0]
+        for obj in panel_pomp_objs:
+            if not isinstance(obj, type(first)):
+                raise TypeError("All merged objects must be of type PanelPomp.")
+            if obj.canonical_param_names != first.canonical_param_names:
+                raise ValueError("All PanelPomp objects must have the same canonical_param_names.")
+            if obj.canonical_shared_param_names != first.canonical_shared_param_names:
+                raise ValueError("All PanelPomp objects must have the same canonical_shared_param_names.")
+            if obj.canonical_unit_param_names != first.canonical_unit_param_names:
+                raise ValueError("All PanelPomp objects must have the same canonical_unit_param_names.")
+            if obj.get_unit_names() != first.get_unit_names():
+                raise ValueError("All PanelPomp objects must have the same unit names.")
+            if len(obj.results_history) != len(first.results_history):
+                raise ValueError("All histories must have the same number of entries to be merged.")
+        merged = copy.deepcopy(first)
+        merged.theta = PanelParameters.merge(*[obj.theta for obj in panel_pomp_objs])
+        merged.results_history = [_merge_panel_results([obj.results_history[i] for obj in panel_pomp_objs])
+                                  for i in range(len(first.results_history))]
+        merged.fresh_key = None if first.fresh_key is None else np.array(first.fresh_key, np.uint32)
+        return merged
+
+    def __getstate__(self):
+        state = self.__dict__.copy()
+        key = state.pop("fresh_key", None)
+        if key is not None:
+            state["_fresh_key_data"] = np.array(key, dtype=np.uint32)
+        return state
+
+    def __setstate__(self, state):
+        state = dict(state)
+        data = state.pop("_fresh_key_data", None)
+        self.__dict__.update(state)
+        if data is not None:
+            arr = np.asarray(data)
+            if arr.shape == (2,) and arr.dtype.kind in "ui" and not np.any(arr < 0) and not np.any(arr > 0xFFFFFFFF):
+                self.fresh_key = arr.astype(np.uint32)
+            else:
+                warnings.warn(f"Failed to reconstruct fresh_key: shape {arr.shape} dtype {arr.dtype}",
+                              UserWarning, stacklevel=2)
+                self.fresh_key = None
+        elif "fresh_key" not in self.__dict__:
+            self.fresh_key = None
 
     def to_struct(self):
         return F.PanelStruct(model=self.model, units=[p.series for p in self.unit_objects.values()],

@@ -42,6 +42,75 @@ def _resolve_model(model, rinit, rproc, dmeas, rmeas):
     return m
 
 
+def _same(a, b):
+    """Deep equality over what a Pomp holds: arrays (NaN equal to NaN), dicts,
+    lists, pandas frames and plain objects compared by their attributes."""
+    if a is b:
+        return True
+    if isinstance(a, np.ndarray) or isinstance(b, np.ndarray):
+        a, b = np.asarray(a), np.asarray(b)
+        if a.shape != b.shape:
+            return False
+        nan_ok = a.dtype.kind in "fc" and b.dtype.kind in "fc"
+        return bool(np.array_equal(a, b, equal_nan=nan_ok))
+    if isinstance(a, (pd.DataFrame, pd.Series)) or isinstance(b, (pd.DataFrame, pd.Series)):
+        return type(a) is type(b) and a.equals(b)
+    if isinstance(a, dict) and isinstance(b, dict):
+        return a.keys() == b.keys() and all(_same(a[k], b[k]) for k in a)
+    if isinstance(a, (list, tuple)) and isinstance(b, (list, tuple)):
+        return type(a) is type(b) and len(a) == len(b) and all(_same(x, y) for x, y in zip(a, b))
+    if isinstance(a, float) and isinstance(b, float):
+        return a == b or (a != a and b != b)
+    if type(a) is not type(b):
+        return False
+    if hasattr(a, "__dict__") and not isinstance(a, type):
+        return _same(vars(a), vars(b))
+    return bool(a == b)
+
+
+_TIMING = ("execution_time", "time")
+
+
+def _same_history(a, b):
+    """Results histories compared entry by entry without their wall-clock
+    times (core/results/result.py:137-157 leaves execution_time out)."""
+    if len(a) != len(b):
+        return False
+    return all(_same({k: v for k, v in x.items() if k not in _TIMING},
+                     {k: v for k, v in y.items() if k not in _TIMING}) for x, y in zip(a, b))
+
+
+# results entries: these keys hold one row per parameter set and are joined
+# along that axis by merge; "key" and "execution_time" are per run
+_PER_THETA = ("theta", "logLiks", "logLik", "CLL", "ESS", "filter_mean", "prediction_mean", "traces",
+              "accepts", "acceptance_rate")
+
+
+def _merge_results(entries):
+    """One results entry from same-index entries of several histories
+    (core/results/result.py:178-213: concatenate along the parameter-set axis;
+    the configuration must agree; the longest execution time; the first key)."""
+    first = entries[0]
+    for e in entries:
+        if e["method"] != first["method"]:
+            raise ValueError("All merged results must share method/kind/panel.")
+        if e.keys() != first.keys():
+            raise ValueError("All merged results must have the same fields.")
+        for k in first:
+            if k not in _PER_THETA and k not in ("key", "execution_time") and not _same(e[k], first[k]):
+                raise ValueError(f"All merged results must have the same {k}.")
+    out = dict(first)
+    for k in first:
+        if k == "theta":
+            out[k] = [dict(t) for e in entries for t in e[k]]
+        elif k in _PER_THETA:
+            out[k] = np.concatenate([np.asarray(e[k]) for e in entries], axis=0)
+    times = [e["execution_time"] for e in entries if e.get("execution_time") is not None]
+    if "execution_time" in first:
+        out["execution_time"] = max(times) if times else None
+    return out
+
+
 class Pomp:
     def __init__(self, ys, theta, t0, rinit=None, rproc=None, dmeas=None, rmeas=None, times=None,
                  nstep=None, dt=None, model=None, accumvars=None, covars=None, key=None):
@@ -294,3 +363,90 @@ class Pomp:
 
     def copy(self):
         return copy.deepcopy(self)
+
+    # -- equality, merge and pickling (core/pomp.py:455-826) ----------------
+    def __eq__(self, other):
+        """Equal parameter names and values, data and Euler grid, state names,
+        t0, model, transform, results history and fresh_key (core/pomp.py:455-540)."""
+        if not isinstance(other, type(self)):
+            return False
+        if self.canonical_param_names != other.canonical_param_names:
+            return False
+        if not _same(self.theta, other.theta):
+            return False
+        if not _same(self.series, other.series):
+            return False
+        if self.statenames != other.statenames or float(self.t0) != float(other.t0):
+            return False
+        if self.model != other.model or not _same(self.par_trans, other.par_trans):
+            return False
+        if not _same_history(self.results_history, other.results_history):
+            return False
+        if (self.fresh_key is None) != (other.fresh_key is None):
+            return False
+        return self.fresh_key is None or _same(np.asarray(self.fresh_key, np.uint32),
+                                               np.asarray(other.fresh_key, np.uint32))
+
+    __hash__ = None
+
+    @staticmethod
+    def merge(*pomp_objs):
+        """One Pomp holding the parameter sets and results of several
+        structurally identical Pomps, in order (core/pomp.py:542-613): results
+        entries at the same index are joined along the parameter-set axis."""
+        if len(pomp_objs) == 0:
+            raise ValueError("At least one Pomp object must be provided.")
+        first = pomp_objs[0]
+        for obj in pomp_objs:
+            if not isinstance(obj, type(first)):
+                raise TypeError("All merged objects must be of type Pomp.")
+            if obj.canonical_param_names != first.canonical_param_names:
+                raise ValueError("All Pomp objects must have the same canonical_param_names.")
+            if obj.statenames != first.statenames:
+                raise ValueError("All Pomp objects must have the same statenames.")
+            if not obj.ys.equals(first.ys) or not _same(obj.series, first.series):
+                raise ValueError("All Pomp objects must have the same ys data.")
+            if obj.t0 != first.t0:
+                raise ValueError("All Pomp objects must have the same t0.")
+            if obj.model != first.model:
+                raise ValueError("All Pomp objects must have the same rinit and rproc.")
+            if not _same(obj.par_trans, first.par_trans):
+                raise ValueError("All Pomp objects must have the same par_trans.")
+            if not obj.theta:
+                raise ValueError("Cannot merge Pomp objects with no parameters.")
+            if len(obj.results_history) != len(first.results_history):
+                raise ValueError("All histories must have the same number of entries to be merged.")
+        merged = copy.deepcopy(first)
+        merged.theta = [dict(t) for obj in pomp_objs for t in obj.theta]
+        merged.results_history = [_merge_results([obj.results_history[i] for obj in pomp_objs])
+                                  for i in range(len(first.results_history))]
+        merged.fresh_key = None if first.fresh_key is None else np.array(first.fresh_key, np.uint32)
+        return merged
+
+    def __getstate__(self):
+        """The key travels as its raw uint32 words (core/pomp.py:615-656); the
+        native model pickles by value (its class and sizes)."""
+        state = self.__dict__.copy()
+        key = state.pop("fresh_key", None)
+        if key is not None:
+            state["_fresh_key_data"] = np.array(key, dtype=np.uint32)
+        return state
+
+    def __setstate__(self, state):
+        """Restore; a malformed key payload warns and leaves fresh_key None
+        (core/pomp.py:658-680)."""
+        state = dict(state)
+        data = state.pop("_fresh_key_data", None)
+        self.__dict__.update(state)
+        if data is not None:
+            try:
+                arr = np.asarray(data)
+                if arr.shape != (2,) or arr.dtype.kind not in "ui" or np.any(arr < 0) or np.any(arr > 0xFFFFFFFF):
+                    raise ValueError(f"key data must be two uint32 words, got shape {arr.shape} "
+                                     f"dtype {arr.dtype}")
+                self.fresh_key = arr.astype(np.uint32)
+            except Exception as e:
+                warnings.warn(f"Failed to reconstruct fresh_key: {e}", UserWarning, stacklevel=2)
+                self.fresh_key = None
+        elif "fresh_key" not in self.__dict__:
+            self.fresh_key = None

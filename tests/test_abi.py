@@ -86,3 +86,18 @@ def test_tuning_settings_round_trip_and_reject_bad_values(lib):
     old = lib.mp_tune_get(7)
     assert lib.mp_tune_set(7, 0) == 0 and lib.mp_tune_get(7) == 0
     assert lib.mp_tune_set(7, old) == 0
+
+
+def test_library_shares_the_one_hip_runtime_in_the_process():
+    # a host call first (key splitting) loads the library; torch's HIP runtime
+    # must then be the only one mapped, or the library would see no device
+    import subprocess
+    import sys
+    from pathlib import Path
+    code = ("from pypomp_amd import random as r; r.split(r.key(1), 2); import torch; "
+            "m = open('/proc/self/maps').read().splitlines(); "
+            "print(len({l.split()[-1] for l in m if 'libamdhip64' in l}))")
+    out = subprocess.run([sys.executable, "-c", code], cwd=Path(__file__).resolve().parents[1],
+                         capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.strip() == "1"

@@ -227,3 +227,39 @@ def test_plain_scan_fallback_and_long_look_back():
         np.testing.assert_array_equal(o["logLik64"], ref["logLik64"])
     batched = F.pfilter(struct, theta[None], J, keys, form="batched")
     np.testing.assert_array_equal(batched["logLik64"], ref["logLik64"])
+
+
+def test_pomp_pickle_continues_the_key_stream_and_merge_joins_runs():
+    # core/pomp.py:542-826: an unpickled Pomp equals the original and draws the
+    # same next keys; merge joins the parameter sets and results of two runs
+    import pickle
+    import pypomp_amd as pp
+    from pypomp_amd.models.lg import LG
+    names = ["A11", "C11", "Q11", "R11", "X0_1"]
+    rw = pp.RWSigma({n: 0.02 for n in names}).geometric_cooling(0.5)
+    P = LG(A=np.array([[0.9]]), ys=lg1_data(0))
+    P.fresh_key = pp.random.key(5)
+    P.pfilter(J=50, reps=2)
+    Q = pickle.loads(pickle.dumps(P))
+    assert P == Q
+    P.pfilter(J=50, reps=2)
+    Q.pfilter(J=50, reps=2)
+    np.testing.assert_array_equal(P.results_history[-1]["logLiks"], Q.results_history[-1]["logLiks"])
+    assert P == Q
+    P.mif(J=20, M=2, rw_sd=rw)
+    Q.mif(J=20, M=2, rw_sd=rw)
+    R = LG(A=np.array([[0.5]]), ys=lg1_data(0))
+    R.fresh_key = pp.random.key(6)
+    for _ in range(2):
+        R.pfilter(J=50, reps=2)
+    R.mif(J=20, M=2, rw_sd=rw)
+    M = type(P).merge(P, R)
+    assert len(M.theta) == 2 and len(M.results_history) == 3
+    res = M.results(0)
+    assert list(res["theta_idx"]) == [0, 1]
+    np.testing.assert_array_equal(res["logLik"].to_numpy()[:1], P.results(0)["logLik"].to_numpy())
+    np.testing.assert_array_equal(res["logLik"].to_numpy()[1:], R.results(0)["logLik"].to_numpy())
+    tr = M.traces()
+    assert set(tr["theta_idx"]) == {0, 1} and set(tr["method"]) == {"pfilter", "mif"}
+    with pytest.raises(ValueError, match="same number of entries"):
+        type(P).merge(P, LG(A=np.array([[0.5]]), ys=lg1_data(0)))

@@ -214,3 +214,37 @@ def test_panel_pomp_methods_follow_the_functional_paths():
     np.testing.assert_allclose(last[last["unit"] == "shared"][["A11", "C11"]].to_numpy()[0], st[0, -1, 1:])
     np.testing.assert_allclose(last[last["unit"] == "unit2"]["logLik"].to_numpy()[0], ut[0, -1, 1, 0])
     assert P.theta.num_replicates() == 1
+
+
+def test_panel_pomp_pickle_and_merge():
+    # panel/panel.py:171-300: an unpickled panel equals the original and draws
+    # the same next keys; merge joins parameter sets and results
+    import pickle
+    import pypomp_amd as pp
+    rw = pp.RWSigma({n: 0.02 for n in LG_NAMES}).geometric_cooling(0.5)
+    P = _panel_pomp()
+    P.fresh_key = tf.key(7)
+    P.pfilter(J=32, reps=2)
+    Q = pickle.loads(pickle.dumps(P))
+    assert P == Q
+    for X in (P, Q):
+        X.pfilter(J=32, reps=2)
+        X.mif(J=16, M=2, rw_sd=rw)
+    np.testing.assert_array_equal(P.results_history[1]["logLik"], Q.results_history[1]["logLik"])
+    np.testing.assert_array_equal(P.results_history[2]["unit_traces"], Q.results_history[2]["unit_traces"])
+    assert P == Q
+    R = _panel_pomp()
+    R.fresh_key = tf.key(8)
+    R.pfilter(J=32, reps=2)
+    R.pfilter(J=32, reps=2)
+    R.mif(J=16, M=2, rw_sd=rw)
+    M = pp.PanelPomp.merge(P, R)
+    assert M.theta.num_replicates() == 2
+    res = M.results(1)
+    assert list(res["theta_idx"]) == [0, 0, 0, 1, 1, 1]
+    np.testing.assert_array_equal(res["logLik"].to_numpy()[:3], P.results(1)["logLik"].to_numpy())
+    np.testing.assert_array_equal(res["logLik"].to_numpy()[3:], R.results(1)["logLik"].to_numpy())
+    assert set(M.traces()["theta_idx"]) == {0, 1}
+    assert P != R
+    with pytest.raises(ValueError, match="same number of entries"):
+        pp.PanelPomp.merge(P, _panel_pomp())

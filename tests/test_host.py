@@ -128,3 +128,144 @@ def test_pomp_pickles_with_its_key_and_history():
     assert Q.theta == P.theta and Q.canonical_param_names == P.canonical_param_names
     assert Q.results_history[0]["method"] == "pfilter"
     assert bytes(Q.model.desc()) == bytes(P.model.desc())
+
+
+def _lg_pomp(a=0.9, key=9):
+    from pypomp_amd import random as prandom
+    from pypomp_amd.models.lg import LG
+    P = LG(A=np.array([[a]]), ys=np.arange(4, dtype=np.float32)[:, None] / 4)
+    P.fresh_key = prandom.key(key)
+    return P
+
+
+def _fake_pfilter_entry(P, seed, n_rows=1):
+    # the shape of a pfilter entry; values are arbitrary host data
+    rng = np.random.default_rng(seed)
+    return {"method": "pfilter", "execution_time": float(seed), "key": np.array([0, seed], np.uint32),
+            "theta": [dict(t) for t in P.theta] * n_rows, "logLiks": rng.normal(size=(n_rows, 3)),
+            "logLik": rng.normal(size=n_rows), "J": 100, "reps": 3, "thresh": 0.0,
+            "CLL": rng.normal(size=(n_rows, 3, 4)).astype(np.float32)}
+
+
+def test_pomp_equality_paths():
+    # core/pomp.py:455-540 and tests/test_pomp/test_pomp_eq_merge.py: every
+    # field that defines a Pomp breaks equality on its own
+    import copy
+    import pickle
+    from pypomp_amd.models.lg import LG
+    P = _lg_pomp()
+    P.results_history.append(_fake_pfilter_entry(P, 1))
+    assert P == pickle.loads(pickle.dumps(P)) and P == copy.deepcopy(P)
+    assert P != "not a pomp"
+    Q = copy.deepcopy(P); Q.theta = [dict(Q.theta[0], A11=0.5)]
+    assert P != Q
+    Q = copy.deepcopy(P); Q.t0 = 10.0
+    assert P != Q
+    Q = copy.deepcopy(P); Q.fresh_key = np.array([0, 99], np.uint32)
+    assert P != Q
+    Q = copy.deepcopy(P); Q.fresh_key = None
+    assert P != Q
+    Q = copy.deepcopy(P); Q.results_history[0]["logLiks"] = Q.results_history[0]["logLiks"] + 1
+    assert P != Q
+    Q = copy.deepcopy(P); Q.series.nstep = Q.series.nstep + 1
+    assert P != Q
+    Q = LG(A=np.array([[0.9]]), ys=np.arange(5, dtype=np.float32)[:, None] / 4)
+    Q.fresh_key, Q.results_history = P.fresh_key, P.results_history
+    assert P != Q                                   # different data
+    R = LG(ys=np.zeros((4, 2), np.float32))
+    assert P != R and P.model != R.model            # different model sizes and names
+    # NaN entries compare equal to themselves (IF2 traces start with a NaN logLik)
+    Q = copy.deepcopy(P); P.results_history[0]["logLiks"][0, 0] = np.nan
+    Q.results_history[0]["logLiks"][0, 0] = np.nan
+    assert P == Q
+
+
+def test_pomp_merge_joins_parameter_sets_and_results():
+    # core/pomp.py:542-613, core/results/result.py:178-213
+    P, Q = _lg_pomp(0.9, 1), _lg_pomp(0.5, 2)
+    P.results_history.append(_fake_pfilter_entry(P, 1))
+    Q.results_history.append(_fake_pfilter_entry(Q, 2, n_rows=2))
+    M = type(P).merge(P, Q)
+    assert [t["A11"] for t in M.theta] == [P.theta[0]["A11"], Q.theta[0]["A11"]]
+    e = M.results_history[0]
+    np.testing.assert_array_equal(e["logLiks"], np.concatenate([P.results_history[0]["logLiks"],
+                                                                Q.results_history[0]["logLiks"]]))
+    assert e["CLL"].shape == (3, 3, 4) and len(e["theta"]) == 3 and e["execution_time"] == 2.0
+    np.testing.assert_array_equal(e["key"], P.results_history[0]["key"])
+    np.testing.assert_array_equal(M.fresh_key, P.fresh_key)
+    assert list(M.results()["theta_idx"]) == [0, 1, 2]
+    # the inputs are unchanged
+    assert len(P.theta) == 1 and P.results_history[0]["logLiks"].shape == (1, 3)
+
+
+def test_pomp_merge_rejects_mismatches():
+    import copy
+    from pypomp_amd.models.lg import LG
+    P = _lg_pomp()
+    with pytest.raises(ValueError, match="At least one"):
+        type(P).merge()
+    with pytest.raises(TypeError):
+        type(P).merge(P, "x")
+    R = LG(ys=np.zeros((4, 2), np.float32))
+    with pytest.raises(ValueError, match="canonical_param_names"):
+        type(P).merge(P, R)
+    Q = LG(A=np.array([[0.9]]), ys=np.arange(4, dtype=np.float32)[:, None] / 8)
+    with pytest.raises(ValueError, match="same ys"):
+        type(P).merge(P, Q)
+    Q = copy.deepcopy(P); Q.t0 = 3.0
+    with pytest.raises(ValueError, match="same t0"):
+        type(P).merge(P, Q)
+    Q = copy.deepcopy(P); Q.results_history.append(_fake_pfilter_entry(Q, 3))
+    with pytest.raises(ValueError, match="same number of entries"):
+        type(P).merge(P, Q)
+    P.results_history.append(_fake_pfilter_entry(P, 4))
+    Q.results_history[0]["J"] = 7
+    with pytest.raises(ValueError, match="same J"):
+        type(P).merge(P, Q)
+
+
+def test_pomp_unpickle_with_bad_key_payload_warns():
+    # core/pomp.py:658-680: a corrupted key payload warns and leaves no key
+    P = _lg_pomp()
+    state = P.__getstate__()
+    np.testing.assert_array_equal(state["_fresh_key_data"], P.fresh_key)
+    assert "fresh_key" not in state
+    state["_fresh_key_data"] = np.zeros(3, np.uint32)
+    Q = object.__new__(type(P))
+    with pytest.warns(UserWarning, match="Failed to reconstruct fresh_key"):
+        Q.__setstate__(state)
+    assert Q.fresh_key is None and Q.theta == P.theta
+    state.pop("_fresh_key_data")
+    R = object.__new__(type(P))
+    R.__setstate__(state)
+    assert R.fresh_key is None
+
+
+def test_panel_parameters_merge_and_equality():
+    import pandas as pd
+    import pypomp_amd as pp
+
+    def params(v):
+        return pp.PanelParameters([{"shared": pd.DataFrame({"shared": [v, 1.0]}, index=["A11", "C11"]),
+                                    "unit_specific": pd.DataFrame({"u1": [0.1, 0.2, v], "u2": [0.1, 0.2, 0.0]},
+                                                                  index=["Q11", "R11", "X0_1"])}])
+    a, b = params(0.5), params(0.7)
+    assert a == params(0.5) and a != b
+    m = pp.PanelParameters.merge(a, b)
+    sh, un = m.arrays(["u1", "u2"])
+    np.testing.assert_allclose(sh[:, 0], [0.5, 0.7])
+    np.testing.assert_allclose(un[:, 0, 2], [0.5, 0.7])
+    other = pp.PanelParameters([{"shared": None, "unit_specific": pd.DataFrame(
+        {"u1": [0.9, 1.0, 0.1, 0.2, 0.0], "u2": [0.9, 1.0, 0.1, 0.2, 0.0]},
+        index=["A11", "C11", "Q11", "R11", "X0_1"])}])
+    with pytest.raises(ValueError, match="same parameter and unit names"):
+        pp.PanelParameters.merge(a, other)
+
+
+def test_pomp_equality_ignores_execution_time():
+    import copy
+    P = _lg_pomp()
+    P.results_history.append(_fake_pfilter_entry(P, 1))
+    Q = copy.deepcopy(P)
+    Q.results_history[0]["execution_time"] = 123.0
+    assert P == Q
