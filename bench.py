@@ -417,7 +417,7 @@ def main():
                     help="run one workload --steps times and print a short JSON line (for profilers)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (replicate filters shard "
+        print(json.dumps({"error": "the engine runs on one GPU; --gpus must be 1 (replicate filters shard "
                                    "across GPUs with no communication, see the design notes)"}))
         sys.exit(2)
 

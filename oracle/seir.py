@@ -1,6 +1,6 @@
 """SEIR model and the binomial sampler restated in NumPy float32 -- TEST ORACLE ONLY.
 
-The model is defined by this build (the reference has no SEIR); it follows the
+The model is defined here (the reference has no SEIR); it follows the
 reference's SIR model pattern (pypomp/models/sir.py:171-249) and draws every
 binomial with the reference's sampler, ``fast_binomial`` / ``binominv``
 (pypomp/random/binom.py:130-522: Giles & Beentjes (2024) order-2 normal

@@ -1,6 +1,6 @@
 """Stochastic volatility with leverage restated in NumPy float32 -- TEST ORACLE ONLY.
 
-Model defined by this build (the MOP workload; the reference's spx model,
+Model defined here (the MOP workload; the reference's spx model,
 pypomp/models/spx.py:66-126, is the pattern).  State (v, eta), theta =
 (mu, phi, sigma, rho, V0, S0):
     v_0 = V0 + S0 z, eta_0 = 0;  v_t = mu + phi (v_{t-1} - mu) + sigma eta_t,

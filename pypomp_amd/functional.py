@@ -671,7 +671,7 @@ def train(struct, thetas_array, J, optimizer, M, eta, alpha, keys, alpha_cooling
     import torch
     from .optim import BFGS, SGD, Adam, FullMatrixAdam, Newton, WeightedNewton
     if isinstance(optimizer, (Newton, WeightedNewton)):
-        raise ValueError(f"Optimizer '{type(optimizer).__name__}' needs the MOP Hessian, which this build does "
+        raise ValueError(f"Optimizer '{type(optimizer).__name__}' needs the MOP Hessian, which this engine does "
                          "not compute; use SGD, Adam, FullMatrixAdam or BFGS")
     if not isinstance(optimizer, (Adam, SGD, FullMatrixAdam, BFGS)):
         raise ValueError(f"Optimizer '{type(optimizer).__name__}' not supported")

@@ -130,7 +130,7 @@ class BFGS(Optimizer):
 @dataclass(frozen=True)
 class Newton(Optimizer):
     """Newton step on the MOP Hessian (core/optimizer.py:270-289).  The Hessian
-    of the MOP estimator is not computed by this build: ``train`` raises."""
+    of the MOP estimator is not computed by this engine: ``train`` raises."""
 
     def init_state(self, theta):
         return ()
@@ -139,7 +139,7 @@ class Newton(Optimizer):
 @dataclass(frozen=True)
 class WeightedNewton(Optimizer):
     """Weighted Newton step on the MOP Hessian (core/optimizer.py:292-330).  The
-    Hessian of the MOP estimator is not computed by this build: ``train``
+    Hessian of the MOP estimator is not computed by this engine: ``train``
     raises."""
 
     def init_state(self, theta):

@@ -6,8 +6,8 @@
   definitions of tests/test_regression/conftest.py:22-115 restated as data.
 * ``workloads.npz`` -- synthetic observation series of the benchmark workloads
   (LG 2-D seeds 1 and 2, SEIR seed 3, SV seed 4, SEIR panel seed 5), simulated
-  by the oracle with the reference's simulate key tree.  They are inputs, not
-  judged outputs.
+  by the oracle with the reference's simulate key tree.  They are inputs to the
+  filters, not expected outputs.
 
 Usage: python tests/golden/make_fixtures.py [--reference /root/reference]
        python tests/golden/make_fixtures.py --only sv[,lg,seir,panel]   (workloads only)
