@@ -202,6 +202,8 @@ int32_t mp_perfilter(const mp_model_desc* h_model, const mp_series_desc* h_serie
 #define MP_TUNE_PLAIN_PROP_THREADS 6 /* plain propagate workgroup size: 256, 512 */
 #define MP_TUNE_PLAIN_SCAN_SPIN 7    /* look-back probes before a scan tile computes a late
                                         predecessor's aggregate itself, 0..2^20 (default 256) */
+#define MP_TUNE_PLAIN_XCD_MAP 8      /* 1 (default): propagate blocks take the particle chunks
+                                        of the scan tiles on their own XCD; 0: block order */
 int32_t mp_tune_set(int32_t param, int32_t value);
 int32_t mp_tune_get(int32_t param);
 

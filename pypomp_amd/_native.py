@@ -28,6 +28,7 @@ EXPORTS = [
 ]
 MP_TUNE_PLAIN_SCAN_ROWS, MP_TUNE_PLAIN_PROP_ROWS, MP_TUNE_BATCHED_BLOCK, MP_TUNE_GRAD_BLOCK = 1, 2, 3, 4
 MP_TUNE_PLAIN_SCAN_THREADS, MP_TUNE_PLAIN_PROP_THREADS, MP_TUNE_PLAIN_SCAN_SPIN = 5, 6, 7
+MP_TUNE_PLAIN_XCD_MAP = 8
 
 
 class ModelDesc(C.Structure):

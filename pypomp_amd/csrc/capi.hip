@@ -22,7 +22,7 @@ hipError_t launch_sched(int algo, const uint32_t* keys, const SeriesSet& ser, co
 hipError_t plain_kernel(int which, const PlainArgs& a, int ntiles, int threads, hipStream_t s);
 hipError_t plain_prepare(const ModelDesc& md, const float* theta, int64_t stride, int R, int flags, char* out,
                          int64_t pstride, hipStream_t s);
-hipError_t plain_reset(uint64_t* lb, int ntiles, PlainState* st, hipStream_t s);
+hipError_t plain_reset(const PlainArgs& a, hipStream_t s);
 size_t batched_small_lds(int J, int block);
 hipError_t launch_order(const SeriesSet& ser, const int32_t* unit, int R, int32_t* order, int32_t* queue,
                         hipStream_t st);
@@ -82,6 +82,8 @@ int g_plain_prop_threads = 512;  // propagate workgroup size (256, 512)
 // look-back probes a scan tile spends on a late predecessor before computing its
 // aggregate itself (0: at once, which forces the fallback path)
 int g_plain_scan_spin = 256;
+// propagate blocks take the chunks of the scan tiles on their own XCD
+int g_plain_xcd_map = 1;
 int g_batched_block = 0;    // batched workgroup size: 0 = automatic, 256, 512, 768
 int g_grad_block = 0;       // MOP-adjoint workgroup size: 0 = automatic, 256, 512
 
@@ -226,7 +228,7 @@ BatchedPlan plan_batched(int algo, const mp_model_desc* m, const mp_series_desc*
 }
 
 struct PlainPlan {
-  int ntiles, rows, nprop, rows_prop, prop_len, scan_threads, prop_threads;
+  int ntiles, rows, nprop, rows_prop, prop_len, scan_threads, prop_threads, xcd_q;
   int64_t off_sched, off_params, off_x0, off_x1, off_lw, off_c, off_pmax, off_pmin, off_lb, off_st, total;
 };
 
@@ -246,6 +248,24 @@ PlainPlan plan_plain(const mp_model_desc* m, const mp_series_desc* s, int R, int
   const int round = num_cus() * (2048 / p.prop_threads);
   if (p.nprop > round / 2) p.nprop = (p.nprop + round - 1) / round * round;
   p.prop_len = (int)(((int64_t)J + p.nprop - 1) / p.nprop);
+  // XCD-local chunks.  Workgroups are dispatched to the XCDs round-robin by
+  // block index, and scan tile k runs on XCD k % 8.  When a scan tile holds a
+  // whole number q of full propagate blocks, the propagate blocks take
+  // block-sized chunks in an order that puts the q chunks of tile k on XCD
+  // k % 8 as well, so the scan reads the log-weights and the next step reads
+  // prefix sums and states from the L2 that holds them.  The block count is
+  // then a multiple of 8 q (trailing blocks may be empty).
+  p.xcd_q = 0;
+  const int cap = p.rows_prop * p.prop_threads;
+  if (g_plain_xcd_map && tile % cap == 0) {
+    const int q = tile / cap, G = kXcds * q;
+    const int n = (J + cap - 1) / cap;
+    if (n >= G) {
+      p.xcd_q = q;
+      p.nprop = (n + G - 1) / G * G;
+      p.prop_len = cap;
+    }
+  }
   const SeriesSet& ser = *reinterpret_cast<const SeriesSet*>(s);
   int64_t o = 0;
   auto take = [&](int64_t bytes) {
@@ -319,6 +339,7 @@ int run_plain_pf(const mp_model_desc* m, const mp_series_desc* s, const float* t
   a.rows_prop = p.rows_prop;
   a.prop_len = p.prop_len;
   a.nprop = p.nprop;
+  a.xcd_q = p.xcd_q;
   a.st = reinterpret_cast<PlainState*>(b + p.off_st);
   a.rows = p.rows;
   a.scan_spin = g_plain_scan_spin;
@@ -335,7 +356,7 @@ int run_plain_pf(const mp_model_desc* m, const mp_series_desc* s, const float* t
     a.cll = cll ? cll + (int64_t)f * s->Tmax : nullptr;
     a.sched = sched_all + 2 * (int64_t)f * slen;
     a.params = params + (int64_t)f * kParamSlot;
-    MP_HIP(plain_reset(a.lb, p.ntiles, a.st, st));
+    MP_HIP(plain_reset(a, st));
     for (int t = 0; t < a.T; ++t) {
       a.t = t;
       MP_HIP(timed(0, st, [&] { return plain_kernel(0, a, p.ntiles, p.prop_threads, st); }));
@@ -699,6 +720,10 @@ int32_t mp_tune_set(int32_t param, int32_t value) {
       if (value < 0 || value > (1 << 20)) return fail(MP_ERR_ARG, "scan spin limit must be 0..2^20");
       g_plain_scan_spin = value;
       return MP_OK;
+    case MP_TUNE_PLAIN_XCD_MAP:
+      if (value != 0 && value != 1) return fail(MP_ERR_ARG, "XCD map must be 0 or 1");
+      g_plain_xcd_map = value;
+      return MP_OK;
     case MP_TUNE_GRAD_BLOCK:
       if (value != 0 && value != 256 && value != 512)
         return fail(MP_ERR_ARG, "adjoint workgroup size must be 0 (automatic), 256 or 512");
@@ -717,6 +742,7 @@ int32_t mp_tune_get(int32_t param) {
     case MP_TUNE_PLAIN_SCAN_THREADS: return g_plain_scan_threads;
     case MP_TUNE_PLAIN_PROP_THREADS: return g_plain_prop_threads;
     case MP_TUNE_PLAIN_SCAN_SPIN: return g_plain_scan_spin;
+    case MP_TUNE_PLAIN_XCD_MAP: return g_plain_xcd_map;
   }
   return -1;
 }

@@ -126,7 +126,22 @@ struct PlainState {
   int uniform_w;       // weights are the constant wconst
   float wconst;
   int collapsed;       // previous step had all weights -inf
+  // Scalars of the step the next launches run, written by k_plain_reset for
+  // the first observation and by each scan's finishing tile for the next one,
+  // so that neither kernel starts with a chain of dependent loads.
+  int64_t obs;         // row of ys / times: obs_off[unit] + t
+  int64_t base;        // schedule index of the first sub-step key: tidx0 + t
+  int ti0;             // first Euler sub-step (index into the unit's dt_ext)
+  int nst;             // Euler sub-steps
+  float tstart, ty;    // start of the interval, observation time
+  float dt0;           // first sub-step length (0 without sub-steps)
+  uint32_t k0, k1;     // first sub-step key
+  uint32_t r0, r1;     // resampling key
 };
+
+// XCDs of an MI355X: the hardware dispatches workgroups over them round-robin
+// by block index, and each has its own L2.
+constexpr int kXcds = 8;
 
 struct PlainArgs {
   ModelDesc md;
@@ -153,6 +168,7 @@ struct PlainArgs {
   int rows_prop;        // particle rows per propagate block
   int prop_len;         // particles per propagate block (<= rows_prop * blockDim)
   int nprop;            // propagate blocks
+  int xcd_q;            // > 0: propagate block b takes chunk xcd_chunk(b, xcd_q); 0: chunk b
   int scan_spin;        // look-back probes before a tile computes a late predecessor's aggregate
 };
 

@@ -38,6 +38,33 @@ __device__ __forceinline__ void st_relaxed(uint64_t* p, uint64_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Chunk of propagate block b when a scan tile holds q chunks: blocks go to the
+// kXcds XCDs round-robin, so within each group of kXcds * q blocks, block r
+// (on XCD r % kXcds) takes chunk q * (r % kXcds) + r / kXcds; the q chunks of
+// scan tile k of the group then all run on XCD k % kXcds, the scan tile's own.
+__device__ __forceinline__ int xcd_chunk(int b, int q) {
+  const int G = kXcds * q, g = b / G, r = b - g * G;
+  return g * G + q * (r % kXcds) + r / kXcds;
+}
+
+// The step scalars of observation t (PlainState's step fields).
+__device__ void load_step(const PlainArgs& a, int t, PlainState& s) {
+  const int64_t obs = a.ser.obs_off[a.unit] + t;
+  const int ti0 = a.ser.tidx0[obs], nst = a.ser.nstep[obs];
+  const int64_t base = ti0 + (int64_t)t;
+  s.obs = obs;
+  s.base = base;
+  s.ti0 = ti0;
+  s.nst = nst;
+  s.tstart = t == 0 ? a.ser.t0 : a.ser.times[obs - 1];
+  s.ty = a.ser.times[obs];
+  s.dt0 = nst > 0 ? a.ser.dt_ext[a.ser.dt_off[a.unit] + ti0] : 0.0f;
+  s.k0 = a.sched[2 * base];
+  s.k1 = a.sched[2 * base + 1];
+  s.r0 = a.sched[2 * (base + nst)];
+  s.r1 = a.sched[2 * (base + nst) + 1];
+}
+
 template <class M>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k_plain_prop(const PlainArgs a) {
   __shared__ double red[32];
@@ -45,14 +72,15 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
   __shared__ uint64_t s_C[kWin];
   const ModelDesc md = a.md;
   const int J = a.J, S = M::FIXED_S ? M::MAXS : md.S, t = a.t, tid = threadIdx.x, B = blockDim.x;
-  const int64_t o0 = a.ser.obs_off[a.unit];
-  const int ti0 = a.ser.tidx0[o0 + t];
-  const int nst = a.ser.nstep[o0 + t];
+  // the step state, with this step's scalars (row, grid, keys) in it
+  const PlainState st = *a.st;
+  const int ti0 = st.ti0;
+  const int nst = st.nst;
   const float* dts = a.ser.dt_ext + a.ser.dt_off[a.unit];
-  const float ty = a.ser.times[o0 + t];
-  const float tstart = t == 0 ? a.ser.t0 : a.ser.times[o0 + t - 1];
-  const float* y = a.ser.ys + (o0 + t) * md.dy;
-  const int64_t base = ti0 + (int64_t)t;
+  const float ty = st.ty;
+  const float tstart = st.tstart;
+  const float* y = a.ser.ys + st.obs * md.dy;
+  const int64_t base = st.base;
   const float fJ = (float)J;
   const float logJ = d_log(fJ);
 
@@ -62,7 +90,6 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
   __syncthreads();
   const typename M::P& P = s_P;
 
-  const PlainState st = *a.st;
   const float* Xc = (t & 1) ? a.X1 : a.X0;
   float* Xn = (t & 1) ? a.X0 : a.X1;
   const float wconst = t == 0 ? d_log(1.0f / fJ) : st.wconst;
@@ -70,7 +97,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
   const bool have_anc = t == 0 ? false : st.have_anc;
 
   float lmax = -__builtin_inff(), lmin = __builtin_inff();
-  const int64_t tile0 = (int64_t)blockIdx.x * a.prop_len;
+  const int chunk = a.xcd_q > 0 ? xcd_chunk(blockIdx.x, a.xcd_q) : (int)blockIdx.x;
+  const int64_t tile0 = (int64_t)chunk * a.prop_len;
   const int64_t tile1 = min((int64_t)J, tile0 + (int64_t)a.prop_len);
   if (tile0 >= J) {  // a trailing block without particles
     if (tid == 0) {
@@ -97,7 +125,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
   const bool pre = M::NOISE > 0 && nst == 1;
   float zpre[kPreRows][NZ];
   if (pre) {
-    const Key sk = uniform_key(a.sched[2 * base], a.sched[2 * base + 1]);
+    const Key sk = uniform_key(st.k0, st.k1);
     MP_UNROLL for (int r = 0; r < kPreRows; ++r) {
       const int64_t j = tile0 + (int64_t)r * B + tid;
       if (r < a.rows_prop && j < tile1) M::draw(P, split_at(sk, (uint64_t)j), zpre[r]);
@@ -131,7 +159,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
       if ((md.accum_mask >> s) & 1u) x[s] = 0.f;
     float tt = tstart;
     if (use_z) {
-      M::step_z(md, P, zp, tt, dts[ti0], x);
+      M::step_z(md, P, zp, tt, st.dt0, x);
     } else {
       for (int k = 0; k < nst; ++k) {
         const Key sk = uniform_key(a.sched[2 * (base + k)], a.sched[2 * (base + k) + 1]);
@@ -207,6 +235,19 @@ __global__ __launch_bounds__(1024) void k_plain_scan(const PlainArgs a) {
   } else {
 #pragma unroll
     for (int r = 0; r < kMaxScanRows; ++r) lwv[r] = (r < rows && j0 + r < J) ? a.lw[j0 + r] : -__builtin_inff();
+  }
+  // The finishing tile's scalars, loaded now so that their dependent loads
+  // overlap the scan: the comb uniform from this step's resampling key, and
+  // the next observation's row, grid and keys.
+  const bool finisher = tile == ntiles - 1 && tid == 0;
+  float ures_next = 0.0f;
+  double loglik_prev = 0.0;
+  PlainState nx{};
+  if (finisher) {
+    const PlainState cur = *a.st;
+    loglik_prev = cur.loglik;
+    ures_next = uniform01(bits_at(Key{cur.r0, cur.r1}, 0));
+    if (a.t + 1 < a.T) load_step(a, a.t + 1, nx);
   }
   // global max / min of the log-weights from the per-block values, eight
   // independent loads in flight per thread
@@ -342,7 +383,7 @@ __global__ __launch_bounds__(1024) void k_plain_scan(const PlainArgs a) {
     const bool collapsed = (m == -__builtin_inff());
     const double llt = step_loglik(m, total, sh);
     const float llf = (float)llt;
-    st->loglik = (a.t == 0 ? 0.0 : st->loglik) + llt;
+    st->loglik = (a.t == 0 ? 0.0 : loglik_prev) + llt;
     if (a.cll) a.cll[a.t] = llf;
     const float nmax = collapsed ? -logJ : m - llf;
     const float nmin = collapsed ? -logJ : mn - llf;
@@ -353,12 +394,21 @@ __global__ __launch_bounds__(1024) void k_plain_scan(const PlainArgs a) {
     st->uniform_w = res;
     st->wconst = 0.0f - logJ;
     st->total = total;
-    const int64_t o0 = a.ser.obs_off[a.unit];
-    const int64_t base = a.ser.tidx0[o0 + a.t] + (int64_t)a.t;
-    const int nst = a.ser.nstep[o0 + a.t];
-    const Key rk{a.sched[2 * (base + nst)], a.sched[2 * (base + nst) + 1]};
-    st->ures = uniform01(bits_at(rk, 0));
-    if (a.t == a.T - 1) *a.loglik_out = st->loglik;
+    st->ures = ures_next;
+    if (a.t + 1 < a.T) {
+      st->obs = nx.obs;
+      st->base = nx.base;
+      st->ti0 = nx.ti0;
+      st->nst = nx.nst;
+      st->tstart = nx.tstart;
+      st->ty = nx.ty;
+      st->dt0 = nx.dt0;
+      st->k0 = nx.k0;
+      st->k1 = nx.k1;
+      st->r0 = nx.r0;
+      st->r1 = nx.r1;
+    }
+    if (a.t == a.T - 1) *a.loglik_out = (a.t == 0 ? 0.0 : loglik_prev) + llt;
   }
   // clear the next step's look-back words
   uint64_t* lbn = a.lb + (int64_t)(par ^ 1) * ntiles;
@@ -401,21 +451,23 @@ hipError_t plain_prepare(const ModelDesc& md, const float* theta, int64_t stride
   return hipErrorInvalidValue;
 }
 
-// Start of a filter: clear the look-back words and the step state.
-__global__ void k_plain_reset(uint64_t* lb, int ntiles, PlainState* st) {
+// Start of a filter: clear the look-back words and the step state, with the
+// first observation's scalars in it.
+__global__ void k_plain_reset(const PlainArgs a) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < ntiles) {
-    lb[i] = 0;
-    lb[ntiles + i] = 0;
+  if (i < a.ntiles) {
+    a.lb[i] = 0;
+    a.lb[a.ntiles + i] = 0;
   }
   if (i == 0) {
     PlainState z{};
-    *st = z;
+    if (a.T > 0) load_step(a, 0, z);
+    *a.st = z;
   }
 }
 
-hipError_t plain_reset(uint64_t* lb, int ntiles, PlainState* st, hipStream_t s) {
-  mp_launch(k_plain_reset, dim3((ntiles + 255) / 256), dim3(256), 0, s, lb, ntiles, st);
+hipError_t plain_reset(const PlainArgs& a, hipStream_t s) {
+  mp_launch(k_plain_reset, dim3((a.ntiles + 255) / 256), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

@@ -86,6 +86,9 @@ def test_tuning_settings_round_trip_and_reject_bad_values(lib):
     old = lib.mp_tune_get(7)
     assert lib.mp_tune_set(7, 0) == 0 and lib.mp_tune_get(7) == 0
     assert lib.mp_tune_set(7, old) == 0
+    assert lib.mp_tune_get(8) == 1 and lib.mp_tune_set(8, 2) != 0
+    assert lib.mp_tune_set(8, 0) == 0 and lib.mp_tune_get(8) == 0
+    assert lib.mp_tune_set(8, 1) == 0
 
 
 def test_library_shares_the_one_hip_runtime_in_the_process():

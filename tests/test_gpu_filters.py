@@ -196,6 +196,33 @@ def test_launch_shapes_do_not_change_results():
             autotune.set(n, v)
 
 
+def test_plain_xcd_chunk_map_does_not_change_results():
+    # Propagate blocks taking the chunks of the scan tiles on their own XCD (the
+    # default) or chunks in block order compute the same bits, with 1, 2, 4 and
+    # 16 propagate chunks per scan tile; all equal the batched form.
+    import pypomp_amd.functional as F
+    from pypomp_amd import _native, autotune
+    lib = _native.lib()
+    struct, theta, _ = lg2_struct(1)
+    keys = tf.split(tf.key(93), 2).reshape(1, 2, 2)
+    J = 200_000
+    ref = F.pfilter(struct, theta[None], J, keys, form="batched")
+    names = ("plain_scan_rows", "plain_scan_threads", "plain_prop_rows", "plain_prop_threads")
+    saved = {n: autotune.get(n) for n in names}
+    try:
+        for shape in ((1, 256, 1, 256), (2, 1024, 2, 512), (4, 1024, 2, 512), (4, 1024, 1, 256)):
+            for n, v in zip(names, shape):
+                autotune.set(n, v)
+            for xmap in (1, 0):
+                _native.check(lib.mp_tune_set(_native.MP_TUNE_PLAIN_XCD_MAP, xmap))
+                out = F.pfilter(struct, theta[None], J, keys, form="plain")
+                np.testing.assert_array_equal(out["logLik64"], ref["logLik64"], err_msg=f"{shape} map {xmap}")
+    finally:
+        for n, v in saved.items():
+            autotune.set(n, v)
+        _native.check(lib.mp_tune_set(_native.MP_TUNE_PLAIN_XCD_MAP, 1))
+
+
 def test_plain_scan_fallback_and_long_look_back():
     # Spin limit 0: a scan tile that finds a predecessor's look-back word
     # unpublished computes that aggregate itself from the log-weights.  With
