@@ -1,8 +1,9 @@
 """Launch-shape autotuner for the engine's kernels.
 
 The library exposes its launch shapes as process-wide settings
-(``mp_tune_set`` / ``mp_tune_get``: plain scan-tile rows, plain propagate-block
-rows, batched workgroup size, MOP-adjoint workgroup size).  ``sweep`` times a caller-supplied workload
+(``mp_tune_set`` / ``mp_tune_get``: plain scan-tile rows and threads, plain
+propagate-block rows and threads, the plain XCD-local chunk order, batched
+workgroup size, MOP-adjoint workgroup size).  ``sweep`` times a caller-supplied workload
 under every combination of candidate values with HIP events on the current
 stream, keeps the fastest, applies it and caches it per (workload key, device),
 in memory and optionally in a JSON file (``PYPOMP_AMD_TUNE_CACHE``).  The
@@ -27,6 +28,8 @@ PARAMS = {
     "grad_block": (_native.MP_TUNE_GRAD_BLOCK, (256, 512)),
     "plain_scan_threads": (_native.MP_TUNE_PLAIN_SCAN_THREADS, (256, 512, 1024)),
     "plain_prop_threads": (_native.MP_TUNE_PLAIN_PROP_THREADS, (256, 512)),
+    # 1: propagate blocks take the particle chunks of the scan tiles on their own XCD
+    "plain_xcd_map": (_native.MP_TUNE_PLAIN_XCD_MAP, (1, 0)),
 }
 
 _lock = threading.Lock()
