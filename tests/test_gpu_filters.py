@@ -223,6 +223,20 @@ def test_plain_xcd_chunk_map_does_not_change_results():
         _native.check(lib.mp_tune_set(_native.MP_TUNE_PLAIN_XCD_MAP, 1))
 
 
+@pytest.mark.parametrize("J", [32767, 32768, 32769])
+def test_plain_xcd_chunk_map_edges(J):
+    # The XCD-local order needs at least 8 q = 32 full propagate chunks of 1024
+    # particles: J = 32767 (32 chunks, the last one short), 32768 (exactly 32
+    # full chunks) and 32769 (33 chunks: 64 blocks, 31 of them empty).
+    import pypomp_amd.functional as F
+    struct, theta, _ = lg2_struct(1)
+    keys = tf.split(tf.key(95), 2).reshape(1, 2, 2)
+    plain = F.pfilter(struct, theta[None], J, keys, form="plain", CLL=True)
+    batched = F.pfilter(struct, theta[None], J, keys, form="batched", CLL=True)
+    np.testing.assert_array_equal(plain["logLik64"], batched["logLik64"])
+    np.testing.assert_array_equal(plain["CLL"], batched["CLL"])
+
+
 def test_plain_scan_fallback_and_long_look_back():
     # Spin limit 0: a scan tile that finds a predecessor's look-back word
     # unpublished computes that aggregate itself from the log-weights.  With
