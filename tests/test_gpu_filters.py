@@ -304,3 +304,68 @@ def test_pomp_pickle_continues_the_key_stream_and_merge_joins_runs():
     assert set(tr["theta_idx"]) == {0, 1} and set(tr["method"]) == {"pfilter", "mif"}
     with pytest.raises(ValueError, match="same number of entries"):
         type(P).merge(P, LG(A=np.array([[0.5]]), ys=lg1_data(0)))
+
+
+def _lg_oo():
+    import pypomp_amd as pp
+    from pypomp_amd.models.lg import LG
+    return LG(A=np.array([[0.9]]), T=5, key=pp.random.key(0))
+
+
+def _reversed(theta):
+    return [{k: row[k] for k in reversed(list(row))} for row in theta]
+
+
+def _oo_pfilter(P, theta):
+    P.pfilter(J=64, key=tf.key(11), theta=theta, CLL=True)
+    return np.asarray(P.results_history[-1]["logLiks"])
+
+
+def _oo_mif(P, theta):
+    import pypomp_amd as pp
+    rw = pp.RWSigma({n: 0.02 for n in P.canonical_param_names}).geometric_cooling(0.5)
+    P.mif(J=64, M=2, rw_sd=rw, key=tf.key(11), theta=theta, n_monitors=1)
+    return np.asarray(P.results_history[-1]["traces"])
+
+
+def _oo_train(P, theta):
+    import pypomp_amd as pp
+    eta = pp.LearningRate({n: 0.2 for n in P.canonical_param_names})
+    P.train(J=64, M=2, eta=eta, key=tf.key(11), theta=theta)
+    return np.asarray(P.results_history[-1]["traces"])
+
+
+def _oo_simulate(P, theta):
+    X, Y = P.simulate(key=tf.key(11), theta=theta, nsim=1)
+    return np.concatenate([X.to_numpy().ravel(), Y.to_numpy().ravel()])
+
+
+@pytest.mark.parametrize("run", [_oo_pfilter, _oo_mif, _oo_train, _oo_simulate],
+                         ids=["pfilter", "mif", "train", "simulate"])
+def test_param_order_invariance(run):
+    # tests/test_properties/test_invariants.py:145-171: results do not depend on
+    # the order theta's keys were supplied in (train with Adam: the reference's
+    # Newton needs the MOP Hessian)
+    P = _lg_oo()
+    theta = [dict(t) for t in P.theta]
+    base = run(P, theta)
+    P.results_history.clear()
+    perm = run(P, _reversed(theta))
+    np.testing.assert_array_equal(base, perm)
+
+
+def test_oo_cll_sum_and_mif_trace_start():
+    # tests/test_properties/test_invariants.py:50-75: the conditional
+    # log-likelihoods sum to the log-likelihood, and iteration 0 of the IF2
+    # trace is the parameter vector IF2 started from
+    import pypomp_amd as pp
+    P = _lg_oo()
+    P.pfilter(J=64, key=tf.key(12), CLL=True, reps=2)
+    r = P.results_history[-1]
+    np.testing.assert_allclose(np.asarray(r["CLL"], np.float64).sum(axis=-1), r["logLiks"], rtol=1e-5, atol=1e-5)
+    names = P.canonical_param_names
+    start = np.array([[t[n] for n in names] for t in P.theta], np.float32)
+    rw = pp.RWSigma({n: 0.02 for n in names}).geometric_cooling(0.5)
+    P.mif(J=64, M=2, rw_sd=rw, key=tf.key(12), n_monitors=1)
+    tr = P.results_history[-1]["traces"]
+    np.testing.assert_allclose(tr[:, 0, 1:], start, rtol=1e-6, atol=1e-6)
