@@ -369,3 +369,42 @@ def test_oo_cll_sum_and_mif_trace_start():
     P.mif(J=64, M=2, rw_sd=rw, key=tf.key(12), n_monitors=1)
     tr = P.results_history[-1]["traces"]
     np.testing.assert_allclose(tr[:, 0, 1:], start, rtol=1e-6, atol=1e-6)
+
+
+def test_object_api_equals_the_functional_calls():
+    # tests/test_properties/test_layer_parity.py:51-135: Pomp.pfilter, Pomp.mif
+    # and Pomp.simulate equal hand-built functional calls with the key derived
+    # from the given key (split(key)[1]), bit for bit
+    import pypomp_amd as pp
+    import pypomp_amd.functional as F
+    from pypomp_amd import random as prandom
+    P = _lg_oo()
+    names = P.canonical_param_names
+    th = P.model.theta_array(P.theta)
+    key = tf.key(13)
+    new = prandom.split(key)[1]
+
+    P.pfilter(J=64, key=key, reps=2, CLL=True, ESS=True)
+    r = P.results_history[-1]
+    exp = F.pfilter(P.to_struct(), th, 64, prandom.split(new, 2).reshape(1, 2, 2), 0.0, CLL=True, ESS=True)
+    np.testing.assert_array_equal(r["logLiks"], exp["logLik"])
+    np.testing.assert_array_equal(r["CLL"], exp["CLL"])
+    np.testing.assert_array_equal(r["ESS"], exp["ESS"])
+
+    rw = pp.RWSigma({n: 0.02 for n in names}).geometric_cooling(0.5)
+    th = P.model.theta_array(P.theta)
+    P.mif(J=64, M=2, rw_sd=rw, key=key)
+    tr = P.results_history[-1]["traces"]
+    lls, traces, _ = F.mif(P.to_struct(), np.repeat(th[:, None, :], 64, axis=1), rw, 2, 64,
+                           prandom.split(new, 1), 0.0)
+    assert np.all(np.isnan(tr[:, 0, 0]))
+    np.testing.assert_array_equal(tr[:, 1:, 0], lls)
+    np.testing.assert_array_equal(tr[:, :, 1:], traces)
+
+    th = P.model.theta_array(P.theta)
+    X, Y = P.simulate(key=key, nsim=3)
+    Xs, Ys = F.simulate(P.to_struct(), th, 3, prandom.split(new, 1))
+    np.testing.assert_array_equal(Y.drop(columns=["theta_idx", "sim", "time"]).to_numpy(),
+                                  np.asarray(Ys).reshape(-1, Ys.shape[-1]))
+    np.testing.assert_array_equal(X.drop(columns=["theta_idx", "sim", "time"]).to_numpy(),
+                                  np.asarray(Xs).reshape(-1, Xs.shape[-1]))
