@@ -117,6 +117,11 @@ class Pomp:
         if covars is not None:
             raise NotImplementedError("covariates are not supported by the native models")
         self.model = _resolve_model(model, rinit, rproc, dmeas, rmeas)
+        # the plugin slots supplied (all four with ``model=``): a filter needs
+        # dmeas and simulate needs rmeas, as in the reference
+        given = [c.role for c in (rinit, rproc, dmeas, rmeas) if c is not None]
+        self._slots = ("rinit", "rproc", "dmeas", "rmeas") if model is not None else tuple(
+            r for r in ("rinit", "rproc", "dmeas", "rmeas") if r in given)
         if accumvars is not None and tuple(accumvars) != tuple(self.model.accumvars):
             raise ValueError("accumvars are fixed by the native model: "
                              f"{tuple(self.model.accumvars)}")
@@ -136,6 +141,31 @@ class Pomp:
         # estimation-scale transforms a panel applies around IF2 (panel/panel.py:146);
         # None: the model's own (the filters always apply the model's)
         self.par_trans = None
+
+    # -- plugin slots (None when not supplied) -----------------------------
+    def _slot(self, role):
+        return getattr(self.model, role) if role in getattr(self, "_slots", (role,)) else None
+
+    @property
+    def rinit(self):
+        return self._slot("rinit")
+
+    @property
+    def rproc(self):
+        return self._slot("rproc")
+
+    @property
+    def dmeas(self):
+        return self._slot("dmeas")
+
+    @property
+    def rmeas(self):
+        return self._slot("rmeas")
+
+    def _need_dmeas(self):
+        # core/estimation_mixin.py:181, 365, 554
+        if self.dmeas is None:
+            raise ValueError("self.dmeas cannot be None")
 
     # -- parameters -------------------------------------------------------
     @property
@@ -177,6 +207,7 @@ class Pomp:
     def pfilter(self, J, key=None, theta=None, thresh=0.0, reps=1, CLL=False, ESS=False,
                 filter_mean=False, prediction_mean=False, track_time=True, form="batched"):
         """Bootstrap particle filter (estimation_mixin.py:101-272)."""
+        self._need_dmeas()
         start = time.time()
         thresh = float(max(0.0, thresh))
         if J < 1:
@@ -199,6 +230,7 @@ class Pomp:
 
     def mif(self, J, M, rw_sd, key=None, theta=None, thresh=0.0, n_monitors=0, track_time=True):
         """IF2 (estimation_mixin.py:274-444)."""
+        self._need_dmeas()
         start = time.time()
         if set(rw_sd.param_names) != set(self.canonical_param_names):
             raise ValueError("rw_sd.sigmas keys must match canonical_param_names up to reordering. "
@@ -225,6 +257,7 @@ class Pomp:
         """Gradient ascent on the MOP objective with the native adjoint
         (estimation_mixin.py:446-634): theta to the estimation scale, F.train,
         traces back to the natural scale."""
+        self._need_dmeas()
         from .optim import Adam, LearningRate
         start = time.time()
         optimizer = optimizer or Adam()
@@ -253,6 +286,8 @@ class Pomp:
         """States and observations from the model (estimation_mixin.py:1081-1223):
         long frames (theta_idx, sim, time, variables), or with ``as_pomp`` a copy
         whose ys are the first parameter set's simulation."""
+        if self.rmeas is None:  # core/estimation_mixin.py:1168
+            raise ValueError("self.rmeas cannot be None. Did you forget to supply it to the object or method?")
         if as_pomp:
             if nsim > 1:
                 warnings.warn("as_pomp is True, but nsim > 1. Only 1 simulation will be performed as_pomp "
@@ -285,6 +320,7 @@ class Pomp:
         """Particle marginal Metropolis-Hastings, one chain per parameter set
         (estimation_mixin.py:823-943).  ``dprior(theta: dict) -> float`` is the
         log prior on the natural scale (flat when None)."""
+        self._need_dmeas()
         start = time.time()
         if J < 1:
             raise ValueError("J should be greater than 0.")
@@ -305,6 +341,7 @@ class Pomp:
 
     def mop(self, J, alpha, key=None, theta=None):
         """MOP(alpha) log-likelihood estimate at natural-scale theta (value only)."""
+        self._need_dmeas()
         thetas = self._theta_array(theta)
         new, _ = self._update_fresh_key(key)
         keys = prandom.split(new, thetas.shape[0])
@@ -380,6 +417,8 @@ class Pomp:
             return False
         if self.model != other.model or not _same(self.par_trans, other.par_trans):
             return False
+        if getattr(self, "_slots", None) != getattr(other, "_slots", None):
+            return False
         if not _same_history(self.results_history, other.results_history):
             return False
         if (self.fresh_key is None) != (other.fresh_key is None):
@@ -408,8 +447,13 @@ class Pomp:
                 raise ValueError("All Pomp objects must have the same ys data.")
             if obj.t0 != first.t0:
                 raise ValueError("All Pomp objects must have the same t0.")
-            if obj.model != first.model:
+            if obj.model != first.model or (obj.rinit is None) != (first.rinit is None) or \
+                    (obj.rproc is None) != (first.rproc is None):
                 raise ValueError("All Pomp objects must have the same rinit and rproc.")
+            if (obj.dmeas is None) != (first.dmeas is None):
+                raise ValueError("All Pomp objects must have the same dmeas (both None or both not None).")
+            if (obj.rmeas is None) != (first.rmeas is None):
+                raise ValueError("All Pomp objects must have the same rmeas (both None or both not None).")
             if not _same(obj.par_trans, first.par_trans):
                 raise ValueError("All Pomp objects must have the same par_trans.")
             if not obj.theta:

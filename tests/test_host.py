@@ -269,3 +269,30 @@ def test_pomp_equality_ignores_execution_time():
     Q = copy.deepcopy(P)
     Q.results_history[0]["execution_time"] = 123.0
     assert P == Q
+
+
+def test_pomp_missing_plugins_raise_the_reference_errors():
+    # core/estimation_mixin.py:181, 365, 554, 1168 and
+    # tests/test_pomp/test_pomp_validation.py:159-209: a Pomp built without dmeas
+    # cannot filter, one without rmeas cannot simulate
+    import pypomp_amd as pp
+    P = _lg_pomp()
+    m = P.model
+    Q = pp.Pomp(ys=P.ys, theta=P.theta, t0=0.0, rinit=m.rinit, rproc=m.rproc, rmeas=m.rmeas, nstep=1)
+    Q.fresh_key = P.fresh_key
+    assert Q.dmeas is None and Q.rmeas is not None and P.dmeas is not None
+    rw = pp.RWSigma({n: 0.01 for n in Q.canonical_param_names})
+    eta = pp.LearningRate({n: 0.1 for n in Q.canonical_param_names})
+    for call in (lambda: Q.pfilter(J=10), lambda: Q.mif(J=10, M=1, rw_sd=rw), lambda: Q.train(J=10, M=1, eta=eta),
+                 lambda: Q.mop(J=10, alpha=0.9), lambda: Q.pmcmc(J=10, M=1, proposal=None)):
+        with pytest.raises(ValueError, match="self.dmeas cannot be None"):
+            call()
+    R = pp.Pomp(ys=P.ys, theta=P.theta, t0=0.0, rinit=m.rinit, rproc=m.rproc, dmeas=m.dmeas, nstep=1)
+    R.fresh_key = P.fresh_key
+    with pytest.raises(ValueError, match="self.rmeas cannot be None"):
+        R.simulate()
+    assert P != R and R != Q
+    with pytest.raises(ValueError, match="same dmeas"):
+        type(P).merge(R, Q)
+    with pytest.raises(ValueError, match="same rmeas"):
+        type(P).merge(P, R)
