@@ -39,23 +39,27 @@ class Series:
 
 
 def make_series(ys, times, t0, nstep=None, dt=None):
-    """Fixed ``nstep`` or fixed ``dt`` Euler grid, as ``_calc_steps`` (helpers.py:191-215)."""
+    """Fixed ``nstep`` or fixed ``dt`` Euler grid, as ``_calc_steps`` (helpers.py:158-215):
+    the loop rebinds ``dt`` to each interval's result, so the next interval
+    starts from the previous interval's step, not the caller's."""
     times = np.asarray(times, dtype=np.float64)
     times0 = np.concatenate([[float(t0)], times])
     ns, dts = [], []
     tol = np.sqrt(np.finfo(float).eps)
+    h = None if dt is None else float(dt)
     for i in range(len(times)):
         t1, t2 = float(times0[i]), float(times0[i + 1])
         if nstep is not None:
-            ns.append(int(nstep)); dts.append((t2 - t1) / int(nstep))
+            n, h = int(nstep), (t2 - t1) / int(nstep)
+        elif t1 >= t2:
+            n, h = 0, 0.0
+        elif t1 + h >= t2:
+            n, h = 1, t2 - t1
         else:
-            if t1 >= t2:
-                ns.append(0); dts.append(0.0)
-            elif t1 + dt >= t2:
-                ns.append(1); dts.append(t2 - t1)
-            else:
-                n = int(np.ceil((t2 - t1) / dt / (1 + tol)))
-                ns.append(n); dts.append((t2 - t1) / n)
+            n = int(np.ceil((t2 - t1) / h / (1 + tol)))
+            h = (t2 - t1) / n
+        ns.append(n)
+        dts.append(h)
     ns = np.array(ns, dtype=np.int64)
     dt_ext = np.repeat(np.array(dts), ns).astype(F32)
     return Series(ys=np.asarray(ys, dtype=F32).reshape(len(times), -1),

@@ -117,11 +117,20 @@ class Pomp:
         if covars is not None:
             raise NotImplementedError("covariates are not supported by the native models")
         self.model = _resolve_model(model, rinit, rproc, dmeas, rmeas)
+        slots = {"rinit": rinit, "rproc": rproc, "dmeas": dmeas, "rmeas": rmeas}
+        for name, comp in slots.items():
+            if comp is not None and comp.role != name:
+                raise ValueError(f"{comp!r} was passed as {name}; it is the model's {comp.role}")
+        if model is None:
+            # core/pomp.py:207-208 (rinit and rproc are required), :319-320
+            if rinit is None or rproc is None:
+                raise TypeError("Pomp needs rinit and rproc (or model=)")
+            if dmeas is None and rmeas is None:
+                raise ValueError("You must supply at least one of dmeas or rmeas")
         # the plugin slots supplied (all four with ``model=``): a filter needs
         # dmeas and simulate needs rmeas, as in the reference
-        given = [c.role for c in (rinit, rproc, dmeas, rmeas) if c is not None]
         self._slots = ("rinit", "rproc", "dmeas", "rmeas") if model is not None else tuple(
-            r for r in ("rinit", "rproc", "dmeas", "rmeas") if r in given)
+            r for r, c in slots.items() if c is not None)
         if accumvars is not None and tuple(accumvars) != tuple(self.model.accumvars):
             raise ValueError("accumvars are fixed by the native model: "
                              f"{tuple(self.model.accumvars)}")

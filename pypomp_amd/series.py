@@ -14,28 +14,33 @@ from . import _native
 
 
 def calc_steps(times0, dt=None, nstep=None):
-    """(nstep_array, dt_array) per interval of ``times0`` (helpers.py:191-215)."""
+    """(nstep_array, dt_array) per interval of ``times0`` (helpers.py:158-215).
+
+    With ``dt`` the step size is carried from interval to interval as the
+    reference's loop does: each interval starts from the previous interval's
+    adjusted step (``(t2 - t1) / n``), not from the caller's ``dt``.  A
+    zero-length interval leaves a step of 0, so a later positive interval
+    divides by zero and raises ``ZeroDivisionError``, as in the reference."""
     if dt is not None and nstep is not None:
         raise ValueError("Only nstep or dt can be provided, not both")
     if dt is None and nstep is None:
         raise ValueError("Either dt or nstep must be provided")
     tol = np.sqrt(np.finfo(float).eps)
     ns, dts = [], []
+    step = None if dt is None else float(dt)
     for i in range(len(times0) - 1):
         t1, t2 = float(times0[i]), float(times0[i + 1])
         if nstep is not None:
-            ns.append(int(nstep))
-            dts.append((t2 - t1) / int(nstep))
+            n, step = int(nstep), (t2 - t1) / int(nstep)
         elif t1 >= t2:
-            ns.append(0)
-            dts.append(0.0)
-        elif t1 + float(dt) >= t2:
-            ns.append(1)
-            dts.append(t2 - t1)
+            n, step = 0, 0.0
+        elif t1 + step >= t2:
+            n, step = 1, t2 - t1
         else:
-            n = int(np.ceil((t2 - t1) / float(dt) / (1 + tol)))
-            ns.append(n)
-            dts.append((t2 - t1) / n)
+            n = int(np.ceil((t2 - t1) / step / (1 + tol)))
+            step = (t2 - t1) / n
+        ns.append(n)
+        dts.append(step)
     return np.array(ns, dtype=np.int64), np.array(dts, dtype=np.float64)
 
 

@@ -23,7 +23,10 @@ def workloads():
     return {k: d[k] for k in d.files}
 
 
-def assert_close_tol(got, expected, atol=1e-6, rtol=1e-4):
+def assert_close_tol(got, expected, atol=1e-6, rtol=1e-5):
+    """Parity bar against the reference's regression values and the oracle:
+    1e-5 relative (the stated float32 log-likelihood target), with 1e-6
+    absolute for entries near zero."""
     got = np.asarray(got, dtype=float).ravel()
     expected = np.asarray(expected, dtype=float).ravel()
     assert got.shape == expected.shape, (got.shape, expected.shape)

@@ -45,9 +45,23 @@ def test_euler_grid_dt_and_nstep():
     times0 = np.array([-1.0, 0, 0.5, 1, 1.5, 2, 2.5, 3, 3.5, 4, 4.5, 5])
     ns, dts = calc_steps(times0, dt=0.5)
     assert ns.tolist() == [2] + [1] * 10 and np.allclose(dts, [0.5] * 11)
-    ns, dts = calc_steps(np.array([0.0, 1.0, 1.0, 3.0]), dt=0.3)
-    assert ns.tolist() == [4, 0, 7]
-    assert np.allclose(dts, [0.25, 0.0, 2.0 / 7])
+    # the reference carries each interval's adjusted step into the next one
+    # (helpers.py:205-214 rebinds dt): [0, 1] takes 4 steps of 0.25, so [1, 3]
+    # takes 8 steps of 0.25, not ceil(2 / 0.3) = 7
+    ns, dts = calc_steps(np.array([0.0, 1.0, 3.0]), dt=0.3)
+    assert ns.tolist() == [4, 8] and np.allclose(dts, [0.25, 0.25])
+    # a short first interval shrinks the step of every later one
+    ns, dts = calc_steps(np.array([0.0, 0.1, 5.1]), dt=1.0)
+    assert ns.tolist() == [1, 50] and np.allclose(dts, [0.1, 0.1])
+    # a zero-length interval leaves a step of 0: the next positive interval
+    # divides by zero, as in the reference
+    with pytest.raises(ZeroDivisionError):
+        calc_steps(np.array([0.0, 1.0, 1.0, 3.0]), dt=0.3)
+    ns, dts = calc_steps(np.array([0.0, 1.0, 1.0]), dt=0.3)
+    assert ns.tolist() == [4, 0] and np.allclose(dts, [0.25, 0.0])
+    # the reference's own grid case (tests/test_core/test_helpers.py:10-31)
+    ns, dts = calc_steps(np.array([-1.0] + [0.5 * i for i in range(11)]), dt=0.5)
+    assert ns.tolist() == [2] + [1] * 10 and np.allclose(np.repeat(dts, ns), 0.5)
     ns, dts = calc_steps(np.array([0.0, 1.0, 3.0]), nstep=7)
     assert ns.tolist() == [7, 7] and np.allclose(dts, [1 / 7, 2 / 7])
     with pytest.raises(ValueError):
@@ -57,6 +71,16 @@ def test_euler_grid_dt_and_nstep():
     s = Series.build(np.zeros((3, 1)), [1.0, 2.0, 4.0], 0.0, dt=0.5)
     assert s.nstep.tolist() == [2, 2, 4] and s.dt_ext.shape == (8,)
     assert np.isclose(s.dt_ext.astype(np.float64).sum(), 4.0)
+
+
+@pytest.mark.parametrize("times,t0,dt", [([1.0, 3.0], 0.0, 0.3), ([0.1, 5.1], 0.0, 1.0),
+                                         ([0.7, 1.9, 2.0, 7.3], 0.0, 0.45), ([1.0, 1.0], 0.0, 0.3)])
+def test_oracle_euler_grid_equals_the_host_grid(times, t0, dt):
+    from oracle import filters as fl
+    s = Series.build(np.zeros((len(times), 1)), times, t0, dt=dt)
+    o = fl.make_series(np.zeros((len(times), 1)), times, t0, dt=dt)
+    assert s.nstep.tolist() == o.nstep.tolist()
+    np.testing.assert_array_equal(s.dt_ext, o.dt_ext)
 
 
 def test_rwsigma_validation_and_order():
@@ -296,3 +320,24 @@ def test_pomp_missing_plugins_raise_the_reference_errors():
         type(P).merge(R, Q)
     with pytest.raises(ValueError, match="same rmeas"):
         type(P).merge(P, R)
+
+
+def test_pomp_slot_rules_follow_the_reference():
+    # core/pomp.py:207-208 (rinit and rproc are required arguments) and
+    # :319-320 (at least one of dmeas or rmeas); a component goes in its own slot
+    import pypomp_amd as pp
+    P = _lg_pomp()
+    m = P.model
+    kw = dict(ys=P.ys, theta=P.theta, t0=0.0, nstep=1)
+    with pytest.raises(ValueError, match="at least one of dmeas or rmeas"):
+        pp.Pomp(rinit=m.rinit, rproc=m.rproc, **kw)
+    with pytest.raises(TypeError, match="rinit and rproc"):
+        pp.Pomp(rproc=m.rproc, dmeas=m.dmeas, **kw)
+    with pytest.raises(TypeError, match="rinit and rproc"):
+        pp.Pomp(rinit=m.rinit, dmeas=m.dmeas, **kw)
+    with pytest.raises(ValueError, match="passed as dmeas"):
+        pp.Pomp(rinit=m.rinit, rproc=m.rproc, dmeas=m.rmeas, **kw)
+    with pytest.raises(ValueError, match="passed as rinit"):
+        pp.Pomp(rinit=m.rproc, rproc=m.rproc, dmeas=m.dmeas, **kw)
+    ok = pp.Pomp(rinit=m.rinit, rproc=m.rproc, dmeas=m.dmeas, **kw)
+    assert ok.rinit is not None and ok.rproc is not None and ok.rmeas is None
