@@ -127,6 +127,22 @@ int32_t mp_pfilter(const mp_model_desc* h_model, const mp_series_desc* h_series,
                    float thresh, int32_t flags, int32_t form, double* loglik, float* cll, float* ess,
                    float* fmean, float* pmean, void* workspace, size_t ws_bytes, void* stream);
 
+/* mp_pfilter without a workspace argument, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  Same arguments,
+ * results and errors as mp_pfilter; its scratch is a fixed array of
+ * MP_NOALLOC_SCRATCH_BYTES in the library, one per GPU, held until the process
+ * ends.  A call whose workspace size (mp_filter_workspace_size) exceeds it
+ * returns MP_ERR_ARG before launching anything.  The call allocates nothing
+ * and does not record kernel timings.  Every call on one GPU uses that GPU's
+ * one array: calls that overlap on a GPU (on different streams or threads)
+ * race and must be serialised by the caller. */
+#define MP_NOALLOC_SCRATCH_BYTES (64u * 1024u * 1024u)
+int32_t mp_pfilter_noalloc(const mp_model_desc* h_model, const mp_series_desc* h_series, const float* theta,
+                           int64_t theta_stride, const uint32_t* keys, const int32_t* unit, int32_t R, int32_t J,
+                           float thresh, int32_t flags, int32_t form, double* loglik, float* cll, float* ess,
+                           float* fmean, float* pmean, void* stream);
+
 /* MOP(alpha) log-likelihood value: _mop_internal (core/algorithms/mop.py:19-135);
  * theta on the estimation scale.  Returns +loglik (the reference's F.mop
  * returns its negative, functional/mop.py:58-62). */

@@ -22,13 +22,14 @@ EXPORTS = [
     "mp_last_error", "mp_abi_version", "mp_host_threefry2x32", "mp_host_split",
     "mp_random_bits", "mp_uniform", "mp_normal", "mp_split",
     "mp_resample_workspace_size", "mp_resample_systematic",
-    "mp_filter_workspace_size", "mp_pfilter", "mp_mop", "mp_perfilter",
+    "mp_filter_workspace_size", "mp_pfilter", "mp_pfilter_noalloc", "mp_mop", "mp_perfilter",
     "mp_mop_grad_workspace_size", "mp_mop_grad", "mp_simulate", "mp_host_from_est",
     "mp_profile_enable", "mp_profile_read", "mp_tune_set", "mp_tune_get",
 ]
 MP_TUNE_PLAIN_SCAN_ROWS, MP_TUNE_PLAIN_PROP_ROWS, MP_TUNE_BATCHED_BLOCK, MP_TUNE_GRAD_BLOCK = 1, 2, 3, 4
 MP_TUNE_PLAIN_SCAN_THREADS, MP_TUNE_PLAIN_PROP_THREADS, MP_TUNE_PLAIN_SCAN_SPIN = 5, 6, 7
 MP_TUNE_PLAIN_XCD_MAP = 8
+MP_NOALLOC_SCRATCH_BYTES = 64 * 1024 * 1024
 
 
 class ModelDesc(C.Structure):
@@ -80,6 +81,8 @@ def lib():
     L.mp_filter_workspace_size.restype = sz
     L.mp_pfilter.argtypes = [C.POINTER(ModelDesc), C.POINTER(SeriesDesc), f32p, i64, u32p, vp, i32, i32,
                              C.c_float, i32, i32, f64p, f32p, f32p, f32p, f32p, vp, sz, vp]
+    L.mp_pfilter_noalloc.argtypes = [C.POINTER(ModelDesc), C.POINTER(SeriesDesc), f32p, i64, u32p, vp, i32, i32,
+                                     C.c_float, i32, i32, f64p, f32p, f32p, f32p, f32p, vp]
     L.mp_mop.argtypes = [C.POINTER(ModelDesc), C.POINTER(SeriesDesc), f32p, i64, u32p, vp, i32, i32,
                          C.c_float, i32, f64p, vp, sz, vp]
     L.mp_perfilter.argtypes = [C.POINTER(ModelDesc), C.POINTER(SeriesDesc), f32p, u32p, vp, i32, i32,

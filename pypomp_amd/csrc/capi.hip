@@ -42,6 +42,13 @@ using namespace mp;
 // the timing events the next mp_launch attaches to its dispatch (engine.hpp)
 thread_local LaunchTiming mp::g_launch_timing;
 
+// Scratch of the no-allocation entry point (mp_pfilter_noalloc): one copy per
+// GPU, held from the library's load until the process ends.  In host code
+// `scratch` names a host-side stand-in, not device memory: the device address
+// comes from hipGetSymbolAddress, once per device, and only that address is
+// handed to kernels.
+static __device__ __align__(256) char scratch[MP_NOALLOC_SCRATCH_BYTES];
+
 static_assert(sizeof(mp_model_desc) == sizeof(ModelDesc), "model desc layout");
 static_assert(sizeof(mp_series_desc) == sizeof(SeriesSet), "series desc layout");
 
@@ -111,11 +118,15 @@ hipEvent_t take_event() {
   return e;
 }
 
+// Set while a no-allocation entry point runs: its launches are not timed,
+// because timing would create HIP events.
+thread_local bool g_no_alloc = false;
+
 // Runs `launch` with the timing events handed to its kernel launch when
 // profiling is on; a launch that does not take them is not timed.
 template <class F>
 hipError_t timed(int kind, hipStream_t, F&& launch) {
-  if (!g_prof_on) return launch();
+  if (!g_prof_on || g_no_alloc) return launch();
   std::lock_guard<std::mutex> lk(g_prof_mu);
   ProfRec r{kind, take_event(), take_event()};
   if (!r.a || !r.b) return launch();
@@ -435,6 +446,28 @@ int run_mop_grad(const mp_model_desc* m, const mp_series_desc* s, const float* t
   return MP_OK;
 }
 
+// Device address of `scratch` on the current device, looked up on the first
+// call made on that device and kept per device.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_address(void** out) {
+  int dev = -1;
+  MP_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(MP_ERR_ARG, "device index %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if (!p) return fail(MP_ERR_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return MP_OK;
+}
+
 // the model's from_est run on the host CPU (detmath is host-and-device code)
 template <class M>
 void host_from_est_rows(const ModelDesc& md, float* theta, int64_t n) {
@@ -565,6 +598,29 @@ int32_t mp_pfilter(const mp_model_desc* h_model, const mp_series_desc* h_series,
   a.fmean = fmean;
   a.pmean = pmean;
   return run_batched(ALGO_PF, h_model, h_series, a, workspace, ws_bytes, st);
+}
+
+int32_t mp_pfilter_noalloc(const mp_model_desc* h_model, const mp_series_desc* h_series, const float* theta,
+                           int64_t theta_stride, const uint32_t* keys, const int32_t* unit, int32_t R, int32_t J,
+                           float thresh, int32_t flags, int32_t form, double* loglik, float* cll, float* ess,
+                           float* fmean, float* pmean, void* stream) {
+  int rc = check_common(h_model, h_series, R, J, keys);
+  if (rc) return rc;
+  const size_t need = mp_filter_workspace_size(MP_ALGO_PF, h_model, h_series, R, J, form);
+  if (need == 0) return fail(MP_ERR_ARG, "workspace size query failed");
+  if (need > (size_t)MP_NOALLOC_SCRATCH_BYTES)
+    return fail(MP_ERR_ARG,
+                "mp_pfilter_noalloc: this call needs %zu bytes of scratch, more than the %zu the "
+                "no-allocation entry point holds; use mp_pfilter with a workspace",
+                need, (size_t)MP_NOALLOC_SCRATCH_BYTES);
+  void* ws = nullptr;
+  rc = scratch_address(&ws);
+  if (rc) return rc;
+  g_no_alloc = true;
+  rc = mp_pfilter(h_model, h_series, theta, theta_stride, keys, unit, R, J, thresh, flags, form, loglik, cll, ess,
+                  fmean, pmean, ws, (size_t)MP_NOALLOC_SCRATCH_BYTES, stream);
+  g_no_alloc = false;
+  return rc;
 }
 
 int32_t mp_mop(const mp_model_desc* h_model, const mp_series_desc* h_series, const float* theta, int64_t theta_stride,
