@@ -33,6 +33,8 @@ extern "C" {
 #define MP_MODEL_LG 1   /* pypomp/models/linear_gaussian.py:139-191 */
 #define MP_MODEL_SEIR 2 /* Euler-binomial SEIR, NegBin reports */
 #define MP_MODEL_SV 3   /* stochastic volatility with leverage */
+#define MP_MODEL_SPX 4  /* pypomp/models/spx.py:56-151: Heston-type volatility with the previous
+                           return as covariate y_prev; per-particle or @vectorized rproc */
 
 /* algorithms / forms */
 #define MP_ALGO_PF 0
@@ -60,6 +62,10 @@ typedef struct mp_model_desc {
   int32_t identity_trans; /* 1: the estimation scale is the natural scale (the
                           reference's ParTrans() with no transforms,
                           core/par_trans.py:70-111); 0: the model's own transforms */
+  int32_t vectorized;  /* 1: the model's rproc is @vectorized (model_mechanics.py:
+                          311-355, 553-568): each Euler sub-step draws particle j's
+                          variates at flat index j of one draw from the sub-step key,
+                          instead of from split(key, J)[j]; models that define it only */
 } mp_model_desc;
 
 /* Observation series of U units (U = 1 for a single Pomp).  Unit u owns rows
@@ -79,6 +85,18 @@ typedef struct mp_series_desc {
   int32_t U;
   int32_t Tmax;           /* max_u T_u */
   int64_t max_euler;      /* max_u sum_t nstep */
+  /* Covariates interpolated onto each unit's Euler grid (reference:
+   * covars_extended from _calc_ys_covars / _calc_interp_covars,
+   * core/algorithms/helpers.py:248-299): unit u owns E_u + 1 rows from
+   * cv_off[u], E_u = its number of Euler steps; row tidx0[t] + k holds the
+   * covariates of sub-step k of interval t (model_mechanics.py:894-906), row
+   * tidx0[t] + nstep[t] those of observation t (pfilter.py:124-128), row 0 those
+   * of rinit (pfilter.py:28-35).  NULL with dc = 0. */
+  const float* covars;    /* (sum_u (E_u + 1), dc) */
+  const int64_t* cv_off;  /* (U) */
+  int32_t dc;             /* covariates per row */
+  int32_t max_nstep;      /* max over intervals of nstep (the reference's
+                             _max_steps_per_interval, helpers.py:296) */
 } mp_series_desc;
 
 const char* mp_last_error(void);

@@ -11,6 +11,10 @@ at the start of each observation interval), ``rinit(theta, keys, t0)``,
 ``rproc_step(X, theta, keys, t, dt)``, ``dmeas(y, X, theta, t)`` and
 ``rmeas(X, theta, keys, t)``; theta is (d,) or per particle (J, d), on the
 natural scale.  ``from_est``/``to_est`` map parameter arrays between scales.
+A model with ``uses_covars`` also takes ``cov``, the covariate row of the
+call (the reference's covars_extended[t_idx]); a model with ``vectorized``
+gets the Euler sub-step key itself instead of per-particle keys
+(core/model_mechanics.py:553-568).
 """
 
 from dataclasses import dataclass
@@ -32,13 +36,40 @@ class Series:
     t0: float
     nstep: np.ndarray       # (T,) int
     dt_ext: np.ndarray      # (sum nstep,) float32
+    covars: np.ndarray = None  # (sum nstep + 1, dc) float32 on the grid, or None
 
     @property
     def T(self):
         return self.ys.shape[0]
 
 
-def make_series(ys, times, t0, nstep=None, dt=None):
+def interp_covars(t, ctimes, covars, order="linear"):
+    """``_interp_covars`` (core/algorithms/helpers.py:218-262), float64:
+    "linear" takes the bracketing covariate times (the first or last pair
+    beyond the table, extrapolating) and a + (b - a) frac; "constant" the
+    right-continuous step function, the first value before the table."""
+    if order not in ("linear", "constant"):
+        raise ValueError(f"Unsupported interpolation order: '{order}'. Must be 'linear' or 'constant'.")
+    t = np.atleast_1d(np.asarray(t, dtype=np.float64))
+    ct = np.asarray(ctimes, dtype=np.float64)
+    cv = np.asarray(covars, dtype=np.float64)
+    cv2 = cv.reshape(cv.shape[0], -1)
+    out = np.empty((t.shape[0], cv2.shape[1]))
+    n = ct.shape[0]
+    for q, tq in enumerate(t):
+        if order == "linear":
+            up = int(np.searchsorted(ct, tq, side="left"))
+            up = min(max(up, 1), n - 1)
+            lo = up - 1
+            frac = (tq - ct[lo]) / (ct[up] - ct[lo])
+            out[q] = cv2[lo] + (cv2[up] - cv2[lo]) * frac
+        else:
+            idx = min(max(int(np.searchsorted(ct, tq, side="right")), 1), n) - 1
+            out[q] = cv2[idx]
+    return out if cv.ndim > 1 else out[:, 0]
+
+
+def make_series(ys, times, t0, nstep=None, dt=None, ctimes=None, covars=None, order="linear"):
     """Fixed ``nstep`` or fixed ``dt`` Euler grid, as ``_calc_steps`` (helpers.py:158-215):
     the loop rebinds ``dt`` to each interval's result, so the next interval
     starts from the previous interval's step, not the caller's."""
@@ -62,8 +93,33 @@ def make_series(ys, times, t0, nstep=None, dt=None):
         dts.append(h)
     ns = np.array(ns, dtype=np.int64)
     dt_ext = np.repeat(np.array(dts), ns).astype(F32)
+    cv = None
+    if covars is not None:
+        # grid of _calc_interp_covars (helpers.py:265-276): each interval's
+        # sub-step start times, then the last observation time
+        grid = [float(times0[i]) + k * dts[i] for i in range(len(ns)) for k in range(int(ns[i]))]
+        grid.append(float(times0[-1]))
+        cvals = np.asarray(covars, dtype=np.float64)
+        cv = interp_covars(np.array(grid), ctimes, cvals.reshape(cvals.shape[0], -1), order).astype(F32)
     return Series(ys=np.asarray(ys, dtype=F32).reshape(len(times), -1),
-                  times=times.astype(F32), t0=float(t0), nstep=ns, dt_ext=dt_ext)
+                  times=times.astype(F32), t0=float(t0), nstep=ns, dt_ext=dt_ext, covars=cv)
+
+
+def _cov_kw(model, series, i):
+    """The covariate row of grid index i, for a model that reads covariates."""
+    if not getattr(model, "uses_covars", False):
+        return {}
+    return {"cov": series.covars[i]}
+
+
+def rinit_at(model, th, pkeys, series):
+    """``rinit`` with the covariates of grid row 0 (pfilter.py:28-35)."""
+    return model.rinit(th, pkeys, series.t0, **_cov_kw(model, series, 0))
+
+
+def dmeas_at(model, y, X, th, t, series, t_idx):
+    """``dmeas`` with the covariates of grid row t_idx (pfilter.py:124-133)."""
+    return model.dmeas(y, X, th, t, **_cov_kw(model, series, t_idx))
 
 
 def rprocess_interp(model, X, theta, key, series, t, t_idx, nstep):
@@ -78,9 +134,10 @@ def rprocess_interp(model, X, theta, key, series, t, t_idx, nstep):
     for _ in range(int(nstep)):
         k2 = tf.split(key)
         nxt, sk = k2[0], k2[1]
-        pkeys = tf.split(sk, J)
+        # @vectorized: the sub-step key itself (step_vec, model_mechanics.py:553-568)
+        pkeys = sk if getattr(model, "vectorized", False) else tf.split(sk, J)
         dt = series.dt_ext[t_idx]
-        X = model.rproc_step(X, theta, pkeys, t, dt)
+        X = model.rproc_step(X, theta, pkeys, t, dt, **_cov_kw(model, series, t_idx))
         t = F32(t + dt)
         t_idx += 1
         key = nxt
@@ -141,7 +198,7 @@ def pfilter(model, theta, series, key, J, thresh=0.0, resampler="fixed",
     th = model.from_est(theta) if should_trans else theta
     K = tf.split(key, J + 1)
     key, pkeys = K[0], K[1:]
-    X = model.rinit(th, pkeys, series.t0)
+    X = rinit_at(model, th, pkeys, series)
     nw = np.full(J, d_log(F32(1.0) / F32(J)), dtype=F32)
     engine = resampler != "reference"
     loglik = 0.0 if engine else F32(0.0)
@@ -154,7 +211,7 @@ def pfilter(model, theta, series, key, J, thresh=0.0, resampler="fixed",
         key, sub = k2[0], k2[1]
         X, t_idx = rprocess_interp(model, X, th, sub, series, t, t_idx, series.nstep[i])
         t = series.times[i]
-        meas = model.dmeas(series.ys[i], X, th, t).astype(F32)
+        meas = dmeas_at(model, series.ys[i], X, th, t, series, t_idx).astype(F32)
         w = (nw + meas).astype(F32)
         nw, ll_t = normalize_weights(w, engine)
         loglik = loglik + ll_t if engine else F32(loglik + ll_t)
@@ -194,7 +251,7 @@ def mop(model, theta_est, series, key, J, alpha, resampler="fixed", return_trace
     th = model.from_est(np.asarray(theta_est, dtype=F32))
     K = tf.split(key, J + 1)
     key, pkeys = K[0], K[1:]
-    X = model.rinit(th, pkeys, series.t0)
+    X = rinit_at(model, th, pkeys, series)
     wF = np.full(J, d_log(F32(1.0) / F32(J)), dtype=F32)
     engine = resampler != "reference"
     loglik = 0.0 if engine else F32(0.0)
@@ -207,7 +264,7 @@ def mop(model, theta_est, series, key, J, alpha, resampler="fixed", return_trace
         key, sub = k2[0], k2[1]
         X, t_idx = rprocess_interp(model, X, th, sub, series, t, t_idx, series.nstep[i])
         t = series.times[i]
-        meas = model.dmeas(series.ys[i], X, th, t).astype(F32)
+        meas = dmeas_at(model, series.ys[i], X, th, t, series, t_idx).astype(F32)
         if engine:
             mm = np.max(meas)
             loglik = loglik + (step_loglik(mm, int(np.sum(fixed_weights(meas), dtype=np.uint64)), J) - logJ)
@@ -276,7 +333,7 @@ def perfilter(model, m, thetas, series, key, J, rw, thresh=0.0, resampler="fixed
     thetas = (thetas + sic * tf.normal(sub, J * d).reshape(J, d)).astype(F32)
     K = tf.split(key, J + 1)
     key, pkeys = K[0], K[1:]
-    X = model.rinit(model.from_est(thetas), pkeys, series.t0)
+    X = rinit_at(model, model.from_est(thetas), pkeys, series)
     step_keys = tf.split(key, T + 1)[1:]
     t, t_idx = F32(series.t0), 0
     logthresh = d_log(F32(thresh)) if thresh > 0 else -np.inf
@@ -290,7 +347,7 @@ def perfilter(model, m, thetas, series, key, J, rw, thresh=0.0, resampler="fixed
         th_nat = model.from_est(thetas)
         X, t_idx = rprocess_interp(model, X, th_nat, kproc, series, t, t_idx, series.nstep[i])
         t = series.times[i]
-        meas = model.dmeas(series.ys[i], X, th_nat, t).astype(F32)
+        meas = dmeas_at(model, series.ys[i], X, th_nat, t, series, t_idx).astype(F32)
         meas = np.nan_to_num(meas, nan=nan_sub).astype(F32)
         w = (nw + meas).astype(F32)
         nw, ll_t = normalize_weights(w, engine)
@@ -416,7 +473,7 @@ def simulate(model, theta, series, key, nsim=1):
     theta = np.asarray(theta, dtype=F32)
     K = tf.split(key, nsim + 1)
     key, skeys = K[0], K[1:]
-    X = model.rinit(theta, skeys, series.t0)
+    X = rinit_at(model, theta, skeys, series)
     Xs, Ys = [X.copy()], []
     t, t_idx = F32(series.t0), 0
     times0 = np.concatenate([[series.t0], series.times]).astype(F32)
@@ -427,7 +484,7 @@ def simulate(model, theta, series, key, nsim=1):
         t = times0[i]
         K = tf.split(key, nsim + 1)
         key, mkeys = K[0], K[1:]
-        Y = model.rmeas(X, theta, mkeys, t)
+        Y = model.rmeas(X, theta, mkeys, t, **_cov_kw(model, series, t_idx))
         Xs.append(X.copy())
         Ys.append(Y)
     return np.stack(Xs), np.stack(Ys)

@@ -13,7 +13,7 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libmipomp.so"
 
 MP_OK = 0
-MP_MODEL_LG, MP_MODEL_SEIR, MP_MODEL_SV = 1, 2, 3
+MP_MODEL_LG, MP_MODEL_SEIR, MP_MODEL_SV, MP_MODEL_SPX = 1, 2, 3, 4
 MP_ALGO_PF, MP_ALGO_MOP, MP_ALGO_PER = 0, 1, 2
 MP_FORM_BATCHED, MP_FORM_PLAIN = 0, 1
 MP_CLL, MP_ESS, MP_FILTER_MEAN, MP_PREDICTION_MEAN, MP_SHOULD_TRANS = 1, 2, 4, 8, 16
@@ -35,14 +35,15 @@ MP_NOALLOC_SCRATCH_BYTES = 64 * 1024 * 1024
 class ModelDesc(C.Structure):
     _fields_ = [("kind", C.c_int32), ("S", C.c_int32), ("d", C.c_int32), ("dy", C.c_int32),
                 ("dx", C.c_int32), ("lg_sep_init", C.c_int32), ("accum_mask", C.c_uint32),
-                ("identity_trans", C.c_int32)]
+                ("identity_trans", C.c_int32), ("vectorized", C.c_int32)]
 
 
 class SeriesDesc(C.Structure):
     _fields_ = [("ys", C.c_void_p), ("times", C.c_void_p), ("nstep", C.c_void_p),
                 ("tidx0", C.c_void_p), ("dt_ext", C.c_void_p), ("obs_off", C.c_void_p),
                 ("dt_off", C.c_void_p), ("t0", C.c_float), ("U", C.c_int32),
-                ("Tmax", C.c_int32), ("max_euler", C.c_int64)]
+                ("Tmax", C.c_int32), ("max_euler", C.c_int64), ("covars", C.c_void_p),
+                ("cv_off", C.c_void_p), ("dc", C.c_int32), ("max_nstep", C.c_int32)]
 
 
 class NativeError(RuntimeError):

@@ -14,7 +14,7 @@
 #include "blockops.hpp"
 #include "engine.hpp"
 #include "resample.hpp"
-#include "seir.hpp"
+#include "mech.hpp"
 
 namespace mp {
 
@@ -189,9 +189,9 @@ __device__ __forceinline__ void run_filter(const BatchArgs& a, const int f, char
         if (!md.identity_trans) M::from_est(md, th);
         typename M::P Pj;
         M::prepare(md, th, Pj);
-        M::rinit(md, Pj, pk, a.ser.t0, x);
+        m_rinit<M>(md, Pj, pk, a.ser.t0, x, cov_row(a.ser, u, 0));
       } else {
-        M::rinit(md, P, pk, a.ser.t0, x);
+        m_rinit<M>(md, P, pk, a.ser.t0, x, cov_row(a.ser, u, 0));
       }
       MP_UNROLL for (int s = 0; s < WM; ++s) if (s < W) X[0][(int64_t)s * J + j] = x[s];
     }
@@ -294,10 +294,10 @@ __device__ __forceinline__ void run_filter(const BatchArgs& a, const int f, char
         for (int k = 0; k < nst; ++k) {
           const Key sk = ld_key(sch, base + k);
           const float dt = dts[ti0 + k];
-          M::step(md, Pj, split_at(sk, (uint64_t)j), tt, dt, x);
+          m_step<M>(md, Pj, sk, (uint64_t)j, tt, dt, x, cov_row(a.ser, u, ti0 + k));
           tt = tt + dt;
         }
-        meas = M::dmeas(md, Pj, y, x, ty);
+        meas = m_dmeas<M>(md, Pj, y, x, ty, cov_row(a.ser, u, ti0 + nst));
       }
       if (ALGO == ALGO_PER) {
         if (meas != meas) meas = nan_sub;
@@ -485,6 +485,7 @@ static hipError_t launch_batched_algo(const BatchArgs& a, int block, int grid, s
       return launch_batched_t<LG4, ALGO>(a, block, grid, lds, st);
     case MODEL_SV: return launch_batched_t<SV, ALGO>(a, block, grid, lds, st);
     case MODEL_SEIR: return launch_batched_t<SEIR, ALGO>(a, block, grid, lds, st);
+    case MODEL_SPX: return launch_batched_t<SPX, ALGO>(a, block, grid, lds, st);
   }
   return hipErrorInvalidValue;
 }

@@ -10,7 +10,7 @@
 
 #include "../../include/mipomp.h"
 #include "engine.hpp"
-#include "seir.hpp"
+#include "mech.hpp"
 
 namespace mp {
 hipError_t launch_bits(const uint32_t* keys, int64_t nkeys, int64_t n, void* out, int mode, hipStream_t s);
@@ -77,7 +77,6 @@ int hip_fail(hipError_t e, const char* where) {
     if (e_ != hipSuccess) return hip_fail(e_, #call);  \
   } while (0)
 
-constexpr int kBatchedBlock = 256;
 constexpr int64_t kParamSlot = 512;  // bytes per filter of prepared model parameters
 
 // Launch-shape parameters (set by the autotuner through mp_tune_set).
@@ -181,17 +180,28 @@ int check_model(const mp_model_desc* m) {
     case MP_MODEL_SEIR:
       if (m->S != 5 || m->d != 10 || m->dy != 1) return fail(MP_ERR_ARG, "SEIR model expects S=5, d=10, dy=1");
       break;
+    case MP_MODEL_SPX:
+      if (m->S != 2 || m->d != 6 || m->dy != 1) return fail(MP_ERR_ARG, "SPX model expects S=2, d=6, dy=1");
+      break;
     default:
       return fail(MP_ERR_ARG, "unknown model kind %d", m->kind);
   }
+  if (m->vectorized && m->kind != MP_MODEL_SPX)
+    return fail(MP_ERR_ARG, "model kind %d has no @vectorized rproc", m->kind);
   return MP_OK;
 }
+
+// covariates a model reads per grid row
+int model_covariates(const mp_model_desc* m) { return m->kind == MP_MODEL_SPX ? SPX::DC : 0; }
 
 int check_series(const mp_series_desc* s) {
   if (!s) return fail(MP_ERR_ARG, "series descriptor is NULL");
   if (s->U < 1 || s->Tmax < 1) return fail(MP_ERR_ARG, "series needs U >= 1 and Tmax >= 1");
   if (!s->ys || !s->times || !s->nstep || !s->tidx0 || !s->dt_ext || !s->obs_off || !s->dt_off)
     return fail(MP_ERR_ARG, "series descriptor has a NULL array");
+  if (s->dc < 0) return fail(MP_ERR_ARG, "negative covariate count (dc = %d)", s->dc);
+  if (s->dc > 0 && (!s->covars || !s->cv_off)) return fail(MP_ERR_ARG, "dc = %d but covars or cv_off is NULL", s->dc);
+  if (s->max_nstep < 0) return fail(MP_ERR_ARG, "negative max_nstep");
   return MP_OK;
 }
 
@@ -485,7 +495,7 @@ void host_from_est_rows(const ModelDesc& md, float* theta, int64_t n) {
 extern "C" {
 
 const char* mp_last_error(void) { return g_err.c_str(); }
-int32_t mp_abi_version(void) { return 3; }
+int32_t mp_abi_version(void) { return 4; }
 
 int32_t mp_host_threefry2x32(const uint32_t* h_keys, const uint32_t* h_ctr, uint32_t* h_out, int64_t n) {
   if (n < 0 || (n > 0 && (!h_keys || !h_ctr || !h_out))) return fail(MP_ERR_ARG, "bad threefry arguments");
@@ -562,6 +572,9 @@ static int check_common(const mp_model_desc* m, const mp_series_desc* s, int R, 
   if (rc) return rc;
   rc = check_series(s);
   if (rc) return rc;
+  if (s->dc < model_covariates(m))
+    return fail(MP_ERR_ARG, "model kind %d reads %d covariates per row, the series has %d", m->kind,
+                model_covariates(m), s->dc);
   if (R < 1) return fail(MP_ERR_ARG, "need at least one filter (R = %d)", R);
   if (J < 1) return fail(MP_ERR_ARG, "J should be greater than 0 (J = %d)", J);
   if (!keys) return fail(MP_ERR_ARG, "keys is NULL");
@@ -666,8 +679,10 @@ int32_t mp_mop_grad(const mp_model_desc* h_model, const mp_series_desc* h_series
   if (h_series->U != 1) return fail(MP_ERR_ARG, "mp_mop_grad runs single-series filters (U = 1)");
   if (!mop_grad_supported(*reinterpret_cast<const ModelDesc*>(h_model)))
     return fail(MP_ERR_ARG, "model kind %d (dx %d, dy %d) has no adjoint", h_model->kind, h_model->dx, h_model->dy);
-  if (h_series->max_euler != h_series->Tmax)
-    return fail(MP_ERR_ARG, "mp_mop_grad needs exactly one Euler step per observation interval");
+  if (h_series->max_euler != h_series->Tmax || h_series->max_nstep != 1)
+    return fail(MP_ERR_ARG, "mp_mop_grad needs exactly one Euler step per observation interval "
+                "(total %lld over %d intervals, at most %d per interval)", (long long)h_series->max_euler,
+                h_series->Tmax, h_series->max_nstep);
   hipStream_t st = (hipStream_t)stream;
   if (form == MP_FORM_PLAIN) {
     for (int f = 0; f < R; ++f) {
@@ -691,6 +706,7 @@ int32_t mp_host_from_est(const mp_model_desc* h_model, float* h_theta, int64_t n
     case MODEL_LG: host_from_est_rows<LG4>(md, h_theta, n); break;
     case MODEL_SV: host_from_est_rows<SV>(md, h_theta, n); break;
     case MODEL_SEIR: host_from_est_rows<SEIR>(md, h_theta, n); break;
+    case MODEL_SPX: host_from_est_rows<SPX>(md, h_theta, n); break;
   }
   return MP_OK;
 }

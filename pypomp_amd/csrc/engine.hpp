@@ -46,7 +46,20 @@ struct SeriesSet {
   int U;
   int Tmax;
   int64_t max_euler;  // max over units of the number of Euler steps
+  // Covariates on each unit's Euler grid: unit u's rows start at cv_off[u];
+  // row tidx0[t] + k is the k-th sub-step of interval t and row
+  // tidx0[t] + nstep[t] the observation after it (the reference's
+  // covars_extended, one row per grid time plus the last observation time).
+  const float* covars;     // (sum_u (E_u + 1), dc), or null when dc == 0
+  const int64_t* cv_off;   // (U), or null when dc == 0
+  int dc;                  // covariates per row
+  int max_nstep;           // max over intervals of nstep
 };
+
+// Row of covariates at grid index i of unit u (null without covariates).
+__device__ __forceinline__ const float* cov_row(const SeriesSet& s, int u, int64_t i) {
+  return s.dc ? s.covars + (s.cv_off[u] + i) * s.dc : nullptr;
+}
 
 // Key-schedule words (keys) per filter: PF/MOP sub-step keys + one resampling
 // key per observation; IF2 adds a perturbation key per observation and two

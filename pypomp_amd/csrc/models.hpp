@@ -9,7 +9,7 @@
 
 namespace mp {
 
-enum ModelKind : int { MODEL_LG = 1, MODEL_SEIR = 2, MODEL_SV = 3 };
+enum ModelKind : int { MODEL_LG = 1, MODEL_SEIR = 2, MODEL_SV = 3, MODEL_SPX = 4 };
 
 // Host-filled description of a model instance (plain data, passed by value).
 struct ModelDesc {
@@ -21,6 +21,7 @@ struct ModelDesc {
   int lg_sep_init;  // LG: separate initial Cholesky factor P follows X0
   uint32_t accum_mask;
   int identity_trans;  // estimation scale == natural scale
+  int vectorized;      // @vectorized rproc: one draw per sub-step key, particle j at index j
 };
 
 __device__ __forceinline__ int tri(int d) { return d * (d + 1) / 2; }

@@ -20,7 +20,7 @@
 // written by an earlier launch.
 #include "blockops.hpp"
 #include "engine.hpp"
-#include "seir.hpp"
+#include "mech.hpp"
 
 namespace mp {
 
@@ -128,7 +128,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
     const Key sk = uniform_key(st.k0, st.k1);
     MP_UNROLL for (int r = 0; r < kPreRows; ++r) {
       const int64_t j = tile0 + (int64_t)r * B + tid;
-      if (r < a.rows_prop && j < tile1) M::draw(P, split_at(sk, (uint64_t)j), zpre[r]);
+      if (r < a.rows_prop && j < tile1) m_draw<M>(md, P, sk, (uint64_t)j, zpre[r]);
     }
   }
   if (have_anc) __syncthreads();
@@ -144,7 +144,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
     float x[M::MAXS];
     int64_t src = j;
     if (t == 0) {
-      M::rinit(md, P, split_at(Key{a.fkey[0], a.fkey[1]}, 1 + (uint64_t)j), a.ser.t0, x);
+      m_rinit<M>(md, P, split_at(Key{a.fkey[0], a.fkey[1]}, 1 + (uint64_t)j), a.ser.t0, x,
+                 cov_row(a.ser, a.unit, 0));
     } else {
       if (have_anc) {
         const uint64_t tj = comb_threshold(st.ures, j, J, st.total);
@@ -159,16 +160,16 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
       if ((md.accum_mask >> s) & 1u) x[s] = 0.f;
     float tt = tstart;
     if (use_z) {
-      M::step_z(md, P, zp, tt, st.dt0, x);
+      m_step_z<M>(md, P, zp, tt, st.dt0, x, cov_row(a.ser, a.unit, ti0));
     } else {
       for (int k = 0; k < nst; ++k) {
         const Key sk = uniform_key(a.sched[2 * (base + k)], a.sched[2 * (base + k) + 1]);
         const float dt = dts[ti0 + k];
-        M::step(md, P, split_at(sk, (uint64_t)j), tt, dt, x);
+        m_step<M>(md, P, sk, (uint64_t)j, tt, dt, x, cov_row(a.ser, a.unit, ti0 + k));
         tt = tt + dt;
       }
     }
-    const float w = wprev + M::dmeas(md, P, y, x, ty);
+    const float w = wprev + m_dmeas<M>(md, P, y, x, ty, cov_row(a.ser, a.unit, ti0 + nst));
     MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) Xn[(int64_t)s * J + j] = x[s];
     a.lw[j] = w;
     lmax = nanmax(lmax, w);
@@ -447,6 +448,7 @@ hipError_t plain_prepare(const ModelDesc& md, const float* theta, int64_t stride
       return plain_prepare_t<LG4>(md, theta, stride, R, flags, out, pstride, s);
     case MODEL_SV: return plain_prepare_t<SV>(md, theta, stride, R, flags, out, pstride, s);
     case MODEL_SEIR: return plain_prepare_t<SEIR>(md, theta, stride, R, flags, out, pstride, s);
+    case MODEL_SPX: return plain_prepare_t<SPX>(md, theta, stride, R, flags, out, pstride, s);
   }
   return hipErrorInvalidValue;
 }
@@ -487,6 +489,7 @@ hipError_t plain_kernel(int which, const PlainArgs& a, int ntiles, int threads, 
       return plain_kernel_t<LG4>(which, a, ntiles, threads, s);
     case MODEL_SV: return plain_kernel_t<SV>(which, a, ntiles, threads, s);
     case MODEL_SEIR: return plain_kernel_t<SEIR>(which, a, ntiles, threads, s);
+    case MODEL_SPX: return plain_kernel_t<SPX>(which, a, ntiles, threads, s);
   }
   return hipErrorInvalidValue;
 }

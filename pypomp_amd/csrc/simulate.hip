@@ -13,7 +13,7 @@
 // Outputs in the reference's internal layout, coalesced over simulations:
 // X (R, T+1, S, nsim), Y (R, T, dy, nsim).
 #include "engine.hpp"
-#include "seir.hpp"
+#include "mech.hpp"
 
 namespace mp {
 
@@ -39,7 +39,7 @@ __global__ __launch_bounds__(256) void k_simulate(const SimArgs a) {
   const Key k0 = sim_key(a.keys, r);
   Key run = split_at(k0, 0);
   float x[M::MAXS];
-  M::rinit(md, P, split_at(k0, 1 + (uint64_t)s), a.ser.t0, x);
+  m_rinit<M>(md, P, split_at(k0, 1 + (uint64_t)s), a.ser.t0, x, cov_row(a.ser, 0, 0));
   float* Xr = a.X + (int64_t)r * (T + 1) * S * ns;
   float* Yr = a.Y + (int64_t)r * T * dy * ns;
   MP_UNROLL for (int c = 0; c < M::MAXS; ++c) if (c < S) Xr[c * ns + s] = x[c];
@@ -53,7 +53,7 @@ __global__ __launch_bounds__(256) void k_simulate(const SimArgs a) {
     for (int k = 0; k < nst; ++k) {
       const Key nxt = split_at(sub, 0), sk = split_at(sub, 1);
       const float dt = a.ser.dt_ext[ti0 + k];
-      M::step(md, P, split_at(sk, (uint64_t)s), tt, dt, x);
+      m_step<M>(md, P, sk, (uint64_t)s, tt, dt, x, cov_row(a.ser, 0, ti0 + k));
       tt = tt + dt;
       sub = nxt;
     }
@@ -61,7 +61,7 @@ __global__ __launch_bounds__(256) void k_simulate(const SimArgs a) {
     const Key mk = split_at(run, 1 + (uint64_t)s);
     run = split_at(run, 0);
     float y[kSimMaxDy];
-    M::rmeas(md, P, mk, x, t, y);
+    m_rmeas<M>(md, P, mk, x, t, y, cov_row(a.ser, 0, ti0 + nst));
     float* Xo = Xr + (int64_t)(i + 1) * S * ns;
     MP_UNROLL for (int c = 0; c < M::MAXS; ++c) if (c < S) Xo[c * ns + s] = x[c];
     float* Yo = Yr + (int64_t)i * dy * ns;
@@ -87,6 +87,7 @@ hipError_t launch_simulate(const SimArgs& a, hipStream_t st) {
       return launch_simulate_t<LG4>(a, st);
     case MODEL_SV: return launch_simulate_t<SV>(a, st);
     case MODEL_SEIR: return launch_simulate_t<SEIR>(a, st);
+    case MODEL_SPX: return launch_simulate_t<SPX>(a, st);
   }
   return hipErrorInvalidValue;
 }

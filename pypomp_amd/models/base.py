@@ -41,6 +41,11 @@ class NativeModel:
     # True: parameters are estimated on their natural scale (the reference's
     # ParTrans() with no transforms, core/par_trans.py:70-111)
     identity_trans = False
+    # True: the model's rproc is @vectorized (one draw per Euler sub-step for
+    # all particles, core/model_mechanics.py:311-355); models that define it only
+    vectorized = False
+    # covariates the model reads, by name (the reference's covar_names)
+    covar_names: list = []
 
     param_names: list = []
     statenames: list = []
@@ -79,7 +84,8 @@ class NativeModel:
             mask |= 1 << int(a)
         return _native.ModelDesc(kind=self.kind, S=self.S, d=self.d, dy=self.dy, dx=self.dx,
                                  lg_sep_init=self.lg_sep_init, accum_mask=mask,
-                                 identity_trans=int(bool(self.identity_trans)))
+                                 identity_trans=int(bool(self.identity_trans)),
+                                 vectorized=int(bool(self.vectorized)))
 
     # host mirrors of the device transforms (float32)
     def to_est(self, theta):

@@ -2,7 +2,7 @@
 core/estimation_mixin.py:101-444, core/analysis_mixin.py:22-70).
 
 Same constructor slots (ys, theta, t0, rinit/rproc/dmeas/rmeas, nstep or dt,
-accumvars) and the same key handling (``fresh_key``, core/pomp.py:391-408).
+accumvars, covars and their interpolation order) and the same key handling (``fresh_key``, core/pomp.py:391-408).
 The process/measurement plugins must be the components of one native model
 (``pypomp_amd.models``); a plain Python callable has no device form and is
 rejected at construction.
@@ -113,9 +113,9 @@ def _merge_results(entries):
 
 class Pomp:
     def __init__(self, ys, theta, t0, rinit=None, rproc=None, dmeas=None, rmeas=None, times=None,
-                 nstep=None, dt=None, model=None, accumvars=None, covars=None, key=None):
-        if covars is not None:
-            raise NotImplementedError("covariates are not supported by the native models")
+                 nstep=None, dt=None, model=None, accumvars=None, covars=None, key=None, order="linear"):
+        if covars is not None and not isinstance(covars, pd.DataFrame):
+            raise TypeError("covars must be a pandas DataFrame or None")
         self.model = _resolve_model(model, rinit, rproc, dmeas, rmeas)
         slots = {"rinit": rinit, "rproc": rproc, "dmeas": dmeas, "rmeas": rmeas}
         for name, comp in slots.items():
@@ -142,7 +142,25 @@ class Pomp:
             times = np.arange(1, ys.shape[0] + 1, dtype=float)
         if nstep is None and dt is None:
             raise ValueError("Either dt or nstep must be provided")
-        self.series = Series.build(ys, times, t0, nstep=nstep, dt=dt)
+        # covariates (core/pomp.py:219-335): the frame's index holds the
+        # covariate times; the columns the model reads, by name, are
+        # interpolated onto the Euler grid
+        need = list(self.model.covar_names)
+        self.covars = covars
+        self.covar_names = [] if covars is None else list(covars.columns)
+        self.order = order
+        ctimes = cvals = None
+        if need:
+            if covars is None:
+                raise ValueError(f"the model reads the covariates {need}: pass them as covars=")
+            missing = [c for c in need if c not in covars.columns]
+            if missing:
+                raise ValueError(f"covars lacks the columns {missing} the model reads")
+            ctimes = np.asarray(covars.index, dtype=float)
+            cvals = covars[need].to_numpy(dtype=np.float64)
+        elif order not in ("linear", "constant"):
+            raise ValueError(f"Unsupported interpolation order: '{order}'. Must be 'linear' or 'constant'.")
+        self.series = Series.build(ys, times, t0, nstep=nstep, dt=dt, ctimes=ctimes, covars=cvals, order=order)
         self.t0 = float(t0)
         self.theta = self._theta_list(theta)
         self.fresh_key = key
@@ -421,6 +439,8 @@ class Pomp:
         if not _same(self.theta, other.theta):
             return False
         if not _same(self.series, other.series):
+            return False
+        if not _same(getattr(self, "covars", None), getattr(other, "covars", None)):
             return False
         if self.statenames != other.statenames or float(self.t0) != float(other.t0):
             return False

@@ -41,7 +41,7 @@ def test_python_binding_lists_every_declared_symbol():
 
 
 def test_abi_version(lib):
-    assert lib.mp_abi_version() == 3
+    assert lib.mp_abi_version() == 4
 
 
 def test_host_split_matches_oracle():
