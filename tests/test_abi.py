@@ -104,3 +104,26 @@ def test_library_shares_the_one_hip_runtime_in_the_process():
                          capture_output=True, text=True, timeout=600)
     assert out.returncode == 0, out.stderr
     assert out.stdout.strip() == "1"
+
+
+def test_mop_grad_rejects_a_grid_with_several_steps_in_one_interval(lib):
+    # nstep [0, 2, 1] has as many Euler steps as intervals, but the adjoint
+    # needs exactly one per interval: the per-interval maximum catches it
+    # before anything is launched (the pointers are never dereferenced)
+    from pypomp_amd import _native
+    fake = C.c_void_p(256)
+    md = _native.ModelDesc(kind=_native.MP_MODEL_SV, S=2, d=6, dy=1)
+    sd = _native.SeriesDesc(ys=fake, times=fake, nstep=fake, tidx0=fake, dt_ext=fake, obs_off=fake, dt_off=fake,
+                            t0=0.0, U=1, Tmax=3, max_euler=3, max_nstep=2)
+    rc = lib.mp_mop_grad(md, sd, fake, 6, fake, 4, 100, 0.9, 10, 0, fake, fake, fake, 1 << 20, None)
+    assert rc == 1
+    assert b"one Euler step per observation interval" in lib.mp_last_error()
+    md_spx = _native.ModelDesc(kind=_native.MP_MODEL_SPX, S=2, d=6, dy=1, vectorized=1)
+    sd.max_nstep = 1
+    rc = lib.mp_pfilter(md_spx, sd, fake, 6, fake, None, 1, 10, 0.0, 0, 0, fake, None, None, None, None, fake,
+                        1 << 20, None)
+    assert rc == 1 and b"reads 1 covariates per row, the series has 0" in lib.mp_last_error()
+    md_sv = _native.ModelDesc(kind=_native.MP_MODEL_SV, S=2, d=6, dy=1, vectorized=1)
+    rc = lib.mp_pfilter(md_sv, sd, fake, 6, fake, None, 1, 10, 0.0, 0, 0, fake, None, None, None, None, fake,
+                        1 << 20, None)
+    assert rc == 1 and b"no @vectorized rproc" in lib.mp_last_error()
