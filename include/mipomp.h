@@ -184,6 +184,22 @@ int32_t mp_mop_grad(const mp_model_desc* h_model, const mp_series_desc* h_series
                     int64_t theta_stride, const uint32_t* keys, int32_t R, int32_t J, float alpha, int32_t K,
                     int32_t form, double* loglik, float* grad, void* workspace, size_t ws_bytes, void* stream);
 
+/* Hessian of the MOP(alpha) log-likelihood with respect to theta_est: the
+ * second derivative jax.hessian takes of _mop_internal_mean
+ * (core/algorithms/ad_helpers.py:8, 56-75; the reference's Hessian is
+ * -hess / T), ancestry and random numbers held fixed, the stop-gradient
+ * weights held at theta (mop.py:125).  Forward mode over hyper-dual numbers,
+ * one workgroup per (row, parameter pair); the key tree and resampling of
+ * mp_mop, whose value loglik (R) float64 reproduces.  grad (R, d) and hess
+ * (R, d, d) float32, hess symmetric.  U = 1.  Models: SV, and LG with
+ * dx = dy = 1 or 2; others return MP_ERR_ARG.  Newton and WeightedNewton
+ * (core/optimizer.py:270-330) of the train loop call it once per step. */
+size_t mp_mop_hess_workspace_size(const mp_model_desc* h_model, const mp_series_desc* h_series, int32_t R,
+                                  int32_t J);
+int32_t mp_mop_hess(const mp_model_desc* h_model, const mp_series_desc* h_series, const float* theta,
+                    int64_t theta_stride, const uint32_t* keys, int32_t R, int32_t J, float alpha, double* loglik,
+                    float* grad, float* hess, void* workspace, size_t ws_bytes, void* stream);
+
 /* The model's estimation-to-natural transform (from_est) applied on the host
  * to n rows of d parameters in place, with the device's arithmetic (the same
  * deterministic exp/tanh/logistic), so host-side traces and priors see the

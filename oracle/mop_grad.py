@@ -41,7 +41,7 @@ def sv_noise(series, key, J):
     return z0, np.stack(zs)
 
 
-def sv_mop_objective(theta_est, ys, z0, zs, anc, alpha, frozen_m=None):
+def sv_mop_objective(theta_est, ys, z0, zs, anc, alpha, frozen_m=None, record=None):
     """MOP log-likelihood (float64 torch).  anc[t] = ancestors chosen after step t.
     With ``frozen_m`` (T, J) the stop-gradient terms use those fixed values (the
     frozen-ancestry objective whose gradient at theta0 is the MOP gradient)."""
@@ -59,6 +59,8 @@ def sv_mop_objective(theta_est, ys, z0, zs, anc, alpha, frozen_m=None):
         v = mu + phi * (v - mu) + sig * z
         u = float(ys[t]) * torch.exp(-0.5 * v) - rho * z
         m = -0.5 * np.log(2 * np.pi) - 0.5 * torch.log(omr2) - 0.5 * v - 0.5 * u * u / omr2
+        if record is not None:
+            record.append(m.detach().clone())
         ll = ll + torch.logsumexp(wP + m, 0) - torch.logsumexp(wP, 0)
         if t < zs.shape[0] - 1:
             a = torch.as_tensor(anc[t], dtype=torch.long)
@@ -171,8 +173,9 @@ def lg_from_est_torch(th_est, spec, identity=False):
     return out
 
 
-def lg_mop_objective(theta_est, ys, z0, zs, anc, alpha, spec, identity=False):
-    """MOP log-likelihood of the LG model (float64 torch), ancestry and noise fixed."""
+def lg_mop_objective(theta_est, ys, z0, zs, anc, alpha, spec, identity=False, frozen_m=None, record=None):
+    """MOP log-likelihood of the LG model (float64 torch), ancestry and noise fixed;
+    ``frozen_m`` (T, J) fixes the stop-gradient values as in sv_mop_objective."""
     th = lg_from_est_torch(theta_est, spec, identity)
     A, C, LQ, LR, X0, L0 = lg_unpack_torch(th, spec)
     J = z0.shape[0]
@@ -191,10 +194,13 @@ def lg_mop_objective(theta_est, ys, z0, zs, anc, alpha, spec, identity=False):
         r = torch.as_tensor(ys[t], dtype=torch.float64)[None, :] - x @ C.T
         s = torch.linalg.solve_triangular(Lc, r.T, upper=False).T
         m = -0.5 * (s * s).sum(1) - 0.5 * dy * np.log(2 * np.pi) - logdet
+        if record is not None:
+            record.append(m.detach().clone())
         ll = ll + torch.logsumexp(wP + m, 0) - torch.logsumexp(wP, 0)
         if t < zs.shape[0] - 1:
             a = torch.as_tensor(anc[t], dtype=torch.long)
-            wF = (wP + m - m.detach())[a]
+            sg = m.detach() if frozen_m is None else torch.as_tensor(frozen_m[t], dtype=torch.float64)
+            wF = (wP + m - sg)[a]
     return ll
 
 
@@ -210,6 +216,73 @@ def lg_mop_value_and_grad(model, theta_est, ys, key, J, alpha, identity=False):
     ll = lg_mop_objective(th, ys, z0, zs, anc, alpha, model.spec, identity)
     ll.backward()
     return float(val), float(ll.detach()), th.grad.numpy().copy()
+
+
+# ---- Hessian (jax.hessian of _mop_internal_mean, ad_helpers.py:8, 56-75) ----
+
+def _frozen_densities(objective, th0, T):
+    """The densities m_t at th0 (T, J), recorded by one evaluation."""
+    rec = []
+    with torch.no_grad():
+        objective(th0, rec)
+    return np.stack([r.numpy() for r in rec])
+
+
+def hessian_and_fd(objective, theta_est, T, h=1e-6):
+    """(value, gradient, Hessian) of ``objective(th, frozen_m=None)`` at
+    theta_est by torch's double backward -- the stop-gradient values are the
+    densities at theta_est, held fixed, as the tangent of stop_gradient is zero
+    in jax.hessian's forward-over-reverse -- and the central differences of the
+    frozen objective's autograd gradient (step h), the pin of the Hessian."""
+    th0 = torch.tensor(np.asarray(theta_est, np.float64))
+    frozen = _frozen_densities(lambda th, rec: objective(th, record=rec), th0, T)
+    f = lambda th: objective(th, frozen_m=frozen)  # noqa: E731
+    H = torch.autograd.functional.hessian(f, th0).numpy()
+    thg = th0.clone().requires_grad_(True)
+    val = f(thg)
+    val.backward()
+    g = thg.grad.numpy().copy()
+
+    def grad_at(th):
+        t = torch.tensor(th, requires_grad=True)
+        f(t).backward()
+        return t.grad.numpy().copy()
+
+    d = th0.shape[0]
+    fd = np.zeros((d, d))
+    for j in range(d):
+        e = np.zeros(d)
+        e[j] = h
+        fd[:, j] = (grad_at(th0.numpy() + e) - grad_at(th0.numpy() - e)) / (2 * h)
+    return float(val.detach()), g, H, fd
+
+
+def sv_hessian_case(theta_est, ys, key, J, alpha):
+    """The SV MOP objective at theta_est with the oracle's ancestry and noise:
+    a callable objective(th, frozen_m=None, record=None) for hessian_and_fd."""
+    from .sv import SVModel
+    ser = fl.make_series(ys, np.arange(1, len(ys) + 1.0), 0.0, nstep=1)
+    _, tr = fl.mop(SVModel(), theta_est, ser, key, J, alpha, return_trace=True)
+    z0, zs = sv_noise(ser, key, J)
+    anc = np.stack(tr["ancestors"])
+
+    def objective(th, frozen_m=None, record=None):
+        return sv_mop_objective(th, ys[:, 0], z0, zs, anc, alpha, frozen_m=frozen_m, record=record)
+
+    return objective
+
+
+def lg_hessian_case(model, theta_est, ys, key, J, alpha):
+    """The LG MOP objective (oracle LG model) as in sv_hessian_case."""
+    ser = fl.make_series(ys, np.arange(1, len(ys) + 1.0), 0.0, nstep=1)
+    _, tr = fl.mop(model, theta_est, ser, key, J, alpha, return_trace=True)
+    z0, zs = lg_noise(ser, key, J, model.spec.dx)
+    anc = np.stack(tr["ancestors"])
+
+    def objective(th, frozen_m=None, record=None):
+        return lg_mop_objective(th, ys, z0, zs, anc, alpha, model.spec, frozen_m=frozen_m, record=record)
+
+    return objective
 
 
 # ---- train (core/algorithms/train.py:38-180) ----------------------------

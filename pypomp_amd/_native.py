@@ -23,7 +23,7 @@ EXPORTS = [
     "mp_random_bits", "mp_uniform", "mp_normal", "mp_split",
     "mp_resample_workspace_size", "mp_resample_systematic",
     "mp_filter_workspace_size", "mp_pfilter", "mp_pfilter_noalloc", "mp_mop", "mp_perfilter",
-    "mp_mop_grad_workspace_size", "mp_mop_grad", "mp_simulate", "mp_host_from_est",
+    "mp_mop_grad_workspace_size", "mp_mop_grad", "mp_mop_hess_workspace_size", "mp_mop_hess", "mp_simulate", "mp_host_from_est",
     "mp_profile_enable", "mp_profile_read", "mp_tune_set", "mp_tune_get",
 ]
 MP_TUNE_PLAIN_SCAN_ROWS, MP_TUNE_PLAIN_PROP_ROWS, MP_TUNE_BATCHED_BLOCK, MP_TUNE_GRAD_BLOCK = 1, 2, 3, 4
@@ -92,6 +92,10 @@ def lib():
     L.mp_mop_grad_workspace_size.restype = sz
     L.mp_mop_grad.argtypes = [C.POINTER(ModelDesc), C.POINTER(SeriesDesc), f32p, i64, u32p, i32, i32, C.c_float,
                               i32, i32, f64p, f32p, vp, sz, vp]
+    L.mp_mop_hess_workspace_size.argtypes = [C.POINTER(ModelDesc), C.POINTER(SeriesDesc), i32, i32]
+    L.mp_mop_hess_workspace_size.restype = sz
+    L.mp_mop_hess.argtypes = [C.POINTER(ModelDesc), C.POINTER(SeriesDesc), f32p, i64, u32p, i32, i32, C.c_float,
+                              f64p, f32p, f32p, vp, sz, vp]
     L.mp_simulate.argtypes = [C.POINTER(ModelDesc), C.POINTER(SeriesDesc), f32p, i64, u32p, vp, i32, i32, i32,
                               f32p, f32p, vp]
     L.mp_host_from_est.argtypes = [C.POINTER(ModelDesc), vp, i64]
@@ -108,6 +112,7 @@ def lib():
     L.mp_resample_workspace_size.restype = sz
     L.mp_filter_workspace_size.restype = sz
     L.mp_mop_grad_workspace_size.restype = sz
+    L.mp_mop_hess_workspace_size.restype = sz
     _lib = L
     return L
 

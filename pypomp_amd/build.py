@@ -15,7 +15,7 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 OUT_DIR = HERE / "_lib"
 LIB = OUT_DIR / "libmipomp.so"
-SOURCES = ["capi.hip", "prims.hip", "batched.hip", "plain.hip", "mopgrad.hip", "simulate.hip"]
+SOURCES = ["capi.hip", "prims.hip", "batched.hip", "plain.hip", "mopgrad.hip", "mophess.hip", "simulate.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -51,7 +51,7 @@ def _compile(src, obj):
 def build(verbose=False):
     OUT_DIR.mkdir(exist_ok=True)
     objs = [OUT_DIR / (Path(s).stem + ".o") for s in SOURCES]
-    with ThreadPoolExecutor(max_workers=min(4, len(SOURCES))) as ex:
+    with ThreadPoolExecutor(max_workers=min(7, len(SOURCES))) as ex:
         logs = list(ex.map(_compile, SOURCES, objs))
     if verbose:
         for s, log in zip(SOURCES, logs):

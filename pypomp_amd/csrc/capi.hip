@@ -35,6 +35,8 @@ int mop_grad_noise_words(const ModelDesc& md);
 bool mop_grad_supported(const ModelDesc& md);
 hipError_t launch_simulate(const SimArgs& a, hipStream_t st);
 size_t mop_grad_lds_bytes(int S, int J, int block, bool lds_work);
+hipError_t launch_mop_hess(const HessArgs& a, int grid, hipStream_t st);
+bool mop_hess_supported(const ModelDesc& md);
 }  // namespace mp
 
 using namespace mp;
@@ -478,6 +480,27 @@ int scratch_address(void** out) {
   return MP_OK;
 }
 
+// MOP Hessian: key schedules of the R rows, then one scratch slice per
+// resident workgroup (the grid strides over the (row, pair) tasks).
+struct HessPlan {
+  int64_t sched_len, sched_bytes, per_block;
+  int grid;
+  size_t total;
+};
+
+HessPlan plan_hess(const mp_model_desc* m, const mp_series_desc* s, int R, int J) {
+  HessPlan p{};
+  const SeriesSet& ser = *reinterpret_cast<const SeriesSet*>(s);
+  p.sched_len = sched_stride(ALGO_MOP, ser);
+  p.sched_bytes = align256((int64_t)R * p.sched_len * 8);
+  p.per_block = hess_scratch_layout(m->S, J).total;
+  const int64_t tasks = (int64_t)R * (m->d * (m->d + 1) / 2);
+  const int64_t cap = (int64_t)num_cus() * 4;
+  p.grid = (int)(tasks < cap ? tasks : cap);
+  p.total = (size_t)p.sched_bytes + (size_t)p.grid * (size_t)p.per_block + 256;
+  return p;
+}
+
 // the model's from_est run on the host CPU (detmath is host-and-device code)
 template <class M>
 void host_from_est_rows(const ModelDesc& md, float* theta, int64_t n) {
@@ -694,6 +717,48 @@ int32_t mp_mop_grad(const mp_model_desc* h_model, const mp_series_desc* h_series
   }
   return run_mop_grad(h_model, h_series, theta, theta_stride, keys, R, J, alpha, K, loglik, grad, workspace,
                       ws_bytes, st);
+}
+
+size_t mp_mop_hess_workspace_size(const mp_model_desc* h_model, const mp_series_desc* h_series, int32_t R,
+                                  int32_t J) {
+  if (!h_model || !h_series || R < 1 || J < 1) return 0;
+  return plan_hess(h_model, h_series, R, J).total;
+}
+
+int32_t mp_mop_hess(const mp_model_desc* h_model, const mp_series_desc* h_series, const float* theta,
+                    int64_t theta_stride, const uint32_t* keys, int32_t R, int32_t J, float alpha, double* loglik,
+                    float* grad, float* hess, void* workspace, size_t ws_bytes, void* stream) {
+  int rc = check_common(h_model, h_series, R, J, keys);
+  if (rc) return rc;
+  if (!theta || !loglik || !grad || !hess || !workspace) return fail(MP_ERR_ARG, "NULL theta/loglik/grad/hess/workspace");
+  if (h_series->U != 1) return fail(MP_ERR_ARG, "mp_mop_hess runs single-series filters (U = 1)");
+  if (!mop_hess_supported(*reinterpret_cast<const ModelDesc*>(h_model)))
+    return fail(MP_ERR_ARG, "model kind %d (dx %d, dy %d) has no Hessian", h_model->kind, h_model->dx,
+                h_model->dy);
+  const HessPlan p = plan_hess(h_model, h_series, R, J);
+  if (ws_bytes < p.total) return fail(MP_ERR_WORKSPACE, "workspace %zu < required %zu bytes", ws_bytes, p.total);
+  hipStream_t st = (hipStream_t)stream;
+  char* base = align_ptr(workspace);
+  HessArgs a{};
+  a.md = *reinterpret_cast<const ModelDesc*>(h_model);
+  a.ser = *reinterpret_cast<const SeriesSet*>(h_series);
+  a.theta = theta;
+  a.theta_stride = theta_stride;
+  a.keys = keys;
+  a.sched = reinterpret_cast<uint32_t*>(base);
+  a.sched_len = p.sched_len;
+  a.R = R;
+  a.J = J;
+  a.alpha = alpha;
+  a.loglik = loglik;
+  a.grad = grad;
+  a.hess = hess;
+  a.ws = base + p.sched_bytes;
+  a.ws_stride = p.per_block;
+  MP_HIP(timed(3, st, [&] { return launch_sched(ALGO_MOP, keys, a.ser, nullptr, p.sched_len,
+                                                const_cast<uint32_t*>(a.sched), R, st); }));
+  MP_HIP(timed(2, st, [&] { return launch_mop_hess(a, p.grid, st); }));
+  return MP_OK;
 }
 
 int32_t mp_host_from_est(const mp_model_desc* h_model, float* h_theta, int64_t n) {

@@ -203,6 +203,43 @@ struct GradArgs {
   int lds_work;    // per-step working arrays in LDS (else in the global scratch)
 };
 
+// MOP Hessian launch (one workgroup per (parameter row, parameter pair)).
+struct HessArgs {
+  ModelDesc md;
+  SeriesSet ser;
+  const float* theta;  // (R, d) estimation scale
+  int64_t theta_stride;
+  const uint32_t* keys;
+  const uint32_t* sched;
+  int64_t sched_len;
+  int R, J;
+  float alpha;
+  double* loglik;  // (R)
+  float* grad;     // (R, d): d loglik / d theta_est
+  float* hess;     // (R, d, d): d^2 loglik / d theta_est^2
+  char* ws;        // per resident workgroup: hyper-dual particle buffers
+  int64_t ws_stride;
+};
+
+// Scratch of one Hessian workgroup: states and weights as hyper-dual numbers
+// (32 bytes each), ping-pong, plus the log-weights and prefix sums.
+struct HessScratch {
+  int64_t off_x0, off_x1, off_w0, off_w1, off_lw, off_c, total;
+};
+constexpr int64_t kHdBytes = 32;
+__host__ __device__ inline HessScratch hess_scratch_layout(int S, int J) {
+  HessScratch s;
+  const int64_t xs = align256((int64_t)S * J * kHdBytes), ws = align256((int64_t)J * kHdBytes);
+  s.off_x0 = 0;
+  s.off_x1 = xs;
+  s.off_w0 = 2 * xs;
+  s.off_w1 = 2 * xs + ws;
+  s.off_lw = 2 * xs + 2 * ws;
+  s.off_c = s.off_lw + align256((int64_t)J * 4);
+  s.total = s.off_c + align256((int64_t)J * 8);
+  return s;
+}
+
 // Simulation launch (one thread per (row, simulation)); U = 1.
 constexpr int kSimMaxDy = 4;
 struct SimArgs {

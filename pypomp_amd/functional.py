@@ -614,6 +614,37 @@ def mop_grad(struct, thetas_array, J, alpha, keys, K=50, form="batched"):
     return ll.cpu().numpy(), g.cpu().numpy()
 
 
+def mop_hess(struct, thetas_array, J, alpha, keys):
+    """MOP(alpha) value, gradient and Hessian with respect to the
+    estimation-scale parameters (the native forward-mode kernel).  thetas (n, d)
+    on the estimation scale, keys (n, 2).  Returns (loglik (n,) float64, grad
+    (n, d), hess (n, d, d)) of the log-likelihood; the reference's
+    ``_jhess_mop`` is the Hessian of -loglik / T (core/algorithms/ad_helpers.py:56-75)."""
+    import torch
+    dev = _native.require_device()
+    thetas = np.ascontiguousarray(np.asarray(thetas_array, dtype=F32))
+    if thetas.ndim == 1:
+        thetas = thetas[None]
+    keys = np.asarray(keys, dtype=np.uint32).reshape(-1, 2)
+    R, d = keys.shape[0], struct.model.d
+    if thetas.shape[0] != R:
+        raise ValueError("thetas and keys disagree on the number of rows")
+    ser = struct.device_series()
+    md = struct.model.desc()
+    th = torch.from_numpy(thetas).to(dev)
+    kt = _keys_tensor(keys, dev)
+    ll = torch.empty(R, dtype=torch.float64, device=dev)
+    g = torch.empty((R, d), dtype=torch.float32, device=dev)
+    H = torch.empty((R, d, d), dtype=torch.float32, device=dev)
+    n = _native.lib().mp_mop_hess_workspace_size(md, ser.desc, R, J)
+    ws = torch.empty(int(n), dtype=torch.uint8, device=dev)
+    _native.check(_native.lib().mp_mop_hess(md, ser.desc, _native.ptr(th), d, _native.ptr(kt), R, J, float(alpha),
+                                            _native.ptr(ll), _native.ptr(g), _native.ptr(H), _native.ptr(ws), int(n),
+                                            _native.stream_ptr()))
+    torch.cuda.current_stream().synchronize()
+    return ll.cpu().numpy(), g.cpu().numpy(), H.cpu().numpy()
+
+
 def _cosine_cooling(i, M, c):
     F = np.float32
     return F(F(c) + (F(1) - F(c)) * F(0.5) * (F(1) + np.cos(F(np.pi) * F(i) / F(M), dtype=F)))

@@ -101,3 +101,50 @@ def test_armijo_backtracking():
     # an objective that never decreases (or is NaN) stops after max_itn halvings
     eta = mg.armijo(lambda _: np.nan, np.float32(1.0), x, g, dr, 1.0, max_itn=10)
     assert eta == np.float32(0.5 ** 10)
+
+
+# ---- Hessian (jax.hessian of _mop_internal_mean, ad_helpers.py:8, 56-75) ----
+
+@pytest.mark.parametrize("alpha", [0.97, 0.0])
+def test_sv_hessian_is_symmetric_and_the_derivative_of_the_gradient(sv_case, alpha):
+    ys, th = sv_case
+    obj = mg.sv_hessian_case(th, ys, tf.split(tf.key(9), 1)[0], 64, alpha)
+    val, g, H, fd = mg.hessian_and_fd(obj, th, len(ys))
+    np.testing.assert_allclose(H, H.T, rtol=1e-10, atol=1e-10 * np.abs(H).max())
+    np.testing.assert_allclose(H, fd, rtol=1e-5, atol=1e-6 * np.abs(H).max())
+    # its gradient is the oracle's MOP gradient
+    _, _, grad, _ = mg.sv_mop_reference_grad(th, ys, tf.split(tf.key(9), 1)[0], 64, alpha)
+    np.testing.assert_allclose(g, grad, rtol=1e-9, atol=1e-12)
+
+
+def test_lg_hessian_is_the_derivative_of_the_gradient():
+    from oracle import lg as olg
+    ys = workloads()["lg2_seed1_ys"][:30]
+    model = olg.LGModel(olg.LGSpec(2, 2, separate_init=True))
+    th = model.to_est(workloads()["lg2_seed1_theta"])
+    obj = mg.lg_hessian_case(model, th, ys, tf.split(tf.key(10), 1)[0], 48, 0.9)
+    val, g, H, fd = mg.hessian_and_fd(obj, th, len(ys))
+    assert H.shape == (19, 19)
+    np.testing.assert_allclose(H, H.T, rtol=1e-10, atol=1e-10 * np.abs(H).max())
+    np.testing.assert_allclose(H, fd, rtol=1e-5, atol=1e-6 * np.abs(H).max())
+
+
+def test_hessian_differs_from_differencing_the_moving_stop_gradient(sv_case):
+    # the stop-gradient values are fixed at theta: re-freezing them at each
+    # shifted point (differences of the MOP gradient itself) gives another
+    # matrix when alpha > 0 -- the property the Hessian kernel must respect
+    import torch
+    ys, th = sv_case
+    key = tf.split(tf.key(11), 1)[0]
+    obj = mg.sv_hessian_case(th, ys, key, 64, 0.97)
+    _, _, H, _ = mg.hessian_and_fd(obj, th, len(ys))
+    h = 1e-4
+    moving = np.zeros_like(H)
+    for j in range(len(th)):
+        cols = []
+        for sgn in (1, -1):
+            t = torch.tensor(np.asarray(th, np.float64) + sgn * h * np.eye(len(th))[j], requires_grad=True)
+            obj(t).backward()
+            cols.append(t.grad.numpy().copy())
+        moving[:, j] = (cols[0] - cols[1]) / (2 * h)
+    assert not np.allclose(H, moving, rtol=1e-3, atol=1e-3 * np.abs(H).max())
