@@ -228,7 +228,7 @@ def _frozen_densities(objective, th0, T):
     return np.stack([r.numpy() for r in rec])
 
 
-def hessian_and_fd(objective, theta_est, T, h=1e-6):
+def hessian_and_fd(objective, theta_est, T, h=1e-6, fd=True):
     """(value, gradient, Hessian) of ``objective(th, frozen_m=None)`` at
     theta_est by torch's double backward -- the stop-gradient values are the
     densities at theta_est, held fixed, as the tangent of stop_gradient is zero
@@ -248,6 +248,8 @@ def hessian_and_fd(objective, theta_est, T, h=1e-6):
         f(t).backward()
         return t.grad.numpy().copy()
 
+    if not fd:
+        return float(val.detach()), g, H, None
     d = th0.shape[0]
     fd = np.zeros((d, d))
     for j in range(d):
@@ -342,8 +344,25 @@ def armijo(obj, curr, pt, grad, direction, eta, c=0.1, max_itn=10, frac=0.5):
         itn += 1
 
 
+def pinv_hermitian(H):
+    """``jnp.linalg.pinv(H, hermitian=True)`` (core/optimizer.py:285, 322):
+    singular values at or below 10 d eps32 times the largest are dropped."""
+    d = H.shape[-1]
+    return np.linalg.pinv(np.asarray(H, np.float64), rcond=10.0 * d * float(np.finfo(np.float32).eps),
+                          hermitian=True)
+
+
+def newton_weight(step):
+    """WeightedNewton's history weight (core/optimizer.py:315-318), float32:
+    i^log(i) / (i + 1)^log(i + 1)."""
+    F32 = np.float32
+    i = F32(step)
+    return F32(np.power(i, np.log(i, dtype=F32), dtype=F32)
+               / np.power(i + F32(1), np.log(i + F32(1), dtype=F32), dtype=F32))
+
+
 def train(value_and_grad, theta_est0, key, T, M, eta, alpha, alpha_cooling=1.0, beta1=0.9, beta2=0.999,
-          eps=1e-8, clip_norm=None, optimizer="adam", ls=None, c=0.1, max_ls_itn=10):
+          eps=1e-8, clip_norm=None, optimizer="adam", ls=None, c=0.1, max_ls_itn=10, hessian=None):
     """``_train_internal`` / ``_train_scan_step`` (core/algorithms/train.py:38-180)
     with one monitor (n_monitors = 1: the MOP value of the same draw is the
     reported objective) and the optimizer ``optimizer``: "adam"
@@ -352,6 +371,9 @@ def train(value_and_grad, theta_est0, key, T, M, eta, alpha, alpha_cooling=1.0, 
     alpha)`` returns (loglik, d loglik / d theta).  eta (M, d).  ``ls``: None,
     or ``neg_loglik_pf(theta, key)``, the bootstrap filter's negative
     log-likelihood with the iteration's MOP key, for the Armijo line search.
+    "newton" and "wnewton" (core/optimizer.py:270-330) take ``hessian(theta,
+    key, alpha)``, the MOP log-likelihood's Hessian, drawn with the
+    iteration's Hessian subkey (train.py:132-141).
     Returns (neg_logliks (M+1,), NaN first, traces (M+1, d)), float32 like the
     reference."""
     F32 = np.float32
@@ -360,6 +382,7 @@ def train(value_and_grad, theta_est0, key, T, M, eta, alpha, alpha_cooling=1.0, 
     m1 = np.zeros_like(th)
     v1 = np.zeros_like(th) if optimizer != "fmadam" else np.zeros((d, d), F32)
     hess, prev = np.eye(d, dtype=F32), np.zeros_like(th)
+    prev_h = np.eye(d, dtype=F32)
     nll, traces = [np.nan], [th.copy()]
     for m in range(M):
         a_m = F32(F32(1) - (F32(1) - F32(alpha)) * cosine_cooling(m, M, alpha_cooling))
@@ -370,8 +393,18 @@ def train(value_and_grad, theta_est0, key, T, M, eta, alpha, alpha_cooling=1.0, 
         nll.append(F32(-ll))
         if clip_norm is not None:
             grad = np.clip(grad, -clip_norm, clip_norm).astype(F32)
-        key = tf.split(key)[0]  # the Hessian subkey (unused by these optimizers)
-        if optimizer == "adam":
+        k2 = tf.split(key)
+        key, hkey = k2[0], k2[1]  # the Hessian subkey
+        if optimizer in ("newton", "wnewton"):
+            Hm = (-np.asarray(hessian(th, hkey, float(a_m)), np.float64) / T).astype(F32)  # of the mean objective
+            if optimizer == "newton" or m == 0:
+                Hw = Hm
+            else:
+                wt = newton_weight(m)
+                Hw = (wt * prev_h + (F32(1) - wt) * Hm).astype(F32)
+            direction = (-(pinv_hermitian(Hw) @ grad.astype(np.float64))).astype(F32)
+            prev_h = Hm
+        elif optimizer == "adam":
             m1 = (F32(beta1) * m1 + F32(1 - beta1) * grad).astype(F32)
             v1 = (F32(beta2) * v1 + F32(1 - beta2) * grad * grad).astype(F32)
             mh = (m1 / (F32(1) - F32(beta1) ** F32(m + 1))).astype(F32)

@@ -614,6 +614,29 @@ def mop_grad(struct, thetas_array, J, alpha, keys, K=50, form="batched"):
     return ll.cpu().numpy(), g.cpu().numpy()
 
 
+class _HessBuffers:
+    """Device buffers of repeated mp_mop_hess calls on n rows (the train loop)."""
+
+    def __init__(self, struct, n, J, dev):
+        import torch
+        self.ser = struct.device_series()
+        self.md = struct.model.desc()
+        self.n, self.J, self.d = n, J, struct.model.d
+        self.ll = torch.empty(n, dtype=torch.float64, device=dev)
+        self.g = torch.empty((n, self.d), dtype=torch.float32, device=dev)
+        self.H = torch.empty((n, self.d, self.d), dtype=torch.float32, device=dev)
+        self.nws = int(_native.lib().mp_mop_hess_workspace_size(self.md, self.ser.desc, n, J))
+        self.ws = torch.empty(self.nws, dtype=torch.uint8, device=dev)
+
+    def run(self, th, keys, alpha):
+        """Hessian (n, d, d) of the log-likelihood at th (n, d) est. scale, keys (n, 2) int32 view."""
+        _native.check(_native.lib().mp_mop_hess(
+            self.md, self.ser.desc, _native.ptr(th), self.d, _native.ptr(keys), self.n, self.J, float(alpha),
+            _native.ptr(self.ll), _native.ptr(self.g), _native.ptr(self.H), _native.ptr(self.ws), self.nws,
+            _native.stream_ptr()))
+        return self.H
+
+
 def mop_hess(struct, thetas_array, J, alpha, keys):
     """MOP(alpha) value, gradient and Hessian with respect to the
     estimation-scale parameters (the native forward-mode kernel).  thetas (n, d)
@@ -629,20 +652,10 @@ def mop_hess(struct, thetas_array, J, alpha, keys):
     R, d = keys.shape[0], struct.model.d
     if thetas.shape[0] != R:
         raise ValueError("thetas and keys disagree on the number of rows")
-    ser = struct.device_series()
-    md = struct.model.desc()
-    th = torch.from_numpy(thetas).to(dev)
-    kt = _keys_tensor(keys, dev)
-    ll = torch.empty(R, dtype=torch.float64, device=dev)
-    g = torch.empty((R, d), dtype=torch.float32, device=dev)
-    H = torch.empty((R, d, d), dtype=torch.float32, device=dev)
-    n = _native.lib().mp_mop_hess_workspace_size(md, ser.desc, R, J)
-    ws = torch.empty(int(n), dtype=torch.uint8, device=dev)
-    _native.check(_native.lib().mp_mop_hess(md, ser.desc, _native.ptr(th), d, _native.ptr(kt), R, J, float(alpha),
-                                            _native.ptr(ll), _native.ptr(g), _native.ptr(H), _native.ptr(ws), int(n),
-                                            _native.stream_ptr()))
+    buf = _HessBuffers(struct, R, J, dev)
+    buf.run(torch.from_numpy(thetas).to(dev), _keys_tensor(keys, dev), alpha)
     torch.cuda.current_stream().synchronize()
-    return ll.cpu().numpy(), g.cpu().numpy(), H.cpu().numpy()
+    return buf.ll.cpu().numpy(), buf.g.cpu().numpy(), buf.H.cpu().numpy()
 
 
 def _cosine_cooling(i, M, c):
@@ -701,11 +714,9 @@ def train(struct, thetas_array, J, optimizer, M, eta, alpha, keys, alpha_cooling
     M+1, d)), float32."""
     import torch
     from .optim import BFGS, SGD, Adam, FullMatrixAdam, Newton, WeightedNewton
-    if isinstance(optimizer, (Newton, WeightedNewton)):
-        raise ValueError(f"Optimizer '{type(optimizer).__name__}' needs the MOP Hessian, which this engine does "
-                         "not compute; use SGD, Adam, FullMatrixAdam or BFGS")
-    if not isinstance(optimizer, (Adam, SGD, FullMatrixAdam, BFGS)):
+    if not isinstance(optimizer, (Adam, SGD, FullMatrixAdam, BFGS, Newton, WeightedNewton)):
         raise ValueError(f"Optimizer '{type(optimizer).__name__}' not supported")
+    newton = isinstance(optimizer, (Newton, WeightedNewton))
     if n_monitors < 1 and optimizer.ls:
         raise ValueError("Line search requires at least one monitor")
     dev = _native.require_device()
@@ -721,8 +732,9 @@ def train(struct, thetas_array, J, optimizer, M, eta, alpha, keys, alpha_cooling
             raise ValueError("train needs the same number of Euler steps in every observation interval")
     T = struct.units[0].T
     eta_arr = torch.from_numpy(eta.to_array(struct.param_names, M)).to(dev)
-    # host key schedule: MOP keys (M, n, 2) and the running keys
+    # host key schedule: MOP keys (M, n, 2), Hessian keys and the running keys
     mop_keys = np.zeros((M, n, 2), np.uint32)
+    hess_keys = np.zeros((M, n, 2), np.uint32)
     mon_keys = []
     for r in range(n):
         k = keys[r]
@@ -734,7 +746,8 @@ def train(struct, thetas_array, J, optimizer, M, eta, alpha, keys, alpha_cooling
                 ks = prandom.split(k, n_monitors + 1)
                 k, sub = ks[0], ks[1:]
                 row_mon.append(sub)
-            k = prandom.split(k, 2)[0]
+            k2 = prandom.split(k, 2)
+            k, hess_keys[m, r] = k2[0], k2[1]  # train.py:132
         mon_keys.append(row_mon)
     ser = struct.device_series()
     md = struct.model.desc()
@@ -742,6 +755,9 @@ def train(struct, thetas_array, J, optimizer, M, eta, alpha, keys, alpha_cooling
     nws = _native.lib().mp_mop_grad_workspace_size(md, ser.desc, n, J, K)
     ws = torch.empty(int(nws), dtype=torch.uint8, device=dev)
     kt = torch.from_numpy(mop_keys.view(np.int32)).to(dev)
+    if newton:
+        hkt = torch.from_numpy(hess_keys.view(np.int32)).to(dev)
+        hws = _HessBuffers(struct, n, J, dev)
     th = torch.from_numpy(th0).to(dev)
     state = optimizer.init_state(th)
     lls = torch.empty((M, n), dtype=torch.float64, device=dev)
@@ -761,7 +777,13 @@ def train(struct, thetas_array, J, optimizer, M, eta, alpha, keys, alpha_cooling
             nll[:, m + 1] = -res["logLik"].mean(axis=1, dtype=F32)
         if optimizer.clip_norm is not None:
             grad = torch.clamp(grad, -optimizer.clip_norm, optimizer.clip_norm)
-        direction, state = optimizer.step(grad, state, m, eta_i=eta_arr[m])
+        if newton:
+            # Hessian of the mean objective -loglik / T at theta, drawn with the
+            # iteration's Hessian subkey (train.py:134-141)
+            hess = hws.run(th, hkt[m], a_m) / (-np.float32(T))
+            direction, state = optimizer.step(grad, state, m, eta_i=eta_arr[m], hess=hess)
+        else:
+            direction, state = optimizer.step(grad, state, m, eta_i=eta_arr[m])
         if optimizer.scale:
             direction = direction / torch.linalg.norm(direction, dim=-1, keepdim=True)
         if optimizer.ls:

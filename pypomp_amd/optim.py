@@ -127,23 +127,55 @@ class BFGS(Optimizer):
         return -(new @ grad[:, :, None])[:, :, 0], (new, grad)
 
 
+def _pinv_hermitian(H):
+    """``jnp.linalg.pinv(H, hermitian=True)`` over a batch (n, d, d): singular
+    values at or below 10 d eps32 times the largest are dropped."""
+    import torch
+    d = H.shape[-1]
+    return torch.linalg.pinv(H, rtol=10.0 * d * float(np.finfo(np.float32).eps), hermitian=True)
+
+
 @dataclass(frozen=True)
 class Newton(Optimizer):
-    """Newton step on the MOP Hessian (core/optimizer.py:270-289).  The Hessian
-    of the MOP estimator is not computed by this engine: ``train`` raises."""
+    """Newton step on the MOP Hessian (core/optimizer.py:270-289): direction =
+    -pinv(H) grad with H the Hessian of the mean objective, from the native
+    Hessian kernel (``train`` passes it as ``hess``)."""
+    needs_hessian = True
 
     def init_state(self, theta):
         return ()
+
+    def step(self, grad, state, step_num, eta_i=None, hess=None):
+        if hess is None:
+            raise ValueError("Newton optimizer requires compute_hessian_fn")
+        return -(_pinv_hermitian(hess) @ grad[:, :, None])[:, :, 0], ()
 
 
 @dataclass(frozen=True)
 class WeightedNewton(Optimizer):
-    """Weighted Newton step on the MOP Hessian (core/optimizer.py:292-330).  The
-    Hessian of the MOP estimator is not computed by this engine: ``train``
-    raises."""
+    """Weighted Newton step (core/optimizer.py:292-330): from the second step
+    on, H_w = w H_prev + (1 - w) H with w = i^log(i) / (i + 1)^log(i + 1),
+    i the step number, and H_prev the previous step's Hessian (the identity
+    before the first); direction = -pinv(H_w) grad."""
+    needs_hessian = True
 
     def init_state(self, theta):
-        return ()
+        import torch
+        n, d = theta.shape
+        return (torch.eye(d, dtype=theta.dtype, device=theta.device).expand(n, d, d).clone(),)
+
+    def step(self, grad, state, step_num, eta_i=None, hess=None):
+        if hess is None:
+            raise ValueError("WeightedNewton optimizer requires compute_hessian_fn")
+        (prev,) = state
+        if step_num == 0:
+            hw = hess
+        else:
+            F = np.float32
+            i = F(step_num)
+            wt = F(np.power(i, np.log(i, dtype=F), dtype=F) / np.power(i + F(1), np.log(i + F(1), dtype=F), dtype=F))
+            hw = float(wt) * prev + float(F(1) - wt) * hess
+        return -(_pinv_hermitian(hw) @ grad[:, :, None])[:, :, 0], (hess,)
 
 
 class LearningRate:

@@ -341,3 +341,41 @@ def test_pomp_slot_rules_follow_the_reference():
         pp.Pomp(rinit=m.rproc, rproc=m.rproc, dmeas=m.dmeas, **kw)
     ok = pp.Pomp(rinit=m.rinit, rproc=m.rproc, dmeas=m.dmeas, **kw)
     assert ok.rinit is not None and ok.rproc is not None and ok.rmeas is None
+
+
+@pytest.mark.parametrize("name", ["newton", "wnewton"])
+def test_newton_steps_follow_the_reference_arithmetic(name):
+    # core/optimizer.py:270-330 on identical inputs (torch on the CPU): direction
+    # -pinv(H_w) grad, H_w = H at step 0 and w H_prev + (1 - w) H after, with
+    # w = i^log i / (i + 1)^log(i + 1); singular values below 10 d eps32 of the
+    # largest dropped
+    import torch
+    from oracle import mop_grad as mg
+    from pypomp_amd.optim import Newton, WeightedNewton
+    rng = np.random.default_rng(3)
+    n, d = 2, 5
+    opt = Newton() if name == "newton" else WeightedNewton()
+    state = opt.init_state(torch.zeros((n, d)))
+    prev = [np.eye(d, dtype=np.float32)] * n
+    for step in range(4):
+        A = rng.normal(size=(n, d, d)).astype(np.float32)
+        H = (A @ np.transpose(A, (0, 2, 1)) + np.eye(d, dtype=np.float32)).astype(np.float32)
+        if step == 2:  # an indefinite one
+            H[:, 0, 0] = -H[:, 0, 0]
+        g = rng.normal(size=(n, d)).astype(np.float32)
+        got, state = opt.step(torch.from_numpy(g), state, step, hess=torch.from_numpy(H))
+        for r in range(n):
+            if name == "newton" or step == 0:
+                Hw = H[r]
+            else:
+                w = mg.newton_weight(step)
+                Hw = (w * prev[r] + (np.float32(1) - w) * H[r]).astype(np.float32)
+            exp = -(mg.pinv_hermitian(Hw) @ g[r].astype(np.float64))
+            np.testing.assert_allclose(got[r].numpy(), exp, rtol=2e-4, atol=1e-5 * np.abs(exp).max())
+        prev = list(H)
+    # a singular Hessian: the null direction is dropped, not inverted
+    Hs = np.diag([2.0, 1.0, 0.0, 0.0, 0.0]).astype(np.float32)[None]
+    d0, _ = Newton().step(torch.ones((1, d)), (), 0, hess=torch.from_numpy(Hs))
+    np.testing.assert_allclose(d0[0].numpy(), [-0.5, -1.0, 0, 0, 0], atol=1e-6)
+    with pytest.raises(ValueError, match="compute_hessian_fn"):
+        opt.step(torch.ones((1, d)), opt.init_state(torch.zeros((1, d))), 0)
