@@ -224,8 +224,12 @@ int32_t mp_simulate(const mp_model_desc* h_model, const mp_series_desc* h_series
 /* One IF2 iteration m: _perfilter_internal (core/algorithms/mif.py:88-258) for R
  * swarms.  thetas_in/thetas_out (R, J, d) on the estimation scale; sigmas and
  * sigmas_init (d) as RWSigma.sigmas_array / sigmas_init_array
- * (core/rw_sigma.py:184-191); geometric cooling factor cool = a^(1/50)
- * (rw_sigma.py:210-213).  theta_mean (R, d): mean of thetas_out over J.
+ * (core/rw_sigma.py:184-191).  Cooling (RWSigma.cooling_factor,
+ * rw_sigma.py:193-225): cooling NULL takes the geometric schedule
+ * cool^(nt/T_u + m) with cool = a^(1/50), evaluated in-kernel; otherwise
+ * cooling (sum T_u) holds this iteration's factors c(nt, m, T_u) at the
+ * observation rows of the series (cosine, hyperbolic and custom schedules,
+ * evaluated by the caller).  theta_mean (R, d): mean of thetas_out over J.
  * ancestry (R, J) int32 or NULL: the composed resampling ancestry of the final
  * particles (mif.py:237-245, return_ancestry; panel IF2 with block=False,
  * core/algorithms/panel_mif.py:224-231).  Panel units of one call (unit[f])
@@ -234,9 +238,9 @@ int32_t mp_simulate(const mp_model_desc* h_model, const mp_series_desc* h_series
  * unit first from a device work queue, so uneven units balance over the CUs. */
 int32_t mp_perfilter(const mp_model_desc* h_model, const mp_series_desc* h_series, const float* thetas_in,
                      const uint32_t* keys, const int32_t* unit, int32_t R, int32_t J, float thresh,
-                     const float* sigmas, const float* sigmas_init, float cool, int32_t m, float* thetas_out,
-                     float* theta_mean, double* loglik, int32_t* ancestry, void* workspace, size_t ws_bytes,
-                     void* stream);
+                     const float* sigmas, const float* sigmas_init, float cool, const float* cooling, int32_t m,
+                     float* thetas_out, float* theta_mean, double* loglik, int32_t* ancestry, void* workspace,
+                     size_t ws_bytes, void* stream);
 
 /* ---- launch-shape tuning ------------------------------------------------------
  * No counterpart in the reference (XLA picks its own launch shapes).  Process-

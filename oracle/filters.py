@@ -297,10 +297,17 @@ def lse(a):
 
 @dataclass
 class RWSigma:
-    """Random-walk sigmas and geometric cooling (core/rw_sigma.py:184-225)."""
+    """Random-walk sigmas and the cooling schedules (core/rw_sigma.py:184-225):
+    geometric (base a), cosine (final factor c over M iterations), hyperbolic
+    (rate s) or custom (fn(nt, m, ntimes))."""
     sigmas: np.ndarray      # (d,) canonical order
     init_mask: np.ndarray   # (d,) 1.0 for initial-value parameters
     a: float = 0.5
+    cooling: str = "geometric"
+    c: float = None
+    M: int = None
+    s: float = None
+    fn: object = None
 
     @property
     def sigmas_array(self):
@@ -311,7 +318,16 @@ class RWSigma:
         return (self.sigmas.astype(F32) * self.init_mask.astype(F32)).astype(F32)
 
     def cooling_factor(self, nt, m, ntimes):
+        """frac = nt / ntimes + m in float32, then the schedule (rw_sigma.py:209-223)."""
         frac = F32(F32(nt) / F32(ntimes) + F32(m))
+        if self.cooling == "cosine":
+            progress = F32(frac / F32(self.M))
+            half = F32((1.0 - self.c) * 0.5)   # a Python-float product, rounded once
+            return F32(F32(self.c) + F32(half * F32(F32(1) + np.cos(F32(F32(np.pi) * progress), dtype=F32))))
+        if self.cooling == "hyperbolic":
+            return F32(F32(1) / F32(F32(1) + F32(F32(self.s) * frac)))
+        if self.cooling == "custom":
+            return F32(self.fn(nt, m, ntimes))
         factor = F32(self.a ** (1.0 / 50.0))
         return F32(d_pow(factor, frac))
 

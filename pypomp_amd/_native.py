@@ -87,7 +87,7 @@ def lib():
     L.mp_mop.argtypes = [C.POINTER(ModelDesc), C.POINTER(SeriesDesc), f32p, i64, u32p, vp, i32, i32,
                          C.c_float, i32, f64p, vp, sz, vp]
     L.mp_perfilter.argtypes = [C.POINTER(ModelDesc), C.POINTER(SeriesDesc), f32p, u32p, vp, i32, i32,
-                               C.c_float, f32p, f32p, C.c_float, i32, f32p, f32p, f64p, vp, vp, sz, vp]
+                               C.c_float, f32p, f32p, C.c_float, f32p, i32, f32p, f32p, f64p, vp, vp, sz, vp]
     L.mp_mop_grad_workspace_size.argtypes = [C.POINTER(ModelDesc), C.POINTER(SeriesDesc), i32, i32, i32]
     L.mp_mop_grad_workspace_size.restype = sz
     L.mp_mop_grad.argtypes = [C.POINTER(ModelDesc), C.POINTER(SeriesDesc), f32p, i64, u32p, i32, i32, C.c_float,

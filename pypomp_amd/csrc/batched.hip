@@ -169,8 +169,12 @@ __device__ __forceinline__ void run_filter(const BatchArgs& a, const int f, char
     if (ALGO == ALGO_PER) {
       isub = ld_key(sch, 0);
       rbase = ld_key(sch, 1);
-      float fr = 0.0f / (float)T + (float)a.mif_m;
-      c0 = d_pow(a.cool_factor, fr);
+      if (a.cool_table) {
+        c0 = a.cool_table[o0];
+      } else {
+        float fr = 0.0f / (float)T + (float)a.mif_m;
+        c0 = d_pow(a.cool_factor, fr);
+      }
     } else {
       rbase = ld_key(a.keys, f);
     }
@@ -219,8 +223,12 @@ __device__ __forceinline__ void run_filter(const BatchArgs& a, const int f, char
     float cft = 0.f;
     Key kpert{0, 0};
     if (ALGO == ALGO_PER) {
-      float fr = (float)t / (float)T + (float)a.mif_m;
-      cft = d_pow(a.cool_factor, fr);
+      if (a.cool_table) {
+        cft = a.cool_table[o0 + t];
+      } else {
+        float fr = (float)t / (float)T + (float)a.mif_m;
+        cft = d_pow(a.cool_factor, fr);
+      }
       kpert = ld_key(sch, base - 1);
     }
     float lmax = -__builtin_inff(), lmin = __builtin_inff();

@@ -1,6 +1,7 @@
 // extern "C" entry points of libmipomp.so (declared in include/mipomp.h):
 // argument checks, workspace carving, launch sequencing.  No C++ exception
 // crosses the boundary; errors become status codes plus a per-thread message.
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -252,7 +253,8 @@ BatchedPlan plan_batched(int algo, const mp_model_desc* m, const mp_series_desc*
 
 struct PlainPlan {
   int ntiles, rows, nprop, rows_prop, prop_len, scan_threads, prop_threads, xcd_q;
-  int64_t off_sched, off_params, off_x0, off_x1, off_lw, off_c, off_pmax, off_pmin, off_lb, off_st, total;
+  int ndiag;
+  int64_t off_sched, off_params, off_x0, off_x1, off_lw, off_c, off_pmax, off_pmin, off_lb, off_st, off_diag, total;
 };
 
 PlainPlan plan_plain(const mp_model_desc* m, const mp_series_desc* s, int R, int J) {
@@ -306,6 +308,9 @@ PlainPlan plan_plain(const mp_model_desc* m, const mp_series_desc* s, int R, int
   p.off_pmin = take((int64_t)p.nprop * 4);
   p.off_lb = take((int64_t)p.ntiles * 16);
   p.off_st = take(sizeof(PlainState));
+  // step diagnostics: one block per 1024 particles, at most 1024 blocks
+  p.ndiag = (int)std::min<int64_t>(1024, ((int64_t)J + 1023) / 1024);
+  p.off_diag = take((int64_t)p.ndiag * kDiagWords * 8);
   p.total = o + 256;
   return p;
 }
@@ -336,7 +341,7 @@ int run_batched(int algo, const mp_model_desc* m, const mp_series_desc* s, Batch
 
 int run_plain_pf(const mp_model_desc* m, const mp_series_desc* s, const float* theta, int64_t theta_stride,
                  const uint32_t* keys, int R, int J, float logthresh, int flags, double* loglik, float* cll,
-                 void* ws, size_t ws_bytes, hipStream_t st) {
+                 float* ess, float* fmean, float* pmean, void* ws, size_t ws_bytes, hipStream_t st) {
   if (s->U != 1) return fail(MP_ERR_ARG, "the plain form runs single-series filters (U = 1)");
   const PlainPlan p = plan_plain(m, s, R, J);
   if (ws_bytes < (size_t)p.total) return fail(MP_ERR_WORKSPACE, "workspace %zu < required %lld bytes", ws_bytes,
@@ -366,6 +371,9 @@ int run_plain_pf(const mp_model_desc* m, const mp_series_desc* s, const float* t
   a.st = reinterpret_cast<PlainState*>(b + p.off_st);
   a.rows = p.rows;
   a.scan_spin = g_plain_scan_spin;
+  a.diag_part = reinterpret_cast<double*>(b + p.off_diag);
+  a.ndiag = p.ndiag;
+  const bool diag = (flags & (MP_ESS | MP_FILTER_MEAN | MP_PREDICTION_MEAN)) != 0;
   const int64_t slen = sched_stride(ALGO_PF, a.ser);
   uint32_t* sched_all = reinterpret_cast<uint32_t*>(b + p.off_sched);
   MP_HIP(timed(3, st, [&] { return launch_sched(ALGO_PF, keys, a.ser, nullptr, slen, sched_all, R, st); }));
@@ -377,6 +385,9 @@ int run_plain_pf(const mp_model_desc* m, const mp_series_desc* s, const float* t
     a.fkey = keys + 2 * f;
     a.loglik_out = loglik + f;
     a.cll = cll ? cll + (int64_t)f * s->Tmax : nullptr;
+    a.ess = (flags & MP_ESS) ? ess + (int64_t)f * s->Tmax : nullptr;
+    a.fmean = (flags & MP_FILTER_MEAN) ? fmean + (int64_t)f * s->Tmax * m->S : nullptr;
+    a.pmean = (flags & MP_PREDICTION_MEAN) ? pmean + (int64_t)f * s->Tmax * m->S : nullptr;
     a.sched = sched_all + 2 * (int64_t)f * slen;
     a.params = params + (int64_t)f * kParamSlot;
     MP_HIP(plain_reset(a, st));
@@ -384,6 +395,7 @@ int run_plain_pf(const mp_model_desc* m, const mp_series_desc* s, const float* t
       a.t = t;
       MP_HIP(timed(0, st, [&] { return plain_kernel(0, a, p.ntiles, p.prop_threads, st); }));
       MP_HIP(timed(1, st, [&] { return plain_kernel(1, a, p.ntiles, p.scan_threads, st); }));
+      if (diag) MP_HIP(plain_kernel(2, a, p.ntiles, 256, st));
     }
   }
   return MP_OK;
@@ -617,11 +629,9 @@ int32_t mp_pfilter(const mp_model_desc* h_model, const mp_series_desc* h_series,
   if ((flags & MP_PREDICTION_MEAN) && !pmean) return fail(MP_ERR_ARG, "MP_PREDICTION_MEAN set but pmean is NULL");
   hipStream_t st = (hipStream_t)stream;
   if (form == MP_FORM_PLAIN) {
-    if (flags & (MP_ESS | MP_FILTER_MEAN | MP_PREDICTION_MEAN))
-      return fail(MP_ERR_ARG, "the plain form reports logLik and CLL only");
     if (unit) return fail(MP_ERR_ARG, "the plain form runs single-series filters");
     return run_plain_pf(h_model, h_series, theta, theta_stride, keys, R, J, log_thresh(thresh), flags, loglik,
-                        (flags & MP_CLL) ? cll : nullptr, workspace, ws_bytes, st);
+                        (flags & MP_CLL) ? cll : nullptr, ess, fmean, pmean, workspace, ws_bytes, st);
   }
   BatchArgs a = base_args(h_model, h_series, keys, unit, R, J);
   a.theta = theta;
@@ -808,8 +818,9 @@ int32_t mp_simulate(const mp_model_desc* h_model, const mp_series_desc* h_series
 
 int32_t mp_perfilter(const mp_model_desc* h_model, const mp_series_desc* h_series, const float* thetas_in,
                      const uint32_t* keys, const int32_t* unit, int32_t R, int32_t J, float thresh, const float* sigmas,
-                     const float* sigmas_init, float cool, int32_t m, float* thetas_out, float* theta_mean,
-                     double* loglik, int32_t* ancestry, void* workspace, size_t ws_bytes, void* stream) {
+                     const float* sigmas_init, float cool, const float* cooling, int32_t m, float* thetas_out,
+                     float* theta_mean, double* loglik, int32_t* ancestry, void* workspace, size_t ws_bytes,
+                     void* stream) {
   int rc = check_common(h_model, h_series, R, J, keys);
   if (rc) return rc;
   if (!thetas_in || !thetas_out || !theta_mean || !loglik || !sigmas || !sigmas_init || !workspace)
@@ -821,6 +832,7 @@ int32_t mp_perfilter(const mp_model_desc* h_model, const mp_series_desc* h_serie
   a.sigmas = sigmas;
   a.sigmas_init = sigmas_init;
   a.cool_factor = cool;
+  a.cool_table = cooling;
   a.mif_m = m;
   a.theta_out = thetas_out;
   a.theta_mean = theta_mean;

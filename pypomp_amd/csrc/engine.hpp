@@ -93,6 +93,7 @@ struct BatchArgs {
   const float* sigmas;      // (d) non-initial sigmas
   const float* sigmas_init; // (d)
   float cool_factor;        // geometric cooling factor a^(1/50) (float32 of the double)
+  const float* cool_table;  // or the factors c(nt, m) of this iteration per observation row (other schedules)
   int mif_m;                // iteration index m
   float* theta_out;         // (R, J, d) swarm after the final resample (est. scale)
   float* theta_mean;        // (R, d) mean of theta_out
@@ -152,6 +153,10 @@ struct PlainState {
   uint32_t r0, r1;     // resampling key
 };
 
+// Words of one block's diagnostic sums: ESS, filter mean and prediction mean
+// of up to 7 state words.
+constexpr int kDiagWords = 16;
+
 // XCDs of an MI355X: the hardware dispatches workgroups over them round-robin
 // by block index, and each has its own L2.
 constexpr int kXcds = 8;
@@ -176,6 +181,11 @@ struct PlainArgs {
   PlainState* st;
   double* loglik_out;   // (1) written after the last step
   float* cll;           // (T) or null
+  float* ess;           // (T) or null
+  float* fmean;         // (T, S) or null
+  float* pmean;         // (T, S) or null
+  double* diag_part;    // (ndiag, kDiagWords) per-block diagnostic sums
+  int ndiag;            // diagnostic blocks
   int rows;             // particle rows per scan tile (tile = rows * blockDim)
   int ntiles;           // scan tiles
   int rows_prop;        // particle rows per propagate block

@@ -416,6 +416,56 @@ __global__ __launch_bounds__(1024) void k_plain_scan(const PlainArgs a) {
   for (int i = tid; i < ntiles; i += B) lbn[i] = 0;
 }
 
+// Diagnostics of step t (pfilter.py:146-156), after its scan: ESS = 1 / sum
+// exp(2 nw), filter mean = sum x exp(nw), prediction mean = mean x, with nw =
+// lw - llf (-log J after a collapse) the normalised weights before
+// resampling; float32 terms as the batched form forms them, summed in float64.
+// Each block writes its partial sums; k_plain_diag_final adds them in block
+// order (a fixed order: the same bits for every run).
+template <class M>
+__global__ __launch_bounds__(256) void k_plain_diag(const PlainArgs a) {
+  __shared__ double red[kRedUsable / 8 + 1];
+  constexpr int NV = 1 + 2 * M::MAXS;
+  static_assert(NV <= kDiagWords, "diagnostic words");
+  const int J = a.J, S = M::FIXED_S ? M::MAXS : a.md.S, tid = threadIdx.x, B = blockDim.x;
+  const float* Xn = (a.t & 1) ? a.X0 : a.X1;
+  const PlainState st = *a.st;
+  const float logJ = d_log((float)J);
+  double acc[NV];
+  MP_UNROLL for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+  for (int64_t j = (int64_t)blockIdx.x * B + tid; j < J; j += (int64_t)gridDim.x * B) {
+    const float nw = st.collapsed ? -logJ : a.lw[j] - st.llf;
+    const float e = expf(nw);
+    acc[0] += (double)(e * e);
+    MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) {
+      const float x = Xn[(int64_t)s * J + j];
+      acc[1 + s] += (double)(x * e);
+      acc[1 + M::MAXS + s] += (double)x;
+    }
+  }
+  block_sum_n<NV>(acc, red);
+  if (tid == 0) MP_UNROLL for (int k = 0; k < NV; ++k) a.diag_part[(int64_t)blockIdx.x * kDiagWords + k] = acc[k];
+}
+
+template <class M>
+__global__ __launch_bounds__(64) void k_plain_diag_final(const PlainArgs a) {
+  constexpr int NV = 1 + 2 * M::MAXS;
+  const int k = threadIdx.x, S = M::FIXED_S ? M::MAXS : a.md.S;
+  if (k >= NV) return;
+  double v = 0.0;
+  for (int b = 0; b < a.ndiag; ++b) v += a.diag_part[(int64_t)b * kDiagWords + k];
+  const int t = a.t;
+  if (k == 0) {
+    if (a.ess) a.ess[t] = (float)(1.0 / v);
+  } else if (k <= M::MAXS) {
+    const int s = k - 1;
+    if (a.fmean && s < S) a.fmean[(int64_t)t * S + s] = (float)v;
+  } else {
+    const int s = k - 1 - M::MAXS;
+    if (a.pmean && s < S) a.pmean[(int64_t)t * S + s] = (float)(v / (double)a.J);
+  }
+}
+
 // Prepared model parameters of each filter (one thread per filter).
 template <class M>
 __global__ void k_plain_prepare(ModelDesc md, const float* theta, int64_t stride, int R, int flags, char* out,
@@ -473,11 +523,17 @@ hipError_t plain_reset(const PlainArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-// which = 0: propagate kernel, 1: scan kernel
+// which = 0: propagate kernel, 1: scan kernel, 2: step diagnostics
 template <class M>
 static hipError_t plain_kernel_t(int which, const PlainArgs& a, int ntiles, int threads, hipStream_t s) {
-  if (which == 0) mp_launch(k_plain_prop<M>, dim3(a.nprop), dim3(threads), 0, s, a);
-  else mp_launch(k_plain_scan<M>, dim3(ntiles), dim3(threads), 0, s, a);
+  if (which == 0) {
+    mp_launch(k_plain_prop<M>, dim3(a.nprop), dim3(threads), 0, s, a);
+  } else if (which == 1) {
+    mp_launch(k_plain_scan<M>, dim3(ntiles), dim3(threads), 0, s, a);
+  } else {
+    mp_launch(k_plain_diag<M>, dim3(a.ndiag), dim3(256), 0, s, a);
+    mp_launch(k_plain_diag_final<M>, dim3(1), dim3(64), 0, s, a);
+  }
   return hipGetLastError();
 }
 

@@ -39,6 +39,24 @@ class PompStruct:
         return self._dev[k]
 
 
+# The plain form (one filter at a time over the whole GPU) pays two launches
+# per observation; the batched form runs one workgroup per filter.  A filter
+# of at least this many particles fills the GPU on its own, so when fewer
+# filters than CUs are asked for, the plain form runs them faster.
+PLAIN_MIN_PARTICLES = 65536
+
+
+def choose_form(R, J, multi_unit=False):
+    """The execution form for R filters of J particles: "plain" when the
+    filters are single-series, fewer than the GPU's compute units and each has
+    at least PLAIN_MIN_PARTICLES particles; otherwise "batched"."""
+    import torch
+    if multi_unit or J < PLAIN_MIN_PARTICLES:
+        return "batched"
+    cus = torch.cuda.get_device_properties(_native.require_device()).multi_processor_count
+    return "plain" if R < cus else "batched"
+
+
 def _form_id(form):
     if form in ("batched", _native.MP_FORM_BATCHED):
         return _native.MP_FORM_BATCHED
@@ -62,10 +80,12 @@ def _workspace(algo, md, ser, R, J, form, dev):
 
 
 def pfilter(struct, thetas_array, J, keys, thresh=0.0, CLL=False, ESS=False, filter_mean=False,
-            prediction_mean=False, form="batched", should_trans=False, unit=None):
+            prediction_mean=False, form="auto", should_trans=False, unit=None):
     """Bootstrap particle filter.  thetas (n, d) natural scale, keys (n, reps, 2);
     ``unit`` (n,) selects each parameter set's unit of a multi-unit struct.
-    Returns a dict with ``logLik`` (n, reps) and the requested diagnostics."""
+    ``form``: "batched", "plain" or "auto" (``choose_form``); both forms give
+    the same results bit for bit.  Returns a dict with ``logLik`` (n, reps) and
+    the requested diagnostics."""
     import torch
     dev = _native.require_device()
     thetas = np.asarray(thetas_array, dtype=F32)
@@ -81,6 +101,8 @@ def pfilter(struct, thetas_array, J, keys, thresh=0.0, CLL=False, ESS=False, fil
         raise ValueError("J should be greater than 0.")
     thresh = float(max(0.0, thresh))
     R = n * reps
+    if form == "auto":
+        form = choose_form(R, J, multi_unit=unit is not None or len(struct.units) > 1)
     ser = struct.device_series()
     md = struct.model.desc()
     th = torch.from_numpy(np.repeat(thetas, reps, axis=0)).to(dev)
@@ -301,19 +323,33 @@ class _PerRunner:
         t = lambda a: torch.from_numpy(np.ascontiguousarray(np.asarray(a, dtype=F32))).to(dev)  # noqa: E731
         self.sig, self.sig0 = t(rw.sigmas_array), t(rw.sigmas_init_array)
         self.factor = float(rw.factor)
+        self.rw = rw
         self.thresh = float(max(0.0, thresh))
         self.unit = None if unit is None else torch.from_numpy(np.ascontiguousarray(unit, dtype=np.int32)).to(dev)
         self.ws, self.nws = _workspace(_native.MP_ALGO_PER, self.md, self.ser, self.R, self.J,
                                        _native.MP_FORM_BATCHED, dev)
         self.anc = torch.empty((self.R, self.J), dtype=torch.int32, device=dev) if ancestry else None
 
-    def run(self, tin, tout, keys, m, tmean, ll):
-        """One iteration: tin -> tout (R, J, d), keys (R, 2) int32 view, means (R, d), loglik (R)."""
+    def cooling(self, ms):
+        """Device tables (len(ms), sum T_u) of the schedule's factors for the
+        iterations ms, or None for the geometric schedule."""
+        import torch
+        if self.rw.geometric:
+            return None
+        tab = np.stack([self.rw.cooling_table(m, self.ser.T_units) for m in ms])
+        return torch.from_numpy(np.ascontiguousarray(tab)).to(self.dev)
+
+    def run(self, tin, tout, keys, m, tmean, ll, cool=None):
+        """One iteration: tin -> tout (R, J, d), keys (R, 2) int32 view, means
+        (R, d), loglik (R); ``cool`` this iteration's factor table (device) for
+        a non-geometric schedule (made here when not given)."""
+        if cool is None and not self.rw.geometric:
+            cool = self.cooling([m])[0]
         _native.check(_native.lib().mp_perfilter(
             self.md, self.ser.desc, _native.ptr(tin), _native.ptr(keys), _native.ptr(self.unit), self.R, self.J,
-            self.thresh, _native.ptr(self.sig), _native.ptr(self.sig0), self.factor, int(m), _native.ptr(tout),
-            _native.ptr(tmean), _native.ptr(ll), _native.ptr(self.anc), _native.ptr(self.ws), self.nws,
-            _native.stream_ptr()))
+            self.thresh, _native.ptr(self.sig), _native.ptr(self.sig0), self.factor, _native.ptr(cool), int(m),
+            _native.ptr(tout), _native.ptr(tmean), _native.ptr(ll), _native.ptr(self.anc), _native.ptr(self.ws),
+            self.nws, _native.stream_ptr()))
 
     def iterate(self, theta0, mkeys, graph=False):
         """M iterations from theta0 (R, J, d) est. scale (device tensor), keys
@@ -326,10 +362,12 @@ class _PerRunner:
         bufs = [theta0.contiguous(), torch.empty_like(theta0)]
         means = torch.empty((M, self.R, self.d), dtype=torch.float32, device=self.dev)
         lls = torch.empty((M, self.R), dtype=torch.float64, device=self.dev)
+        cools = self.cooling(range(M))
 
         def body():
             for m in range(M):
-                self.run(bufs[m % 2], bufs[(m + 1) % 2], kt[m], m, means[m], lls[m])
+                self.run(bufs[m % 2], bufs[(m + 1) % 2], kt[m], m, means[m], lls[m],
+                         None if cools is None else cools[m])
 
         if graph and M > 0:
             g = torch.cuda.CUDAGraph()
@@ -337,7 +375,7 @@ class _PerRunner:
                 body()
             # the graph holds raw addresses: keep every buffer it reads or
             # writes alive for as long as the graph may be replayed
-            self.graph, self.graph_buffers = g, (kt, bufs, means, lls)
+            self.graph, self.graph_buffers = g, (kt, bufs, means, lls, cools)
             g.replay()
         else:
             body()
@@ -595,13 +633,23 @@ def mop_grad(struct, thetas_array, J, alpha, keys, K=50, form="batched"):
     R = keys.shape[0]
     if thetas.shape[0] != R:
         raise ValueError("thetas and keys disagree on the number of rows")
+    ll, g = _mop_grad_dev(struct, torch.from_numpy(thetas).to(dev), J, alpha, _keys_tensor(keys, dev), K, form)
+    torch.cuda.current_stream().synchronize()
+    return ll.cpu().numpy(), g.cpu().numpy()
+
+
+def _mop_grad_dev(struct, th, J, alpha, kt, K=50, form="batched"):
+    """mp_mop_grad on device tensors: th (R, d) float32 estimation scale, kt
+    (R, 2) int32 view of the keys; returns device (loglik (R,) float64, grad
+    (R, d) float32), enqueued on the current stream without synchronising."""
+    import torch
     for u in struct.units:
         if not np.all(u.nstep == 1):
             raise ValueError("the MOP adjoint needs exactly one Euler step per observation interval")
+    dev = th.device
+    R = th.shape[0]
     ser = struct.device_series()
     md = struct.model.desc()
-    th = torch.from_numpy(thetas).to(dev)
-    kt = _keys_tensor(keys, dev)
     ll = torch.empty(R, dtype=torch.float64, device=dev)
     g = torch.empty((R, struct.model.d), dtype=torch.float32, device=dev)
     fid = _form_id(form)
@@ -610,8 +658,7 @@ def mop_grad(struct, thetas_array, J, alpha, keys, K=50, form="batched"):
     _native.check(_native.lib().mp_mop_grad(md, ser.desc, _native.ptr(th), struct.model.d, _native.ptr(kt), R, J,
                                             float(alpha), int(K), fid, _native.ptr(ll), _native.ptr(g),
                                             _native.ptr(ws), int(n), _native.stream_ptr()))
-    torch.cuda.current_stream().synchronize()
-    return ll.cpu().numpy(), g.cpu().numpy()
+    return ll, g
 
 
 class _HessBuffers:
@@ -812,14 +859,24 @@ class _MopFn:
             class MopFunction(torch.autograd.Function):
                 @staticmethod
                 def forward(ctx, theta_est, struct, J, alpha, keys, K):
-                    ll, g = mop_grad(struct, theta_est.detach().cpu().numpy(), J, alpha, keys, K)
-                    ctx.save_for_backward(torch.from_numpy(g).to(theta_est.device))
-                    return torch.from_numpy(ll).to(device=theta_est.device, dtype=torch.float64)
+                    # device-resident: theta and keys stay on the GPU and nothing
+                    # synchronises (keys given as a device int32/uint32 (n, 2)
+                    # tensor are used in place; NumPy keys are copied up once)
+                    dev = _native.require_device()
+                    th = theta_est.detach().to(device=dev, dtype=torch.float32).contiguous()
+                    if isinstance(keys, torch.Tensor):
+                        kt = keys.to(device=dev).contiguous().view(torch.int32).reshape(-1, 2)
+                    else:
+                        kt = _keys_tensor(keys, dev)
+                    ll, g = _mop_grad_dev(struct, th, J, alpha, kt, K)
+                    ctx.save_for_backward(g)
+                    return ll.to(theta_est.device)
 
                 @staticmethod
                 def backward(ctx, grad_out):
                     (g,) = ctx.saved_tensors
-                    return (grad_out[:, None].to(g.dtype) * g, None, None, None, None, None)
+                    gb = grad_out.to(device=g.device, dtype=g.dtype)[:, None] * g
+                    return (gb.to(grad_out.device), None, None, None, None, None)
 
             cls.fn = MopFunction
         return cls.fn

@@ -232,8 +232,10 @@ class Pomp:
 
     # -- algorithms ---------------------------------------------------------
     def pfilter(self, J, key=None, theta=None, thresh=0.0, reps=1, CLL=False, ESS=False,
-                filter_mean=False, prediction_mean=False, track_time=True, form="batched"):
-        """Bootstrap particle filter (estimation_mixin.py:101-272)."""
+                filter_mean=False, prediction_mean=False, track_time=True, form="auto"):
+        """Bootstrap particle filter (estimation_mixin.py:101-272); ``form`` as
+        in functional.pfilter ("auto": plain for a few filters of many
+        particles, batched otherwise)."""
         self._need_dmeas()
         start = time.time()
         thresh = float(max(0.0, thresh))

@@ -101,7 +101,7 @@ def test_lg2_matches_oracle(form):
     import pypomp_amd.functional as F
     struct, theta, ys = lg2_struct(1)
     keys = tf.split(tf.key(1234), 3)
-    flags = dict(CLL=True) if form == "plain" else dict(CLL=True, filter_mean=True, prediction_mean=True)
+    flags = dict(CLL=True, ESS=True, filter_mean=True, prediction_mean=True)
     out = F.pfilter(struct, theta[None], 1000, keys.reshape(1, 3, 2), form=form, **flags)
     model = olg.LGModel(olg.LGSpec(2, 2, separate_init=True))
     ser = fl.make_series(ys, np.arange(1, ys.shape[0] + 1.0), 0.0, nstep=1)
@@ -111,11 +111,12 @@ def test_lg2_matches_oracle(form):
         # log-likelihood is bit-exact
         assert float(out["logLik64"][0, r]) == o["logLik64"], (r, out["logLik64"][0, r], o["logLik64"])
         np.testing.assert_array_equal(out["CLL"][0, r], np.array(o["CLL"], dtype=np.float32))
-        if form == "batched":
-            # identical particle trajectories: same predicted and filtered means
-            np.testing.assert_allclose(out["prediction_mean"][0, r], np.array(o["prediction_mean"]),
-                                       rtol=1e-4, atol=1e-5)
-            np.testing.assert_allclose(out["filter_mean"][0, r], np.array(o["filter_mean"]), rtol=1e-4, atol=1e-5)
+        # identical particle trajectories: the same predicted and filtered means
+        # and ESS (float64 sums on the GPU, float32 sums in the oracle)
+        np.testing.assert_allclose(out["prediction_mean"][0, r], np.array(o["prediction_mean"]),
+                                   rtol=1e-5, atol=1e-5)
+        np.testing.assert_allclose(out["filter_mean"][0, r], np.array(o["filter_mean"]), rtol=1e-5, atol=1e-5)
+        np.testing.assert_allclose(out["ESS"][0, r], np.array(o["ESS"]), rtol=1e-5)
 
 
 def test_plain_and_batched_choose_the_same_ancestors():
@@ -408,3 +409,66 @@ def test_object_api_equals_the_functional_calls():
                                   np.asarray(Ys).reshape(-1, Ys.shape[-1]))
     np.testing.assert_array_equal(X.drop(columns=["theta_idx", "sim", "time"]).to_numpy(),
                                   np.asarray(Xs).reshape(-1, Xs.shape[-1]))
+
+
+@pytest.mark.parametrize("schedule", ["cosine", "hyperbolic", "custom"])
+@pytest.mark.parametrize("graph", [False, True])
+def test_mif_cooling_schedules_match_oracle(schedule, graph):
+    # the non-geometric schedules (core/rw_sigma.py:193-225, 308-376) reach the
+    # kernel as per-observation factor tables: traces and final swarm equal the
+    # oracle's mif, eagerly and replayed from a HIP graph
+    import pypomp_amd as pp
+    import pypomp_amd.functional as F
+    names = ["A11", "C11", "Q11", "R11", "X0_1"]
+    base = pp.RWSigma({n: 0.05 for n in names}, init_names=["X0_1"])
+    fn = lambda nt, m, ntimes: 0.8 ** (m + nt / ntimes)  # noqa: E731
+    rw = {"cosine": base.cosine_cooling(0.2, 4), "hyperbolic": base.hyperbolic_cooling(0.7),
+          "custom": base.custom_cooling(fn)}[schedule]
+    init = np.array([n == "X0_1" for n in names], np.float64)
+    kw = {"cosine": dict(c=0.2, M=4), "hyperbolic": dict(s=0.7), "custom": dict(fn=fn)}[schedule]
+    orw = fl.RWSigma(np.full(5, 0.05), init, a=0.5, cooling=schedule, **kw)
+    J, M = 40, 4
+    th = np.repeat(lg1_theta()[None, None, :], J, axis=1)
+    keys = tf.split(tf.key(44), 2)
+    lls, tr, final = F.mif(lg1_struct(), np.repeat(th, 2, axis=0), rw, M, J, keys, graph=graph)
+    model = olg.LGModel(olg.LGSpec(1, 1))
+    ser = fl.make_series(lg1_data(0), np.arange(1, 5.0), 0.0, nstep=1)
+    for r in range(2):
+        ell, etr, efinal = fl.mif(model, th[0], ser, keys[r], J, M, orw)
+        # the swarm bit for bit; its means (float64 sums here, float32 in the
+        # oracle) to the parity bar
+        np.testing.assert_array_equal(final[r], efinal)
+        assert_close_tol(tr[r], etr)
+        np.testing.assert_allclose(lls[r], ell, rtol=1e-5)
+    # another schedule gives another path
+    _, tr_geo, _ = F.mif(lg1_struct(), np.repeat(th, 2, axis=0), base.geometric_cooling(0.5), M, J, keys)
+    assert not np.array_equal(tr, tr_geo)
+
+
+def test_auto_form_runs_diagnostics_in_the_plain_form():
+    # Pomp.pfilter(J = 2^18, ESS, means) takes the plain kernels ("auto": a few
+    # filters of many particles) and equals the batched form
+    import pypomp_amd.functional as F
+    from pypomp_amd import _native
+    from pypomp_amd.models import LGModel
+    from pypomp_amd.series import Series
+    _, theta, ys = lg2_struct(2)
+    struct = F.PompStruct(model=LGModel(2, 2, separate_init=True),
+                          units=[Series.build(ys[:40], np.arange(1, 41.0), 0.0, nstep=1)])
+    J = 1 << 18
+    assert F.choose_form(2, J) == "plain" and F.choose_form(2, 4096) == "batched"
+    assert F.choose_form(512, J) == "batched" and F.choose_form(2, J, multi_unit=True) == "batched"
+    keys = tf.split(tf.key(61), 2).reshape(1, 2, 2)
+    flags = dict(CLL=True, ESS=True, filter_mean=True, prediction_mean=True)
+    _native.profile_enable(True)
+    _native.profile_read()
+    a = F.pfilter(struct, theta[None], J, keys, **flags)
+    prof = _native.profile_read()
+    _native.profile_enable(False)
+    assert prof["plain_prop"][1] == 2 * 40 and prof["batched"][1] == 0
+    b = F.pfilter(struct, theta[None], J, keys, form="batched", **flags)
+    np.testing.assert_array_equal(a["logLik64"], b["logLik64"])
+    np.testing.assert_array_equal(a["CLL"], b["CLL"])
+    for k in ("ESS", "filter_mean", "prediction_mean"):
+        np.testing.assert_allclose(a[k], b[k], rtol=1e-6, atol=1e-7)
+    assert np.all(a["ESS"] > 0) and np.all(a["ESS"] <= J * 1.0001)

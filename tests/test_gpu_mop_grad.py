@@ -80,6 +80,25 @@ def test_autograd_function_backward():
     np.testing.assert_allclose(t.grad.numpy(), -g / 50, rtol=1e-6)
 
 
+def test_autograd_function_on_device_tensors():
+    # theta and keys as device tensors: the forward stays on the GPU and the
+    # gradient lands on theta's device
+    import torch
+    import pypomp_amd.functional as F
+    struct, _ = sv_struct(50)
+    th = osv.SVModel().to_est(osv.default_theta())
+    keys = tf.split(tf.key(9), 2)
+    t = torch.tensor(np.stack([th, th]), dtype=torch.float32, device="cuda", requires_grad=True)
+    kd = torch.from_numpy(keys.view(np.int32)).cuda()
+    ll = F.mop_loglik_autograd(t, struct, 128, 0.97, kd, 10)
+    assert ll.device.type == "cuda" and ll.dtype == torch.float64
+    (-ll.sum() / 50).backward()
+    assert t.grad.device.type == "cuda"
+    v, g = F.mop_grad(struct, np.stack([th, th]), 128, 0.97, keys, K=10)
+    np.testing.assert_array_equal(ll.detach().cpu().numpy(), v)
+    np.testing.assert_allclose(t.grad.cpu().numpy(), -g / 50, rtol=1e-6)
+
+
 def test_workgroup_sizes_agree():
     # the value is bit-exact across workgroup sizes; the float64 gradient sums
     # follow the thread layout, so they agree to rounding

@@ -379,3 +379,38 @@ def test_newton_steps_follow_the_reference_arithmetic(name):
     np.testing.assert_allclose(d0[0].numpy(), [-0.5, -1.0, 0, 0, 0], atol=1e-6)
     with pytest.raises(ValueError, match="compute_hessian_fn"):
         opt.step(torch.ones((1, d)), opt.init_state(torch.zeros((1, d))), 0)
+
+
+def test_cooling_schedules_follow_the_reference():
+    # core/rw_sigma.py:193-225 and the reference's tests/test_core/test_rw_sigma_cooling.py:
+    # factor 1 at t = 0, the final value at t = M (cosine), 1 / (1 + s t) (hyperbolic)
+    import pickle
+    from oracle import filters as fl
+    rw = RWSigma({"a": 0.1, "b": 0.2}, init_names=["b"])
+    cos = rw.cosine_cooling(0.1, 10)
+    assert np.isclose(cos.cooling_factor(0, 0, 50), 1.0) and np.isclose(cos.cooling_factor(0, 10, 50), 0.1)
+    assert np.isclose(cos.cooling_factor(25, 4, 50), 0.1 + 0.9 * 0.5 * (1 + np.cos(np.pi * 4.5 / 10)), rtol=1e-6)
+    hyp = rw.hyperbolic_cooling(0.3)
+    assert np.isclose(hyp.cooling_factor(10, 2, 40), 1 / (1 + 0.3 * 2.25), rtol=1e-6)
+    cus = rw.custom_cooling(lambda nt, m, ntimes: 0.9 ** (m + nt / ntimes))
+    assert np.isclose(cus.cooling_factor(5, 1, 10), 0.9 ** 1.5, rtol=1e-6)
+    for bad in (lambda: rw.cosine_cooling(1.5, 10), lambda: rw.cosine_cooling(0.5, 0),
+                lambda: rw.hyperbolic_cooling(-1.0), lambda: rw.cooled(-1.0)):
+        with pytest.raises(ValueError):
+            bad()
+    np.testing.assert_allclose(rw.cooled(0.5).sigmas_all, [0.05, 0.1])
+    # the tables the kernel reads equal the oracle's restatement, row for row
+    # (the geometric schedule is evaluated in the kernel, with the oracle's d_pow)
+    o = dict(cosine=dict(c=0.1, M=10), hyperbolic=dict(s=0.3))
+    for kind, h in (("cosine", cos), ("hyperbolic", hyp)):
+        orw = fl.RWSigma(np.array([0.1, 0.2]), np.array([0.0, 1.0]), a=0.5, cooling=kind, **o.get(kind, {}))
+        for m in (0, 3):
+            got = h.cooling_factor(np.arange(37), m, 37)
+            exp = np.array([orw.cooling_factor(i, m, 37) for i in range(37)], np.float32)
+            np.testing.assert_array_equal(got, exp)
+    tab = cos.cooling_table(2, [3, 5])
+    np.testing.assert_array_equal(tab[:3], cos.cooling_factor(np.arange(3), 2, 3))
+    np.testing.assert_array_equal(tab[3:], cos.cooling_factor(np.arange(5), 2, 5))
+    assert rw.geometric_cooling(0.5).cooling_table(0, [3]) is None
+    back = pickle.loads(pickle.dumps(cus))
+    assert np.isclose(back.cooling_factor(5, 1, 10), 0.9 ** 1.5, rtol=1e-6)
