@@ -46,18 +46,25 @@ MEASURED_TRAFFIC_PANEL = 61.02e9
 # object also carries the VALU issue rate implied by the live kernel time.
 VALU_INSTR_PER_PSTEP = {"plain_prop": 667.0, "plain_scan": 153.0, "seir": 10289.0, "mop_grad": 762.0,
                         "panel_if2": 14436.0}
-# one wave64 VALU instruction per CU per cycle: 256 CUs x 2.4 GHz
-VALU_PEAK_GINSTR = 256 * 2.4
+# The VALU issue ceiling, measured in this run by the calibration probes
+# (valu_probe): the best of five instruction mixes (integer add/xor with FMAs
+# issues about 1.7 wave64 instructions per CU-cycle on gfx950, above one per
+# cycle) and the threefry mix (add/rotate/xor, about 1.05 per CU-cycle), the
+# ceiling of the random-number-bound kernels.  Set by main() before any
+# workload; the single-issue figure 256 CUs x 2.4 GHz stands in until then.
+VALU_PEAK = {"best": 256 * 2.4, "threefry": 256 * 2.4, "source": "single issue, 256 CUs x 2.4 GHz"}
 
 
 def valu_rate(instr_per_pstep, psteps, seconds):
     """VALU issue rate (wave64 Ginstr/s) of a kernel that ran `psteps` particle-steps
-    in `seconds`, from its measured instruction count per particle-step."""
+    in `seconds`, from its measured instruction count per particle-step, against
+    the calibrated ceilings."""
     if not seconds or instr_per_pstep is None:
         return None
     g = instr_per_pstep * psteps / 64.0 / seconds / 1e9
-    return {"achieved": g, "peak": VALU_PEAK_GINSTR, "unit": "Ginstr/s (wave64)", "frac": g / VALU_PEAK_GINSTR,
-            "instr_per_particle_step": instr_per_pstep}
+    return {"achieved": g, "peak": VALU_PEAK["best"], "unit": "Ginstr/s (wave64)", "frac": g / VALU_PEAK["best"],
+            "threefry_mix_peak": VALU_PEAK["threefry"], "frac_of_threefry_mix": g / VALU_PEAK["threefry"],
+            "peak_source": VALU_PEAK["source"], "instr_per_particle_step": instr_per_pstep}
 
 
 METRIC = "particle-steps/s (pfilter & MOP-grad, batched vs plain GPU); loglik rel err"
@@ -127,7 +134,7 @@ class BatchedSEIR:
     """Config: SEIR (7 Euler-binomial sub-steps/week), J=20000, T=260, 256 replicate
     filters in one persistent launch (batched form)."""
 
-    def __init__(self, J=20000, reps=256):
+    def __init__(self, J=20000, reps=256, form="batched"):
         import torch
         import pypomp_amd.functional as F
         from pypomp_amd import _native
@@ -147,10 +154,11 @@ class BatchedSEIR:
         self.keys = torch.from_numpy(np.ascontiguousarray(keys).view(np.int32)).to(self.dev)
         self.th = torch.from_numpy(np.ascontiguousarray(self.theta)).to(self.dev)
         self.ll = torch.empty(reps, dtype=torch.float64, device=self.dev)
-        n = _native.lib().mp_filter_workspace_size(_native.MP_ALGO_PF, self.md, self.ser.desc, reps, J,
-                                                   _native.MP_FORM_BATCHED)
+        self.fid = _native.MP_FORM_PLAIN if form == "plain" else _native.MP_FORM_BATCHED
+        n = _native.lib().mp_filter_workspace_size(_native.MP_ALGO_PF, self.md, self.ser.desc, reps, J, self.fid)
         self.ws = torch.empty(int(n), dtype=torch.uint8, device=self.dev)
         self.nws = int(n)
+        self.form = form
 
     @property
     def particle_steps(self):
@@ -160,7 +168,7 @@ class BatchedSEIR:
         from pypomp_amd import _native
         _native.check(_native.lib().mp_pfilter(
             self.md, self.ser.desc, _native.ptr(self.th), 0, _native.ptr(self.keys), None, self.R, self.J, 0.0,
-            0, _native.MP_FORM_BATCHED, _native.ptr(self.ll), None, None, None, None, _native.ptr(self.ws),
+            0, self.fid, _native.ptr(self.ll), None, None, None, None, _native.ptr(self.ws),
             self.nws, _native.stream_ptr()))
 
     def bytes_per_particle_step(self):
@@ -170,7 +178,8 @@ class BatchedSEIR:
 
     @property
     def measured_traffic(self):
-        return MEASURED_TRAFFIC_SEIR if (self.J == 20000 and self.R == 256 and self.T == 260) else None
+        full = self.J == 20000 and self.R == 256 and self.T == 260 and self.form == "batched"
+        return MEASURED_TRAFFIC_SEIR if full else None
 
 
 class MopGradSV:
@@ -315,6 +324,8 @@ def time_workload(wl, steps, warmup):
     prof = _native.profile_read()
     _native.profile_enable(False)
     ms, n = prof["batched"]
+    if n == 0 and prof["plain_prop"][1] > 0:  # plain form: propagate + scan per observation
+        ms, n = prof["plain_prop"][0] + prof["plain_scan"][0], prof["plain_prop"][1]
     if n == 0:  # launches inside a replayed graph are not event-bracketed
         ms, n = el / steps * 1e3, getattr(wl, "M", 1)
     avg = ms / max(n, 1)
@@ -336,10 +347,38 @@ def kalman_loglik(seed):
     return json.loads(path.read_text())[f"lg2_seed{seed}"]
 
 
-def cpu_baseline(target_s=12.0):
+def host_info():
+    """The CPU the baselines ran on: model name (lscpu's), logical CPUs, the ones
+    this process may use, and the OpenMP threads the C restatement starts."""
+    from oracle import cpu
+    model = None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+            "omp_threads": cpu.num_threads()}
+
+
+def median_runs(run, runs=5):
+    """Median of ``runs`` timed calls of ``run`` (after one untimed warm-up)."""
+    run()
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        run()
+        times.append(time.perf_counter() - t0)
+    return float(np.median(times)), times
+
+
+def cpu_baseline(target_s=2.5):
     """C restatement of the bootstrap filter (oracle/cpu_pf.c, OpenMP, one replicate
     per thread as the reference's CPU sharding) on a bounded sample of the workload:
-    as many J=1e6 replicates as threads, T calibrated to ~target_s seconds."""
+    as many J=1e6 replicates as threads, T calibrated to ~target_s seconds per run;
+    the median of 5 runs after a warm-up."""
     from oracle import cpu
     from oracle import threefry as tf
     st, theta, ys = lg2_struct(2)
@@ -350,32 +389,60 @@ def cpu_baseline(target_s=12.0):
     cpu.lg_pfilter(theta, 2, ys[:2], J, keys)
     per_step = (time.perf_counter() - t0) / 2
     T = int(max(2, min(ys.shape[0], target_s / max(per_step, 1e-6))))
-    t0 = time.perf_counter()
-    cpu.lg_pfilter(theta, 2, ys[:T], J, keys)
-    dt = time.perf_counter() - t0
+    dt, times = median_runs(lambda: cpu.lg_pfilter(theta, 2, ys[:T], J, keys))
     return {"value": cores * J * T / dt, "unit": "particle-steps/s", "cores": cores, "kind": "port",
             "sample": f"C restatement (oracle/cpu_pf.c, -O3 -march=x86-64-v3, OpenMP), LG 2-D pfilter, "
-                      f"{cores} replicates x J={J} x T={T} ({dt:.1f} s)"}
+                      f"{cores} replicates x J={J} x T={T}, median of 5 runs ({dt:.2f} s each)",
+            "runs_s": times, "host": host_info()}
 
 
-def cpu_mop_grad_baseline(J=1000, T=1000, reps=4):
-    """Torch (CPU, float64) autograd of the MOP estimator -- the reference's jax.grad
-    route restated (oracle/mop_grad.py) -- on a few replicates, one after another."""
-    import torch
-    from oracle import mop_grad as mg
+def cpu_mop_grad_baseline(J=1000, T=1000):
+    """C restatement of the MOP(0.97) value and hand adjoint gradient
+    (oracle/cpu_pf.c cpu_sv_mop_grad, pinned against the float64 autograd
+    oracle), OpenMP with one replicate per thread, two replicates per thread;
+    the median of 5 runs after a warm-up."""
+    from oracle import cpu
     from oracle import sv as osv
     from oracle import threefry as tf
     from tests.helpers import workloads
     ys = workloads()["sv_seed4_ys"][:T]
     th = osv.SVModel().to_est(osv.default_theta())
+    cores = cpu.num_threads()
+    reps = 2 * cores
     keys = tf.split(tf.key(4), reps)
-    t0 = time.perf_counter()
-    for r in range(reps):
-        mg.sv_mop_value_and_grad(th, ys, keys[r], J, 0.97)
-    dt = time.perf_counter() - t0
-    return {"value": reps * J * T / dt, "unit": "particle-steps/s", "cores": torch.get_num_threads(),
-            "kind": "port", "sample": f"NumPy forward + torch float64 autograd of MOP(0.97), SV, "
-                                      f"J={J}, T={T}, {reps} replicates ({dt:.1f} s)"}
+    dt, times = median_runs(lambda: cpu.sv_mop_grad(th, ys, J, 0.97, keys))
+    return {"value": reps * J * T / dt, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+            "sample": f"C restatement with the hand adjoint (oracle/cpu_pf.c, -O3 -march=x86-64-v3, OpenMP), "
+                      f"MOP(0.97) value + gradient, SV, J={J}, T={T}, {reps} replicates, median of 5 runs "
+                      f"({dt:.2f} s each)", "runs_s": times, "host": host_info()}
+
+
+# VALU instructions one wave issues per probe iteration (rocprofv3 SQ_INSTS_VALU
+# / waves / iterations, profiles/valu_peak_calibration.md): kind 0 FMA stream,
+# 1 threefry integer mix, 2 FMA + exp2.
+PROBE_VALU_PER_ITER = {0: 8.0, 1: 24.0, 2: 16.0, 3: 23.0, 4: 15.0}
+PROBE_KINDS = {0: "fma", 1: "int_threefry", 2: "fma_exp2", 3: "fma_int", 4: "int_add_xor"}
+
+
+def valu_probe(iters=100_000, blocks=2048):
+    """The VALU issue calibration kernels (mp_valu_probe) timed with HIP events:
+    instructions per second from the per-iteration counts the counters measured."""
+    import torch
+    from pypomp_amd import _native
+    out = torch.empty(blocks * 256, dtype=torch.float32, device="cuda")
+    res = {}
+    for kind, name in PROBE_KINDS.items():
+        _native.check(_native.lib().mp_valu_probe(kind, iters, blocks, _native.ptr(out), _native.stream_ptr()))
+        torch.cuda.synchronize()
+        _native.profile_enable(True)
+        _native.check(_native.lib().mp_valu_probe(kind, iters, blocks, _native.ptr(out), _native.stream_ptr()))
+        torch.cuda.synchronize()
+        ms, n = _native.profile_read()["valu_probe"]
+        _native.profile_enable(False)
+        waves = blocks * 256 // 64
+        g = PROBE_VALU_PER_ITER[kind] * iters * waves / (ms * 1e-3) / 1e9
+        res[name] = {"ms": ms, "Ginstr_per_s": g, "per_cu_cycle": g / (256 * 2.4)}
+    return res
 
 
 def run_only(args):
@@ -388,9 +455,13 @@ def run_only(args):
         dst = torch.empty_like(src)
         step = lambda: dst.copy_(src)  # noqa: E731
         work, unit = src.numel() * 4, "bytes copied"
+    elif args.only == "valu_peak":
+        print(json.dumps({"only": "valu_peak", "probe": valu_probe()}), flush=True)
+        return
     else:
         wl = {"plain": lambda: PlainLG(reps=args.reps), "seir": BatchedSEIR, "mop_batched": MopGradSV,
               "mop_plain": lambda: MopGradSV("plain", reps=16), "panel": PanelIF2,
+              "seir_plain": lambda: BatchedSEIR(reps=32, form="plain"),
               "panel_eager": lambda: PanelIF2(M=args.reps if args.reps < 50 else 50, graph=False)}[args.only]()
         step, work, unit = wl.step, wl.particle_steps, "particle-steps"
     for _ in range(args.warmup):
@@ -413,7 +484,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--reps", type=int, default=64)
     ap.add_argument("--no-extra", action="store_true", help="skip the batched workloads")
-    ap.add_argument("--only", choices=["plain", "seir", "mop_batched", "mop_plain", "panel", "panel_eager", "copy"],
+    ap.add_argument("--only", choices=["plain", "seir", "seir_plain", "mop_batched", "mop_plain", "panel",
+                                       "panel_eager", "copy", "valu_peak"],
                     help="run one workload --steps times and print a short JSON line (for profilers)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -426,6 +498,11 @@ def main():
     torch.cuda.set_device(0)
     if args.only:
         return run_only(args)
+    log("VALU issue calibration probes")
+    calib = valu_probe()
+    VALU_PEAK.update(best=max(v["Ginstr_per_s"] for v in calib.values()),
+                     threefry=calib["int_threefry"]["Ginstr_per_s"],
+                     source="mp_valu_probe in this run: best of 5 instruction mixes; threefry mix")
     wl = PlainLG(reps=args.reps)
     log(f"workload LG 2-D plain: J={wl.J} T={wl.T} reps={wl.R}")
     for _ in range(args.warmup):
@@ -472,7 +549,12 @@ def main():
     extra = {}
     if not args.no_extra:
         log("SEIR batched")
-        extra["seir_pfilter_batched"] = time_workload(BatchedSEIR(), 1, 1)
+        sb = time_workload(BatchedSEIR(), 1, 1)
+        log("SEIR plain (32 of the filters, one at a time)")
+        sp = time_workload(BatchedSEIR(reps=32, form="plain"), 1, 0)
+        sb["vs_plain"] = sb["value"] / sp["value"]
+        extra["seir_pfilter_batched"] = sb
+        extra["seir_pfilter_plain"] = sp
         log("MOP grad batched")
         mb = time_workload(MopGradSV("batched"), 2, 1)
         log("MOP grad plain (16 filters, one launch each)")
@@ -482,6 +564,7 @@ def main():
         extra["mop_grad_plain"] = mp_
         log("panel IF2 (100 SEIR units, 8 chains, J=2000, M=50, one HIP graph)")
         extra["panel_if2_graph"] = time_workload(PanelIF2(), 1, 1)
+
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     if cpu is not None and not args.no_extra:
         cg = cpu_mop_grad_baseline()
@@ -494,9 +577,12 @@ def main():
         "config": {"workload": "pfilter LG 2-D, J=1e6, T=200, 64 replicates one at a time (plain form)",
                    "J": J, "T": T, "replicates": R, "form": "plain"},
         "roofline": roofline, "cpu_baseline": cpu,
-        "loglik_rel_err": rel,  # mean over the replicates against the exact Kalman value
+        # Monte Carlo error of the replicates' mean log-likelihood against the exact
+        # FP64 Kalman value (an accuracy check, not parity with the reference)
+        "loglik_mc_rel_err_vs_kalman": rel,
         "loglik": {"mean": float(ll.mean()), "sd": float(ll.std(ddof=1)), "kalman_fp64": exact},
         "kernels": kernels,
+        "valu_calibration": calib,
         "extra": extra,
     }
     print(json.dumps(out), flush=True)

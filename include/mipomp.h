@@ -265,9 +265,19 @@ int32_t mp_tune_get(int32_t param);
  * No counterpart in the reference (it has no profiler hooks).  While enabled,
  * every kernel launch of the entry points above is bracketed by HIP events on
  * its stream; mp_profile_read synchronises on them and returns, per kind
- * (0 plain propagate, 1 plain scan, 2 batched filter, 3 key schedule), the
+ * (0 plain propagate, 1 plain scan, 2 batched filter, 3 key schedule,
+ * 4 VALU probe), the
  * summed milliseconds and launch counts, then clears the records. */
 int32_t mp_profile_enable(int32_t on);
+
+/* VALU issue calibration, no counterpart in the reference: blocks x 256
+ * threads each run iters rounds of eight independent dependency chains of
+ * one instruction mix (kind 0: float FMAs; 1: threefry's integer add, rotate
+ * and xor; 2: FMAs with base-2 exponentials; 3: FMAs with integer add and xor;
+ * 4: integer add and xor alone), writing one float per thread to
+ * out (blocks x 256).  Timed as kind 4 of mp_profile_read; with the VALU
+ * instruction counters it gives the issue ceiling the kernels are held to. */
+int32_t mp_valu_probe(int32_t kind, int64_t iters, int32_t blocks, float* out, void* stream);
 int32_t mp_profile_read(double* h_ms, int64_t* h_count, int32_t nkinds);
 
 #ifdef __cplusplus

@@ -24,6 +24,7 @@ def lib():
         vp = C.c_void_p
         L.cpu_lg_pfilter.argtypes = [vp, C.c_int, vp, C.c_int, C.c_int64, vp, C.c_int, vp]
         L.cpu_sv_mop.argtypes = [vp, vp, C.c_int, C.c_int64, C.c_float, vp, C.c_int, vp]
+        L.cpu_sv_mop_grad.argtypes = [vp, vp, C.c_int, C.c_int64, C.c_float, vp, C.c_int, vp, vp]
         L.cpu_num_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -51,6 +52,19 @@ def sv_mop(theta_est, ys, J, alpha, keys):
     rc = lib().cpu_sv_mop(_p(theta_est), _p(ys), ys.shape[0], J, float(alpha), _p(keys), keys.shape[0], _p(out))
     assert rc == 0
     return out
+
+
+def sv_mop_grad(theta_est, ys, J, alpha, keys):
+    """(loglik (R,), grad (R, 6)) of the SV MOP by the C hand adjoint."""
+    theta_est = np.ascontiguousarray(theta_est, np.float32)
+    ys = np.ascontiguousarray(np.asarray(ys, np.float32).ravel())
+    keys = np.ascontiguousarray(np.asarray(keys, np.uint32).reshape(-1, 2))
+    out = np.zeros(keys.shape[0], np.float64)
+    g = np.zeros((keys.shape[0], 6), np.float64)
+    rc = lib().cpu_sv_mop_grad(_p(theta_est), _p(ys), ys.shape[0], J, float(alpha), _p(keys), keys.shape[0],
+                               _p(out), _p(g))
+    assert rc == 0
+    return out, g
 
 
 def num_threads():

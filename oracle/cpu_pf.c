@@ -515,6 +515,157 @@ int cpu_sv_mop(const float* theta_est, const float* ys, int T, int64_t J, float 
   return err;
 }
 
+/* MOP(alpha) value and gradient of the SV model with respect to theta_est, by
+ * a hand-written reverse sweep (the derivative jax.grad takes through
+ * _mop_internal_mean, core/algorithms/ad_helpers.py:6-8, times -T).  The
+ * forward is sv_mop_one's and keeps every step's states v_t, noise z_t,
+ * densities m_t and ancestry a_t.  Value-wise the weights wP are one value for
+ * all particles, so with pi_t = softmax(m_t):
+ *   d loglik = sum_t sum_i (pi_t,i - 1/J) alpha delta_{t-1},i + pi_t,i dm_t,i,
+ *   delta_t,j = alpha delta_{t-1},a_t(j) + dm_t,a_t(j)      (wF's tangent)
+ * whose adjoint runs backwards: lambda_{t-1,i} = alpha (pi_t,i - 1/J +
+ * C_t,i), C_t,i = sum over the children j of i (a_t(j) = i) of lambda_t,j, and
+ * each density's weight is g_t,i = pi_t,i + C_t,i.  The state adjoint adds
+ * g dm/dv and phi times the children's state adjoints.  Sums in double. */
+static double sv_mop_grad_one(const float* th_est, const float* ys, int T, int64_t J, float alpha, key_t2 key,
+                              float* Vt, float* Zt, float* Mt, int32_t* At, float* V0s, float* Z0s, double* lam,
+                              double* lamc, double* xb, double* xbc, double* grad) {
+  const float mu = th_est[0], phi = d_tanh(th_est[1]), sig = d_exp(th_est[2]), rho = d_tanh(th_est[3]);
+  const float V0 = th_est[4], S0 = d_exp(th_est[5]);
+  const float omr2 = 1.0f - rho * rho, k = 1.0f / omr2, c0 = -0.918938533f - 0.5f * d_log(omr2);
+  key_t2 run = split_at(key, 0);
+  for (int64_t j = 0; j < J; ++j) {
+    Z0s[j] = normal_at(split_at(key, 1 + (uint64_t)j), 0);
+    V0s[j] = V0 + S0 * Z0s[j];
+  }
+  const double logJ = (double)d_log((float)J);
+  double ll = 0.0;
+  uint64_t* C = (uint64_t*)malloc(sizeof(uint64_t) * J);
+  float* m_max = (float*)malloc(sizeof(float) * T);
+  for (int t = 0; t < T; ++t) {
+    key_t2 sub = split_at(run, 1);
+    run = split_at(run, 0);
+    key_t2 sk = split_at(sub, 1);
+    float* v_t = Vt + (int64_t)t * J;
+    const float* v_p = t == 0 ? V0s : Vt + (int64_t)(t - 1) * J;
+    const int32_t* a_p = t == 0 ? NULL : At + (int64_t)(t - 1) * J;
+    float m = -INFINITY;
+    for (int64_t j = 0; j < J; ++j) {
+      float v = v_p[a_p ? a_p[j] : j];
+      float z = normal_at(split_at(sk, (uint64_t)j), 0);
+      v = mu + phi * (v - mu) + sig * z;
+      float u = ys[t] * d_exp(-0.5f * v) - rho * z;
+      float g = c0 - 0.5f * v - 0.5f * u * u * k;
+      v_t[j] = v;
+      Zt[(int64_t)t * J + j] = z;
+      Mt[(int64_t)t * J + j] = g;
+      if (g > m) m = g;
+    }
+    m_max[t] = m;
+    const uint64_t total = fixed_prefix(Mt + (int64_t)t * J, m, J, C);
+    ll += step_loglik(m, total, J) - logJ;
+    key_t2 rk = split_at(run, 1);
+    run = split_at(run, 0);
+    if (t < T - 1) resample(C, total, J, uniform01(bits_at(rk, 0)), At + (int64_t)t * J);
+  }
+  /* reverse sweep; grad in natural parameters (mu, phi, sigma, rho, V0, S0) */
+  double gn[6] = {0, 0, 0, 0, 0, 0};
+  const double kd = k, rhod = rho, phid = phi, mud = mu;
+  for (int64_t j = 0; j < J; ++j) {
+    lam[j] = 0.0;
+    xb[j] = 0.0;
+  }
+  for (int t = T - 1; t >= 0; --t) {
+    const float* m_t = Mt + (int64_t)t * J;
+    const float* v_t = Vt + (int64_t)t * J;
+    const float* z_t = Zt + (int64_t)t * J;
+    /* children sums through a_t of lambda_t and of the next states' adjoints */
+    for (int64_t j = 0; j < J; ++j) {
+      lamc[j] = 0.0;
+      xbc[j] = 0.0;
+    }
+    if (t < T - 1) {
+      const int32_t* a_t = At + (int64_t)t * J;
+      for (int64_t j = 0; j < J; ++j) {
+        lamc[a_t[j]] += lam[j];
+        xbc[a_t[j]] += phid * xb[j];
+      }
+    }
+    double s = 0.0;
+    for (int64_t j = 0; j < J; ++j) s += exp((double)m_t[j] - (double)m_max[t]);
+    const double iJ = 1.0 / (double)J;
+    for (int64_t j = 0; j < J; ++j) {
+      const double pi = exp((double)m_t[j] - (double)m_max[t]) / s;
+      const double g = pi + lamc[j];
+      /* density derivatives at (v, eta = z) */
+      const double v = v_t[j], z = z_t[j];
+      const double ry = (double)ys[t] * exp(-0.5 * v);
+      const double u = ry - rhod * z;
+      const double dm_dv = -0.5 + 0.5 * u * ry * kd;
+      gn[3] += g * (rhod * kd + u * z * kd - u * u * rhod * kd * kd);
+      /* state adjoint of v_t,j, then the step that made it */
+      const double xbar = g * dm_dv + xbc[j];
+      const double vprev = t == 0 ? (double)V0s[j] : (double)Vt[(int64_t)(t - 1) * J + At[(int64_t)(t - 1) * J + j]];
+      gn[0] += xbar * (1.0 - phid);
+      gn[1] += xbar * (vprev - mud);
+      gn[2] += xbar * z;
+      xb[j] = xbar;
+      lam[j] = t > 0 ? (double)alpha * (pi - iJ + lamc[j]) : 0.0;
+    }
+    if (t == 0) {
+      /* rinit: v_init = V0 + S0 z0, its adjoint phi times the first step's */
+      for (int64_t j = 0; j < J; ++j) {
+        const double xi = phid * xb[j];
+        gn[4] += xi;
+        gn[5] += xi * (double)Z0s[j];
+      }
+    }
+  }
+  /* chain rule to the estimation scale */
+  grad[0] = gn[0];
+  grad[1] = gn[1] * (1.0 - phid * phid);
+  grad[2] = gn[2] * (double)sig;
+  grad[3] = gn[3] * (1.0 - rhod * rhod);
+  grad[4] = gn[4];
+  grad[5] = gn[5] * (double)S0;
+  free(C);
+  free(m_max);
+  return ll;
+}
+
+/* R independent SV MOP value + gradient computations, one per OpenMP thread;
+ * grad (R, 6) d loglik / d theta_est. */
+int cpu_sv_mop_grad(const float* theta_est, const float* ys, int T, int64_t J, float alpha, const uint32_t* keys,
+                    int R, double* loglik, double* grad) {
+  int err = 0;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int r = 0; r < R; ++r) {
+    const int64_t TJ = (int64_t)T * J;
+    float* Vt = (float*)malloc(sizeof(float) * TJ);
+    float* Zt = (float*)malloc(sizeof(float) * TJ);
+    float* Mt = (float*)malloc(sizeof(float) * TJ);
+    int32_t* At = (int32_t*)malloc(sizeof(int32_t) * TJ);
+    float* V0s = (float*)malloc(sizeof(float) * J);
+    float* Z0s = (float*)malloc(sizeof(float) * J);
+    double* w = (double*)malloc(sizeof(double) * 4 * J);
+    if (!Vt || !Zt || !Mt || !At || !V0s || !Z0s || !w) {
+      err = 1;
+    } else {
+      key_t2 k = {keys[2 * r], keys[2 * r + 1]};
+      loglik[r] = sv_mop_grad_one(theta_est, ys, T, J, alpha, k, Vt, Zt, Mt, At, V0s, Z0s, w, w + J, w + 2 * J,
+                                  w + 3 * J, grad + 6 * (int64_t)r);
+    }
+    free(Vt);
+    free(Zt);
+    free(Mt);
+    free(At);
+    free(V0s);
+    free(Z0s);
+    free(w);
+  }
+  return err;
+}
+
 int cpu_num_threads(void) {
 #ifdef _OPENMP
   extern int omp_get_max_threads(void);

@@ -24,7 +24,7 @@ EXPORTS = [
     "mp_resample_workspace_size", "mp_resample_systematic",
     "mp_filter_workspace_size", "mp_pfilter", "mp_pfilter_noalloc", "mp_mop", "mp_perfilter",
     "mp_mop_grad_workspace_size", "mp_mop_grad", "mp_mop_hess_workspace_size", "mp_mop_hess", "mp_simulate", "mp_host_from_est",
-    "mp_profile_enable", "mp_profile_read", "mp_tune_set", "mp_tune_get",
+    "mp_profile_enable", "mp_profile_read", "mp_tune_set", "mp_tune_get", "mp_valu_probe",
 ]
 MP_TUNE_PLAIN_SCAN_ROWS, MP_TUNE_PLAIN_PROP_ROWS, MP_TUNE_BATCHED_BLOCK, MP_TUNE_GRAD_BLOCK = 1, 2, 3, 4
 MP_TUNE_PLAIN_SCAN_THREADS, MP_TUNE_PLAIN_PROP_THREADS, MP_TUNE_PLAIN_SCAN_SPIN = 5, 6, 7
@@ -103,6 +103,7 @@ def lib():
     L.mp_tune_get.argtypes = [i32]
     L.mp_profile_enable.argtypes = [i32]
     L.mp_profile_read.argtypes = [vp, vp, i32]
+    L.mp_valu_probe.argtypes = [i32, i64, i32, vp, vp]
     for n in EXPORTS:
         fn = getattr(L, n)
         if fn.restype is C.c_int or fn.restype is None:
@@ -142,7 +143,7 @@ def require_device():
     return torch.device("cuda", torch.cuda.current_device())
 
 
-PROFILE_KINDS = ("plain_prop", "plain_scan", "batched", "sched")
+PROFILE_KINDS = ("plain_prop", "plain_scan", "batched", "sched", "valu_probe")
 
 
 def profile_enable(on=True):

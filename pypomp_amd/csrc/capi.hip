@@ -37,6 +37,7 @@ bool mop_grad_supported(const ModelDesc& md);
 hipError_t launch_simulate(const SimArgs& a, hipStream_t st);
 size_t mop_grad_lds_bytes(int S, int J, int block, bool lds_work);
 hipError_t launch_mop_hess(const HessArgs& a, int grid, hipStream_t st);
+hipError_t launch_valu_probe(int kind, int64_t iters, int blocks, float* out, hipStream_t s);
 bool mop_hess_supported(const ModelDesc& md);
 }  // namespace mp
 
@@ -99,7 +100,7 @@ int g_grad_block = 0;       // MOP-adjoint workgroup size: 0 = automatic, 256, 5
 // Optional per-kernel timing (mp_profile_*): the events are attached to the
 // kernel's own dispatch packet (hipExtLaunchKernel through mp_launch), so they
 // time that kernel alone.  Kinds: 0 plain propagate, 1 plain scan, 2 batched
-// filter, 3 key schedule.
+// filter, 3 key schedule, 4 VALU calibration probe.
 struct ProfRec {
   int kind;
   hipEvent_t a, b;
@@ -894,6 +895,14 @@ int32_t mp_tune_get(int32_t param) {
     case MP_TUNE_PLAIN_XCD_MAP: return g_plain_xcd_map;
   }
   return -1;
+}
+
+int32_t mp_valu_probe(int32_t kind, int64_t iters, int32_t blocks, float* out, void* stream) {
+  if (kind < 0 || kind > 4) return fail(MP_ERR_ARG, "probe kind must be 0..4");
+  if (iters < 1 || blocks < 1 || !out) return fail(MP_ERR_ARG, "need iters >= 1, blocks >= 1 and an output");
+  hipStream_t st = (hipStream_t)stream;
+  MP_HIP(timed(4, st, [&] { return launch_valu_probe(kind, iters, blocks, out, st); }));
+  return MP_OK;
 }
 
 int32_t mp_profile_enable(int32_t on) {
