@@ -222,7 +222,8 @@ class MopGradSV:
     def step(self):
         from pypomp_amd import _native
         _native.check(_native.lib().mp_mop_grad(
-            self.md, self.ser.desc, _native.ptr(self.th), 0, _native.ptr(self.keys), self.R, self.J, 0.97, self.K,
+            self.md, self.ser.desc, _native.ptr(self.th), 0, _native.ptr(self.keys), None, self.R, self.J, 0.97,
+            self.K,
             self.fid, _native.ptr(self.ll), _native.ptr(self.g), _native.ptr(self.ws), self.nws,
             _native.stream_ptr()))
 

@@ -173,16 +173,20 @@ int32_t mp_mop(const mp_model_desc* h_model, const mp_series_desc* h_series, con
  * jax.grad computes through _mop_internal_mean (ad_helpers.py:6-8; the
  * reference's gradient is -grad / T), ancestry and random numbers held fixed.
  * The forward keeps a checkpoint every K observations; the reverse sweep
- * recomputes each K-step block from it.  One Euler step per observation,
- * U = 1.  loglik (R) float64, grad (R, d) float32.  form MP_FORM_PLAIN runs the
+ * recomputes each K-step block from it.  One Euler step per observation.
+ * unit (R) the panel unit of each filter (its own rows and length; the
+ * panel train, core/algorithms/train.py:191-415, differentiates one MOP per
+ * unit), or NULL for unit 0 of a one-unit series.  loglik (R) float64, grad
+ * (R, d) float32.  form MP_FORM_PLAIN runs the
  * filters one launch at a time.  Models with an adjoint: SV, and LG with
  * dx = dy = 1 or dx = dy = 2; others return MP_ERR_ARG.  The train loop
  * (core/algorithms/train.py:38-180) calls this once per gradient step. */
 size_t mp_mop_grad_workspace_size(const mp_model_desc* h_model, const mp_series_desc* h_series, int32_t R,
                                   int32_t J, int32_t K);
 int32_t mp_mop_grad(const mp_model_desc* h_model, const mp_series_desc* h_series, const float* theta,
-                    int64_t theta_stride, const uint32_t* keys, int32_t R, int32_t J, float alpha, int32_t K,
-                    int32_t form, double* loglik, float* grad, void* workspace, size_t ws_bytes, void* stream);
+                    int64_t theta_stride, const uint32_t* keys, const int32_t* unit, int32_t R, int32_t J,
+                    float alpha, int32_t K, int32_t form, double* loglik, float* grad, void* workspace,
+                    size_t ws_bytes, void* stream);
 
 /* Hessian of the MOP(alpha) log-likelihood with respect to theta_est: the
  * second derivative jax.hessian takes of _mop_internal_mean

@@ -426,3 +426,96 @@ def train(value_and_grad, theta_est0, key, T, M, eta, alpha, alpha_cooling=1.0, 
             th = (th + np.asarray(eta[m], F32) * direction).astype(F32)
         traces.append(th.copy())
     return np.array(nll, dtype=F32), np.stack(traces)
+
+
+def _adam_direction(grad, m1, v1, step, beta1=0.9, beta2=0.999, eps=1e-8):
+    """Adam (core/optimizer.py:119-153) on an array of any shape, float32."""
+    F32 = np.float32
+    m1 = (F32(beta1) * m1 + F32(1 - beta1) * grad).astype(F32)
+    v1 = (F32(beta2) * v1 + F32(1 - beta2) * grad * grad).astype(F32)
+    mh = (m1 / (F32(1) - F32(beta1) ** F32(step + 1))).astype(F32)
+    vh = (v1 / (F32(1) - F32(beta2) ** F32(step + 1))).astype(F32)
+    return (-mh / (np.sqrt(vh) + F32(eps))).astype(F32), m1, v1
+
+
+def panel_train(value_and_grad, shared0, unit0, perm, keys, T, M, eta_shared, eta_unit, alpha, alpha_cooling=1.0,
+                chunk_size=1, optimizer="adam", clip_norm=None, scale=False):
+    """One replicate of ``_panel_train_internal`` (core/algorithms/train.py:
+    190-415) on the estimation scale.  shared0 (ns,), unit0 (U, nu) in the
+    panel order (shared names, then unit names); ``perm`` maps the panel order
+    to the model's (theta_model = theta_panel[perm]); keys (M, U, 2), the MOP
+    key of unit u at iteration m.  ``value_and_grad(theta_model, key, alpha,
+    u)`` returns (loglik_u, d loglik_u / d theta_model).
+
+    Per iteration m, alpha_m = 1 - (1 - alpha) c_m (cosine cooling), and per
+    chunk of ``chunk_size`` units, in order: every unit of the chunk is
+    differentiated at the current shared values; the chunk loss is
+    sum_u (-loglik_u) / (chunk_size T), its shared gradient the sum over the
+    chunk, the unit gradients scaled by chunk_size (train.py:348-351); the
+    shared optimizer steps with the running chunk count, the unit optimizer
+    with m (train.py:357-360); shared += eta_shared[m] / n_chunks dir,
+    unit += eta_unit[m] dir.  The iteration's objective is the mean over
+    chunks of the chunk loss times T U.  Returns (neg_logliks (M+1,), NaN
+    first, shared (M+1, ns), unit (M+1, U, nu)), float32."""
+    F32 = np.float32
+    sh = np.asarray(shared0, F32).copy()
+    un = np.asarray(unit0, F32).copy()
+    U, nu = un.shape
+    ns = sh.shape[0]
+    cs = int(chunk_size)
+    n_chunks = U // cs
+    inv = np.argsort(perm)
+
+    def init(shape_rows, d):
+        if optimizer == "fmadam":
+            return np.zeros(shape_rows + (d,), F32), np.zeros(shape_rows + (d, d), F32)
+        return np.zeros(shape_rows + (d,), F32), np.zeros(shape_rows + (d,), F32)
+
+    def direction(grad, state, step):
+        m1, v1 = state
+        if optimizer == "sgd":
+            return -grad, state
+        if optimizer == "adam":
+            dr, m1, v1 = _adam_direction(grad, m1, v1, step)
+            return dr, (m1, v1)
+        if optimizer == "fmadam":
+            out = [fmadam_direction(grad[i], m1[i], v1[i], step) for i in range(grad.shape[0])]
+            return (np.stack([o[0] for o in out]), (np.stack([o[1] for o in out]), np.stack([o[2] for o in out])))
+        raise ValueError(f"Optimizer '{optimizer}' not supported for panel train")
+
+    st_sh = init((1,), ns)
+    st_un = [init((cs,), nu) for _ in range(n_chunks)]
+    step = 0
+    nll, sh_tr, un_tr = [np.nan], [sh.copy()], [un.copy()]
+    for m in range(M):
+        a_m = F32(F32(1) - (F32(1) - F32(alpha)) * cosine_cooling(m, M, alpha_cooling))
+        chunk_nll = []
+        for c in range(n_chunks):
+            units = range(c * cs, (c + 1) * cs)
+            res, g_sh, g_un = [], np.zeros(ns), []
+            for u in units:
+                th = np.concatenate([sh, un[u]])[perm]
+                ll, g = value_and_grad(th, keys[m, u], float(a_m), u)
+                gp = np.asarray(g, np.float64)[inv]           # d loglik / d theta_panel
+                res.append(F32(-ll))
+                g_sh = g_sh - gp[:ns] / (cs * T)
+                g_un.append(-gp[ns:] / (cs * T))
+            loss = F32(np.sum(np.array(res, F32), dtype=F32) / F32(cs * T))
+            chunk_nll.append(F32(loss * F32(T * U)))
+            g_sh = g_sh.astype(F32)[None]
+            g_un = (np.array(g_un) * cs).astype(F32)
+            if clip_norm is not None:
+                g_sh = np.clip(g_sh, -clip_norm, clip_norm).astype(F32)
+                g_un = np.clip(g_un, -clip_norm, clip_norm).astype(F32)
+            d_sh, st_sh = direction(g_sh, st_sh, step) if ns else (g_sh, st_sh)
+            d_un, st_un[c] = direction(g_un, st_un[c], m) if nu else (g_un, st_un[c])
+            if scale:
+                d_sh = (d_sh / np.maximum(np.linalg.norm(d_sh, axis=-1, keepdims=True), F32(1e-8))).astype(F32)
+                d_un = (d_un / np.maximum(np.linalg.norm(d_un, axis=-1, keepdims=True), F32(1e-8))).astype(F32)
+            sh = (sh + (np.asarray(eta_shared[m], F32) / F32(n_chunks)) * d_sh[0]).astype(F32)
+            un[c * cs:(c + 1) * cs] = (un[c * cs:(c + 1) * cs] + np.asarray(eta_unit[m], F32) * d_un).astype(F32)
+            step += 1
+        nll.append(F32(np.mean(np.array(chunk_nll, F32), dtype=F32)))
+        sh_tr.append(sh.copy())
+        un_tr.append(un.copy())
+    return np.array(nll, F32), np.stack(sh_tr), np.stack(un_tr)

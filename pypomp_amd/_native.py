@@ -90,8 +90,8 @@ def lib():
                                C.c_float, f32p, f32p, C.c_float, f32p, i32, f32p, f32p, f64p, vp, vp, sz, vp]
     L.mp_mop_grad_workspace_size.argtypes = [C.POINTER(ModelDesc), C.POINTER(SeriesDesc), i32, i32, i32]
     L.mp_mop_grad_workspace_size.restype = sz
-    L.mp_mop_grad.argtypes = [C.POINTER(ModelDesc), C.POINTER(SeriesDesc), f32p, i64, u32p, i32, i32, C.c_float,
-                              i32, i32, f64p, f32p, vp, sz, vp]
+    L.mp_mop_grad.argtypes = [C.POINTER(ModelDesc), C.POINTER(SeriesDesc), f32p, i64, u32p, vp, i32, i32,
+                              C.c_float, i32, i32, f64p, f32p, vp, sz, vp]
     L.mp_mop_hess_workspace_size.argtypes = [C.POINTER(ModelDesc), C.POINTER(SeriesDesc), i32, i32]
     L.mp_mop_hess_workspace_size.restype = sz
     L.mp_mop_hess.argtypes = [C.POINTER(ModelDesc), C.POINTER(SeriesDesc), f32p, i64, u32p, i32, i32, C.c_float,

@@ -440,14 +440,15 @@ GradPlan plan_grad(const mp_model_desc* m, const mp_series_desc* s, int R, int J
 }
 
 int run_mop_grad(const mp_model_desc* m, const mp_series_desc* s, const float* theta, int64_t theta_stride,
-                 const uint32_t* keys, int R, int J, float alpha, int K, double* loglik, float* grad, void* ws,
-                 size_t ws_bytes, hipStream_t st) {
+                 const uint32_t* keys, const int32_t* unit, int R, int J, float alpha, int K, double* loglik,
+                 float* grad, void* ws, size_t ws_bytes, hipStream_t st) {
   const GradPlan p = plan_grad(m, s, R, J, K);
   if (ws_bytes < p.total) return fail(MP_ERR_WORKSPACE, "workspace %zu < required %zu bytes", ws_bytes, p.total);
   char* base = align_ptr(ws);
   GradArgs a{};
   a.md = *reinterpret_cast<const ModelDesc*>(m);
   a.ser = *reinterpret_cast<const SeriesSet*>(s);
+  a.unit = unit;
   a.theta = theta;
   a.theta_stride = theta_stride;
   a.keys = keys;
@@ -462,7 +463,7 @@ int run_mop_grad(const mp_model_desc* m, const mp_series_desc* s, const float* t
   a.ws = base + p.sched_bytes;
   a.ws_stride = p.per_filter;
   a.lds_work = p.lds_work;
-  MP_HIP(timed(3, st, [&] { return launch_sched(ALGO_MOP, keys, a.ser, nullptr, p.sched_len,
+  MP_HIP(timed(3, st, [&] { return launch_sched(ALGO_MOP, keys, a.ser, unit, p.sched_len,
                                                 const_cast<uint32_t*>(a.sched), R, st); }));
   const int grid = R < num_cus() * 8 ? R : num_cus() * 8;
   hipError_t e = timed(2, st, [&] { return launch_mop_grad(a, p.block, grid, st); });
@@ -704,13 +705,14 @@ size_t mp_mop_grad_workspace_size(const mp_model_desc* h_model, const mp_series_
 }
 
 int32_t mp_mop_grad(const mp_model_desc* h_model, const mp_series_desc* h_series, const float* theta,
-                    int64_t theta_stride, const uint32_t* keys, int32_t R, int32_t J, float alpha, int32_t K,
-                    int32_t form, double* loglik, float* grad, void* workspace, size_t ws_bytes, void* stream) {
+                    int64_t theta_stride, const uint32_t* keys, const int32_t* unit, int32_t R, int32_t J,
+                    float alpha, int32_t K, int32_t form, double* loglik, float* grad, void* workspace,
+                    size_t ws_bytes, void* stream) {
   int rc = check_common(h_model, h_series, R, J, keys);
   if (rc) return rc;
   if (!theta || !loglik || !grad || !workspace) return fail(MP_ERR_ARG, "NULL theta/loglik/grad/workspace");
   if (K < 1) return fail(MP_ERR_ARG, "checkpoint interval K must be >= 1 (K = %d)", K);
-  if (h_series->U != 1) return fail(MP_ERR_ARG, "mp_mop_grad runs single-series filters (U = 1)");
+  if (h_series->U != 1 && !unit) return fail(MP_ERR_ARG, "a series of %d units needs a unit per filter", h_series->U);
   if (!mop_grad_supported(*reinterpret_cast<const ModelDesc*>(h_model)))
     return fail(MP_ERR_ARG, "model kind %d (dx %d, dy %d) has no adjoint", h_model->kind, h_model->dx, h_model->dy);
   if (h_series->max_euler != h_series->Tmax || h_series->max_nstep != 1)
@@ -720,13 +722,14 @@ int32_t mp_mop_grad(const mp_model_desc* h_model, const mp_series_desc* h_series
   hipStream_t st = (hipStream_t)stream;
   if (form == MP_FORM_PLAIN) {
     for (int f = 0; f < R; ++f) {
-      rc = run_mop_grad(h_model, h_series, theta + f * theta_stride, theta_stride, keys + 2 * f, 1, J, alpha, K,
-                        loglik + f, grad + (int64_t)f * h_model->d, workspace, ws_bytes, st);
+      rc = run_mop_grad(h_model, h_series, theta + f * theta_stride, theta_stride, keys + 2 * f,
+                        unit ? unit + f : nullptr, 1, J, alpha, K, loglik + f, grad + (int64_t)f * h_model->d,
+                        workspace, ws_bytes, st);
       if (rc) return rc;
     }
     return MP_OK;
   }
-  return run_mop_grad(h_model, h_series, theta, theta_stride, keys, R, J, alpha, K, loglik, grad, workspace,
+  return run_mop_grad(h_model, h_series, theta, theta_stride, keys, unit, R, J, alpha, K, loglik, grad, workspace,
                       ws_bytes, st);
 }
 

@@ -199,6 +199,7 @@ struct PlainArgs {
 struct GradArgs {
   ModelDesc md;
   SeriesSet ser;
+  const int32_t* unit; // (R) panel unit of each filter, or null (unit 0)
   const float* theta;  // (R, d) estimation scale
   int64_t theta_stride;
   const uint32_t* keys;

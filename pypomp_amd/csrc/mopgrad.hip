@@ -102,10 +102,15 @@ template <class M>
 __device__ __forceinline__ void mop_grad_filter(const GradArgs& a, const int f, char* buf, char* wbuf, double* red,
                                                 uint64_t* tab) {
   const ModelDesc md = a.md;
-  const int J = a.J, S = M::FIXED_S ? M::MAXS : md.S, d = md.d, T = a.ser.Tmax, K = a.K;
+  // the filter's panel unit: its own rows of the series, its own length; the
+  // scratch is laid out for the longest unit
+  const int u = a.unit ? a.unit[f] : 0;
+  const int64_t o0 = a.ser.obs_off[u];
+  const int T = (int)(a.ser.obs_off[u + 1] - o0);
+  const int J = a.J, S = M::FIXED_S ? M::MAXS : md.S, d = md.d, K = a.K;
   const int tid = threadIdx.x, B = blockDim.x;
   constexpr int NZ = M::NOISE > 0 ? M::NOISE : 1;
-  const GradScratch L = grad_scratch_layout(S, M::NOISE, J, T, K, a.lds_work);
+  const GradScratch L = grad_scratch_layout(S, M::NOISE, J, a.ser.Tmax, K, a.lds_work);
   const GradWork W = grad_work_layout(S, J);
   float* ckx = reinterpret_cast<float*>(buf + L.ckx);
   int32_t* ancl = reinterpret_cast<int32_t*>(buf + L.anc);
@@ -120,10 +125,10 @@ __device__ __forceinline__ void mop_grad_filter(const GradArgs& a, const int f, 
   float* lama = reinterpret_cast<float*>(wbuf + W.lama);
   int32_t* ap = reinterpret_cast<int32_t*>(wbuf + W.ap);  // this step's ancestry
   const int64_t SJ = (int64_t)S * J;
-  const float* ys = a.ser.ys;
-  const float* times = a.ser.times;
-  const int32_t* tidx0 = a.ser.tidx0;
-  const float* dts = a.ser.dt_ext;
+  const float* ys = a.ser.ys + o0 * md.dy;
+  const float* times = a.ser.times + o0;
+  const int32_t* tidx0 = a.ser.tidx0 + o0;
+  const float* dts = a.ser.dt_ext + a.ser.dt_off[u];
   const uint32_t* sch = a.sched + 2 * (int64_t)f * a.sched_len;
   const Key fkey = uniform_key(a.keys[2 * f], a.keys[2 * f + 1]);
   const float fJ = (float)J, logJ = d_log(fJ), invJ = 1.0f / fJ;

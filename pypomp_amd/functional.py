@@ -71,6 +71,19 @@ def _keys_tensor(keys, dev):
     return torch.from_numpy(k.view(np.int32)).to(dev)
 
 
+def _unit_tensor(unit, R, dev, U):
+    """(R,) int32 device tensor of panel unit indices in [0, U), or None."""
+    import torch
+    if unit is None:
+        if U != 1:
+            raise ValueError(f"a struct of {U} units needs the unit of every row")
+        return None
+    u = np.array(np.broadcast_to(np.asarray(unit, dtype=np.int32), (R,)))
+    if u.size and (u.min() < 0 or u.max() >= U):
+        raise ValueError(f"unit indices must lie in [0, {U})")
+    return torch.from_numpy(u).to(dev)
+
+
 def _workspace(algo, md, ser, R, J, form, dev):
     import torch
     n = _native.lib().mp_filter_workspace_size(algo, md, ser.desc, R, J, form)
@@ -281,8 +294,9 @@ def pmcmc(struct, thetas_array, proposal, M, J, keys, thresh=0.0, dprior=None):
     return ll_tr, lp_tr, nat_tr, accepts
 
 
-def mop(struct, thetas_array, J, alpha, keys, form="batched", return_float64=False):
-    """MOP(alpha) objective: thetas (n, d) on the estimation scale, keys (n, 2).
+def mop(struct, thetas_array, J, alpha, keys, form="batched", return_float64=False, unit=None):
+    """MOP(alpha) objective: thetas (n, d) on the estimation scale, keys (n, 2),
+    unit (n,) the panel unit each row filters (None: a one-unit struct).
     Returns the NEGATIVE log-likelihood estimates (n,), as the reference."""
     import torch
     dev = _native.require_device()
@@ -301,9 +315,10 @@ def mop(struct, thetas_array, J, alpha, keys, form="batched", return_float64=Fal
     fid = _form_id(form)
     ws, nws = _workspace(_native.MP_ALGO_MOP, md, ser, 1 if fid == _native.MP_FORM_PLAIN else R, J,
                          _native.MP_FORM_BATCHED, dev)
-    _native.check(_native.lib().mp_mop(md, ser.desc, _native.ptr(th), struct.model.d, _native.ptr(kt), None,
-                                       R, J, float(alpha), fid, _native.ptr(ll), _native.ptr(ws), nws,
-                                       _native.stream_ptr()))
+    ut = _unit_tensor(unit, R, dev, len(struct.units))
+    _native.check(_native.lib().mp_mop(md, ser.desc, _native.ptr(th), struct.model.d, _native.ptr(kt),
+                                       None if ut is None else _native.ptr(ut), R, J, float(alpha), fid,
+                                       _native.ptr(ll), _native.ptr(ws), nws, _native.stream_ptr()))
     torch.cuda.current_stream().synchronize()
     v = -ll.cpu().numpy()
     return v if return_float64 else v.astype(F32)
@@ -619,10 +634,11 @@ def panel_mif(struct, shared_array, unit_array, rw_sd, M, J, keys, thresh=0.0, n
     return sh_tr, un_tr, np.ascontiguousarray(fs), np.ascontiguousarray(fu)
 
 
-def mop_grad(struct, thetas_array, J, alpha, keys, K=50, form="batched"):
+def mop_grad(struct, thetas_array, J, alpha, keys, K=50, form="batched", unit=None):
     """MOP(alpha) value and gradient by the native adjoint.  thetas (n, d) on the
-    estimation scale, keys (n, 2).  Returns (loglik (n,) float64, grad (n, d)
-    float32) with grad = d loglik / d theta_est.  The reference's
+    estimation scale, keys (n, 2), unit (n,) the panel unit of each row (None:
+    a one-unit struct).  Returns (loglik (n,) float64, grad (n, d) float32)
+    with grad = d loglik / d theta_est.  The reference's
     ``_mop_internal_mean`` is -loglik / T (core/algorithms/mop.py:148-158)."""
     import torch
     dev = _native.require_device()
@@ -633,14 +649,16 @@ def mop_grad(struct, thetas_array, J, alpha, keys, K=50, form="batched"):
     R = keys.shape[0]
     if thetas.shape[0] != R:
         raise ValueError("thetas and keys disagree on the number of rows")
-    ll, g = _mop_grad_dev(struct, torch.from_numpy(thetas).to(dev), J, alpha, _keys_tensor(keys, dev), K, form)
+    ll, g = _mop_grad_dev(struct, torch.from_numpy(thetas).to(dev), J, alpha, _keys_tensor(keys, dev), K, form,
+                          _unit_tensor(unit, R, dev, len(struct.units)))
     torch.cuda.current_stream().synchronize()
     return ll.cpu().numpy(), g.cpu().numpy()
 
 
-def _mop_grad_dev(struct, th, J, alpha, kt, K=50, form="batched"):
+def _mop_grad_dev(struct, th, J, alpha, kt, K=50, form="batched", unit=None):
     """mp_mop_grad on device tensors: th (R, d) float32 estimation scale, kt
-    (R, 2) int32 view of the keys; returns device (loglik (R,) float64, grad
+    (R, 2) int32 view of the keys, unit (R,) int32 device tensor of panel units
+    or None (a one-unit struct); returns device (loglik (R,) float64, grad
     (R, d) float32), enqueued on the current stream without synchronising."""
     import torch
     for u in struct.units:
@@ -655,9 +673,10 @@ def _mop_grad_dev(struct, th, J, alpha, kt, K=50, form="batched"):
     fid = _form_id(form)
     n = _native.lib().mp_mop_grad_workspace_size(md, ser.desc, 1 if fid == _native.MP_FORM_PLAIN else R, J, K)
     ws = torch.empty(int(n), dtype=torch.uint8, device=dev)
-    _native.check(_native.lib().mp_mop_grad(md, ser.desc, _native.ptr(th), struct.model.d, _native.ptr(kt), R, J,
-                                            float(alpha), int(K), fid, _native.ptr(ll), _native.ptr(g),
-                                            _native.ptr(ws), int(n), _native.stream_ptr()))
+    _native.check(_native.lib().mp_mop_grad(md, ser.desc, _native.ptr(th), struct.model.d, _native.ptr(kt),
+                                            None if unit is None else _native.ptr(unit), R, J, float(alpha),
+                                            int(K), fid, _native.ptr(ll), _native.ptr(g), _native.ptr(ws), int(n),
+                                            _native.stream_ptr()))
     return ll, g
 
 
@@ -813,9 +832,9 @@ def train(struct, thetas_array, J, optimizer, M, eta, alpha, keys, alpha_cooling
     nll = np.full((n, M + 1), np.nan, dtype=F32)
     for m in range(M):
         a_m = float(F32(F32(1) - (F32(1) - F32(alpha)) * _cosine_cooling(m, M, alpha_cooling)))
-        _native.check(_native.lib().mp_mop_grad(md, ser.desc, _native.ptr(th), d, _native.ptr(kt[m]), n, J, a_m,
-                                                int(K), fid, _native.ptr(lls[m]), _native.ptr(g), _native.ptr(ws),
-                                                int(nws), _native.stream_ptr()))
+        _native.check(_native.lib().mp_mop_grad(md, ser.desc, _native.ptr(th), d, _native.ptr(kt[m]), None, n, J,
+                                                a_m, int(K), fid, _native.ptr(lls[m]), _native.ptr(g),
+                                                _native.ptr(ws), int(nws), _native.stream_ptr()))
         grad = -g / np.float32(T)  # gradient of the mean objective, -loglik / T
         if n_monitors > 1:
             means = th.cpu().numpy()
@@ -845,6 +864,126 @@ def train(struct, thetas_array, J, optimizer, M, eta, alpha, keys, alpha_cooling
     if n_monitors == 1:
         nll[:, 1:] = (-lls.cpu().numpy().T).astype(F32)
     return nll, torch.stack(traces, dim=1).cpu().numpy()
+
+
+def panel_train(struct, shared_array, unit_array, J, optimizer, M, eta, alpha, keys, alpha_cooling=1.0,
+                chunk_size=1, K=50):
+    """Gradient ascent on the panel MOP objective (functional/train.py:112-242;
+    core/algorithms/train.py:190-415).  shared_array (n, n_shared) and
+    unit_array (n, U, n_spec) natural scale, keys (n, M, U, 2): the MOP key of
+    unit u at iteration m.  Per iteration m (alpha_m by cosine cooling) the
+    units are taken ``chunk_size`` at a time, in order; each chunk is one
+    adjoint launch over its (replicate, unit) filters at the current shared
+    values.  The chunk loss is sum_u (-loglik_u) / (chunk_size T); its shared
+    gradient sums over the chunk and the unit gradients are scaled by
+    chunk_size; the shared optimizer steps with the running chunk count, the
+    unit optimizer with m; shared += eta_shared[m] / n_chunks direction, unit
+    += eta_spec[m] direction.  Only SGD, Adam and FullMatrixAdam are taken,
+    as the reference.  Returns (neg_logliks (n, M+1), NaN first, the mean over
+    chunks of the chunk loss times T U; shared history (n, M+1, n_shared);
+    unit history (n, M+1, U, n_spec)), natural scale, float32."""
+    import torch
+    from .optim import SGD, Adam, FullMatrixAdam
+    if not isinstance(optimizer, (SGD, Adam, FullMatrixAdam)):
+        raise ValueError(f"Optimizer '{type(optimizer).__name__}' not supported for panel train")
+    model = struct.model
+    U, ns, nu = struct.U, len(struct.shared_param_names), len(struct.unit_param_names)
+    sh = np.asarray(shared_array, dtype=F32)
+    un = np.asarray(unit_array, dtype=F32)
+    if un.ndim == 2:
+        sh, un = sh.reshape(1, -1), un[None]
+    n = un.shape[0]
+    if sh.shape != (n, ns) or un.shape != (n, U, nu):
+        raise ValueError("shared_array must be (n, n_shared) and unit_array (n, U, n_spec)")
+    keys = np.asarray(keys, dtype=np.uint32)
+    if keys.size != n * M * U * 2:
+        raise ValueError(f"keys must hold (n, M, U) = ({n}, {M}, {U}) keys")
+    cs = max(1, int(chunk_size))
+    if U % cs:
+        raise ValueError(f"chunk_size {cs} does not divide the {U} units")
+    n_chunks = U // cs
+    lengths = {u.T for u in struct.units}
+    if len(lengths) != 1:
+        raise ValueError("panel train needs units with the same number of observations")
+    T = lengths.pop()
+    for u in struct.units:
+        if not np.all(u.nstep == 1):
+            raise ValueError("the MOP adjoint needs exactly one Euler step per observation interval")
+    dev = _native.require_device()
+    perm = struct.perm
+    inv = np.argsort(perm)
+    pt = model if struct.par_trans is None else struct.par_trans
+    eta_sh = torch.from_numpy(np.asarray(eta.to_array(struct.shared_param_names, M), F32).reshape(M, ns)
+                              if ns else np.zeros((M, 0), F32)).to(dev)
+    eta_un = torch.from_numpy(np.asarray(eta.to_array(struct.unit_param_names, M), F32).reshape(M, nu)
+                              if nu else np.zeros((M, 0), F32)).to(dev)
+    full = np.concatenate([np.broadcast_to(sh[:, None, :], (n, U, ns)), un], axis=2)
+    full_est = pt.to_est(full[..., perm])[..., inv]
+    sh_t = torch.from_numpy(np.ascontiguousarray(full_est[:, 0, :ns])).to(dev)
+    un_t = torch.from_numpy(np.ascontiguousarray(full_est[:, :, ns:])).to(dev)
+
+    # filter rows of chunk c: (replicate r, unit c cs + j) at row r cs + j
+    R = n * cs
+    kc = np.ascontiguousarray(keys.reshape(n, M, n_chunks, cs, 2).transpose(1, 2, 0, 3, 4).reshape(
+        M, n_chunks, R, 2))
+    kt = torch.from_numpy(kc.view(np.int32)).to(dev)
+    rows = [torch.from_numpy(np.tile(np.arange(c * cs, (c + 1) * cs, dtype=np.int32), n)).to(dev)
+            for c in range(n_chunks)]
+    perm_t = torch.from_numpy(perm).to(dev)
+    inv_t = torch.from_numpy(inv).to(dev)
+    ser = struct.device_series()
+    md = model.desc()
+    d = model.d
+    nws = int(_native.lib().mp_mop_grad_workspace_size(md, ser.desc, R, J, K))
+    ws = torch.empty(nws, dtype=torch.uint8, device=dev)
+    lls = torch.empty((M, n_chunks, R), dtype=torch.float64, device=dev)
+    g = torch.empty((R, d), dtype=torch.float32, device=dev)
+    st_sh = optimizer.init_state(sh_t) if ns else None
+    st_un = [optimizer.init_state(un_t[:, c * cs:(c + 1) * cs].reshape(R, nu)) if nu else None
+             for c in range(n_chunks)]
+    scale = float(F32(cs * T))
+    sh_hist, un_hist = [sh_t.clone()], [un_t.clone()]
+    step = 0
+    for m in range(M):
+        a_m = float(F32(F32(1) - (F32(1) - F32(alpha)) * _cosine_cooling(m, M, alpha_cooling)))
+        for c in range(n_chunks):
+            th = torch.cat([sh_t.repeat_interleave(cs, dim=0), un_t[:, c * cs:(c + 1) * cs].reshape(R, nu)],
+                           dim=1).index_select(1, perm_t).contiguous()
+            _native.check(_native.lib().mp_mop_grad(md, ser.desc, _native.ptr(th), d, _native.ptr(kt[m, c]),
+                                                    _native.ptr(rows[c]), R, J, a_m, int(K), _native.MP_FORM_BATCHED,
+                                                    _native.ptr(lls[m, c]), _native.ptr(g), _native.ptr(ws), nws,
+                                                    _native.stream_ptr()))
+            gr = -g.index_select(1, inv_t) / scale  # d chunk loss / d theta, panel order
+            g_sh = gr[:, :ns].reshape(n, cs, ns).sum(dim=1)
+            g_un = gr[:, ns:] * float(cs)
+            if optimizer.clip_norm is not None:
+                g_sh = torch.clamp(g_sh, -optimizer.clip_norm, optimizer.clip_norm)
+                g_un = torch.clamp(g_un, -optimizer.clip_norm, optimizer.clip_norm)
+            if ns:
+                d_sh, st_sh = optimizer.step(g_sh, st_sh, step)
+                if optimizer.scale:
+                    d_sh = d_sh / torch.clamp(torch.linalg.norm(d_sh, dim=-1, keepdim=True), min=1e-8)
+                sh_t = sh_t + (eta_sh[m] / float(n_chunks))[None, :] * d_sh
+            if nu:
+                d_un, st_un[c] = optimizer.step(g_un, st_un[c], m)
+                if optimizer.scale:
+                    d_un = d_un / torch.clamp(torch.linalg.norm(d_un, dim=-1, keepdim=True), min=1e-8)
+                un_t = un_t.clone()
+                un_t[:, c * cs:(c + 1) * cs] += (eta_un[m][None, :] * d_un).reshape(n, cs, nu)
+            step += 1
+        sh_hist.append(sh_t.clone())
+        un_hist.append(un_t.clone())
+    torch.cuda.current_stream().synchronize()
+    # objective: per chunk sum_u (-loglik_u) / (cs T) times T U, float32, mean over chunks
+    res = (-lls.cpu().numpy()).astype(F32).reshape(M, n_chunks, n, cs)
+    chunk = (res.sum(axis=3, dtype=F32) / F32(cs * T) * F32(T * U)).astype(F32)
+    nll = np.full((n, M + 1), np.nan, dtype=F32)
+    nll[:, 1:] = chunk.mean(axis=1, dtype=F32).T
+    sh_tr = torch.stack(sh_hist, dim=1).cpu().numpy()
+    un_tr = torch.stack(un_hist, dim=1).cpu().numpy()
+    fullp = np.concatenate([np.broadcast_to(sh_tr[:, :, None, :], (n, M + 1, U, ns)), un_tr], axis=-1)
+    natp = pt.from_est(fullp[..., perm])[..., inv]
+    return nll, np.ascontiguousarray(natp[:, :, 0, :ns]), np.ascontiguousarray(natp[..., ns:])
 
 
 class _MopFn:

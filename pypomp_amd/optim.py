@@ -2,6 +2,7 @@
 core/learning_rate.py).  The update runs on device tensors (float32, as the
 reference), so an M-step run synchronises with the host only at its end."""
 
+from collections.abc import Mapping
 from dataclasses import dataclass
 
 import numpy as np
@@ -179,35 +180,161 @@ class WeightedNewton(Optimizer):
 
 
 class LearningRate:
-    """Per-parameter learning rates, constant or per-iteration schedules
-    (core/learning_rate.py)."""
+    """Per-parameter learning rates (core/learning_rate.py:11-365): each a
+    constant or a schedule of one length M, kept as ``rates_all_arr`` of shape
+    (d,) or (M, d) in the order the rates were given; ``to_array`` gives the
+    (M, k) float32 schedule of any k of the names, in the order asked, and the
+    decay methods multiply every rate by a per-iteration factor."""
 
     def __init__(self, rates):
-        if not isinstance(rates, dict) or not rates:
-            raise ValueError("rates must be a non-empty dictionary")
-        clean = {}
+        if not isinstance(rates, Mapping):
+            raise ValueError("rates must be a Mapping (e.g., dict)")
+        vals, lengths = [], set()
         for n, v in rates.items():
             if not isinstance(n, str):
-                raise ValueError("All keys in rates must be strings")
-            arr = np.asarray(v, dtype=np.float64)
-            if arr.ndim > 1 or np.any(arr < 0) or not np.all(np.isfinite(arr)):
-                raise ValueError(f"learning rate of '{n}' must be a non-negative float or 1-D schedule")
-            clean[n] = arr
-        self.param_names = tuple(clean)
-        self.rates = clean
+                raise ValueError("All keys in rates must be strings (parameter names)")
+            if isinstance(v, (int, float, np.number)):
+                vals.append(float(v))
+            elif isinstance(v, (list, np.ndarray)):
+                a = np.asarray(v, dtype=np.float64)
+                if a.ndim != 1:
+                    raise ValueError(f"Learning rate schedule for '{n}' must be 1D")
+                vals.append(a)
+                lengths.add(a.size)
+            else:
+                raise TypeError(f"Learning rate for '{n}' must be float or 1D sequence, got {type(v).__name__}")
+        if len(lengths) > 1:
+            raise ValueError(f"All 1D learning rate schedules must have the same length. Got lengths: "
+                             f"{sorted(lengths)}")
+        self.param_names = tuple(rates.keys())
+        if not vals:
+            self.rates_all_arr = np.empty((0,), np.float64)
+        elif lengths:
+            L = lengths.pop()
+            self.rates_all_arr = np.column_stack([np.full(L, v) if isinstance(v, float) else v for v in vals])
+        else:
+            self.rates_all_arr = np.asarray(vals, dtype=np.float64)
+
+    @classmethod
+    def _of(cls, names, arr):
+        out = object.__new__(cls)
+        out.param_names = tuple(names)
+        out.rates_all_arr = arr
+        return out
+
+    def _column(self, i):
+        a = self.rates_all_arr
+        if a.ndim == 1:
+            return float(a[i])
+        col = a[:, i]
+        return float(col[0]) if col.size and np.all(col == col[0]) else col
+
+    def _varying(self):
+        a = self.rates_all_arr
+        for i, n in enumerate(self.param_names):
+            if a.ndim == 2 and not np.all(a[:, i] == a[0, i]):
+                return n
+        return self.param_names[0] if self.param_names else "PARAM_NAMES_NONE"
 
     def to_array(self, param_names, M):
-        """(M, d) float32 schedule in the order ``param_names``."""
-        if set(param_names) != set(self.param_names):
-            raise ValueError("LearningRate parameter names must match the model's parameter names, "
-                             f"got {sorted(self.param_names)}, expected {sorted(param_names)}")
-        cols = []
+        """(M, len(param_names)) float32 schedule in the order ``param_names``;
+        every name must have a rate, a schedule must have length M."""
+        idx = {n: i for i, n in enumerate(self.param_names)}
         for n in param_names:
-            r = self.rates[n]
-            if r.ndim == 0:
-                cols.append(np.full(M, float(r)))
+            if n not in idx:
+                raise ValueError(f"Parameter '{n}' not found in learning rates")
+        order = np.array([idx[n] for n in param_names], dtype=np.int64)
+        a = self.rates_all_arr
+        if a.ndim == 1:
+            out = np.tile(a[order], (max(M, 1), 1))
+        else:
+            if a.shape[0] != M:
+                raise ValueError(f"Learning rate schedule for '{self._varying()}' has length {a.shape[0]}, "
+                                 f"expected M={M}")
+            out = a[:, order]
+        return np.ascontiguousarray(out, dtype=np.float32)
+
+    def _decayed(self, factor, M, kind):
+        a = self.rates_all_arr
+        if a.ndim == 2 and a.shape[0] != M:
+            raise ValueError(f"Cannot apply {kind} decay of length {M} to schedule of length {a.shape[0]} for "
+                             f"'{self._varying()}'")
+        return LearningRate._of(self.param_names, factor[:, None] * (a[None, :] if a.ndim == 1 else a))
+
+    def cosine_decay(self, final_factor, M):
+        """Rates times f + (1 - f) (1 + cos(pi m / M)) / 2, m = 0 .. M-1."""
+        if not 0 <= final_factor <= 1:
+            raise ValueError("final_factor should be between 0 and 1")
+        m = np.arange(M)
+        return self._decayed(final_factor + (1.0 - final_factor) * 0.5 * (1.0 + np.cos(np.pi * m / M)), M,
+                             "cosine")
+
+    def geometric_decay(self, decay_rate, M):
+        """Rates times decay_rate^m."""
+        if not 0 <= decay_rate <= 1:
+            raise ValueError("decay_rate should be between 0 and 1")
+        return self._decayed(decay_rate ** np.arange(M), M, "geometric")
+
+    def linear_decay(self, final_factor, M):
+        """Rates times a linear ramp from 1 to final_factor over M iterations."""
+        if not 0 <= final_factor <= 1:
+            raise ValueError("final_factor should be between 0 and 1")
+        return self._decayed(np.linspace(1.0, final_factor, M), M, "linear")
+
+    def hyperbolic_decay(self, decay_rate, M):
+        """Rates times 1 / (1 + decay_rate m)."""
+        if decay_rate < 0:
+            raise ValueError("decay_rate should be non-negative")
+        return self._decayed(1.0 / (1.0 + decay_rate * np.arange(M)), M, "hyperbolic")
+
+    @property
+    def rates(self):
+        return {n: self._column(i) for i, n in enumerate(self.param_names)}
+
+    def __getitem__(self, name):
+        if name not in self.param_names:
+            raise KeyError(f"Parameter '{name}' not found in learning rates.")
+        return self._column(self.param_names.index(name))
+
+    def __contains__(self, name):
+        return name in self.param_names
+
+    def __len__(self):
+        return len(self.param_names)
+
+    def __iter__(self):
+        return iter(self.param_names)
+
+    def keys(self):
+        return self.rates.keys()
+
+    def values(self):
+        return self.rates.values()
+
+    def items(self):
+        return self.rates.items()
+
+    def get(self, name, default=None):
+        return self[name] if name in self.param_names else default
+
+    def __eq__(self, other):
+        return (isinstance(other, LearningRate) and self.param_names == other.param_names
+                and np.array_equal(self.rates_all_arr, other.rates_all_arr))
+
+    __hash__ = None
+
+    def __str__(self):
+        parts = []
+        for n, v in self.rates.items():
+            if isinstance(v, float):
+                parts.append(f"'{n}': {v:.4g}")
+            elif v.size == 0:
+                parts.append(f"'{n}': []")
+            elif v.size <= 5:
+                parts.append(f"'{n}': [" + ", ".join(f"{x:.4g}" for x in v) + "]")
             else:
-                if r.shape[0] < M:
-                    raise ValueError(f"schedule of '{n}' is shorter than M = {M}")
-                cols.append(r[:M])
-        return np.stack(cols, axis=1).astype(np.float32)
+                parts.append(f"'{n}': [{v[0]:.4g} ... {v[-1]:.4g}] (len={v.size})")
+        return "LearningRate(\n    " + "\n    ".join(parts) + "\n)"
+
+    __repr__ = __str__
+

@@ -4,9 +4,10 @@ panel/estimation_mixin.py:384-773, panel/analysis_mixin.py:74-200).
 A panel holds unit ``Pomp`` objects of one native model with shared and
 unit-specific parameters (``PanelParameters``, one or more replicate sets).
 ``pfilter`` runs every (set, replicate, unit) filter in one launch; ``mif`` is
-panel IF2 (functional.panel_mif).  Keys follow the reference:
-``fresh_key``-derived key split into n_sets * reps * U filter keys for pfilter
-and into n_sets chain keys for mif.
+panel IF2 (functional.panel_mif); ``train`` is panel MOP gradient ascent
+(functional.panel_train).  Keys follow the reference: ``fresh_key``-derived
+key split into n_sets * reps * U filter keys for pfilter, into n_sets chain
+keys for mif and into n_sets * M * U MOP keys for train.
 """
 
 import copy
@@ -276,6 +277,38 @@ class PanelPomp:
         self.results_history.append(dict(method="mif", theta=theta, shared_traces=st, unit_traces=ut, units=units,
                                           J=J, M=M, key=old_key, time=time.time() - start))
 
+    def train(self, J, M, eta, chunk_size=1, optimizer=None, alpha=0.97, key=None, theta=None, alpha_cooling=1.0):
+        """Panel MOP gradient ascent (panel/estimation_mixin.py:775-990): keys
+        split(key, n_sets M U) as (n_sets, M, U), F.panel_train, traces with the
+        panel objective's log-likelihood in the shared traces' column 0 and NaN
+        in the unit traces' column 0; theta becomes the final values."""
+        from .optim import Adam, LearningRate
+        start = time.time()
+        optimizer = optimizer or Adam()
+        theta = self.theta if theta is None else theta
+        n = theta.num_replicates()
+        new_key, old_key = self._update_fresh_key(key)
+        if J < 1 or M < 1:
+            raise ValueError("J and M must be greater than 0.")
+        if not isinstance(eta, LearningRate):
+            raise TypeError("eta must be a LearningRate object")
+        units = self.get_unit_names()
+        U = len(units)
+        chunk_size = max(1, int(chunk_size))
+        sh, un = theta.arrays(units)
+        keys = prandom.split(new_key, n * M * U).reshape(n, M, U, 2)
+        nll, sh_h, un_h = F.panel_train(self.to_struct(), sh, un, J, optimizer, M, eta, alpha, keys,
+                                        alpha_cooling=alpha_cooling, chunk_size=chunk_size)
+        st = np.concatenate([-nll[:, :, None], sh_h], axis=-1)
+        ut = np.concatenate([np.full((n, M + 1, U, 1), np.nan, np.float32), un_h], axis=-1)
+        self.theta = PanelParameters.from_arrays(sh_h[:, -1], un_h[:, -1], self.canonical_shared_param_names,
+                                                 self.canonical_unit_param_names, units)
+        self.theta.logLik_unit = np.full((n, U), np.nan)
+        self.results_history.append(dict(method="train", theta=theta, shared_traces=st, unit_traces=ut, units=units,
+                                          J=J, M=M, eta=eta, optimizer=optimizer, alpha=alpha,
+                                          alpha_cooling=alpha_cooling, chunk_size=chunk_size, key=old_key,
+                                          time=time.time() - start))
+
     def results(self, index=-1):
         """Tidy frame: one row per (parameter set, unit) plus a 'shared' row with
         the panel total; columns theta_idx, unit, logLik, se, then parameters."""
@@ -301,23 +334,23 @@ class PanelPomp:
         return pd.DataFrame(rows)
 
     def traces(self):
-        """Long-format traces of every mif in the history: theta_idx, unit
-        ('shared' for the panel total and shared parameters), iteration, method,
-        logLik, se, parameters."""
+        """Long-format traces of every mif and train in the history: theta_idx,
+        unit ('shared' for the panel total and shared parameters), iteration,
+        method, logLik, se, parameters."""
         rows = []
         it0 = 0
         for r in self.results_history:
-            if r["method"] != "mif":
+            if r["method"] not in ("mif", "train"):
                 continue
             st, ut = r["shared_traces"], r["unit_traces"]
             for i in range(st.shape[0]):
                 for m in range(st.shape[1]):
-                    row = {"theta_idx": i, "unit": "shared", "iteration": it0 + m, "method": "mif",
+                    row = {"theta_idx": i, "unit": "shared", "iteration": it0 + m, "method": r["method"],
                            "logLik": float(st[i, m, 0]), "se": np.nan}
                     row.update({nm: float(st[i, m, 1 + k]) for k, nm in enumerate(self.canonical_shared_param_names)})
                     rows.append(row)
                     for u, name in enumerate(r["units"]):
-                        row = {"theta_idx": i, "unit": name, "iteration": it0 + m, "method": "mif",
+                        row = {"theta_idx": i, "unit": name, "iteration": it0 + m, "method": r["method"],
                                "logLik": float(ut[i, m, u, 0]), "se": np.nan}
                         row.update({nm: float(ut[i, m, u, 1 + k])
                                     for k, nm in enumerate(self.canonical_unit_param_names)})

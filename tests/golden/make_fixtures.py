@@ -9,7 +9,7 @@
   by the oracle with the reference's simulate key tree.  They are inputs to the
   filters, not expected outputs.
 
-Usage: python tests/golden/make_fixtures.py [--reference /root/reference]
+Usage: python tests/golden/make_fixtures.py [--reference /root/reference] [--regression-only]
        python tests/golden/make_fixtures.py --only sv[,lg,seir,panel]   (workloads only)
 """
 
@@ -46,7 +46,8 @@ def regression(ref):
     out = {}
     for name in ["test_simulate_regression", "test_pfilter_regression", "test_pfilter_diagnostics_regression",
                  "test_mop_regression", "test_mif_regression", "test_mif_full_trace_regression",
-                 "test_panel_pfilter_regression", "test_panel_mif_regression", "test_train_regression"]:
+                 "test_panel_pfilter_regression", "test_panel_mif_regression", "test_train_regression",
+                 "test_panel_train_regression"]:
         d = name.replace("_diagnostics", "").replace("_full_trace", "")
         path = base / d / f"{name}.csv"
         out[name] = read_csv(path)
@@ -79,7 +80,12 @@ def main():
     ap.add_argument("--reference", default="/root/reference")
     ap.add_argument("--only", default="", help="comma list of workloads to refresh (lg, seir, sv, panel); "
                     "keeps the others and the regression values")
+    ap.add_argument("--regression-only", action="store_true", help="rewrite reference_regression.json only")
     args = ap.parse_args()
+    if args.regression_only:
+        (HERE / "reference_regression.json").write_text(json.dumps(regression(args.reference), indent=1))
+        print("wrote", HERE / "reference_regression.json")
+        return
     only = set(filter(None, args.only.split(",")))
     arrays = {}
     if only:

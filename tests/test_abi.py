@@ -115,7 +115,7 @@ def test_mop_grad_rejects_a_grid_with_several_steps_in_one_interval(lib):
     md = _native.ModelDesc(kind=_native.MP_MODEL_SV, S=2, d=6, dy=1)
     sd = _native.SeriesDesc(ys=fake, times=fake, nstep=fake, tidx0=fake, dt_ext=fake, obs_off=fake, dt_off=fake,
                             t0=0.0, U=1, Tmax=3, max_euler=3, max_nstep=2)
-    rc = lib.mp_mop_grad(md, sd, fake, 6, fake, 4, 100, 0.9, 10, 0, fake, fake, fake, 1 << 20, None)
+    rc = lib.mp_mop_grad(md, sd, fake, 6, fake, None, 4, 100, 0.9, 10, 0, fake, fake, fake, 1 << 20, None)
     assert rc == 1
     assert b"one Euler step per observation interval" in lib.mp_last_error()
     md_spx = _native.ModelDesc(kind=_native.MP_MODEL_SPX, S=2, d=6, dy=1, vectorized=1)

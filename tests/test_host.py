@@ -414,3 +414,72 @@ def test_cooling_schedules_follow_the_reference():
     assert rw.geometric_cooling(0.5).cooling_table(0, [3]) is None
     back = pickle.loads(pickle.dumps(cus))
     assert np.isclose(back.cooling_factor(5, 1, 10), 0.9 ** 1.5, rtol=1e-6)
+
+
+def test_panel_train_argument_errors():
+    # checked on the host before any device work (core/algorithms/train.py:
+    # 196-199; panel/estimation_mixin.py:852-882)
+    import pandas as pd
+    import pypomp_amd as pp
+    import pypomp_amd.functional as F
+    from pypomp_amd.models import LGModel
+    units = [Series.build(np.zeros((4, 1)), np.arange(1, 5.0), 0.0, nstep=1) for _ in range(3)]
+    struct = F.PanelStruct(model=LGModel(1, 1), units=units, shared_param_names=["A11"],
+                           unit_param_names=["C11", "Q11", "R11", "X0_1"])
+    sh, un = np.ones((1, 1), np.float32), np.ones((1, 3, 4), np.float32)
+    eta = pp.LearningRate({n: 0.01 for n in LGModel(1, 1).param_names})
+    keys = np.zeros((1, 2, 3, 2), np.uint32)
+    with pytest.raises(ValueError, match="not supported for panel train"):
+        F.panel_train(struct, sh, un, 10, pp.BFGS(), 2, eta, 0.9, keys)
+    with pytest.raises(ValueError, match="does not divide"):
+        F.panel_train(struct, sh, un, 10, pp.Adam(), 2, eta, 0.9, keys, chunk_size=2)
+    with pytest.raises(ValueError, match="keys must hold"):
+        F.panel_train(struct, sh, un, 10, pp.Adam(), 2, eta, 0.9, keys[:, :1])
+    with pytest.raises(ValueError, match="shared_array must be"):
+        F.panel_train(struct, np.ones((1, 2), np.float32), un, 10, pp.Adam(), 2, eta, 0.9, keys)
+    uneven = F.PanelStruct(model=LGModel(1, 1), units=units[:2] + [Series.build(np.zeros((5, 1)), np.arange(1, 6.0),
+                                                                                  0.0, nstep=1)],
+                           shared_param_names=["A11"], unit_param_names=["C11", "Q11", "R11", "X0_1"])
+    with pytest.raises(ValueError, match="same number of observations"):
+        F.panel_train(uneven, sh, un, 10, pp.Adam(), 2, eta, 0.9, keys)
+    from pypomp_amd.models import LG
+    A = np.array([[0.9]])
+    P = pp.PanelPomp({"a": LG(A=A, ys=np.zeros((4, 1))), "b": LG(A=A, ys=np.ones((4, 1)))}, pp.PanelParameters([{
+        "shared": pd.DataFrame({"shared": [0.9, 1.0]}, index=["A11", "C11"]),
+        "unit_specific": pd.DataFrame({"a": [0.1, 0.3, 0.0], "b": [0.1, 0.3, 0.0]}, index=["Q11", "R11", "X0_1"])}]))
+    with pytest.raises(TypeError, match="LearningRate"):
+        P.train(J=10, M=2, eta=0.01, key=tf.key(1))
+    with pytest.raises(ValueError, match="greater than 0"):
+        P.train(J=0, M=2, eta=eta, key=tf.key(1))
+
+
+def test_learning_rate_schedules_and_lookup():
+    # core/learning_rate.py: subsets in any order, schedules of length M only,
+    # and the four decays as closed forms
+    import pypomp_amd as pp
+    lr = pp.LearningRate({"a": 0.1, "b": [0.1, 0.2, 0.3]})
+    np.testing.assert_array_equal(lr.to_array(["b"], 3)[:, 0], np.float32([0.1, 0.2, 0.3]))
+    np.testing.assert_array_equal(lr.to_array(["b", "a"], 3)[:, 1], np.float32([0.1] * 3))
+    assert lr["a"] == 0.1 and "b" in lr and len(lr) == 2 and lr.get("z", 7) == 7
+    with pytest.raises(ValueError, match="has length 3, expected M=4"):
+        lr.to_array(["a"], 4)
+    with pytest.raises(ValueError, match="not found in learning rates"):
+        lr.to_array(["z"], 3)
+    with pytest.raises(ValueError, match="same length"):
+        pp.LearningRate({"a": [0.1], "b": [0.1, 0.2]})
+    with pytest.raises(TypeError, match="must be float or 1D sequence"):
+        pp.LearningRate({"a": "x"})
+    base = pp.LearningRate({"a": 0.2, "b": 0.4})
+    M, m = 6, np.arange(6)
+    cases = {
+        "cosine": (base.cosine_decay(0.1, M), 0.1 + 0.9 * 0.5 * (1 + np.cos(np.pi * m / M))),
+        "geometric": (base.geometric_decay(0.5, M), 0.5 ** m),
+        "linear": (base.linear_decay(0.25, M), 1 - 0.75 * m / (M - 1)),
+        "hyperbolic": (base.hyperbolic_decay(2.0, M), 1 / (1 + 2.0 * m)),
+    }
+    for kind, (lrd, f) in cases.items():
+        np.testing.assert_allclose(lrd.to_array(["a", "b"], M), np.stack([0.2 * f, 0.4 * f], 1), rtol=1e-6,
+                                   err_msg=kind)
+    assert base.cosine_decay(0.1, M) == base.cosine_decay(0.1, M) != base
+    with pytest.raises(ValueError, match="Cannot apply geometric decay of length 5"):
+        base.cosine_decay(0.1, M).geometric_decay(0.5, 5)

@@ -167,3 +167,28 @@ def test_train(golden, lg1):
     g = golden["test_train_regression"]
     assert_close_tol(nll[-1], g["final_neg_loglik"])
     assert_close_tol(tr[-1], g["final_theta"])
+
+
+def test_panel_train(golden, lg1):
+    # the reference's F.panel_train on the lg_panel_struct fixture: units on
+    # data seeds 0 and 1, shared A11 and C11, three Adam steps, eta 0.01,
+    # alpha 0.97, chunk_size 1, keys split(key(20260811), M U); the panel
+    # transform is the identity, so the natural values are the estimation-scale
+    # start and the unit mechanics still apply the model's transforms
+    from oracle import mop_grad as mg
+    model, _ = lg1
+    data = [lg1_data(0), lg1_data(1)]
+
+    def vg(th, k, alpha, u):
+        val, _, g = mg.lg_mop_value_and_grad(model, th, data[u], k, 12, alpha)
+        return val, g
+
+    th = lg1_theta()
+    keys = tf.split(tf.key(20260811), 6).reshape(3, 2, 2)
+    nll, sh, un = mg.panel_train(vg, th[:2], np.stack([th[2:], th[2:]]), np.arange(5), keys, 4, 3,
+                                 np.full((3, 2), 0.01, np.float32), np.full((3, 3), 0.01, np.float32), 0.97)
+    g = golden["test_panel_train_regression"]
+    assert np.isnan(nll[0])
+    assert_close_tol(nll[-1], g["final_neg_loglik"])
+    assert_close_tol(sh[-1], g["final_shared"])
+    assert_close_tol(un[-1], g["final_unit"])
