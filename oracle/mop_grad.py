@@ -519,3 +519,30 @@ def panel_train(value_and_grad, shared0, unit0, perm, keys, T, M, eta_shared, et
         sh_tr.append(sh.copy())
         un_tr.append(un.copy())
     return np.array(nll, F32), np.stack(sh_tr), np.stack(un_tr)
+
+
+def panel_mop(value_and_grad, shared, unit, perm, keys, T, chunk_size=1):
+    """``_chunked_panel_mop_internal`` (core/algorithms/mop.py:183-270) and its
+    gradient (``_vg_chunked_panel_mop_internal``, :273-275) on the estimation
+    scale: theta_u = [shared, unit_u][perm] in the model's order, res_u =
+    -loglik_u (float32), the chunk sums added in order in float32 and the
+    total divided by U T.  ``value_and_grad(theta_model, key, u)`` returns
+    (loglik_u, d loglik_u / d theta_model).  Returns (value, d/d shared,
+    d/d unit)."""
+    F32 = np.float32
+    sh = np.asarray(shared, F32)
+    un = np.asarray(unit, F32)
+    U, ns = un.shape[0], sh.shape[0]
+    inv = np.argsort(perm)
+    res, g_sh, g_un = [], np.zeros(ns), []
+    for u in range(U):
+        ll, g = value_and_grad(np.concatenate([sh, un[u]])[perm], keys[u], u)
+        gp = -np.asarray(g, np.float64)[inv] / (U * T)
+        res.append(F32(-ll))
+        g_sh = g_sh + gp[:ns]
+        g_un.append(gp[ns:])
+    res = np.array(res, F32).reshape(U // chunk_size, chunk_size)
+    total = F32(0)
+    for c in range(res.shape[0]):
+        total = F32(total + res[c].sum(dtype=F32))
+    return F32(total / F32(U * T)), g_sh.astype(F32), np.array(g_un).astype(F32)

@@ -866,6 +866,46 @@ def train(struct, thetas_array, J, optimizer, M, eta, alpha, keys, alpha_cooling
     return nll, torch.stack(traces, dim=1).cpu().numpy()
 
 
+def panel_mop(struct, shared_array, unit_array, J, alpha, keys, chunk_size=1, grad=False, K=50):
+    """Panel MOP objective (core/algorithms/mop.py:183-275,
+    ``_chunked_panel_mop_internal`` and its value-and-gradient): shared
+    (n_shared,) and unit (U, n_spec) on the estimation scale, keys (U, 2), one
+    per unit.  Every unit's MOP filter runs in one launch; the value is
+    sum_u (-loglik_u) / (U T), summed chunk by chunk in float32 as the
+    reference's scan does.  With ``grad`` returns (value, d value / d shared
+    (n_shared,), d value / d unit (U, n_spec)) by the native adjoint.  Units
+    must have one length and ``chunk_size`` must divide U (the reference
+    reshapes the units into chunks)."""
+    import torch
+    U, ns, nu = struct.U, len(struct.shared_param_names), len(struct.unit_param_names)
+    sh = np.asarray(shared_array, dtype=F32).reshape(ns)
+    un = np.asarray(unit_array, dtype=F32).reshape(U, nu)
+    keys = np.asarray(keys, dtype=np.uint32).reshape(U, 2)
+    cs = int(chunk_size)
+    if cs < 1 or U % cs:
+        raise ValueError(f"chunk_size {cs} does not divide the {U} units")
+    lengths = {u.T for u in struct.units}
+    if len(lengths) != 1:
+        raise ValueError("the panel MOP needs units with the same number of observations")
+    T = lengths.pop()
+    perm = struct.perm
+    th = np.ascontiguousarray(np.concatenate([np.broadcast_to(sh[None], (U, ns)), un], axis=1)[:, perm])
+    unit = np.arange(U, dtype=np.int32)
+    if grad:
+        ll, g = mop_grad(struct, th, J, alpha, keys, K=K, unit=unit)
+    else:
+        ll = -mop(struct, th, J, alpha, keys, return_float64=True, unit=unit)
+    res = (-ll).astype(F32).reshape(U // cs, cs)
+    total = F32(0)
+    for c in range(U // cs):
+        total = F32(total + res[c].sum(dtype=F32))
+    value = F32(total / F32(U * T))
+    if not grad:
+        return value
+    gp = (-g.astype(np.float64)[:, np.argsort(perm)] / (U * T)).astype(F32)  # d value / d theta, panel order
+    return value, gp[:, :ns].sum(axis=0, dtype=F32), np.ascontiguousarray(gp[:, ns:])
+
+
 def panel_train(struct, shared_array, unit_array, J, optimizer, M, eta, alpha, keys, alpha_cooling=1.0,
                 chunk_size=1, K=50):
     """Gradient ascent on the panel MOP objective (functional/train.py:112-242;

@@ -148,3 +148,30 @@ def test_hessian_differs_from_differencing_the_moving_stop_gradient(sv_case):
             cols.append(t.grad.numpy().copy())
         moving[:, j] = (cols[0] - cols[1]) / (2 * h)
     assert not np.allclose(H, moving, rtol=1e-3, atol=1e-3 * np.abs(H).max())
+
+
+def test_panel_mop_reduces_to_the_unit_objective():
+    # one unit: the panel objective is the unit's -loglik / T and its gradient
+    # the unit's; two copies of a unit with one key: the same value, the
+    # shared gradient summed over both and each unit gradient halved
+    from oracle import lg as olg
+    from tests.helpers import lg1_data, lg1_theta
+    model = olg.LGModel(olg.LGSpec(1, 1))
+    ys = np.concatenate([lg1_data(0), lg1_data(1)])
+    th = model.to_est(lg1_theta())
+    key = tf.split(tf.key(3), 1)[0]
+
+    def vg(t, k, u):
+        val, _, g = mg.lg_mop_value_and_grad(model, t, ys, k, 50, 0.9)
+        return val, g
+
+    perm = np.array([0, 1, 2, 3, 4])
+    ll, g = vg(th, key, 0)
+    v1, s1, u1 = mg.panel_mop(vg, th[:2], th[None, 2:], perm, key[None], 8)
+    assert v1 == np.float32(np.float32(-ll) / np.float32(8))
+    np.testing.assert_allclose(s1, -g[:2] / 8, rtol=1e-6)
+    np.testing.assert_allclose(u1[0], -g[2:] / 8, rtol=1e-6)
+    v2, s2, u2 = mg.panel_mop(vg, th[:2], np.stack([th[2:], th[2:]]), perm, np.stack([key, key]), 8, chunk_size=2)
+    assert v2 == v1
+    np.testing.assert_allclose(s2, s1, rtol=1e-6)
+    np.testing.assert_allclose(u2, np.stack([u1[0], u1[0]]) / 2, rtol=1e-6)
