@@ -252,14 +252,15 @@ int32_t mp_perfilter(const mp_model_desc* h_model, const mp_series_desc* h_serie
  * query the size after setting).  The Python autotuner
  * (pypomp_amd/autotune.py) sweeps them on the caller's workload and caches the
  * best values. */
-#define MP_TUNE_PLAIN_SCAN_ROWS 1 /* consecutive particles per scan thread, 1..16 */
+#define MP_TUNE_PLAIN_SCAN_ROWS 1 /* consecutive particles per scan thread: 1, 2, 4, 8, 16 */
 #define MP_TUNE_PLAIN_PROP_ROWS 2 /* rows of 256 particles per propagate block, 1..16 */
 #define MP_TUNE_BATCHED_BLOCK 3   /* batched workgroup size: 0 automatic, 256, 512, 768 */
 #define MP_TUNE_GRAD_BLOCK 4      /* MOP-adjoint workgroup size: 0 automatic, 256, 512 */
 #define MP_TUNE_PLAIN_SCAN_THREADS 5 /* plain scan workgroup size: 256, 512, 1024 */
 #define MP_TUNE_PLAIN_PROP_THREADS 6 /* plain propagate workgroup size: 256, 512 */
-#define MP_TUNE_PLAIN_SCAN_SPIN 7    /* look-back probes before a scan tile computes a late
-                                        predecessor's aggregate itself, 0..2^20 (default 256) */
+#define MP_TUNE_PLAIN_LDS_TILES 7    /* scan tiles whose ends each propagate block forms in LDS,
+                                        0..1024 (default 1024); more tiles (or 0) take a separate
+                                        launch per step that writes them to global memory */
 #define MP_TUNE_PLAIN_XCD_MAP 8      /* 1 (default): propagate blocks take the particle chunks
                                         of the scan tiles on their own XCD; 0: block order */
 int32_t mp_tune_set(int32_t param, int32_t value);

@@ -89,9 +89,9 @@ int g_plain_scan_rows = 4;  // particles per scan thread (1..16)
 int g_plain_prop_rows = 2;  // particle rows per propagate block (1..16)
 int g_plain_scan_threads = 1024;  // scan workgroup size (256, 512, 1024)
 int g_plain_prop_threads = 512;  // propagate workgroup size (256, 512)
-// look-back probes a scan tile spends on a late predecessor before computing its
-// aggregate itself (0: at once, which forces the fallback path)
-int g_plain_scan_spin = 256;
+// scan tiles whose ends every propagate block forms in LDS (above it, or at 0,
+// a k_plain_ends launch per step writes them to global memory)
+int g_plain_lds_tiles = kMaxLdsTiles;
 // propagate blocks take the chunks of the scan tiles on their own XCD
 int g_plain_xcd_map = 1;
 int g_batched_block = 0;    // batched workgroup size: 0 = automatic, 256, 512, 768
@@ -253,9 +253,11 @@ BatchedPlan plan_batched(int algo, const mp_model_desc* m, const mp_series_desc*
 }
 
 struct PlainPlan {
-  int ntiles, rows, nprop, rows_prop, prop_len, scan_threads, prop_threads, xcd_q;
+  int ntiles, rows, tile_len, nprop, rows_prop, prop_len, scan_threads, prop_threads, xcd_q;
   int ndiag;
-  int64_t off_sched, off_params, off_x0, off_x1, off_lw, off_c, off_pmax, off_pmin, off_lb, off_st, off_diag, total;
+  bool global_ends;
+  int64_t off_sched, off_params, off_x0, off_x1, off_lw, off_c, off_agg, off_ends, off_pmax, off_pmin, off_mm,
+      off_steps, off_st, off_diag, total;
 };
 
 PlainPlan plan_plain(const mp_model_desc* m, const mp_series_desc* s, int R, int J) {
@@ -263,8 +265,15 @@ PlainPlan plan_plain(const mp_model_desc* m, const mp_series_desc* s, int R, int
   p.scan_threads = g_plain_scan_threads;
   p.prop_threads = g_plain_prop_threads;
   p.rows = g_plain_scan_rows;
+  // Past kMaxLdsTiles tiles the scan threads take more particles each (up to
+  // 16), so that propagate blocks can still form the tile ends in LDS.
+  const int lds_cap = std::min(g_plain_lds_tiles, kMaxLdsTiles);
+  while (p.rows < 16 && (int64_t)J > (int64_t)lds_cap * p.rows * p.scan_threads && lds_cap > 0) p.rows *= 2;
+  if (p.rows > 16) p.rows = 16;
   const int tile = p.rows * p.scan_threads;
+  p.tile_len = tile;
   p.ntiles = (J + tile - 1) / tile;
+  p.global_ends = p.ntiles > lds_cap;
   p.rows_prop = g_plain_prop_rows;
   p.nprop = (J + p.rows_prop * p.prop_threads - 1) / (p.rows_prop * p.prop_threads);
   // Whole rounds of resident workgroups (8 waves per SIMD: 2048 threads per
@@ -305,9 +314,12 @@ PlainPlan plan_plain(const mp_model_desc* m, const mp_series_desc* s, int R, int
   p.off_x1 = take((int64_t)m->S * J * 4);
   p.off_lw = take((int64_t)J * 4);
   p.off_c = take((int64_t)J * 8);
+  p.off_agg = take((int64_t)p.ntiles * 8);
+  p.off_ends = p.global_ends ? take((int64_t)p.ntiles * 8) : -1;
   p.off_pmax = take((int64_t)p.nprop * 4);
   p.off_pmin = take((int64_t)p.nprop * 4);
-  p.off_lb = take((int64_t)p.ntiles * 16);
+  p.off_mm = take(4 * sizeof(float));
+  p.off_steps = take((int64_t)std::max(1, ser.Tmax) * (int64_t)sizeof(PlainStep));
   p.off_st = take(sizeof(PlainState));
   // step diagnostics: one block per 1024 particles, at most 1024 blocks
   p.ndiag = (int)std::min<int64_t>(1024, ((int64_t)J + 1023) / 1024);
@@ -361,17 +373,21 @@ int run_plain_pf(const mp_model_desc* m, const mp_series_desc* s, const float* t
   a.X1 = reinterpret_cast<float*>(b + p.off_x1);
   a.lw = reinterpret_cast<float*>(b + p.off_lw);
   a.C = reinterpret_cast<uint64_t*>(b + p.off_c);
+  a.agg = reinterpret_cast<uint64_t*>(b + p.off_agg);
+  a.ends = p.global_ends ? reinterpret_cast<uint64_t*>(b + p.off_ends) : nullptr;
   a.pmax = reinterpret_cast<float*>(b + p.off_pmax);
   a.pmin = reinterpret_cast<float*>(b + p.off_pmin);
-  a.lb = reinterpret_cast<uint64_t*>(b + p.off_lb);
+  a.mm = reinterpret_cast<float*>(b + p.off_mm);
+  a.steps = reinterpret_cast<PlainStep*>(b + p.off_steps);
   a.ntiles = p.ntiles;
+  a.tile_len = p.tile_len;
+  a.tile_shift = __builtin_ctz((unsigned)p.tile_len);
   a.rows_prop = p.rows_prop;
   a.prop_len = p.prop_len;
   a.nprop = p.nprop;
   a.xcd_q = p.xcd_q;
   a.st = reinterpret_cast<PlainState*>(b + p.off_st);
   a.rows = p.rows;
-  a.scan_spin = g_plain_scan_spin;
   a.diag_part = reinterpret_cast<double*>(b + p.off_diag);
   a.ndiag = p.ndiag;
   const bool diag = (flags & (MP_ESS | MP_FILTER_MEAN | MP_PREDICTION_MEAN)) != 0;
@@ -398,6 +414,7 @@ int run_plain_pf(const mp_model_desc* m, const mp_series_desc* s, const float* t
       MP_HIP(timed(1, st, [&] { return plain_kernel(1, a, p.ntiles, p.scan_threads, st); }));
       if (diag) MP_HIP(plain_kernel(2, a, p.ntiles, 256, st));
     }
+    MP_HIP(plain_kernel(3, a, p.ntiles, 64, st));
   }
   return MP_OK;
 }
@@ -848,7 +865,8 @@ int32_t mp_perfilter(const mp_model_desc* h_model, const mp_series_desc* h_serie
 int32_t mp_tune_set(int32_t param, int32_t value) {
   switch (param) {
     case MP_TUNE_PLAIN_SCAN_ROWS:
-      if (value < 1 || value > 16) return fail(MP_ERR_ARG, "scan rows must be 1..16");
+      if (value < 1 || value > 16 || (value & (value - 1)))
+        return fail(MP_ERR_ARG, "scan rows must be 1, 2, 4, 8 or 16");
       g_plain_scan_rows = value;
       return MP_OK;
     case MP_TUNE_PLAIN_PROP_ROWS:
@@ -869,9 +887,10 @@ int32_t mp_tune_set(int32_t param, int32_t value) {
       if (value != 256 && value != 512) return fail(MP_ERR_ARG, "plain workgroup size must be 256 or 512");
       g_plain_prop_threads = value;
       return MP_OK;
-    case MP_TUNE_PLAIN_SCAN_SPIN:
-      if (value < 0 || value > (1 << 20)) return fail(MP_ERR_ARG, "scan spin limit must be 0..2^20");
-      g_plain_scan_spin = value;
+    case MP_TUNE_PLAIN_LDS_TILES:
+      if (value < 0 || value > kMaxLdsTiles)
+        return fail(MP_ERR_ARG, "LDS tile limit must be 0..%d", kMaxLdsTiles);
+      g_plain_lds_tiles = value;
       return MP_OK;
     case MP_TUNE_PLAIN_XCD_MAP:
       if (value != 0 && value != 1) return fail(MP_ERR_ARG, "XCD map must be 0 or 1");
@@ -894,7 +913,7 @@ int32_t mp_tune_get(int32_t param) {
     case MP_TUNE_GRAD_BLOCK: return g_grad_block;
     case MP_TUNE_PLAIN_SCAN_THREADS: return g_plain_scan_threads;
     case MP_TUNE_PLAIN_PROP_THREADS: return g_plain_prop_threads;
-    case MP_TUNE_PLAIN_SCAN_SPIN: return g_plain_scan_spin;
+    case MP_TUNE_PLAIN_LDS_TILES: return g_plain_lds_tiles;
     case MP_TUNE_PLAIN_XCD_MAP: return g_plain_xcd_map;
   }
   return -1;

@@ -131,26 +131,24 @@ __host__ __device__ inline FilterScratch filter_scratch_layout(int W, int J) {
   return s;
 }
 
-struct PlainState {
-  float llf;           // float32 log-likelihood term of the previous step
-  float ures;          // comb uniform of the previous step's resampling
-  uint64_t total;      // sum of fixed-point weights of the previous step
-  double loglik;       // running FP64 sum over t
-  int have_anc;        // previous step resampled
-  int uniform_w;       // weights are the constant wconst
-  float wconst;
-  int collapsed;       // previous step had all weights -inf
-  // Scalars of the step the next launches run, written by k_plain_reset for
-  // the first observation and by each scan's finishing tile for the next one,
-  // so that neither kernel starts with a chain of dependent loads.
+// Scalars of observation t of a plain-form filter, one row per t, built by
+// k_plain_reset before the first step, so that a kernel starts with one
+// independent load instead of a chain of dependent ones.
+struct PlainStep {
   int64_t obs;         // row of ys / times: obs_off[unit] + t
   int64_t base;        // schedule index of the first sub-step key: tidx0 + t
   int ti0;             // first Euler sub-step (index into the unit's dt_ext)
   int nst;             // Euler sub-steps
   float tstart, ty;    // start of the interval, observation time
   float dt0;           // first sub-step length (0 without sub-steps)
+  float ures_prev;     // comb uniform of step t - 1's resampling key (0 at t = 0)
   uint32_t k0, k1;     // first sub-step key
-  uint32_t r0, r1;     // resampling key
+};
+
+// Running state of a plain-form filter: the FP64 sum of the log-likelihood
+// terms of the steps finished so far, kept by block 0 of each propagate launch.
+struct PlainState {
+  double loglik;
 };
 
 // Words of one block's diagnostic sums: ESS, filter mean and prediction mean
@@ -174,10 +172,14 @@ struct PlainArgs {
   float* X0;
   float* X1;
   float* lw;
-  uint64_t* C;
-  float* pmax;          // (ntiles) per-tile max / min log-weight
+  uint64_t* C;          // (J) prefix sums of the fixed-point weights within each scan tile
+  uint64_t* agg;        // (ntiles) sum of each scan tile's fixed-point weights
+  uint64_t* ends;       // (ntiles) inclusive prefix of agg (k_plain_ends), or null: every
+                        // propagate block forms it in LDS from agg
+  float* pmax;          // (nprop) per propagate block max / min log-weight
   float* pmin;
-  uint64_t* lb;         // (2, ntiles) look-back words: state << 62 | prefix
+  float* mm;            // (2, 2) per step parity: max and min log-weight of the step
+  PlainStep* steps;     // (T) step scalars
   PlainState* st;
   double* loglik_out;   // (1) written after the last step
   float* cll;           // (T) or null
@@ -186,14 +188,19 @@ struct PlainArgs {
   float* pmean;         // (T, S) or null
   double* diag_part;    // (ndiag, kDiagWords) per-block diagnostic sums
   int ndiag;            // diagnostic blocks
-  int rows;             // particle rows per scan tile (tile = rows * blockDim)
+  int rows;             // particles per scan thread (tile = rows * scan workgroup size)
+  int tile_len;         // particles per scan tile (a power of two)
+  int tile_shift;       // log2(tile_len)
   int ntiles;           // scan tiles
   int rows_prop;        // particle rows per propagate block
   int prop_len;         // particles per propagate block (<= rows_prop * blockDim)
   int nprop;            // propagate blocks
   int xcd_q;            // > 0: propagate block b takes chunk xcd_chunk(b, xcd_q); 0: chunk b
-  int scan_spin;        // look-back probes before a tile computes a late predecessor's aggregate
 };
+
+// At most this many scan tiles have their ends formed in LDS by each
+// propagate block; more take a k_plain_ends launch per step.
+constexpr int kMaxLdsTiles = 1024;
 
 // MOP value + adjoint gradient launch (one workgroup per filter).
 struct GradArgs {

@@ -40,6 +40,13 @@ __device__ __forceinline__ void m_step(const ModelDesc& md, const typename M::P&
   }
 }
 
+// Whether a model's draw reads its prepared parameters (a model says so with
+// DRAW_USES_P = false; otherwise it is taken to read them).
+template <class M, class = void>
+struct DrawUsesP : std::true_type {};
+template <class M>
+struct DrawUsesP<M, std::void_t<decltype(M::DRAW_USES_P)>> : std::integral_constant<bool, M::DRAW_USES_P> {};
+
 // The sub-step noise of particle j (models whose noise depends on the key
 // only, NOISE > 0): m_draw + m_step_z == m_step.
 template <class M>

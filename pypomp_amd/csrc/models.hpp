@@ -125,9 +125,11 @@ struct LGT {
   // The step's noise depends on the particle's key only, not on its state:
   // kernels may draw it before the ancestor is known (draw + step_z == step).
   static constexpr int NOISE = MD;
+  // the draw reads the parameters (their dx) unless the size is exact
+  static constexpr bool DRAW_USES_P = !EXACT;
   static constexpr bool HOIST = false;  // no particle-independent step or density terms
   __device__ static void draw(const P& p, Key k, float* z) {
-    MP_UNROLL for (int i = 0; i < MD; ++i) z[i] = i < p.dx ? normal_at(k, i) : 0.f;
+    MP_UNROLL for (int i = 0; i < MD; ++i) z[i] = i < (EXACT ? MD : p.dx) ? normal_at(k, i) : 0.f;
   }
 
   __device__ static void step(const ModelDesc& md, const P& p, Key k, float t, float dt, float* x) {
@@ -336,6 +338,7 @@ struct SV {
 
   // the step's one normal depends on the particle's key only (draw + step_z == step)
   static constexpr int NOISE = 1;
+  static constexpr bool DRAW_USES_P = false;
   static constexpr bool HOIST = false;
   __device__ static void draw(const P&, Key k, float* z) { z[0] = normal_at(k, 0); }
 

@@ -2,41 +2,30 @@
 // two launches per observation (the shape of the reference's per-step XLA
 // loop body, pfilter.py:97-182):
 //
-//  k_plain_prop  -- ancestor search (systematic comb against the previous
-//                   step's prefix sums, staged per tile in LDS) fused with the
+//  k_plain_prop  -- finishes the previous step from its scan tiles' totals
+//                   (log-likelihood term, resampling decision), then the
+//                   ancestor search (systematic comb against the previous
+//                   step's prefix sums, staged per block in LDS) fused with the
 //                   gather, the Euler sub-steps, the measurement density and the
-//                   log-weight; per-tile max/min.
-//  k_plain_scan  -- fixed-point weights and a one-pass inclusive prefix sum
-//                   across tiles with decoupled look-back (bounded waits: a
-//                   tile computes a late predecessor's aggregate itself); the
-//                   tile with the last index finishes the step from the exact
-//                   total: the log-likelihood term (FP64 sum over t), the
-//                   resampling decision and the comb's uniform.
+//                   log-weight; per-block max / min.
+//  k_plain_scan  -- fixed-point weights and their inclusive prefix sums within
+//                   each scan tile, plus the tile's total.  Tiles do not wait
+//                   on each other: the prefix of the tiles before a position is
+//                   formed where it is needed, by the next propagate launch
+//                   (each block from the tile totals, in LDS).  Every tile
+//                   reduces the blocks' max log-weights (tile 0 also the
+//                   minima and publishes both for the next launches).
 //
-// Cross-workgroup traffic uses only relaxed agent-scope atomics on single
-// 64-bit words (look-back state and prefix packed together): on a multi-XCD
-// GPU an agent-scope release/acquire fence writes back / invalidates the whole
-// L2, which is what this design avoids.  Everything else a kernel reads was
-// written by an earlier launch.
+// Every value a kernel reads was written by an earlier launch; no workgroup
+// waits on another.
 #include "blockops.hpp"
 #include "engine.hpp"
 #include "mech.hpp"
 
 namespace mp {
 
-constexpr uint64_t kStateShift = 62;
-constexpr uint64_t kAggState = 1ull << kStateShift, kInclState = 2ull << kStateShift;
-constexpr uint64_t kValueMask = (1ull << kStateShift) - 1;
 constexpr int kWin = 2048;  // LDS search window (uint64 entries)
-constexpr int kMaxScanRows = 16;  // rows of 256 particles per scan tile, at most
-constexpr int kLookback = 4;      // look-back loads per lane and probe
-
-__device__ __forceinline__ uint64_t ld_relaxed(uint64_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_relaxed(uint64_t* p, uint64_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+constexpr int kMaxScanRows = 16;  // particles per scan thread, at most
 
 // Chunk of propagate block b when a scan tile holds q chunks: blocks go to the
 // kXcds XCDs round-robin, so within each group of kXcds * q blocks, block r
@@ -47,22 +36,126 @@ __device__ __forceinline__ int xcd_chunk(int b, int q) {
   return g * G + q * (r % kXcds) + r / kXcds;
 }
 
-// The step scalars of observation t (PlainState's step fields).
-__device__ void load_step(const PlainArgs& a, int t, PlainState& s) {
-  const int64_t obs = a.ser.obs_off[a.unit] + t;
-  const int ti0 = a.ser.tidx0[obs], nst = a.ser.nstep[obs];
-  const int64_t base = ti0 + (int64_t)t;
-  s.obs = obs;
-  s.base = base;
-  s.ti0 = ti0;
-  s.nst = nst;
-  s.tstart = t == 0 ? a.ser.t0 : a.ser.times[obs - 1];
-  s.ty = a.ser.times[obs];
-  s.dt0 = nst > 0 ? a.ser.dt_ext[a.ser.dt_off[a.unit] + ti0] : 0.0f;
-  s.k0 = a.sched[2 * base];
-  s.k1 = a.sched[2 * base + 1];
-  s.r0 = a.sched[2 * (base + nst)];
-  s.r1 = a.sched[2 * (base + nst) + 1];
+// A wave walks the scan tiles' totals in chunks of 256 tiles: lane l holds
+// tiles c0 + 4 l .. c0 + 4 l + 3 of the chunk at c0, its four loads in flight
+// together, and a wave scan of the lane sums orders the chunk.
+constexpr int kLaneTiles = 4, kChunkTiles = kLaneTiles * WAVE;
+
+__device__ __forceinline__ void load_chunk(const PlainArgs& a, int c0, uint64_t (&v)[kLaneTiles]) {
+  const int lane = threadIdx.x & (WAVE - 1);
+#pragma unroll
+  for (int i = 0; i < kLaneTiles; ++i) {
+    const int k = c0 + kLaneTiles * lane + i;
+    v[i] = k < a.ntiles ? a.agg[k] : 0;
+  }
+}
+
+// Sum of the scan tiles' fixed-point totals of a step (the exact total of its
+// weights), by one wave, given the first chunk's values `v` (load_chunk at 0).
+__device__ uint64_t wave_step_total(const PlainArgs& a, const uint64_t (&v)[kLaneTiles]) {
+  if (a.ends) return a.ends[a.ntiles - 1];
+  uint64_t tot = lane_u64(wave_incl_scan_u64(v[0] + v[1] + v[2] + v[3]), WAVE - 1);
+  for (int c0 = kChunkTiles; c0 < a.ntiles; c0 += kChunkTiles) {
+    uint64_t w[kLaneTiles];
+    load_chunk(a, c0, w);
+    tot += lane_u64(wave_incl_scan_u64(w[0] + w[1] + w[2] + w[3]), WAVE - 1);
+  }
+  return tot;
+}
+__device__ uint64_t wave_step_total(const PlainArgs& a) {
+  uint64_t v[kLaneTiles] = {};
+  if (!a.ends) load_chunk(a, 0, v);
+  return wave_step_total(a, v);
+}
+
+// What the end of step t fixes (the reference's _normalize_weights and the
+// resampling criterion, helpers.py:34-57, pfilter.py:136-159), from the exact
+// total of the step's fixed-point weights and its max / min log-weight.
+struct PlainFin {
+  double llt;      // log-likelihood term (float64)
+  float llf;       // the float32 term (the normaliser of the weights)
+  int collapsed;   // every weight was -inf
+  int res;         // the step resamples
+};
+
+__device__ PlainFin plain_finish(const PlainArgs& a, int t, uint64_t total) {
+  const int par = t & 1;
+  const float m = a.mm[2 * par], mn = a.mm[2 * par + 1];
+  const float logJ = d_log((float)a.J);
+  PlainFin f;
+  f.collapsed = (m == -__builtin_inff());
+  f.llt = step_loglik(m, total, shift_for(a.J));
+  f.llf = (float)f.llt;
+  const float nmax = f.collapsed ? -logJ : m - f.llf;
+  const float nmin = f.collapsed ? -logJ : mn - f.llf;
+  f.res = (nmax - nmin) > a.logthresh;
+  return f;
+}
+
+// Inclusive ends `e` of the lane's four tiles of a chunk, after `carry` (the
+// end of the chunks before it), and the chunk's total: one DPP wave scan.
+__device__ __forceinline__ uint64_t chunk_ends(const uint64_t (&v)[kLaneTiles], uint64_t carry,
+                                               uint64_t (&e)[kLaneTiles]) {
+  const uint64_t ls = v[0] + v[1] + v[2] + v[3];
+  const uint64_t inc = wave_incl_scan_u64(ls);
+  uint64_t x = carry + inc - ls;
+#pragma unroll
+  for (int i = 0; i < kLaneTiles; ++i) {
+    x += v[i];
+    e[i] = x;
+  }
+  return lane_u64(inc, WAVE - 1);
+}
+
+// #{i : C_i <= T} over the whole filter (its upper bound), by one wave, for
+// T < total.  C_i is the prefix within scan tile k = i / L plus the end of the
+// tiles before k.  With the ends in global memory a wave search finds k;
+// otherwise the wave forms the ends from the tile totals, chunk by chunk (`v0`
+// holds the first chunk): the ends <= T are the first k of them (they are
+// non-decreasing), counted by ballots, and the base of tile k is end k - 1.
+// Tiles past the last have total 0, so their ends equal the total and exceed T.
+__device__ int64_t wave_global_upper_bound(const PlainArgs& a, const uint64_t (&v0)[kLaneTiles], uint64_t T) {
+  int k;
+  uint64_t base;
+  if (a.ends) {
+    k = (int)wave_upper_bound_u64(a.ends, 0, a.ntiles, T);
+    base = k > 0 ? a.ends[k - 1] : 0;
+  } else {
+    uint64_t carry = 0;
+    for (int c0 = 0;; c0 += kChunkTiles) {
+      uint64_t v[kLaneTiles], e[kLaneTiles];
+      if (c0 == 0) {
+#pragma unroll
+        for (int i = 0; i < kLaneTiles; ++i) v[i] = v0[i];
+      } else {
+        load_chunk(a, c0, v);
+      }
+      const uint64_t ctot = chunk_ends(v, carry, e);
+      int n = 0;
+#pragma unroll
+      for (int i = 0; i < kLaneTiles; ++i) n += __popcll(__ballot(e[i] <= T));
+      k = c0 + n;
+      if (n < kChunkTiles) {
+        base = carry;
+        if (n > 0) {
+          const int l = (n - 1) / kLaneTiles, i = (n - 1) % kLaneTiles;
+          base = lane_u64(i == 0 ? e[0] : i == 1 ? e[1] : i == 2 ? e[2] : e[3], l);
+        }
+        break;
+      }
+      carry += ctot;
+    }
+  }
+  // T < total, so tile k exists and its last prefix (its end) exceeds T
+  const int64_t i0 = (int64_t)k << a.tile_shift, n = min((int64_t)a.tile_len, (int64_t)a.J - i0);
+  return i0 + wave_upper_bound_u64(a.C + i0, 0, n, T - base);
+}
+
+// The prefix sum of particle i over the whole filter: the within-tile prefix
+// plus the end of the tiles before its tile (`ends` in LDS or global memory).
+__device__ __forceinline__ uint64_t global_prefix(const PlainArgs& a, const uint64_t* ends, int64_t i) {
+  const int k = (int)(i >> a.tile_shift);
+  return (k > 0 ? ends[k - 1] : 0) + a.C[i];
 }
 
 template <class M>
@@ -70,10 +163,13 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
   __shared__ double red[32];
   __shared__ int64_t s_win[2];
   __shared__ uint64_t s_C[kWin];
+  __shared__ uint64_t s_end[kMaxLdsTiles];
+  __shared__ uint64_t s_total;
+  __shared__ PlainFin s_fin;
   const ModelDesc md = a.md;
   const int J = a.J, S = M::FIXED_S ? M::MAXS : md.S, t = a.t, tid = threadIdx.x, B = blockDim.x;
-  // the step state, with this step's scalars (row, grid, keys) in it
-  const PlainState st = *a.st;
+  // this step's scalars (row, grid, keys) and the previous step's comb uniform
+  const PlainStep st = a.steps[t];
   const int ti0 = st.ti0;
   const int nst = st.nst;
   const float* dts = a.ser.dt_ext + a.ser.dt_off[a.unit];
@@ -84,42 +180,95 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
   const float fJ = (float)J;
   const float logJ = d_log(fJ);
 
-  // parameters prepared once per filter (k_plain_prepare), staged in LDS
+  // Waves 0 to 3 finish the previous step below (a latency-bound chain of
+  // dependent loads): they issue at a raised priority, so that the other
+  // waves' VALU work on the CU does not delay it, and their first loads (the
+  // first chunk of scan tile totals, block 0's running log-likelihood) are in
+  // flight during the parameter staging.
+  const int wv = tid >> 6, lane = tid & 63;
+  const bool pro = t > 0 && wv < 4;
+  if (pro) __builtin_amdgcn_s_setprio(2);
+  uint64_t v0[kLaneTiles] = {};
+  if (pro && !a.ends) load_chunk(a, 0, v0);
+  const double ll_prev = (pro && blockIdx.x == 0 && wv == 2 && t > 1) ? a.st->loglik : 0.0;
+
+  // Parameters prepared once per filter (k_plain_prepare), staged in LDS.  The
+  // first barrier below publishes them; a model whose noise draw reads them
+  // waits for them here.
   __shared__ typename M::P s_P;
   if (tid == 0) s_P = *reinterpret_cast<const typename M::P*>(a.params);
-  __syncthreads();
+  if constexpr (DrawUsesP<M>::value) __syncthreads();
   const typename M::P& P = s_P;
 
   const float* Xc = (t & 1) ? a.X1 : a.X0;
   float* Xn = (t & 1) ? a.X0 : a.X1;
-  const float wconst = t == 0 ? d_log(1.0f / fJ) : st.wconst;
-  const bool uniform_w = t == 0 ? true : st.uniform_w;
-  const bool have_anc = t == 0 ? false : st.have_anc;
 
   float lmax = -__builtin_inff(), lmin = __builtin_inff();
   const int chunk = a.xcd_q > 0 ? xcd_chunk(blockIdx.x, a.xcd_q) : (int)blockIdx.x;
   const int64_t tile0 = (int64_t)chunk * a.prop_len;
   const int64_t tile1 = min((int64_t)J, tile0 + (int64_t)a.prop_len);
-  if (tile0 >= J) {  // a trailing block without particles
+  if (tile0 >= J) {  // a trailing block without particles (never block 0)
     if (tid == 0) {
       a.pmax[blockIdx.x] = lmax;
       a.pmin[blockIdx.x] = lmin;
     }
     return;
   }
-  // Ancestor search window of this tile.  A(t) = #{i : C_i <= t} is
-  // non-decreasing in the comb position, so every j of the tile has
-  // A(t_j) in [A(t_tile0), A(t_tile1-1)]; that slice of C is staged in LDS.
-  // Waves 0 and 1 find the two ends of the window concurrently.
-  const int wv = tid >> 6;
-  if (have_anc && wv < 2) {
-    const int64_t jj = wv == 0 ? tile0 : tile1 - 1;
-    const int64_t v = wave_upper_bound_u64(a.C, 0, J, comb_threshold(st.ures, jj, J, st.total));
-    if ((tid & 63) == 0) s_win[wv] = v;
+
+  // End of the previous step, t - 1 (from its scan tiles' totals):
+  //  * waves 0 and 1 find the two ends of this block's ancestor window:
+  //    A(T) = #{i : C_i <= T} is non-decreasing in the comb position, so every
+  //    j of the block has A(T_j) in [A(T_tile0), A(T_tile1-1)]; that slice of
+  //    C is staged in LDS below;
+  //  * wave 2 computes the step's log-likelihood term and resampling decision
+  //    (block 0 also adds the term to the running log-likelihood);
+  //  * wave 3 forms the tile ends in LDS for the staging.
+  // The searches run whether or not the step resamples (wave 2 decides that
+  // concurrently); their result is then unused.
+  if (t > 0) {
+    if (wv < 3) {
+      const uint64_t total = wave_step_total(a, v0);
+      if (wv < 2) {
+        const int64_t jj = wv == 0 ? tile0 : tile1 - 1;
+        const uint64_t T = comb_threshold(st.ures_prev, jj, J, total);
+        // every prefix sum is <= total
+        const int64_t v = T >= total ? J : wave_global_upper_bound(a, v0, T);
+        if (lane == 0) s_win[wv] = v;
+      } else {
+        const PlainFin f = plain_finish(a, t - 1, total);
+        if (lane == 0) {
+          s_fin = f;
+          s_total = total;
+          if (blockIdx.x == 0) {
+            a.st->loglik = ll_prev + f.llt;
+            if (a.cll) a.cll[t - 1] = f.llf;
+          }
+        }
+      }
+    } else if (wv == 3 && !a.ends) {
+      uint64_t carry = 0;
+      for (int c0 = 0; c0 < a.ntiles; c0 += kChunkTiles) {
+        uint64_t v[kLaneTiles], e[kLaneTiles];
+        if (c0 == 0) {
+#pragma unroll
+          for (int i = 0; i < kLaneTiles; ++i) v[i] = v0[i];
+        } else {
+          load_chunk(a, c0, v);
+        }
+        carry += chunk_ends(v, carry, e);
+#pragma unroll
+        for (int i = 0; i < kLaneTiles; ++i) {
+          const int k = c0 + kLaneTiles * lane + i;
+          if (k < a.ntiles) s_end[k] = e[i];
+        }
+      }
+    }
   }
+  if (pro) __builtin_amdgcn_s_setprio(0);
   // Models whose step noise depends on the particle's key alone draw it for
   // the first kPreRows rows here: the other waves do this VALU work while
-  // waves 0 and 1 wait on the search's dependent loads.
+  // waves 0 to 3 wait on their loads (and all of them while the parameters
+  // arrive, when the draw does not read them).
   constexpr int kPreRows = 2;
   constexpr int NZ = M::NOISE > 0 ? M::NOISE : 1;
   const bool pre = M::NOISE > 0 && nst == 1;
@@ -131,11 +280,23 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
       if (r < a.rows_prop && j < tile1) m_draw<M>(md, P, sk, (uint64_t)j, zpre[r]);
     }
   }
-  if (have_anc) __syncthreads();
+  __syncthreads();
+  const bool have_anc = t > 0 && s_fin.res;
+  const bool uniform_w = t == 0 || s_fin.res;
+  const bool collapsed = t > 0 && s_fin.collapsed;
+  const float llf = t > 0 ? s_fin.llf : 0.0f;
+  const float wconst = t == 0 ? d_log(1.0f / fJ) : 0.0f - logJ;
+  const uint64_t total = t > 0 ? s_total : 0;
+  const uint64_t* ends = a.ends ? a.ends : s_end;
   const int64_t wlo = have_anc ? s_win[0] : 0, whi = have_anc ? s_win[1] : 0;
   const bool in_lds = have_anc && (whi - wlo) <= kWin;
   if (in_lds) {
-    for (int64_t i = tid; i < whi - wlo; i += B) s_C[i] = a.C[wlo + i];
+    const int n = (int)(whi - wlo);
+    if (a.ends) {
+      for (int i = tid; i < n; i += B) s_C[i] = global_prefix(a, a.ends, wlo + i);
+    } else {
+      for (int i = tid; i < n; i += B) s_C[i] = global_prefix(a, s_end, wlo + i);
+    }
     __syncthreads();
   }
   auto row = [&](int r, bool use_z, const float (&zp)[NZ]) {
@@ -148,14 +309,24 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
                  cov_row(a.ser, a.unit, 0));
     } else {
       if (have_anc) {
-        const uint64_t tj = comb_threshold(st.ures, j, J, st.total);
-        const int64_t an = in_lds ? wlo + (whi > wlo ? upper_bound_fixed(s_C, (int)(whi - wlo), tj) : 0)
-                                  : upper_bound_u64(a.C, wlo, whi, tj);
+        const uint64_t tj = comb_threshold(st.ures_prev, j, J, total);
+        int64_t an;
+        if (in_lds) {
+          an = wlo + (whi > wlo ? upper_bound_fixed(s_C, (int)(whi - wlo), tj) : 0);
+        } else {
+          int64_t lo = wlo, hi = whi;
+          while (lo < hi) {
+            const int64_t mid = lo + ((hi - lo) >> 1);
+            if (global_prefix(a, ends, mid) <= tj) lo = mid + 1;
+            else hi = mid;
+          }
+          an = lo;
+        }
         src = an < J ? an : J - 1;
       }
       MP_UNROLL for (int s = 0; s < M::MAXS; ++s) if (s < S) x[s] = Xc[(int64_t)s * J + src];
     }
-    float wprev = uniform_w ? wconst : (st.collapsed ? -logJ : a.lw[j] - st.llf);
+    float wprev = uniform_w ? wconst : (collapsed ? -logJ : a.lw[j] - llf);
     MP_UNROLL for (int s = 0; s < M::MAXS; ++s)
       if ((md.accum_mask >> s) & 1u) x[s] = 0.f;
     float tt = tstart;
@@ -185,32 +356,17 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
   }
 }
 
-// Aggregate of scan tile `idx` (the sum of its particles' fixed-point weights),
-// computed by one wave: what that tile publishes, computed here instead when it
-// has not published in time (a tile whose workgroup is not running yet).
-__device__ uint64_t wave_tile_aggregate(const float* lw, int64_t idx, int64_t tile_len, int64_t J, float m,
-                                        int sh) {
-  const int lane = threadIdx.x & 63;
-  const int64_t j0 = idx * tile_len, j1 = min(J, j0 + tile_len);
-  uint64_t v = 0;
-  for (int64_t j = j0 + lane; j < j1; j += 64) v += particle_fixed_weight(lw[j], m, sh);
-  return wave_sum_u64(v);
-}
-
 template <class M>
 __global__ __launch_bounds__(1024) void k_plain_scan(const PlainArgs a) {
   __shared__ double red[32];
-  __shared__ uint64_t tab[65];  // wave totals (<= 16) and the tile total
-  __shared__ uint64_t s_prefix;
+  __shared__ uint64_t tab[17];  // wave totals (<= 16), then their exclusive prefix
   const int J = a.J, tid = threadIdx.x, B = blockDim.x, lane = tid & 63, w = tid >> 6;
   const int nw = (B + 63) >> 6;
-  const int ntiles = a.ntiles;
   const int par = a.t & 1;
-  uint64_t* lb = a.lb + (int64_t)par * ntiles;
   const int sh = shift_for(J);
   const int tile = blockIdx.x;
   const int rows = a.rows;
-  const int64_t tile0 = (int64_t)tile * rows * B;
+  const int64_t tile0 = (int64_t)tile * a.tile_len;
   // Each thread owns `rows` consecutive particles j0 .. j0 + rows - 1: it
   // loads them as float4 vectors, scans them serially and takes one wave scan
   // of its total (rather than one wave scan per row).
@@ -218,8 +374,6 @@ __global__ __launch_bounds__(1024) void k_plain_scan(const PlainArgs a) {
   const bool full = j0 + rows <= J;  // all of the thread's particles exist
   const bool vec = full && (rows & 3) == 0;
 
-  // the thread's log-weights, loads issued first so that they are in flight
-  // together with the per-block maxima below
   float lwv[kMaxScanRows];
   if (vec) {
     const float4* p4 = reinterpret_cast<const float4*>(a.lw + j0);
@@ -237,21 +391,8 @@ __global__ __launch_bounds__(1024) void k_plain_scan(const PlainArgs a) {
 #pragma unroll
     for (int r = 0; r < kMaxScanRows; ++r) lwv[r] = (r < rows && j0 + r < J) ? a.lw[j0 + r] : -__builtin_inff();
   }
-  // The finishing tile's scalars, loaded now so that their dependent loads
-  // overlap the scan: the comb uniform from this step's resampling key, and
-  // the next observation's row, grid and keys.
-  const bool finisher = tile == ntiles - 1 && tid == 0;
-  float ures_next = 0.0f;
-  double loglik_prev = 0.0;
-  PlainState nx{};
-  if (finisher) {
-    const PlainState cur = *a.st;
-    loglik_prev = cur.loglik;
-    ures_next = uniform01(bits_at(Key{cur.r0, cur.r1}, 0));
-    if (a.t + 1 < a.T) load_step(a, a.t + 1, nx);
-  }
-  // global max / min of the log-weights from the per-block values, eight
-  // independent loads in flight per thread
+  // the step's max (and, in tile 0, min) log-weight from the propagate
+  // blocks' values, eight independent loads in flight per thread
   float m = -__builtin_inff(), mn = __builtin_inff();
   for (int i0 = 0; i0 < a.nprop; i0 += 8 * B) {
     float vx[8], vn[8];
@@ -259,7 +400,7 @@ __global__ __launch_bounds__(1024) void k_plain_scan(const PlainArgs a) {
     for (int k = 0; k < 8; ++k) {
       const int i = i0 + k * B + tid;
       vx[k] = i < a.nprop ? a.pmax[i] : -__builtin_inff();
-      vn[k] = i < a.nprop ? a.pmin[i] : __builtin_inff();
+      vn[k] = (tile == 0 && i < a.nprop) ? a.pmin[i] : __builtin_inff();
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -267,7 +408,15 @@ __global__ __launch_bounds__(1024) void k_plain_scan(const PlainArgs a) {
       mn = nanmin(mn, vn[k]);
     }
   }
-  block_maxmin(m, mn, red);
+  if (tile == 0) {
+    block_maxmin(m, mn, red);
+    if (tid == 0) {
+      a.mm[2 * par] = m;
+      a.mm[2 * par + 1] = mn;
+    }
+  } else {
+    m = block_nanmax(m, red);
+  }
 
   // fixed-point weights, their running sum over the thread's particles, and
   // one wave scan of the thread totals
@@ -284,85 +433,15 @@ __global__ __launch_bounds__(1024) void k_plain_scan(const PlainArgs a) {
   const uint64_t texcl = winc - run;
   if (lane == 63) tab[w] = winc;
   __syncthreads();
-  // wave 0: exclusive scan of the wave totals, then publish the tile's
-  // aggregate
-  const int n = nw;
+  // wave 0: exclusive scan of the wave totals and the tile's total
   if (w == 0) {
-    const uint64_t v = lane < n ? tab[lane] : 0;
+    const uint64_t v = lane < nw ? tab[lane] : 0;
     const uint64_t inc = wave_incl_scan_u64(v);
-    if (lane < n) tab[lane] = inc - v;
-    const uint64_t aggv = __shfl(inc, WAVE - 1, WAVE);
-    if (lane == 0) {
-      tab[n] = aggv;
-      st_relaxed(&lb[tile], (tile == 0 ? kInclState : kAggState) | aggv);
-    }
-    // Decoupled look-back, kLookback * 64 predecessors per probe with all of
-    // their loads in flight together: lane l inspects tiles base - l - 64 k.
-    // The nearest predecessor with an inclusive prefix ends the walk; the
-    // aggregates in between are summed with a wave reduction.  Up to
-    // kLookback * 64 tiles this is one round trip once the predecessors have
-    // published.  Tiles are block indices, so a predecessor may not be running
-    // yet (its workgroup not dispatched while this one waits): after
-    // a.scan_spin probes without progress the wave computes the missing
-    // aggregates itself from the log-weights.  Every wait is therefore bounded
-    // and no dispatch order is assumed.
-    uint64_t prefix = 0;
-    if (tile > 0) {
-      int bse = tile - 1;
-      int spins = 0;
-      while (true) {
-        uint64_t word[kLookback];
-#pragma unroll
-        for (int k = 0; k < kLookback; ++k) {
-          const int idx = bse - lane - 64 * k;
-          word[k] = idx >= 0 ? ld_relaxed(&lb[idx]) : kInclState;
-        }
-        // offset (lane + 64 k) of the nearest inclusive predecessor
-        int first = 64 * kLookback;
-#pragma unroll
-        for (int k = kLookback - 1; k >= 0; --k) {
-          const uint64_t im = __ballot((word[k] >> kStateShift) == 2);
-          if (im) first = 64 * k + __ffsll((unsigned long long)im) - 1;
-        }
-        uint64_t nr[kLookback];
-        bool ready = true;
-#pragma unroll
-        for (int k = 0; k < kLookback; ++k) {
-          nr[k] = __ballot((word[k] >> kStateShift) == 0 && 64 * k + lane <= first);
-          ready = ready && nr[k] == 0;
-        }
-        if (!ready) {
-          if (spins < a.scan_spin) {
-            ++spins;
-            __builtin_amdgcn_s_sleep(1);
-            continue;
-          }
-#pragma unroll
-          for (int k = 0; k < kLookback; ++k) {
-            uint64_t msk = nr[k];
-            while (msk) {
-              const int l = __ffsll((unsigned long long)msk) - 1;
-              msk &= msk - 1;
-              const uint64_t agg = wave_tile_aggregate(a.lw, bse - l - 64 * k, (int64_t)rows * B, J, m, sh);
-              if (lane == l) word[k] = kAggState | agg;
-            }
-          }
-        }
-        spins = 0;
-        uint64_t v = 0;
-#pragma unroll
-        for (int k = 0; k < kLookback; ++k) v += 64 * k + lane <= first ? (word[k] & kValueMask) : 0;
-        prefix += wave_sum_u64(v);
-        if (first < 64 * kLookback) break;
-        bse -= 64 * kLookback;
-      }
-      if (lane == 0) st_relaxed(&lb[tile], kInclState | (prefix + aggv));
-    }
-    if (lane == 0) s_prefix = prefix;
+    if (lane < nw) tab[lane] = inc - v;
+    if (lane == nw - 1) a.agg[tile] = inc;
   }
   __syncthreads();
-  const uint64_t prefix = s_prefix;
-  const uint64_t base = prefix + tab[w] + texcl;
+  const uint64_t base = tab[w] + texcl;
   if (full && (rows & 1) == 0) {
     ulonglong2* p2 = reinterpret_cast<ulonglong2*>(a.C + j0);
 #pragma unroll
@@ -373,47 +452,50 @@ __global__ __launch_bounds__(1024) void k_plain_scan(const PlainArgs a) {
     for (int r = 0; r < kMaxScanRows; ++r)
       if (r < rows && j0 + r < J) a.C[j0 + r] = base + loc[r];
   }
-  // The tile holding the last index finishes the step: its inclusive prefix is
-  // the exact total of the fixed-point weights.  It touches only the step
-  // state and the other parity's words, which no tile of this launch reads.
-  if (tile != ntiles - 1) return;
-  if (tid == 0) {
-    const uint64_t total = prefix + tab[n];
-    PlainState* st = a.st;
-    const float fJ = (float)J, logJ = d_log(fJ);
-    const bool collapsed = (m == -__builtin_inff());
-    const double llt = step_loglik(m, total, sh);
-    const float llf = (float)llt;
-    st->loglik = (a.t == 0 ? 0.0 : loglik_prev) + llt;
-    if (a.cll) a.cll[a.t] = llf;
-    const float nmax = collapsed ? -logJ : m - llf;
-    const float nmin = collapsed ? -logJ : mn - llf;
-    const bool res = (nmax - nmin) > a.logthresh;
-    st->llf = llf;
-    st->collapsed = collapsed;
-    st->have_anc = res;
-    st->uniform_w = res;
-    st->wconst = 0.0f - logJ;
-    st->total = total;
-    st->ures = ures_next;
-    if (a.t + 1 < a.T) {
-      st->obs = nx.obs;
-      st->base = nx.base;
-      st->ti0 = nx.ti0;
-      st->nst = nx.nst;
-      st->tstart = nx.tstart;
-      st->ty = nx.ty;
-      st->dt0 = nx.dt0;
-      st->k0 = nx.k0;
-      st->k1 = nx.k1;
-      st->r0 = nx.r0;
-      st->r1 = nx.r1;
-    }
-    if (a.t == a.T - 1) *a.loglik_out = (a.t == 0 ? 0.0 : loglik_prev) + llt;
+}
+
+// Inclusive prefix of the scan tiles' totals into a.ends (one workgroup), for
+// filters with more tiles than a propagate block forms in LDS: each thread
+// sums a run of consecutive tiles, a block scan orders the runs, and a second
+// pass writes the ends.
+__global__ __launch_bounds__(1024) void k_plain_ends(const PlainArgs a) {
+  __shared__ uint64_t tab[17];
+  const int tid = threadIdx.x, B = blockDim.x, lane = tid & 63, w = tid >> 6, nw = (B + 63) >> 6;
+  const int q = (a.ntiles + B - 1) / B;
+  const int64_t k0 = (int64_t)tid * q;
+  uint64_t v = 0;
+  for (int i = 0; i < q; ++i)
+    if (k0 + i < a.ntiles) v += a.agg[k0 + i];
+  const uint64_t winc = wave_incl_scan_u64(v);
+  if (lane == 63) tab[w] = winc;
+  __syncthreads();
+  if (w == 0) {
+    const uint64_t x = lane < nw ? tab[lane] : 0;
+    const uint64_t inc = wave_incl_scan_u64(x);
+    if (lane < nw) tab[lane] = inc - x;
   }
-  // clear the next step's look-back words
-  uint64_t* lbn = a.lb + (int64_t)(par ^ 1) * ntiles;
-  for (int i = tid; i < ntiles; i += B) lbn[i] = 0;
+  __syncthreads();
+  uint64_t e = tab[w] + winc - v;
+  for (int i = 0; i < q; ++i)
+    if (k0 + i < a.ntiles) {
+      e += a.agg[k0 + i];
+      a.ends[k0 + i] = e;
+    }
+}
+
+// After the last step: its log-likelihood term completes the filter's
+// log-likelihood (one wave).
+__global__ __launch_bounds__(64) void k_plain_final(const PlainArgs a) {
+  if (a.T == 0) {
+    if (threadIdx.x == 0) *a.loglik_out = 0.0;
+    return;
+  }
+  const uint64_t total = wave_step_total(a);
+  const PlainFin f = plain_finish(a, a.T - 1, total);
+  if (threadIdx.x == 0) {
+    *a.loglik_out = (a.T == 1 ? 0.0 : a.st->loglik) + f.llt;
+    if (a.cll) a.cll[a.T - 1] = f.llf;
+  }
 }
 
 // Diagnostics of step t (pfilter.py:146-156), after its scan: ESS = 1 / sum
@@ -429,7 +511,13 @@ __global__ __launch_bounds__(256) void k_plain_diag(const PlainArgs a) {
   static_assert(NV <= kDiagWords, "diagnostic words");
   const int J = a.J, S = M::FIXED_S ? M::MAXS : a.md.S, tid = threadIdx.x, B = blockDim.x;
   const float* Xn = (a.t & 1) ? a.X0 : a.X1;
-  const PlainState st = *a.st;
+  __shared__ PlainFin s_fin;
+  if (tid < WAVE) {
+    const PlainFin f = plain_finish(a, a.t, wave_step_total(a));
+    if (tid == 0) s_fin = f;
+  }
+  __syncthreads();
+  const PlainFin st = s_fin;
   const float logJ = d_log((float)J);
   double acc[NV];
   MP_UNROLL for (int k = 0; k < NV; ++k) acc[k] = 0.0;
@@ -503,36 +591,55 @@ hipError_t plain_prepare(const ModelDesc& md, const float* theta, int64_t stride
   return hipErrorInvalidValue;
 }
 
-// Start of a filter: clear the look-back words and the step state, with the
-// first observation's scalars in it.
+// Start of a filter: the step table (one thread per observation) and the
+// running state.
 __global__ void k_plain_reset(const PlainArgs a) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < a.ntiles) {
-    a.lb[i] = 0;
-    a.lb[a.ntiles + i] = 0;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < a.T) {
+    const int64_t obs = a.ser.obs_off[a.unit] + t;
+    const int ti0 = a.ser.tidx0[obs], nst = a.ser.nstep[obs];
+    const int64_t base = ti0 + (int64_t)t;
+    PlainStep s;
+    s.obs = obs;
+    s.base = base;
+    s.ti0 = ti0;
+    s.nst = nst;
+    s.tstart = t == 0 ? a.ser.t0 : a.ser.times[obs - 1];
+    s.ty = a.ser.times[obs];
+    s.dt0 = nst > 0 ? a.ser.dt_ext[a.ser.dt_off[a.unit] + ti0] : 0.0f;
+    s.k0 = a.sched[2 * base];
+    s.k1 = a.sched[2 * base + 1];
+    s.ures_prev = 0.0f;
+    if (t > 0) {
+      // step t - 1 resamples with the key after its sub-step keys
+      const int64_t rb = a.ser.tidx0[obs - 1] + (int64_t)(t - 1) + a.ser.nstep[obs - 1];
+      s.ures_prev = uniform01(bits_at(Key{a.sched[2 * rb], a.sched[2 * rb + 1]}, 0));
+    }
+    a.steps[t] = s;
   }
-  if (i == 0) {
-    PlainState z{};
-    if (a.T > 0) load_step(a, 0, z);
-    *a.st = z;
-  }
+  if (t == 0) a.st->loglik = 0.0;
 }
 
 hipError_t plain_reset(const PlainArgs& a, hipStream_t s) {
-  mp_launch(k_plain_reset, dim3((a.ntiles + 255) / 256), dim3(256), 0, s, a);
+  const int n = a.T > 0 ? a.T : 1;
+  mp_launch(k_plain_reset, dim3((n + 255) / 256), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
-// which = 0: propagate kernel, 1: scan kernel, 2: step diagnostics
+// which = 0: propagate kernel, 1: scan kernel (and the tile ends when they are
+// global), 2: step diagnostics, 3: the filter's last step
 template <class M>
 static hipError_t plain_kernel_t(int which, const PlainArgs& a, int ntiles, int threads, hipStream_t s) {
   if (which == 0) {
     mp_launch(k_plain_prop<M>, dim3(a.nprop), dim3(threads), 0, s, a);
   } else if (which == 1) {
     mp_launch(k_plain_scan<M>, dim3(ntiles), dim3(threads), 0, s, a);
-  } else {
+    if (a.ends) mp_launch(k_plain_ends, dim3(1), dim3(1024), 0, s, a);
+  } else if (which == 2) {
     mp_launch(k_plain_diag<M>, dim3(a.ndiag), dim3(256), 0, s, a);
     mp_launch(k_plain_diag_final<M>, dim3(1), dim3(64), 0, s, a);
+  } else {
+    mp_launch(k_plain_final, dim3(1), dim3(64), 0, s, a);
   }
   return hipGetLastError();
 }

@@ -32,6 +32,7 @@ struct SPX {
   static constexpr bool DIFFERENTIABLE = false;
   static constexpr bool HOIST = false;
   static constexpr int NOISE = 1;    // dZ depends on the key only
+  static constexpr bool DRAW_USES_P = false;
   static constexpr bool EXT = true;  // reads covariates; defines the @vectorized draw
   static constexpr int DC = 1;       // covariates read: y_prev
   struct P {

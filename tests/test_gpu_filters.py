@@ -238,37 +238,73 @@ def test_plain_xcd_chunk_map_edges(J):
     np.testing.assert_array_equal(plain["CLL"], batched["CLL"])
 
 
-def test_plain_scan_fallback_and_long_look_back():
-    # Spin limit 0: a scan tile that finds a predecessor's look-back word
-    # unpublished computes that aggregate itself from the log-weights.  With
-    # 256-particle tiles (782 tiles at J = 200000) the look-back also spans
-    # several 256-tile probes.  Both give the bits of the default launch.
-    import pypomp_amd.functional as F
+def _with_plain_tuning(settings, fn):
     from pypomp_amd import _native, autotune
     lib = _native.lib()
-    struct, theta, _ = lg2_struct(1)
-    keys = tf.split(tf.key(91), 2).reshape(1, 2, 2)
-    J = 200_000
-    ref = F.pfilter(struct, theta[None], J, keys, form="plain")
-    saved = {n: autotune.get(n) for n in ("plain_scan_rows", "plain_scan_threads")}
-    spin = lib.mp_tune_get(_native.MP_TUNE_PLAIN_SCAN_SPIN)
+    names = ("plain_scan_rows", "plain_scan_threads")
+    saved = {n: autotune.get(n) for n in names}
+    lds = lib.mp_tune_get(_native.MP_TUNE_PLAIN_LDS_TILES)
     try:
-        autotune.set("plain_scan_rows", 1)
-        autotune.set("plain_scan_threads", 256)
-        long_lb = F.pfilter(struct, theta[None], J, keys, form="plain")
-        _native.check(lib.mp_tune_set(_native.MP_TUNE_PLAIN_SCAN_SPIN, 0))
-        fallback = F.pfilter(struct, theta[None], J, keys, form="plain")
-        for n, v in saved.items():
-            autotune.set(n, v)
-        fallback_default_tiles = F.pfilter(struct, theta[None], J, keys, form="plain")
+        for n, v in settings.items():
+            if n == "lds_tiles":
+                _native.check(lib.mp_tune_set(_native.MP_TUNE_PLAIN_LDS_TILES, v))
+            else:
+                autotune.set(n, v)
+        return fn()
     finally:
         for n, v in saved.items():
             autotune.set(n, v)
-        _native.check(lib.mp_tune_set(_native.MP_TUNE_PLAIN_SCAN_SPIN, spin))
-    for o in (long_lb, fallback, fallback_default_tiles):
-        np.testing.assert_array_equal(o["logLik64"], ref["logLik64"])
-    batched = F.pfilter(struct, theta[None], J, keys, form="batched")
-    np.testing.assert_array_equal(batched["logLik64"], ref["logLik64"])
+        _native.check(lib.mp_tune_set(_native.MP_TUNE_PLAIN_LDS_TILES, lds))
+
+
+@pytest.mark.parametrize("settings", [
+    {"lds_tiles": 0},                                                   # tile ends by a separate launch
+    {"plain_scan_rows": 1, "plain_scan_threads": 256},                  # 782 tiles of 256 in LDS
+    {"plain_scan_rows": 1, "plain_scan_threads": 256, "lds_tiles": 0},  # 782 tiles, global ends
+    {"plain_scan_rows": 1, "plain_scan_threads": 256, "lds_tiles": 100},  # rows widened to 8: 98 tiles
+])
+def test_plain_tile_ends_do_not_change_results(settings):
+    # The propagate blocks take the prefix of the scan tiles before each
+    # position from the tiles' totals, formed in LDS by every block (the
+    # default) or written to global memory by a separate launch per step (more
+    # tiles than the LDS limit); the scan threads widen their runs to stay under
+    # the limit.  Every variant gives the bits of the batched form.
+    import pypomp_amd.functional as F
+    struct, theta, _ = lg2_struct(1)
+    keys = tf.split(tf.key(91), 2).reshape(1, 2, 2)
+    J = 200_000
+    ref = F.pfilter(struct, theta[None], J, keys, form="batched", CLL=True)
+    out = _with_plain_tuning(settings, lambda: F.pfilter(struct, theta[None], J, keys, form="plain", CLL=True))
+    np.testing.assert_array_equal(out["logLik64"], ref["logLik64"])
+    np.testing.assert_array_equal(out["CLL"], ref["CLL"])
+
+
+@pytest.mark.parametrize("lds_tiles", [1024, 0])
+def test_plain_wide_ancestor_windows_match_batched_and_oracle(lds_tiles):
+    # An outlying observation leaves a few hundred particles with non-zero
+    # fixed-point weights, scattered over J = 100000 indices: a block's comb
+    # then spans thousands of particles, more than its LDS window holds, and
+    # the per-particle search over global memory runs.  Plain (both tile-end
+    # paths), batched and the oracle pick the same ancestors.
+    import pypomp_amd.functional as F
+    from pypomp_amd.models import LGModel
+    from pypomp_amd.series import Series
+    _, theta, ys = lg2_struct(1)
+    ys = ys[:6].copy()
+    ys[2] = ys[2] + F32(4.0)
+    times = np.arange(1, ys.shape[0] + 1.0)
+    struct = F.PompStruct(model=LGModel(2, 2, separate_init=True), units=[Series.build(ys, times, 0.0, nstep=1)])
+    keys = tf.split(tf.key(97), 1).reshape(1, 1, 2)
+    J = 100_000
+    batched = F.pfilter(struct, theta[None], J, keys, form="batched", CLL=True)
+    plain = _with_plain_tuning({"lds_tiles": lds_tiles},
+                               lambda: F.pfilter(struct, theta[None], J, keys, form="plain", CLL=True))
+    np.testing.assert_array_equal(plain["logLik64"], batched["logLik64"])
+    np.testing.assert_array_equal(plain["CLL"], batched["CLL"])
+    exp = fl.pfilter(olg.LGModel(olg.LGSpec(2, 2, separate_init=True)), theta,
+                     fl.make_series(ys, times, 0.0, nstep=1), keys[0, 0], J, diagnostics=True)
+    assert float(plain["logLik64"][0, 0]) == exp["logLik64"]
+    np.testing.assert_array_equal(plain["CLL"][0, 0], np.array(exp["CLL"], dtype=F32))
 
 
 def test_pomp_pickle_continues_the_key_stream_and_merge_joins_runs():
