@@ -257,7 +257,7 @@ int32_t mp_perfilter(const mp_model_desc* h_model, const mp_series_desc* h_serie
 #define MP_TUNE_BATCHED_BLOCK 3   /* batched workgroup size: 0 automatic, 256, 512, 768 */
 #define MP_TUNE_GRAD_BLOCK 4      /* MOP-adjoint workgroup size: 0 automatic, 256, 512 */
 #define MP_TUNE_PLAIN_SCAN_THREADS 5 /* plain scan workgroup size: 256, 512, 1024 */
-#define MP_TUNE_PLAIN_PROP_THREADS 6 /* plain propagate workgroup size: 256, 512 */
+#define MP_TUNE_PLAIN_PROP_THREADS 6 /* plain propagate workgroup size: 256, 512, 1024 */
 #define MP_TUNE_PLAIN_LDS_TILES 7    /* scan tiles whose ends each propagate block forms in LDS,
                                         0..1024 (default 1024); more tiles (or 0) take a separate
                                         launch per step that writes them to global memory */

@@ -27,7 +27,7 @@ PARAMS = {
     "batched_block": (_native.MP_TUNE_BATCHED_BLOCK, (256, 512, 768)),
     "grad_block": (_native.MP_TUNE_GRAD_BLOCK, (256, 512)),
     "plain_scan_threads": (_native.MP_TUNE_PLAIN_SCAN_THREADS, (256, 512, 1024)),
-    "plain_prop_threads": (_native.MP_TUNE_PLAIN_PROP_THREADS, (256, 512)),
+    "plain_prop_threads": (_native.MP_TUNE_PLAIN_PROP_THREADS, (256, 512, 1024)),
     # 1: propagate blocks take the particle chunks of the scan tiles on their own XCD
     "plain_xcd_map": (_native.MP_TUNE_PLAIN_XCD_MAP, (1, 0)),
 }

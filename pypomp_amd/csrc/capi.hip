@@ -88,7 +88,7 @@ constexpr int64_t kParamSlot = 512;  // bytes per filter of prepared model param
 int g_plain_scan_rows = 4;  // particles per scan thread (1..16)
 int g_plain_prop_rows = 2;  // particle rows per propagate block (1..16)
 int g_plain_scan_threads = 1024;  // scan workgroup size (256, 512, 1024)
-int g_plain_prop_threads = 512;  // propagate workgroup size (256, 512)
+int g_plain_prop_threads = 512;  // propagate workgroup size (256, 512, 1024)
 // scan tiles whose ends every propagate block forms in LDS (above it, or at 0,
 // a k_plain_ends launch per step writes them to global memory)
 int g_plain_lds_tiles = kMaxLdsTiles;
@@ -253,7 +253,7 @@ BatchedPlan plan_batched(int algo, const mp_model_desc* m, const mp_series_desc*
 }
 
 struct PlainPlan {
-  int ntiles, rows, tile_len, nprop, rows_prop, prop_len, scan_threads, prop_threads, xcd_q;
+  int ntiles, rows, tile_len, nprop, rows_prop, prop_len, scan_threads, prop_threads, xcd_q, win;
   int ndiag;
   bool global_ends;
   int64_t off_sched, off_params, off_x0, off_x1, off_lw, off_c, off_agg, off_ends, off_pmax, off_pmin, off_mm,
@@ -301,6 +301,10 @@ PlainPlan plan_plain(const mp_model_desc* m, const mp_series_desc* s, int R, int
       p.prop_len = cap;
     }
   }
+  // LDS ancestor window: twice a block's particles, 2048 to 4096 entries (16
+  // to 32 KiB), so that a block whose comb spans up to that many parents
+  // searches in LDS (a wider span searches global memory)
+  p.win = std::min(4096, std::max(2048, 2 * p.rows_prop * p.prop_threads));
   const SeriesSet& ser = *reinterpret_cast<const SeriesSet*>(s);
   int64_t o = 0;
   auto take = [&](int64_t bytes) {
@@ -386,6 +390,10 @@ int run_plain_pf(const mp_model_desc* m, const mp_series_desc* s, const float* t
   a.prop_len = p.prop_len;
   a.nprop = p.nprop;
   a.xcd_q = p.xcd_q;
+  a.win = p.win;
+  a.logJ = d_log((float)J);
+  a.log1J = d_log(1.0f / (float)J);
+  a.sh = shift_for(J);
   a.st = reinterpret_cast<PlainState*>(b + p.off_st);
   a.rows = p.rows;
   a.diag_part = reinterpret_cast<double*>(b + p.off_diag);
@@ -884,7 +892,8 @@ int32_t mp_tune_set(int32_t param, int32_t value) {
       g_plain_scan_threads = value;
       return MP_OK;
     case MP_TUNE_PLAIN_PROP_THREADS:
-      if (value != 256 && value != 512) return fail(MP_ERR_ARG, "plain workgroup size must be 256 or 512");
+      if (value != 256 && value != 512 && value != 1024)
+        return fail(MP_ERR_ARG, "plain workgroup size must be 256, 512 or 1024");
       g_plain_prop_threads = value;
       return MP_OK;
     case MP_TUNE_PLAIN_LDS_TILES:

@@ -196,6 +196,9 @@ struct PlainArgs {
   int prop_len;         // particles per propagate block (<= rows_prop * blockDim)
   int nprop;            // propagate blocks
   int xcd_q;            // > 0: propagate block b takes chunk xcd_chunk(b, xcd_q); 0: chunk b
+  int win;              // LDS ancestor-window entries of a propagate block
+  float logJ, log1J;    // d_log(J) and d_log(1 / J) (float32)
+  int sh;               // fixed-point weight shift, shift_for(J)
 };
 
 // At most this many scan tiles have their ends formed in LDS by each

@@ -24,7 +24,6 @@
 
 namespace mp {
 
-constexpr int kWin = 2048;  // LDS search window (uint64 entries)
 constexpr int kMaxScanRows = 16;  // particles per scan thread, at most
 
 // Chunk of propagate block b when a scan tile holds q chunks: blocks go to the
@@ -81,10 +80,10 @@ struct PlainFin {
 __device__ PlainFin plain_finish(const PlainArgs& a, int t, uint64_t total) {
   const int par = t & 1;
   const float m = a.mm[2 * par], mn = a.mm[2 * par + 1];
-  const float logJ = d_log((float)a.J);
+  const float logJ = a.logJ;
   PlainFin f;
   f.collapsed = (m == -__builtin_inff());
-  f.llt = step_loglik(m, total, shift_for(a.J));
+  f.llt = step_loglik(m, total, a.sh);
   f.llf = (float)f.llt;
   const float nmax = f.collapsed ? -logJ : m - f.llf;
   const float nmin = f.collapsed ? -logJ : mn - f.llf;
@@ -151,6 +150,21 @@ __device__ int64_t wave_global_upper_bound(const PlainArgs& a, const uint64_t (&
   return i0 + wave_upper_bound_u64(a.C + i0, 0, n, T - base);
 }
 
+// wave_global_upper_bound for a step whose tiles fit one chunk, from the
+// tile ends `e` (chunk_ends of the first chunk with carry 0).
+__device__ int64_t wave_upper_bound_one_chunk(const PlainArgs& a, const uint64_t (&e)[kLaneTiles], uint64_t T) {
+  int n = 0;
+#pragma unroll
+  for (int i = 0; i < kLaneTiles; ++i) n += __popcll(__ballot(e[i] <= T));
+  uint64_t base = 0;
+  if (n > 0) {
+    const int l = (n - 1) / kLaneTiles, i = (n - 1) % kLaneTiles;
+    base = lane_u64(i == 0 ? e[0] : i == 1 ? e[1] : i == 2 ? e[2] : e[3], l);
+  }
+  const int64_t i0 = (int64_t)n << a.tile_shift, m = min((int64_t)a.tile_len, (int64_t)a.J - i0);
+  return i0 + wave_upper_bound_u64(a.C + i0, 0, m, T - base);
+}
+
 // The prefix sum of particle i over the whole filter: the within-tile prefix
 // plus the end of the tiles before its tile (`ends` in LDS or global memory).
 __device__ __forceinline__ uint64_t global_prefix(const PlainArgs& a, const uint64_t* ends, int64_t i) {
@@ -159,10 +173,10 @@ __device__ __forceinline__ uint64_t global_prefix(const PlainArgs& a, const uint
 }
 
 template <class M>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k_plain_prop(const PlainArgs a) {
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void k_plain_prop(const PlainArgs a) {
   __shared__ double red[32];
   __shared__ int64_t s_win[2];
-  __shared__ uint64_t s_C[kWin];
+  extern __shared__ uint64_t s_C[];  // the ancestor window, a.win entries
   __shared__ uint64_t s_end[kMaxLdsTiles];
   __shared__ uint64_t s_total;
   __shared__ PlainFin s_fin;
@@ -177,8 +191,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
   const float tstart = st.tstart;
   const float* y = a.ser.ys + st.obs * md.dy;
   const int64_t base = st.base;
-  const float fJ = (float)J;
-  const float logJ = d_log(fJ);
+  const float logJ = a.logJ;
 
   // Waves 0 to 3 finish the previous step below (a latency-bound chain of
   // dependent loads): they issue at a raised priority, so that the other
@@ -186,7 +199,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
   // first chunk of scan tile totals, block 0's running log-likelihood) are in
   // flight during the parameter staging.
   const int wv = tid >> 6, lane = tid & 63;
-  const bool pro = t > 0 && wv < 4;
+  const bool pro = t > 0 && wv < 4;  // (wave 3 only when the tile ends span several chunks)
   if (pro) __builtin_amdgcn_s_setprio(2);
   uint64_t v0[kLaneTiles] = {};
   if (pro && !a.ends) load_chunk(a, 0, v0);
@@ -225,8 +238,24 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
   //  * wave 3 forms the tile ends in LDS for the staging.
   // The searches run whether or not the step resamples (wave 2 decides that
   // concurrently); their result is then unused.
+  const bool one_chunk = !a.ends && a.ntiles <= kChunkTiles;
   if (t > 0) {
-    if (wv < 3) {
+    if (wv < 2 && one_chunk) {
+      // the tile ends from one DPP scan; wave 0 also stores them in LDS
+      uint64_t e[kLaneTiles];
+      const uint64_t total = chunk_ends(v0, 0, e);
+      if (wv == 0) {
+#pragma unroll
+        for (int i = 0; i < kLaneTiles; ++i) {
+          const int k = kLaneTiles * lane + i;
+          if (k < a.ntiles) s_end[k] = e[i];
+        }
+      }
+      const uint64_t T = comb_threshold(st.ures_prev, wv == 0 ? tile0 : tile1 - 1, J, total);
+      // every prefix sum is <= total
+      const int64_t v = T >= total ? J : wave_upper_bound_one_chunk(a, e, T);
+      if (lane == 0) s_win[wv] = v;
+    } else if (wv < 3) {
       const uint64_t total = wave_step_total(a, v0);
       if (wv < 2) {
         const int64_t jj = wv == 0 ? tile0 : tile1 - 1;
@@ -245,7 +274,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
           }
         }
       }
-    } else if (wv == 3 && !a.ends) {
+    } else if (wv == 3 && !a.ends && !one_chunk) {
       uint64_t carry = 0;
       for (int c0 = 0; c0 < a.ntiles; c0 += kChunkTiles) {
         uint64_t v[kLaneTiles], e[kLaneTiles];
@@ -285,11 +314,11 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
   const bool uniform_w = t == 0 || s_fin.res;
   const bool collapsed = t > 0 && s_fin.collapsed;
   const float llf = t > 0 ? s_fin.llf : 0.0f;
-  const float wconst = t == 0 ? d_log(1.0f / fJ) : 0.0f - logJ;
+  const float wconst = t == 0 ? a.log1J : 0.0f - logJ;
   const uint64_t total = t > 0 ? s_total : 0;
   const uint64_t* ends = a.ends ? a.ends : s_end;
   const int64_t wlo = have_anc ? s_win[0] : 0, whi = have_anc ? s_win[1] : 0;
-  const bool in_lds = have_anc && (whi - wlo) <= kWin;
+  const bool in_lds = have_anc && (whi - wlo) <= a.win;
   if (in_lds) {
     const int n = (int)(whi - wlo);
     if (a.ends) {
@@ -363,7 +392,7 @@ __global__ __launch_bounds__(1024) void k_plain_scan(const PlainArgs a) {
   const int J = a.J, tid = threadIdx.x, B = blockDim.x, lane = tid & 63, w = tid >> 6;
   const int nw = (B + 63) >> 6;
   const int par = a.t & 1;
-  const int sh = shift_for(J);
+  const int sh = a.sh;
   const int tile = blockIdx.x;
   const int rows = a.rows;
   const int64_t tile0 = (int64_t)tile * a.tile_len;
@@ -518,7 +547,7 @@ __global__ __launch_bounds__(256) void k_plain_diag(const PlainArgs a) {
   }
   __syncthreads();
   const PlainFin st = s_fin;
-  const float logJ = d_log((float)J);
+  const float logJ = a.logJ;
   double acc[NV];
   MP_UNROLL for (int k = 0; k < NV; ++k) acc[k] = 0.0;
   for (int64_t j = (int64_t)blockIdx.x * B + tid; j < J; j += (int64_t)gridDim.x * B) {
@@ -631,7 +660,7 @@ hipError_t plain_reset(const PlainArgs& a, hipStream_t s) {
 template <class M>
 static hipError_t plain_kernel_t(int which, const PlainArgs& a, int ntiles, int threads, hipStream_t s) {
   if (which == 0) {
-    mp_launch(k_plain_prop<M>, dim3(a.nprop), dim3(threads), 0, s, a);
+    mp_launch(k_plain_prop<M>, dim3(a.nprop), dim3(threads), (uint32_t)(a.win * sizeof(uint64_t)), s, a);
   } else if (which == 1) {
     mp_launch(k_plain_scan<M>, dim3(ntiles), dim3(threads), 0, s, a);
     if (a.ends) mp_launch(k_plain_ends, dim3(1), dim3(1024), 0, s, a);
