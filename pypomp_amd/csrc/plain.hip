@@ -77,6 +77,21 @@ struct PlainFin {
   int res;         // the step resamples
 };
 
+// plain_finish for a block that needs only the decision and, when the step
+// does not resample, the normaliser: with no threshold (log threshold -inf) a
+// step resamples unless its max is +inf or NaN (plain_finish's difference is
+// then NaN), in which case step_loglik is NaN; a collapsed step (max -inf)
+// resamples.  The same fields as plain_finish, without its float64 log.
+__device__ PlainFin plain_finish_unthresholded(const PlainArgs& a, int t) {
+  const float m = a.mm[2 * (t & 1)];
+  PlainFin f;
+  f.collapsed = (m == -__builtin_inff());
+  f.res = !(m == __builtin_inff() || m != m);
+  f.llt = f.res ? 0.0 : __builtin_nan("");  // unused when the step resamples
+  f.llf = (float)f.llt;
+  return f;
+}
+
 __device__ PlainFin plain_finish(const PlainArgs& a, int t, uint64_t total) {
   const int par = t & 1;
   const float m = a.mm[2 * par], mn = a.mm[2 * par + 1];
@@ -264,7 +279,11 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
         const int64_t v = T >= total ? J : wave_global_upper_bound(a, v0, T);
         if (lane == 0) s_win[wv] = v;
       } else {
-        const PlainFin f = plain_finish(a, t - 1, total);
+        // block 0 keeps the running log-likelihood and needs the term; with no
+        // threshold the others need only the decision
+        const PlainFin f = (blockIdx.x != 0 && a.logthresh == -__builtin_inff())
+                               ? plain_finish_unthresholded(a, t - 1)
+                               : plain_finish(a, t - 1, total);
         if (lane == 0) {
           s_fin = f;
           s_total = total;
