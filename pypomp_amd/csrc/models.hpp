@@ -54,6 +54,10 @@ struct LGT {
     float logdet, half_n_log2pi;
     int dx, dy;
   };
+  // The dimensions: the constant MD when the size is exact, so that every
+  // guard and index below folds at compile time.
+  __host__ __device__ static int DX(const P& p) { return EXACT ? MD : p.dx; }
+  __host__ __device__ static int DY(const P& p) { return EXACT ? MD : p.dy; }
 
   __host__ __device__ static void from_est(const ModelDesc& md, float* th) {
     const int dx = EXACT ? MD : md.dx, dy = EXACT ? MD : md.dy;
@@ -114,10 +118,10 @@ struct LGT {
 
   __device__ static void rinit(const ModelDesc&, const P& p, Key k, float, float* x) {
     float z[MD];
-    MP_UNROLL for (int i = 0; i < MD; ++i) z[i] = i < p.dx ? normal_at(k, i) : 0.f;
-    MP_UNROLL for (int i = 0; i < MD; ++i) if (i < p.dx) {
+    MP_UNROLL for (int i = 0; i < MD; ++i) z[i] = i < DX(p) ? normal_at(k, i) : 0.f;
+    MP_UNROLL for (int i = 0; i < MD; ++i) if (i < DX(p)) {
       float s = p.L0[i][0] * z[0];
-      MP_UNROLL for (int j = 1; j < MD; ++j) if (j < p.dx) s += p.L0[i][j] * z[j];
+      MP_UNROLL for (int j = 1; j < MD; ++j) if (j < DX(p)) s += p.L0[i][j] * z[j];
       x[i] = p.X0[i] + s;
     }
   }
@@ -129,7 +133,7 @@ struct LGT {
   static constexpr bool DRAW_USES_P = !EXACT;
   static constexpr bool HOIST = false;  // no particle-independent step or density terms
   __device__ static void draw(const P& p, Key k, float* z) {
-    MP_UNROLL for (int i = 0; i < MD; ++i) z[i] = i < (EXACT ? MD : p.dx) ? normal_at(k, i) : 0.f;
+    MP_UNROLL for (int i = 0; i < MD; ++i) z[i] = i < DX(p) ? normal_at(k, i) : 0.f;
   }
 
   __device__ static void step(const ModelDesc& md, const P& p, Key k, float t, float dt, float* x) {
@@ -142,23 +146,23 @@ struct LGT {
     float xn[MD];
     MP_UNROLL for (int i = 0; i < MD; ++i) {
       float a = p.A[i][0] * x[0];
-      MP_UNROLL for (int j = 1; j < MD; ++j) if (j < p.dx) a += p.A[i][j] * x[j];
+      MP_UNROLL for (int j = 1; j < MD; ++j) if (j < DX(p)) a += p.A[i][j] * x[j];
       float s = p.LQ[i][0] * z[0];
-      MP_UNROLL for (int j = 1; j < MD; ++j) if (j < p.dx) s += p.LQ[i][j] * z[j];
+      MP_UNROLL for (int j = 1; j < MD; ++j) if (j < DX(p)) s += p.LQ[i][j] * z[j];
       xn[i] = a + s;
     }
-    MP_UNROLL for (int i = 0; i < MD; ++i) if (i < p.dx) x[i] = xn[i];
+    MP_UNROLL for (int i = 0; i < MD; ++i) if (i < DX(p)) x[i] = xn[i];
   }
 
   __device__ static float dmeas(const ModelDesc&, const P& p, const float* y, const float* x, float) {
     float r[MD], s[MD];
     MP_UNROLL for (int i = 0; i < MD; ++i) {
       float m = p.C[i][0] * x[0];
-      MP_UNROLL for (int j = 1; j < MD; ++j) if (j < p.dx) m += p.C[i][j] * x[j];
-      r[i] = i < p.dy ? y[i] - m : 0.f;
+      MP_UNROLL for (int j = 1; j < MD; ++j) if (j < DX(p)) m += p.C[i][j] * x[j];
+      r[i] = i < DY(p) ? y[i] - m : 0.f;
     }
     float quad = 0.f;
-    MP_UNROLL for (int i = 0; i < MD; ++i) if (i < p.dy) {
+    MP_UNROLL for (int i = 0; i < MD; ++i) if (i < DY(p)) {
       float a = r[i];
       MP_UNROLL for (int k = 0; k < MD; ++k) if (k < i) a -= p.LC[i][k] * s[k];
       s[i] = a / p.LC[i][i];
@@ -170,23 +174,23 @@ struct LGT {
   // y = C x + LR z, z = normal(key, (dy,)) (linear_gaussian.py:181-191)
   __device__ static void rmeas(const ModelDesc&, const P& p, Key k, const float* x, float, float* y) {
     float z[MD];
-    MP_UNROLL for (int i = 0; i < MD; ++i) z[i] = i < p.dy ? normal_at(k, i) : 0.f;
-    MP_UNROLL for (int i = 0; i < MD; ++i) if (i < p.dy) {
+    MP_UNROLL for (int i = 0; i < MD; ++i) z[i] = i < DY(p) ? normal_at(k, i) : 0.f;
+    MP_UNROLL for (int i = 0; i < MD; ++i) if (i < DY(p)) {
       float m = p.C[i][0] * x[0];
-      MP_UNROLL for (int j = 1; j < MD; ++j) if (j < p.dx) m += p.C[i][j] * x[j];
+      MP_UNROLL for (int j = 1; j < MD; ++j) if (j < DX(p)) m += p.C[i][j] * x[j];
       float s = p.LR[i][0] * z[0];
-      MP_UNROLL for (int j = 1; j < MD; ++j) if (j < p.dy) s += p.LR[i][j] * z[j];
+      MP_UNROLL for (int j = 1; j < MD; ++j) if (j < DY(p)) s += p.LR[i][j] * z[j];
       y[i] = m + s;
     }
   }
 
   // ---- derivatives for the MOP adjoint (natural-scale parameters) ----
   // theta offsets: A, C, LQ (lower triangle), LR (lower triangle), X0, [L0]
-  __device__ static int oC(const P& p) { return p.dx * p.dx; }
-  __device__ static int oQ(const P& p) { return oC(p) + p.dy * p.dx; }
-  __device__ static int oR(const P& p) { return oQ(p) + p.dx * (p.dx + 1) / 2; }
-  __device__ static int oX(const P& p) { return oR(p) + p.dy * (p.dy + 1) / 2; }
-  __device__ static int oP(const P& p) { return oX(p) + p.dx; }
+  __device__ static int oC(const P& p) { return DX(p) * DX(p); }
+  __device__ static int oQ(const P& p) { return oC(p) + DY(p) * DX(p); }
+  __device__ static int oR(const P& p) { return oQ(p) + DX(p) * (DX(p) + 1) / 2; }
+  __device__ static int oX(const P& p) { return oR(p) + DY(p) * (DY(p) + 1) / 2; }
+  __device__ static int oP(const P& p) { return oX(p) + DX(p); }
 
   __device__ static float dmeas_aux(const P&, const float*) { return 0.f; }
   __device__ static float dmeas_a(const ModelDesc& md, const P& p, const float* y, const float* x, float) {
@@ -200,24 +204,24 @@ struct LGT {
     float r[MD], s[MD], w[MD], Li[MD][MD], Si[MD][MD];
     MP_UNROLL for (int i = 0; i < MD; ++i) {
       float m = 0.f;
-      MP_UNROLL for (int j = 0; j < MD; ++j) if (j < p.dx) m += p.C[i][j] * x[j];
-      r[i] = i < p.dy ? y[i] - m : 0.f;
+      MP_UNROLL for (int j = 0; j < MD; ++j) if (j < DX(p)) m += p.C[i][j] * x[j];
+      r[i] = i < DY(p) ? y[i] - m : 0.f;
     }
     // s = LC^-1 r, w = LC^-T s; Li = LC^-1 (lower), Si = Li^T Li
     MP_UNROLL for (int i = 0; i < MD; ++i) {
       float a = r[i];
       MP_UNROLL for (int k = 0; k < MD; ++k) if (k < i) a -= p.LC[i][k] * s[k];
-      s[i] = i < p.dy ? a / p.LC[i][i] : 0.f;
+      s[i] = i < DY(p) ? a / p.LC[i][i] : 0.f;
     }
     MP_UNROLL for (int ii = MD - 1; ii >= 0; --ii) {
       float a = s[ii];
-      MP_UNROLL for (int k = 0; k < MD; ++k) if (k > ii && k < p.dy) a -= p.LC[k][ii] * w[k];
-      w[ii] = ii < p.dy ? a / p.LC[ii][ii] : 0.f;
+      MP_UNROLL for (int k = 0; k < MD; ++k) if (k > ii && k < DY(p)) a -= p.LC[k][ii] * w[k];
+      w[ii] = ii < DY(p) ? a / p.LC[ii][ii] : 0.f;
     }
     MP_UNROLL for (int c = 0; c < MD; ++c) MP_UNROLL for (int i = 0; i < MD; ++i) {
       float a = (i == c) ? 1.f : 0.f;
       MP_UNROLL for (int k = 0; k < MD; ++k) if (k < i) a -= p.LC[i][k] * Li[k][c];
-      Li[i][c] = (i < p.dy && c < p.dy) ? a / p.LC[i][i] : 0.f;
+      Li[i][c] = (i < DY(p) && c < DY(p)) ? a / p.LC[i][i] : 0.f;
     }
     MP_UNROLL for (int i = 0; i < MD; ++i) MP_UNROLL for (int j = 0; j < MD; ++j) {
       float a = 0.f;
@@ -226,17 +230,17 @@ struct LGT {
     }
     MP_UNROLL for (int j = 0; j < MD; ++j) {
       float a = 0.f;
-      MP_UNROLL for (int i = 0; i < MD; ++i) if (i < p.dy) a += p.C[i][j] * w[i];
+      MP_UNROLL for (int i = 0; i < MD; ++i) if (i < DY(p)) a += p.C[i][j] * w[i];
       gx[j] = a;
     }
     const double wd = (double)wgt;
     const int oc = oC(p), orr = oR(p);
-    MP_UNROLL for (int i = 0; i < MD; ++i) if (i < p.dy)
-      MP_UNROLL for (int j = 0; j < MD; ++j) if (j < p.dx) gth[oc + i * p.dx + j] += wd * (double)(w[i] * x[j]);
+    MP_UNROLL for (int i = 0; i < MD; ++i) if (i < DY(p))
+      MP_UNROLL for (int j = 0; j < MD; ++j) if (j < DX(p)) gth[oc + i * DX(p) + j] += wd * (double)(w[i] * x[j]);
     // d/dLR_ij = sum_k (w_i w_k - Si_ik) LR_kj, j <= i
-    MP_UNROLL for (int i = 0; i < MD; ++i) if (i < p.dy) MP_UNROLL for (int j = 0; j <= i; ++j) {
+    MP_UNROLL for (int i = 0; i < MD; ++i) if (i < DY(p)) MP_UNROLL for (int j = 0; j <= i; ++j) {
       float a = 0.f;
-      MP_UNROLL for (int k = 0; k < MD; ++k) if (k < p.dy) a += (w[i] * w[k] - Si[i][k]) * p.LR[k][j];
+      MP_UNROLL for (int k = 0; k < MD; ++k) if (k < DY(p)) a += (w[i] * w[k] - Si[i][k]) * p.LR[k][j];
       gth[orr + i * (i + 1) / 2 + j] += wd * (double)a;
     }
   }
@@ -245,16 +249,16 @@ struct LGT {
   __device__ static void step_vjp(const ModelDesc&, const P& p, Key k, const float* xin, const float*,
                                   const float* xbar_out, float* xbar_in, double* gth) {
     float z[MD];
-    MP_UNROLL for (int i = 0; i < MD; ++i) z[i] = i < p.dx ? normal_at(k, i) : 0.f;
+    MP_UNROLL for (int i = 0; i < MD; ++i) z[i] = i < DX(p) ? normal_at(k, i) : 0.f;
     MP_UNROLL for (int j = 0; j < MD; ++j) {
       float a = 0.f;
-      MP_UNROLL for (int i = 0; i < MD; ++i) if (i < p.dx) a += p.A[i][j] * xbar_out[i];
+      MP_UNROLL for (int i = 0; i < MD; ++i) if (i < DX(p)) a += p.A[i][j] * xbar_out[i];
       xbar_in[j] = a;
     }
     const int oq = oQ(p);
-    MP_UNROLL for (int i = 0; i < MD; ++i) if (i < p.dx) {
+    MP_UNROLL for (int i = 0; i < MD; ++i) if (i < DX(p)) {
       const double b = (double)xbar_out[i];
-      MP_UNROLL for (int j = 0; j < MD; ++j) if (j < p.dx) gth[i * p.dx + j] += b * (double)xin[j];
+      MP_UNROLL for (int j = 0; j < MD; ++j) if (j < DX(p)) gth[i * DX(p) + j] += b * (double)xin[j];
       MP_UNROLL for (int j = 0; j <= i; ++j) gth[oq + i * (i + 1) / 2 + j] += b * (double)z[j];
     }
   }
@@ -262,9 +266,9 @@ struct LGT {
   // x_0 = X0 + L0 z (L0 = LQ unless a separate initial factor is given)
   __device__ static void rinit_vjp(const ModelDesc& md, const P& p, Key k, float, const float* xbar, double* gth) {
     float z[MD];
-    MP_UNROLL for (int i = 0; i < MD; ++i) z[i] = i < p.dx ? normal_at(k, i) : 0.f;
+    MP_UNROLL for (int i = 0; i < MD; ++i) z[i] = i < DX(p) ? normal_at(k, i) : 0.f;
     const int ox = oX(p), ol = md.lg_sep_init ? oP(p) : oQ(p);
-    MP_UNROLL for (int i = 0; i < MD; ++i) if (i < p.dx) {
+    MP_UNROLL for (int i = 0; i < MD; ++i) if (i < DX(p)) {
       const double b = (double)xbar[i];
       gth[ox + i] += b;
       MP_UNROLL for (int j = 0; j <= i; ++j) gth[ol + i * (i + 1) / 2 + j] += b * (double)z[j];

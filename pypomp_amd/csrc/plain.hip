@@ -204,7 +204,10 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
   const float* dts = a.ser.dt_ext + a.ser.dt_off[a.unit];
   const float ty = st.ty;
   const float tstart = st.tstart;
-  const float* y = a.ser.ys + st.obs * md.dy;
+  // the observation row, staged in LDS (published by the first barrier below)
+  __shared__ float s_y[kSimMaxDy];
+  if (tid < md.dy && tid < kSimMaxDy) s_y[tid] = a.ser.ys[st.obs * md.dy + tid];
+  const float* y = s_y;
   const int64_t base = st.base;
   const float logJ = a.logJ;
 
