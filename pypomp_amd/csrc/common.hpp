@@ -210,10 +210,11 @@ MP_HD double step_loglik(float m, uint64_t total, int sh) {
 // T_j = floor(unif_j * total) with unif_j = fl32(fl32(u + j) / J), exactly.
 // The division: u + j is 0 or at least 2^-23 and J < 2^31, so the quotient
 // stays far above underflow and div_by is exact (1 / J is the same for every
-// particle and leaves the loops).
+// particle and leaves the loops).  0 <= j < J < 2^31: j converts to float as
+// a 32-bit integer (the same correctly rounded value, one instruction).
 MP_HD uint64_t comb_threshold(float u, int64_t j, int64_t J, uint64_t total) {
   const float fJ = (float)J;
-  float f = div_by(u + (float)j, fJ, 1.0f / fJ);
+  float f = div_by(u + (float)(uint32_t)j, fJ, 1.0f / fJ);
   uint32_t b = __float_as_uint(f);
   uint32_t e = (b >> 23) & 0xffu;
   if (e == 0) return 0;  // f == 0 (subnormal f cannot occur: f >= 2^-23 / J)
