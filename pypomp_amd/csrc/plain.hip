@@ -351,6 +351,10 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
     __syncthreads();
   }
   auto row = [&](int r, bool use_z, const float (&zp)[NZ]) {
+    // A compiler barrier (no instruction): the model parameters are read from
+    // LDS inside each row rather than held in registers across the rows, which
+    // would spill at the 8-wave register budget.
+    __asm__ volatile("" ::: "memory");
     const int64_t j = tile0 + (int64_t)r * B + tid;
     if (j >= tile1) return;
     float x[M::MAXS];
