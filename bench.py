@@ -30,7 +30,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM-side bytes per launch at J = 1e6 from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # passes (profiles/plain_lg2_J1e6_traffic.md): FETCH_SIZE x 2 + WRITE_SIZE x 1, the
 # factors that make a 1 GiB device copy read 1 GiB and write 1 GiB.
-MEASURED_TRAFFIC_J1E6 = {"plain_prop": 30.64e6, "plain_scan": 12.24e6}
+MEASURED_TRAFFIC_J1E6 = {"plain_prop": 30.60e6, "plain_scan": 12.24e6}
 # HBM bytes per launch of the SV MOP-gradient kernel at J=1000, T=1000, K=50, 512
 # filters (FETCH_SIZE x 2 + WRITE_SIZE, copy-calibrated)
 MEASURED_TRAFFIC_MOP = 19.03e9
@@ -44,7 +44,7 @@ MEASURED_TRAFFIC_PANEL = 61.02e9
 # particle-steps per dispatch, rocprofv3 --pmc; profiles/valu_plain_lg2_seir_mopgrad.md).
 # The kernels are VALU- or latency-bound rather than HBM-bound, so each roofline
 # object also carries the VALU issue rate implied by the live kernel time.
-VALU_INSTR_PER_PSTEP = {"plain_prop": 706.0, "plain_scan": 123.0, "seir": 10289.0, "mop_grad": 762.0,
+VALU_INSTR_PER_PSTEP = {"plain_prop": 656.0, "plain_scan": 123.0, "seir": 10289.0, "mop_grad": 762.0,
                         "panel_if2": 14436.0}
 # The VALU issue ceiling, measured in this run by the calibration probes
 # (valu_probe): the best of five instruction mixes (integer add/xor with FMAs
