@@ -211,13 +211,15 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
   const int64_t base = st.base;
   const float logJ = a.logJ;
 
-  // Waves 0 to 3 finish the previous step below (a latency-bound chain of
-  // dependent loads): they issue at a raised priority, so that the other
-  // waves' VALU work on the CU does not delay it, and their first loads (the
-  // first chunk of scan tile totals, block 0's running log-likelihood) are in
-  // flight during the parameter staging.
+  // Waves 0 to 2 (and 3 when the tile ends span several chunks) finish the
+  // previous step below (a latency-bound chain of dependent loads): they issue
+  // at a raised priority, so that the other waves' VALU work on the CU does
+  // not delay it, and their first loads (the first chunk of scan tile totals,
+  // block 0's running log-likelihood) are in flight during the parameter
+  // staging.
   const int wv = tid >> 6, lane = tid & 63;
-  const bool pro = t > 0 && wv < 4;  // (wave 3 only when the tile ends span several chunks)
+  const bool one_chunk = !a.ends && a.ntiles <= kChunkTiles;
+  const bool pro = t > 0 && (wv < 3 || (wv == 3 && !a.ends && !one_chunk));
   if (pro) __builtin_amdgcn_s_setprio(2);
   uint64_t v0[kLaneTiles] = {};
   if (pro && !a.ends) load_chunk(a, 0, v0);
@@ -256,7 +258,6 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
   //  * wave 3 forms the tile ends in LDS for the staging.
   // The searches run whether or not the step resamples (wave 2 decides that
   // concurrently); their result is then unused.
-  const bool one_chunk = !a.ends && a.ntiles <= kChunkTiles;
   if (t > 0) {
     if (wv < 2 && one_chunk) {
       // the tile ends from one DPP scan; wave 0 also stores them in LDS
