@@ -292,7 +292,7 @@ __device__ __forceinline__ void run_filter(const BatchArgs& a, const int f, char
             const Key sk = ld_key(sch, base + k);
             const float dtk = dts[ti0 + k];
             if constexpr (kHoist == 1) M::step_sub(Pj, s_sub[k], split_at(sk, (uint64_t)j), dtk, x);
-            else M::step_sub(Pj, M::substep_tab(Pj, s_sub[k], dtk), split_at(sk, (uint64_t)j), dtk, x);
+            else M::step_tab(Pj, s_sub[k], split_at(sk, (uint64_t)j), dtk, x);
           }
           if constexpr (kHoist == 1) meas = M::dmeas_obs(Pj, s_obs, y, x);
           else meas = M::dmeas_obs(Pj, M::obs_tab(Pj, s_obs, y), y, x);

@@ -55,40 +55,83 @@ __device__ __forceinline__ float ndtri_f32(float u) {
   return 1.41421354f * (p * x);
 }
 
+// What the binomial inversion below needs of p alone: its first steps and the
+// p-only factors of its correction terms and bottom-up stage.  BinomLazy forms
+// each where the inversion uses it (no extra live registers); BinomP holds
+// them, formed once, for draws that share p (a sub-step's E->I and I->R
+// probabilities when every particle has the same parameters).  Both take the
+// same operations, so the draws agree bit for bit.
+struct BinomLazy {
+  float p;  // the probability as given (edge cases)
+  __device__ bool flip() const { return p > 0.5f; }
+  __device__ float ps() const { return fminf(fmaxf(flip() ? 1.0f - p : p, 0.f), 1.f); }  // min(p, 1-p)
+  __device__ float q() const { return 1.0f - ps(); }
+  __device__ float pq() const { return ps() * q(); }
+  __device__ float c6a() const { return 2.0f + 2.0f * ps(); }
+  __device__ float c6b() const { return q() - ps(); }
+  __device__ float c1a() const { return -2.0f + 14.0f * pq(); }
+  __device__ float c1b() const { return -1.0f - 2.0f * pq(); }
+  __device__ float c2() const { return (ps() - q()) * (2.0f + pq()); }
+  __device__ float ratio() const { return ps() / fminf(fmaxf(1.0f - ps(), 1.17549435e-38f), 1.0f); }
+  __device__ float logq() const { return d_log1p(-ps()); }
+};
+
+struct BinomP {
+  float p_, ps_, q_, c6a_, c6b_, c1a_, c1b_, c2_, ratio_, logq_;
+  bool flip_;
+  __device__ float p() const { return p_; }
+  __device__ bool flip() const { return flip_; }
+  __device__ float ps() const { return ps_; }
+  __device__ float q() const { return q_; }
+  __device__ float c6a() const { return c6a_; }
+  __device__ float c6b() const { return c6b_; }
+  __device__ float c1a() const { return c1a_; }
+  __device__ float c1b() const { return c1b_; }
+  __device__ float c2() const { return c2_; }
+  __device__ float ratio() const { return ratio_; }
+  __device__ float logq() const { return logq_; }
+};
+
+__device__ __forceinline__ BinomP binom_p(float p) {
+  const BinomLazy l{p};
+  return BinomP{p, l.ps(), l.q(), l.c6a(), l.c6b(), l.c1a(), l.c1b(), l.c2(), l.ratio(), l.logq(), l.flip()};
+}
+
+__device__ __forceinline__ float prob(const BinomLazy& b) { return b.p; }
+__device__ __forceinline__ float prob(const BinomP& b) { return b.p_; }
+
 // Giles & Beentjes (2024) approximate inverse binomial CDF, order 2, with the
-// exact bottom-up stage of at most 5 terms (binom.py:236-451).  Returns the
-// count as a float; NaN for invalid arguments.
-__device__ __forceinline__ float binominv_f32(float u, float n, float p) {
+// exact bottom-up stage of at most 5 terms (binom.py:236-451), given p as a
+// BinomLazy or BinomP.  Returns the count as a float; NaN for invalid arguments.
+template <class B>
+__device__ __forceinline__ float binominv_t(float u, float n, const B& b) {
   const float tiny = 1.17549435e-38f, eps = 1.1920929e-07f;
+  const float p = prob(b);
   const bool invalid = (n < 0.f) || (p < 0.f) || (p > 1.f) || (u < 0.f) || (u > 1.f);
-  const bool flip = p > 0.5f;
-  float ps = flip ? 1.0f - p : p;
+  const bool flip = b.flip();
+  const float ps = b.ps();
   const float uf = flip ? 1.0f - u : u;
   const float ns = (n < 0.f) ? 1.0f : n;
-  ps = fminf(fmaxf(ps, 0.f), 1.f);
   const float us = fminf(fmaxf(uf, eps), 1.0f - eps);
-  const float q = 1.0f - ps;
+  const float q = b.q();
   const float w = ndtri_f32(us);
   const float w2 = w * w;
   const float np_ = ns * ps;
   const float npq = np_ * q;
   const float sq = sqrtf(fmaxf(npq, tiny));
-  const float pq = ps * q;
   // the numerator is at least 2 (q >= ps): div_by is exact
-  float q0 = np_ + sq * w + div_by(2.0f + 2.0f * ps + (q - ps) * w2, 6.0f, 1.0f / 6.0f);
+  float q0 = np_ + sq * w + div_by(b.c6a() + b.c6b() * w2, 6.0f, 1.0f / 6.0f);
   float w3 = w2 * w;
-  float q1 = q0 + ((-2.0f + 14.0f * pq) * w + (-1.0f - 2.0f * pq) * w3) / fmaxf(72.0f * sq, tiny);
+  float q1 = q0 + (b.c1a() * w + b.c1b() * w3) / fmaxf(72.0f * sq, tiny);
   float w4 = w3 * w;
-  float q2 = q1 + ((ps - q) * (2.0f + pq) * (16.0f - 7.0f * w2 - 3.0f * w4)) / fmaxf(1620.0f * (sq * sq), tiny);
+  float q2 = q1 + (b.c2() * (16.0f - 7.0f * w2 - 3.0f * w4)) / fmaxf(1620.0f * (sq * sq), tiny);
   float k = fminf(fmaxf(floorf(q2), 0.f), ns);
   k = fminf(k, np_ + 6.0f * sq + 5.0f);
   if (k < 10.0f || np_ < 4.0f) {
     // bottom-up exact CDF (binom.py:396-451)
     const float ue = fminf(fmaxf(uf, 0.f), 1.f);
-    const float qc = fminf(fmaxf(1.0f - ps, tiny), 1.0f);
-    const float ratio = ps / qc;
-    const float logq = d_log1p(-ps);
-    float pmf = (ns == 0.f) ? 1.0f : d_exp(ns * logq);
+    const float ratio = b.ratio();
+    float pmf = (ns == 0.f) ? 1.0f : d_exp(ns * b.logq());
     float cdf = pmf;
     bool found = cdf >= ue;
     float res = found ? 0.f : k;
@@ -115,11 +158,15 @@ __device__ __forceinline__ float binominv_f32(float u, float n, float p) {
   return r;
 }
 
-// fast_binomial(key, n, p) with a scalar draw (binom.py:211-233), int result.
-__device__ __forceinline__ int fast_binomial_i(Key k, int n, float p) {
+__device__ __forceinline__ float binominv_f32(float u, float n, float p) { return binominv_t(u, n, BinomLazy{p}); }
+
+// fast_binomial(key, n, p) with a scalar draw (binom.py:211-233), int result,
+// given p as a BinomLazy or BinomP.
+template <class B>
+__device__ __forceinline__ int fast_binomial_i(Key k, int n, const B& b) {
   float u = uniform01(bits_at(k, 0));
   u = fminf(fmaxf(u, 1.17549435e-38f), 0.99999994f);
-  float r = binominv_f32(u, (float)n, p);
+  float r = binominv_t(u, (float)n, b);
   return r != r ? -1 : (int)r;
 }
 
@@ -223,7 +270,8 @@ struct SEIR {
     double lgy1;  // lgamma(y + 1)
   };
   struct Sub {
-    float bt, pEI, pIR;
+    float bt;
+    BinomP bEI, bIR;  // binom_p of the E->I and I->R probabilities
   };
   struct Obs {
     double c;  // lgamma(y + k) - lgamma(k) - lgamma(y + 1)
@@ -234,20 +282,24 @@ struct SEIR {
   __device__ static Sub substep_tab(const P& p, const Tab& tb, float dt) {
     Sub sb;
     sb.bt = p.beta * (1.0f + p.amp * tb.cs);
-    sb.pEI = -d_expm1(-p.sigma * dt);
-    sb.pIR = -d_expm1(-p.gamma * dt);
+    sb.bEI = binom_p(-d_expm1(-p.sigma * dt));
+    sb.bIR = binom_p(-d_expm1(-p.gamma * dt));
     return sb;
   }
 
   __device__ static Sub substep(const P& p, float t, float dt) { return substep_tab(p, tab_sub(t), dt); }
 
-  __device__ static void step_sub(const P& p, const Sub& sb, Key key, float dt, float* x) {
+  // One Euler-binomial sub-step given the seasonal transmission rate bt and
+  // the E->I and I->R probabilities (BinomP from the sub-step table, or
+  // BinomLazy from the particle's own parameters).
+  template <class B>
+  __device__ static void step_core(const P& p, float bt, Key key, float dt, float* x, const B& bEI, const B& bIR) {
     int S = (int)x[0], E = (int)x[1], I = (int)x[2], R = (int)x[3], H = (int)x[4];
-    const float foi = sb.bt * (float)I * p.invN;
+    const float foi = bt * (float)I * p.invN;
     const float pSE = -d_expm1(-foi * dt);
-    const int dSE = fast_binomial_i(split_at(key, 0), S, pSE);
-    const int dEI = fast_binomial_i(split_at(key, 1), E, sb.pEI);
-    const int dIR = fast_binomial_i(split_at(key, 2), I, sb.pIR);
+    const int dSE = fast_binomial_i(split_at(key, 0), S, BinomLazy{pSE});
+    const int dEI = fast_binomial_i(split_at(key, 1), E, bEI);
+    const int dIR = fast_binomial_i(split_at(key, 2), I, bIR);
     S -= dSE;
     E += dSE - dEI;
     I += dEI - dIR;
@@ -260,8 +312,19 @@ struct SEIR {
     x[4] = (float)H;
   }
 
+  // the sub-step from the shared table (every particle has the parameters)
+  __device__ static void step_sub(const P& p, const Sub& sb, Key key, float dt, float* x) {
+    step_core(p, sb.bt, key, dt, x, sb.bEI, sb.bIR);
+  }
+
+  // the sub-step from its time-only table, with the particle's own parameters
+  __device__ static void step_tab(const P& p, const Tab& tb, Key key, float dt, float* x) {
+    const float bt = p.beta * (1.0f + p.amp * tb.cs);
+    step_core(p, bt, key, dt, x, BinomLazy{-d_expm1(-p.sigma * dt)}, BinomLazy{-d_expm1(-p.gamma * dt)});
+  }
+
   __device__ static void step(const ModelDesc&, const P& p, Key key, float t, float dt, float* x) {
-    step_sub(p, substep(p, t, dt), key, dt, x);
+    step_tab(p, tab_sub(t), key, dt, x);
   }
 
   __device__ static TabObs tab_obs(const float* y) { return TabObs{dd_lgamma((double)rintf(y[0]) + 1.0)}; }
