@@ -263,6 +263,9 @@ int32_t mp_perfilter(const mp_model_desc* h_model, const mp_series_desc* h_serie
                                         launch per step that writes them to global memory */
 #define MP_TUNE_PLAIN_XCD_MAP 8      /* 1 (default): propagate blocks take the particle chunks
                                         of the scan tiles on their own XCD; 0: block order */
+#define MP_TUNE_PLAIN_LANES 9        /* replicate filters the plain form runs at once, each on
+                                        its own stream (default priority) with its own buffers:
+                                        1..4 (default 2; the no-allocation entry point takes 1) */
 int32_t mp_tune_set(int32_t param, int32_t value);
 int32_t mp_tune_get(int32_t param);
 

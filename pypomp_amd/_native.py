@@ -29,6 +29,7 @@ EXPORTS = [
 MP_TUNE_PLAIN_SCAN_ROWS, MP_TUNE_PLAIN_PROP_ROWS, MP_TUNE_BATCHED_BLOCK, MP_TUNE_GRAD_BLOCK = 1, 2, 3, 4
 MP_TUNE_PLAIN_SCAN_THREADS, MP_TUNE_PLAIN_PROP_THREADS, MP_TUNE_PLAIN_LDS_TILES = 5, 6, 7
 MP_TUNE_PLAIN_XCD_MAP = 8
+MP_TUNE_PLAIN_LANES = 9
 MP_NOALLOC_SCRATCH_BYTES = 64 * 1024 * 1024
 
 

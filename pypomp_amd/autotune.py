@@ -30,6 +30,8 @@ PARAMS = {
     "plain_prop_threads": (_native.MP_TUNE_PLAIN_PROP_THREADS, (256, 512, 1024)),
     # 1: propagate blocks take the particle chunks of the scan tiles on their own XCD
     "plain_xcd_map": (_native.MP_TUNE_PLAIN_XCD_MAP, (1, 0)),
+    # replicate filters of the plain form run at once, each on its own stream
+    "plain_lanes": (_native.MP_TUNE_PLAIN_LANES, (2, 1, 3, 4)),
 }
 
 _lock = threading.Lock()
@@ -142,7 +144,10 @@ def tune_pfilter(struct, theta, J, R, form="batched", key=None):
     th = torch.from_numpy(np.ascontiguousarray(np.asarray(theta, np.float32))).to(dev)
     kt = _keys_tensor(prandom.split(prandom.key(0), R), dev)
     ll = torch.empty(R, dtype=torch.float64, device=dev)
-    names = ("plain_scan_rows", "plain_prop_rows") if fid == _native.MP_FORM_PLAIN else ("batched_block",)
+    if fid == _native.MP_FORM_PLAIN:
+        names = ("plain_scan_rows", "plain_prop_rows") + (("plain_lanes",) if R > 1 else ())
+    else:
+        names = ("batched_block",)
     holder = {}
 
     def run():

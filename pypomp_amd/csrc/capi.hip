@@ -94,6 +94,11 @@ int g_plain_prop_threads = 512;  // propagate workgroup size (256, 512, 1024)
 int g_plain_lds_tiles = kMaxLdsTiles;
 // propagate blocks take the chunks of the scan tiles on their own XCD
 int g_plain_xcd_map = 1;
+// Replicate filters the plain form runs at once, each with its own buffers on
+// its own stream (1..4): the latency-bound scan of one overlaps the VALU-bound
+// propagate of another.
+constexpr int kMaxLanes = 4;
+int g_plain_lanes = 2;
 int g_batched_block = 0;    // batched workgroup size: 0 = automatic, 256, 512, 768
 int g_grad_block = 0;       // MOP-adjoint workgroup size: 0 = automatic, 256, 512
 
@@ -124,6 +129,8 @@ hipEvent_t take_event() {
 // Set while a no-allocation entry point runs: its launches are not timed,
 // because timing would create HIP events.
 thread_local bool g_no_alloc = false;
+
+constexpr int kMaxDevices = 64;
 
 // Runs `launch` with the timing events handed to its kernel launch when
 // profiling is on; a launch that does not take them is not timed.
@@ -254,14 +261,21 @@ BatchedPlan plan_batched(int algo, const mp_model_desc* m, const mp_series_desc*
 
 struct PlainPlan {
   int ntiles, rows, tile_len, nprop, rows_prop, prop_len, scan_threads, prop_threads, xcd_q, win;
-  int ndiag;
+  int ndiag, lanes;
   bool global_ends;
+  // off_x0 .. off_diag: lane 0's buffers; lane l's sit lane_stride * l further
   int64_t off_sched, off_params, off_x0, off_x1, off_lw, off_c, off_agg, off_ends, off_pmax, off_pmin, off_mm,
-      off_steps, off_st, off_diag, total;
+      off_steps, off_st, off_diag, lane_stride, total;
 };
+
+// Lanes of a plain-form call of R filters: the setting, at most R, and one in
+// the no-allocation entry point (the other lanes' streams are created on first
+// use).
+int plain_lanes(int R) { return g_no_alloc ? 1 : std::min(g_plain_lanes, R); }
 
 PlainPlan plan_plain(const mp_model_desc* m, const mp_series_desc* s, int R, int J) {
   PlainPlan p{};
+  p.lanes = plain_lanes(R);
   p.scan_threads = g_plain_scan_threads;
   p.prop_threads = g_plain_prop_threads;
   p.rows = g_plain_scan_rows;
@@ -314,6 +328,7 @@ PlainPlan plan_plain(const mp_model_desc* m, const mp_series_desc* s, int R, int
   };
   p.off_sched = take((int64_t)R * sched_stride(ALGO_PF, ser) * 8);  // all filters' key schedules
   p.off_params = take((int64_t)R * kParamSlot);                      // prepared parameters
+  const int64_t lane0 = o;
   p.off_x0 = take((int64_t)m->S * J * 4);
   p.off_x1 = take((int64_t)m->S * J * 4);
   p.off_lw = take((int64_t)J * 4);
@@ -328,7 +343,8 @@ PlainPlan plan_plain(const mp_model_desc* m, const mp_series_desc* s, int R, int
   // step diagnostics: one block per 1024 particles, at most 1024 blocks
   p.ndiag = (int)std::min<int64_t>(1024, ((int64_t)J + 1023) / 1024);
   p.off_diag = take((int64_t)p.ndiag * kDiagWords * 8);
-  p.total = o + 256;
+  p.lane_stride = o - lane0;
+  p.total = lane0 + p.lanes * p.lane_stride + 256;
   return p;
 }
 
@@ -356,6 +372,31 @@ int run_batched(int algo, const mp_model_desc* m, const mp_series_desc* s, Batch
   return MP_OK;
 }
 
+// The streams of the plain form's lanes 1.. (lane 0 is the caller's stream)
+// and the events that fork them from the caller's stream and join them back:
+// one set per host thread and device, default priority, created on first use
+// and kept until the process ends.
+struct LaneSync {
+  hipStream_t s[kMaxLanes] = {};
+  hipEvent_t fork = nullptr;
+  hipEvent_t join[kMaxLanes] = {};
+};
+
+int lane_sync(int lanes, LaneSync** out) {
+  static thread_local LaneSync tl[kMaxDevices];
+  int dev = -1;
+  MP_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(MP_ERR_ARG, "device index %d out of range", dev);
+  LaneSync& L = tl[dev];
+  if (!L.fork) MP_HIP(hipEventCreateWithFlags(&L.fork, hipEventDisableTiming));
+  for (int l = 1; l < lanes; ++l) {
+    if (!L.s[l]) MP_HIP(hipStreamCreateWithFlags(&L.s[l], hipStreamNonBlocking));
+    if (!L.join[l]) MP_HIP(hipEventCreateWithFlags(&L.join[l], hipEventDisableTiming));
+  }
+  *out = &L;
+  return MP_OK;
+}
+
 int run_plain_pf(const mp_model_desc* m, const mp_series_desc* s, const float* theta, int64_t theta_stride,
                  const uint32_t* keys, int R, int J, float logthresh, int flags, double* loglik, float* cll,
                  float* ess, float* fmean, float* pmean, void* ws, size_t ws_bytes, hipStream_t st) {
@@ -372,17 +413,6 @@ int run_plain_pf(const mp_model_desc* m, const mp_series_desc* s, const float* t
   a.T = s->Tmax;
   a.flags = flags;
   a.logthresh = logthresh;
-  a.sched = reinterpret_cast<uint32_t*>(b + p.off_sched);
-  a.X0 = reinterpret_cast<float*>(b + p.off_x0);
-  a.X1 = reinterpret_cast<float*>(b + p.off_x1);
-  a.lw = reinterpret_cast<float*>(b + p.off_lw);
-  a.C = reinterpret_cast<uint64_t*>(b + p.off_c);
-  a.agg = reinterpret_cast<uint64_t*>(b + p.off_agg);
-  a.ends = p.global_ends ? reinterpret_cast<uint64_t*>(b + p.off_ends) : nullptr;
-  a.pmax = reinterpret_cast<float*>(b + p.off_pmax);
-  a.pmin = reinterpret_cast<float*>(b + p.off_pmin);
-  a.mm = reinterpret_cast<float*>(b + p.off_mm);
-  a.steps = reinterpret_cast<PlainStep*>(b + p.off_steps);
   a.ntiles = p.ntiles;
   a.tile_len = p.tile_len;
   a.tile_shift = __builtin_ctz((unsigned)p.tile_len);
@@ -394,9 +424,7 @@ int run_plain_pf(const mp_model_desc* m, const mp_series_desc* s, const float* t
   a.logJ = d_log((float)J);
   a.log1J = d_log(1.0f / (float)J);
   a.sh = shift_for(J);
-  a.st = reinterpret_cast<PlainState*>(b + p.off_st);
   a.rows = p.rows;
-  a.diag_part = reinterpret_cast<double*>(b + p.off_diag);
   a.ndiag = p.ndiag;
   const bool diag = (flags & (MP_ESS | MP_FILTER_MEAN | MP_PREDICTION_MEAN)) != 0;
   const int64_t slen = sched_stride(ALGO_PF, a.ser);
@@ -404,27 +432,78 @@ int run_plain_pf(const mp_model_desc* m, const mp_series_desc* s, const float* t
   MP_HIP(timed(3, st, [&] { return launch_sched(ALGO_PF, keys, a.ser, nullptr, slen, sched_all, R, st); }));
   char* params = b + p.off_params;
   MP_HIP(plain_prepare(a.md, theta, theta_stride, R, flags, params, kParamSlot, st));
-  for (int f = 0; f < R; ++f) {
-    a.f = f;
-    a.theta = theta + f * theta_stride;
-    a.fkey = keys + 2 * f;
-    a.loglik_out = loglik + f;
-    a.cll = cll ? cll + (int64_t)f * s->Tmax : nullptr;
-    a.ess = (flags & MP_ESS) ? ess + (int64_t)f * s->Tmax : nullptr;
-    a.fmean = (flags & MP_FILTER_MEAN) ? fmean + (int64_t)f * s->Tmax * m->S : nullptr;
-    a.pmean = (flags & MP_PREDICTION_MEAN) ? pmean + (int64_t)f * s->Tmax * m->S : nullptr;
-    a.sched = sched_all + 2 * (int64_t)f * slen;
-    a.params = params + (int64_t)f * kParamSlot;
-    MP_HIP(plain_reset(a, st));
-    for (int t = 0; t < a.T; ++t) {
-      a.t = t;
-      MP_HIP(timed(0, st, [&] { return plain_kernel(0, a, p.ntiles, p.prop_threads, st); }));
-      MP_HIP(timed(1, st, [&] { return plain_kernel(1, a, p.ntiles, p.scan_threads, st); }));
-      if (diag) MP_HIP(plain_kernel(2, a, p.ntiles, 256, st));
-    }
-    MP_HIP(plain_kernel(3, a, p.ntiles, 64, st));
+  // each lane's buffers
+  PlainArgs A[kMaxLanes] = {a, a, a, a};
+  for (int l = 0; l < p.lanes; ++l) {
+    char* lb = b + l * p.lane_stride;
+    PlainArgs& x = A[l];
+    x.X0 = reinterpret_cast<float*>(lb + p.off_x0);
+    x.X1 = reinterpret_cast<float*>(lb + p.off_x1);
+    x.lw = reinterpret_cast<float*>(lb + p.off_lw);
+    x.C = reinterpret_cast<uint64_t*>(lb + p.off_c);
+    x.agg = reinterpret_cast<uint64_t*>(lb + p.off_agg);
+    x.ends = p.global_ends ? reinterpret_cast<uint64_t*>(lb + p.off_ends) : nullptr;
+    x.pmax = reinterpret_cast<float*>(lb + p.off_pmax);
+    x.pmin = reinterpret_cast<float*>(lb + p.off_pmin);
+    x.mm = reinterpret_cast<float*>(lb + p.off_mm);
+    x.steps = reinterpret_cast<PlainStep*>(lb + p.off_steps);
+    x.st = reinterpret_cast<PlainState*>(lb + p.off_st);
+    x.diag_part = reinterpret_cast<double*>(lb + p.off_diag);
   }
-  return MP_OK;
+  hipStream_t ls[kMaxLanes] = {st, st, st, st};
+  LaneSync* L = nullptr;
+  if (p.lanes > 1) {
+    int rc = lane_sync(p.lanes, &L);
+    if (rc) return rc;
+    MP_HIP(hipEventRecord(L->fork, st));
+    for (int l = 1; l < p.lanes; ++l) {
+      ls[l] = L->s[l];
+      MP_HIP(hipStreamWaitEvent(ls[l], L->fork, 0));
+    }
+  }
+  // Filters f0 .. f0 + nl - 1 run together, filter f0 + l in lane l; each
+  // observation step is the propagate and scan launches of every lane.
+  auto body = [&]() -> int {
+    for (int f0 = 0; f0 < R; f0 += p.lanes) {
+      const int nl = std::min(p.lanes, R - f0);
+      for (int l = 0; l < nl; ++l) {
+        const int f = f0 + l;
+        PlainArgs& x = A[l];
+        x.f = f;
+        x.theta = theta + f * theta_stride;
+        x.fkey = keys + 2 * f;
+        x.loglik_out = loglik + f;
+        x.cll = cll ? cll + (int64_t)f * s->Tmax : nullptr;
+        x.ess = (flags & MP_ESS) ? ess + (int64_t)f * s->Tmax : nullptr;
+        x.fmean = (flags & MP_FILTER_MEAN) ? fmean + (int64_t)f * s->Tmax * m->S : nullptr;
+        x.pmean = (flags & MP_PREDICTION_MEAN) ? pmean + (int64_t)f * s->Tmax * m->S : nullptr;
+        x.sched = sched_all + 2 * (int64_t)f * slen;
+        x.params = params + (int64_t)f * kParamSlot;
+        MP_HIP(plain_reset(x, ls[l]));
+      }
+      for (int t = 0; t < a.T; ++t) {
+        for (int l = 0; l < nl; ++l) {
+          PlainArgs& x = A[l];
+          const hipStream_t q = ls[l];
+          x.t = t;
+          MP_HIP(timed(0, q, [&] { return plain_kernel(0, x, p.ntiles, p.prop_threads, q); }));
+          MP_HIP(timed(1, q, [&] { return plain_kernel(1, x, p.ntiles, p.scan_threads, q); }));
+          if (diag) MP_HIP(plain_kernel(2, x, p.ntiles, 256, q));
+        }
+      }
+      for (int l = 0; l < nl; ++l) MP_HIP(plain_kernel(3, A[l], p.ntiles, 64, ls[l]));
+    }
+    return MP_OK;
+  };
+  const int rc = body();
+  // join even after a failed launch, so the caller's stream still orders
+  // after whatever the other lanes have queued
+  for (int l = 1; l < p.lanes; ++l) {
+    const hipError_t e1 = hipEventRecord(L->join[l], ls[l]);
+    const hipError_t e2 = e1 == hipSuccess ? hipStreamWaitEvent(st, L->join[l], 0) : e1;
+    if (rc == MP_OK && e2 != hipSuccess) return hip_fail(e2, "joining a plain-form lane");
+  }
+  return rc;
 }
 
 BatchArgs base_args(const mp_model_desc* m, const mp_series_desc* s, const uint32_t* keys, const int32_t* unit,
@@ -499,7 +578,6 @@ int run_mop_grad(const mp_model_desc* m, const mp_series_desc* s, const float* t
 
 // Device address of `scratch` on the current device, looked up on the first
 // call made on that device and kept per device.
-constexpr int kMaxDevices = 64;
 std::mutex g_scratch_mu;
 void* g_scratch_ptr[kMaxDevices] = {};
 
@@ -679,6 +757,11 @@ int32_t mp_pfilter_noalloc(const mp_model_desc* h_model, const mp_series_desc* h
                            float* fmean, float* pmean, void* stream) {
   int rc = check_common(h_model, h_series, R, J, keys);
   if (rc) return rc;
+  // set before the size query: the plan depends on it (one plain-form lane)
+  struct NoAllocScope {
+    NoAllocScope() { g_no_alloc = true; }
+    ~NoAllocScope() { g_no_alloc = false; }
+  } no_alloc_scope;
   const size_t need = mp_filter_workspace_size(MP_ALGO_PF, h_model, h_series, R, J, form);
   if (need == 0) return fail(MP_ERR_ARG, "workspace size query failed");
   if (need > (size_t)MP_NOALLOC_SCRATCH_BYTES)
@@ -689,11 +772,8 @@ int32_t mp_pfilter_noalloc(const mp_model_desc* h_model, const mp_series_desc* h
   void* ws = nullptr;
   rc = scratch_address(&ws);
   if (rc) return rc;
-  g_no_alloc = true;
-  rc = mp_pfilter(h_model, h_series, theta, theta_stride, keys, unit, R, J, thresh, flags, form, loglik, cll, ess,
-                  fmean, pmean, ws, (size_t)MP_NOALLOC_SCRATCH_BYTES, stream);
-  g_no_alloc = false;
-  return rc;
+  return mp_pfilter(h_model, h_series, theta, theta_stride, keys, unit, R, J, thresh, flags, form, loglik, cll, ess,
+                    fmean, pmean, ws, (size_t)MP_NOALLOC_SCRATCH_BYTES, stream);
 }
 
 int32_t mp_mop(const mp_model_desc* h_model, const mp_series_desc* h_series, const float* theta, int64_t theta_stride,
@@ -905,6 +985,10 @@ int32_t mp_tune_set(int32_t param, int32_t value) {
       if (value != 0 && value != 1) return fail(MP_ERR_ARG, "XCD map must be 0 or 1");
       g_plain_xcd_map = value;
       return MP_OK;
+    case MP_TUNE_PLAIN_LANES:
+      if (value < 1 || value > kMaxLanes) return fail(MP_ERR_ARG, "plain-form lanes must be 1..%d", kMaxLanes);
+      g_plain_lanes = value;
+      return MP_OK;
     case MP_TUNE_GRAD_BLOCK:
       if (value != 0 && value != 256 && value != 512)
         return fail(MP_ERR_ARG, "adjoint workgroup size must be 0 (automatic), 256 or 512");
@@ -924,6 +1008,7 @@ int32_t mp_tune_get(int32_t param) {
     case MP_TUNE_PLAIN_PROP_THREADS: return g_plain_prop_threads;
     case MP_TUNE_PLAIN_LDS_TILES: return g_plain_lds_tiles;
     case MP_TUNE_PLAIN_XCD_MAP: return g_plain_xcd_map;
+    case MP_TUNE_PLAIN_LANES: return g_plain_lanes;
   }
   return -1;
 }

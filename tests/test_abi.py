@@ -89,6 +89,9 @@ def test_tuning_settings_round_trip_and_reject_bad_values(lib):
     assert lib.mp_tune_get(8) == 1 and lib.mp_tune_set(8, 2) != 0
     assert lib.mp_tune_set(8, 0) == 0 and lib.mp_tune_get(8) == 0
     assert lib.mp_tune_set(8, 1) == 0
+    # plain-form lanes: 1..4, default 2
+    assert lib.mp_tune_get(9) == 2
+    assert lib.mp_tune_set(9, 0) != 0 and lib.mp_tune_set(9, 5) != 0 and lib.mp_tune_get(9) == 2
 
 
 def test_library_shares_the_one_hip_runtime_in_the_process():

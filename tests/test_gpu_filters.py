@@ -241,7 +241,7 @@ def test_plain_xcd_chunk_map_edges(J):
 def _with_plain_tuning(settings, fn):
     from pypomp_amd import _native, autotune
     lib = _native.lib()
-    names = ("plain_scan_rows", "plain_scan_threads")
+    names = ("plain_scan_rows", "plain_scan_threads", "plain_lanes")
     saved = {n: autotune.get(n) for n in names}
     lds = lib.mp_tune_get(_native.MP_TUNE_PLAIN_LDS_TILES)
     try:
@@ -277,6 +277,32 @@ def test_plain_tile_ends_do_not_change_results(settings):
     out = _with_plain_tuning(settings, lambda: F.pfilter(struct, theta[None], J, keys, form="plain", CLL=True))
     np.testing.assert_array_equal(out["logLik64"], ref["logLik64"])
     np.testing.assert_array_equal(out["CLL"], ref["CLL"])
+
+
+@pytest.mark.parametrize("nsets,reps", [(1, 3), (2, 2)])
+def test_plain_lanes_do_not_change_results(nsets, reps):
+    # With lanes the plain form runs replicate filters in groups, each on its
+    # own stream with its own buffers (a short last group leaves lanes idle):
+    # every output, diagnostics included, keeps the bits of one lane.
+    import pypomp_amd.functional as F
+    struct, theta, _ = lg2_struct(1)
+    thetas = np.stack([theta, theta * F32(1.05)])[:nsets]
+    keys = tf.split(tf.key(93), nsets * reps).reshape(nsets, reps, 2)
+    J = 150_000
+
+    def run(lanes):
+        return _with_plain_tuning({"plain_lanes": lanes}, lambda: F.pfilter(
+            struct, thetas, J, keys, form="plain", CLL=True, ESS=True, filter_mean=True, prediction_mean=True))
+
+    one, two = run(1), run(2)
+    for k in ("logLik64", "CLL", "ESS", "filter_mean", "prediction_mean"):
+        np.testing.assert_array_equal(two[k], one[k], err_msg=k)
+    four = run(4)  # three or four filters: every lane of four busy, or one idle
+    for k in ("logLik64", "CLL", "ESS", "filter_mean", "prediction_mean"):
+        np.testing.assert_array_equal(four[k], one[k], err_msg=k)
+    ref = F.pfilter(struct, thetas, J, keys, form="batched", CLL=True)
+    np.testing.assert_array_equal(two["logLik64"], ref["logLik64"])
+    np.testing.assert_array_equal(two["CLL"], ref["CLL"])
 
 
 @pytest.mark.parametrize("lds_tiles", [1024, 0])
