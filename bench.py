@@ -490,6 +490,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--reps", type=int, default=64)
     ap.add_argument("--no-extra", action="store_true", help="skip the batched workloads")
+    ap.add_argument("--lanes", type=int, default=None,
+                    help="plain-form filters run at once, each on its own stream (default: the library's, 2)")
     ap.add_argument("--only", choices=["plain", "seir", "seir_plain", "mop_batched", "mop_plain", "panel",
                                        "panel_eager", "copy", "valu_peak"],
                     help="run one workload --steps times and print a short JSON line (for profilers)")
@@ -502,6 +504,8 @@ def main():
     import torch
     from pypomp_amd import _native, autotune
     torch.cuda.set_device(0)
+    if args.lanes is not None:
+        autotune.set("plain_lanes", args.lanes)
     if args.only:
         return run_only(args)
     log("VALU issue calibration probes")
