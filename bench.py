@@ -589,11 +589,20 @@ def main():
     if not args.no_extra:
         log("SEIR batched")
         sb = time_workload(BatchedSEIR(), 1, 1)
-        log("SEIR plain (32 of the filters, one at a time)")
+        log(f"SEIR plain (32 of the filters, {lanes} at a time)")
         sp = time_workload(BatchedSEIR(reps=32, form="plain"), 1, 0)
+        log("SEIR plain (32 of the filters, one at a time)")
+        autotune.set("plain_lanes", 1)
+        try:
+            sp1 = time_workload(BatchedSEIR(reps=32, form="plain"), 1, 0)
+        finally:
+            autotune.set("plain_lanes", lanes)
         sb["vs_plain"] = sb["value"] / sp["value"]
+        sb["vs_plain_one_lane"] = sb["value"] / sp1["value"]
+        sp["lanes"], sp1["lanes"] = lanes, 1
         extra["seir_pfilter_batched"] = sb
         extra["seir_pfilter_plain"] = sp
+        extra["seir_pfilter_plain_one_lane"] = sp1
         log("MOP grad batched")
         mb = time_workload(MopGradSV("batched"), 2, 1)
         log("MOP grad plain (16 filters, one launch each)")

@@ -252,8 +252,10 @@ int32_t mp_perfilter(const mp_model_desc* h_model, const mp_series_desc* h_serie
  * query the size after setting).  The Python autotuner
  * (pypomp_amd/autotune.py) sweeps them on the caller's workload and caches the
  * best values. */
+/* The four plain-form shapes take 0 (the default) for automatic: scan 1024 x 4 and propagate
+   512 x 2 with one or two lanes, scan 256 x 16 and propagate 256 x 4 with three or four. */
 #define MP_TUNE_PLAIN_SCAN_ROWS 1 /* consecutive particles per scan thread: 1, 2, 4, 8, 16 */
-#define MP_TUNE_PLAIN_PROP_ROWS 2 /* rows of 256 particles per propagate block, 1..16 */
+#define MP_TUNE_PLAIN_PROP_ROWS 2 /* particle rows per propagate block, 1..16 */
 #define MP_TUNE_BATCHED_BLOCK 3   /* batched workgroup size: 0 automatic, 256, 512, 768 */
 #define MP_TUNE_GRAD_BLOCK 4      /* MOP-adjoint workgroup size: 0 automatic, 256, 512 */
 #define MP_TUNE_PLAIN_SCAN_THREADS 5 /* plain scan workgroup size: 256, 512, 1024 */
@@ -265,7 +267,7 @@ int32_t mp_perfilter(const mp_model_desc* h_model, const mp_series_desc* h_serie
                                         of the scan tiles on their own XCD; 0: block order */
 #define MP_TUNE_PLAIN_LANES 9        /* replicate filters the plain form runs at once, each on
                                         its own stream (default priority) with its own buffers:
-                                        1..4 (default 2; the no-allocation entry point takes 1) */
+                                        1..4 (default 4; the no-allocation entry point takes 1) */
 int32_t mp_tune_set(int32_t param, int32_t value);
 int32_t mp_tune_get(int32_t param);
 

@@ -31,7 +31,7 @@ PARAMS = {
     # 1: propagate blocks take the particle chunks of the scan tiles on their own XCD
     "plain_xcd_map": (_native.MP_TUNE_PLAIN_XCD_MAP, (1, 0)),
     # replicate filters of the plain form run at once, each on its own stream
-    "plain_lanes": (_native.MP_TUNE_PLAIN_LANES, (2, 1, 3, 4)),
+    "plain_lanes": (_native.MP_TUNE_PLAIN_LANES, (4, 1, 2, 3)),
 }
 
 _lock = threading.Lock()

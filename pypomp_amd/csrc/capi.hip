@@ -85,10 +85,11 @@ constexpr int64_t kParamSlot = 512;  // bytes per filter of prepared model param
 
 // Launch-shape parameters (set by the autotuner through mp_tune_set).
 // Defaults from a sweep on LG 2-D, J = 1e6 (2.86e10 particle-steps/s).
-int g_plain_scan_rows = 4;  // particles per scan thread (1..16)
-int g_plain_prop_rows = 2;  // particle rows per propagate block (1..16)
-int g_plain_scan_threads = 1024;  // scan workgroup size (256, 512, 1024)
-int g_plain_prop_threads = 512;  // propagate workgroup size (256, 512, 1024)
+// Plain-form launch shapes; 0 = automatic (plain_shape below).
+int g_plain_scan_rows = 0;     // particles per scan thread (1..16)
+int g_plain_prop_rows = 0;     // particle rows per propagate block (1..16)
+int g_plain_scan_threads = 0;  // scan workgroup size (256, 512, 1024)
+int g_plain_prop_threads = 0;  // propagate workgroup size (256, 512, 1024)
 // scan tiles whose ends every propagate block forms in LDS (above it, or at 0,
 // a k_plain_ends launch per step writes them to global memory)
 int g_plain_lds_tiles = kMaxLdsTiles;
@@ -98,7 +99,7 @@ int g_plain_xcd_map = 1;
 // its own stream (1..4): the latency-bound scan of one overlaps the VALU-bound
 // propagate of another.
 constexpr int kMaxLanes = 4;
-int g_plain_lanes = 2;
+int g_plain_lanes = 4;
 int g_batched_block = 0;    // batched workgroup size: 0 = automatic, 256, 512, 768
 int g_grad_block = 0;       // MOP-adjoint workgroup size: 0 = automatic, 256, 512
 
@@ -276,9 +277,15 @@ int plain_lanes(int R) { return g_no_alloc ? 1 : std::min(g_plain_lanes, R); }
 PlainPlan plan_plain(const mp_model_desc* m, const mp_series_desc* s, int R, int J) {
   PlainPlan p{};
   p.lanes = plain_lanes(R);
-  p.scan_threads = g_plain_scan_threads;
-  p.prop_threads = g_plain_prop_threads;
-  p.rows = g_plain_scan_rows;
+  // Automatic shapes, measured on config 2 (J = 1e6): alone or in two lanes,
+  // 4096-particle scan tiles of 1024 threads x 4 and propagate blocks of
+  // 512 x 2; with three or four lanes in flight, smaller workgroups that fit
+  // beside the other lanes' kernels: scan 256 x 16, propagate 256 x 4
+  // (4 lanes: 5.6e10 -> 6.7e10 particle-steps/s; 1 lane: 3.4e10 -> 3.1e10).
+  const bool wide = p.lanes >= 3;
+  p.scan_threads = g_plain_scan_threads ? g_plain_scan_threads : (wide ? 256 : 1024);
+  p.prop_threads = g_plain_prop_threads ? g_plain_prop_threads : (wide ? 256 : 512);
+  p.rows = g_plain_scan_rows ? g_plain_scan_rows : (wide ? 16 : 4);
   // Past kMaxLdsTiles tiles the scan threads take more particles each (up to
   // 16), so that propagate blocks can still form the tile ends in LDS.
   const int lds_cap = std::min(g_plain_lds_tiles, kMaxLdsTiles);
@@ -288,7 +295,7 @@ PlainPlan plan_plain(const mp_model_desc* m, const mp_series_desc* s, int R, int
   p.tile_len = tile;
   p.ntiles = (J + tile - 1) / tile;
   p.global_ends = p.ntiles > lds_cap;
-  p.rows_prop = g_plain_prop_rows;
+  p.rows_prop = g_plain_prop_rows ? g_plain_prop_rows : (wide ? 4 : 2);
   p.nprop = (J + p.rows_prop * p.prop_threads - 1) / (p.rows_prop * p.prop_threads);
   // Whole rounds of resident workgroups (8 waves per SIMD: 2048 threads per
   // CU): past half a round the block count is rounded up to a multiple of a
@@ -953,12 +960,12 @@ int32_t mp_perfilter(const mp_model_desc* h_model, const mp_series_desc* h_serie
 int32_t mp_tune_set(int32_t param, int32_t value) {
   switch (param) {
     case MP_TUNE_PLAIN_SCAN_ROWS:
-      if (value < 1 || value > 16 || (value & (value - 1)))
-        return fail(MP_ERR_ARG, "scan rows must be 1, 2, 4, 8 or 16");
+      if (value < 0 || value > 16 || (value & (value - 1)))
+        return fail(MP_ERR_ARG, "scan rows must be 0 (automatic), 1, 2, 4, 8 or 16");
       g_plain_scan_rows = value;
       return MP_OK;
     case MP_TUNE_PLAIN_PROP_ROWS:
-      if (value < 1 || value > 16) return fail(MP_ERR_ARG, "propagate rows must be 1..16");
+      if (value < 0 || value > 16) return fail(MP_ERR_ARG, "propagate rows must be 0 (automatic) or 1..16");
       g_plain_prop_rows = value;
       return MP_OK;
     case MP_TUNE_BATCHED_BLOCK:
@@ -967,13 +974,13 @@ int32_t mp_tune_set(int32_t param, int32_t value) {
       g_batched_block = value;
       return MP_OK;
     case MP_TUNE_PLAIN_SCAN_THREADS:
-      if (value != 256 && value != 512 && value != 1024)
-        return fail(MP_ERR_ARG, "plain scan workgroup size must be 256, 512 or 1024");
+      if (value != 0 && value != 256 && value != 512 && value != 1024)
+        return fail(MP_ERR_ARG, "plain scan workgroup size must be 0 (automatic), 256, 512 or 1024");
       g_plain_scan_threads = value;
       return MP_OK;
     case MP_TUNE_PLAIN_PROP_THREADS:
-      if (value != 256 && value != 512 && value != 1024)
-        return fail(MP_ERR_ARG, "plain workgroup size must be 256, 512 or 1024");
+      if (value != 0 && value != 256 && value != 512 && value != 1024)
+        return fail(MP_ERR_ARG, "plain workgroup size must be 0 (automatic), 256, 512 or 1024");
       g_plain_prop_threads = value;
       return MP_OK;
     case MP_TUNE_PLAIN_LDS_TILES:

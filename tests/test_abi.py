@@ -80,7 +80,13 @@ def test_tuning_settings_round_trip_and_reject_bad_values(lib):
             assert autotune.get(name) == v
         autotune.set(name, old)
     assert lib.mp_tune_set(3, 300) != 0
-    assert lib.mp_tune_set(1, 0) != 0
+    # the plain-form shapes: 0 is automatic (the default), other values are checked
+    for knob in (1, 2, 5, 6):
+        old = lib.mp_tune_get(knob)
+        assert old == 0
+        assert lib.mp_tune_set(knob, 0) == 0 and lib.mp_tune_get(knob) == 0
+    assert lib.mp_tune_set(1, 3) != 0 and lib.mp_tune_set(2, 17) != 0
+    assert lib.mp_tune_set(5, 128) != 0 and lib.mp_tune_set(6, 768) != 0
     assert lib.mp_tune_set(99, 1) != 0
     assert lib.mp_tune_set(7, -1) != 0
     old = lib.mp_tune_get(7)
@@ -89,9 +95,9 @@ def test_tuning_settings_round_trip_and_reject_bad_values(lib):
     assert lib.mp_tune_get(8) == 1 and lib.mp_tune_set(8, 2) != 0
     assert lib.mp_tune_set(8, 0) == 0 and lib.mp_tune_get(8) == 0
     assert lib.mp_tune_set(8, 1) == 0
-    # plain-form lanes: 1..4, default 2
-    assert lib.mp_tune_get(9) == 2
-    assert lib.mp_tune_set(9, 0) != 0 and lib.mp_tune_set(9, 5) != 0 and lib.mp_tune_get(9) == 2
+    # plain-form lanes: 1..4, default 4
+    assert lib.mp_tune_get(9) == 4
+    assert lib.mp_tune_set(9, 0) != 0 and lib.mp_tune_set(9, 5) != 0 and lib.mp_tune_get(9) == 4
 
 
 def test_library_shares_the_one_hip_runtime_in_the_process():
