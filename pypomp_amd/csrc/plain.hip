@@ -191,8 +191,12 @@ template <class M>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void k_plain_prop(const PlainArgs a) {
   __shared__ double red[32];
   __shared__ int64_t s_win[2];
-  extern __shared__ uint64_t s_C[];  // the ancestor window, a.win entries
-  __shared__ uint64_t s_end[kMaxLdsTiles];
+  // Dynamic LDS: the ancestor window (a.win entries), then, when the tile ends
+  // are not in global memory, the a.ntiles (<= kMaxLdsTiles) tile ends.  Sized
+  // at launch, so that a filter of a few hundred tiles leaves room for eight
+  // 256-thread blocks per CU.
+  extern __shared__ uint64_t s_C[];
+  uint64_t* const s_end = s_C + a.win;
   __shared__ uint64_t s_total;
   __shared__ PlainFin s_fin;
   const ModelDesc md = a.md;
@@ -687,7 +691,8 @@ hipError_t plain_reset(const PlainArgs& a, hipStream_t s) {
 template <class M>
 static hipError_t plain_kernel_t(int which, const PlainArgs& a, int ntiles, int threads, hipStream_t s) {
   if (which == 0) {
-    mp_launch(k_plain_prop<M>, dim3(a.nprop), dim3(threads), (uint32_t)(a.win * sizeof(uint64_t)), s, a);
+    const int64_t lds_words = (int64_t)a.win + (a.ends ? 0 : a.ntiles);
+    mp_launch(k_plain_prop<M>, dim3(a.nprop), dim3(threads), (uint32_t)(lds_words * sizeof(uint64_t)), s, a);
   } else if (which == 1) {
     mp_launch(k_plain_scan<M>, dim3(ntiles), dim3(threads), 0, s, a);
     if (a.ends) mp_launch(k_plain_ends, dim3(1), dim3(1024), 0, s, a);
