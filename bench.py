@@ -34,7 +34,9 @@ HBM_PEAK_GBS = 8000.0
 # HBM-side bytes per launch at J = 1e6 from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # passes (profiles/plain_lg2_J1e6_traffic.md): FETCH_SIZE x 2 + WRITE_SIZE x 1, the
 # factors that make a 1 GiB device copy read 1 GiB and write 1 GiB.
-MEASURED_TRAFFIC_J1E6 = {"plain_prop": 30.60e6, "plain_scan": 12.24e6}
+MEASURED_TRAFFIC_J1E6 = {"plain_prop": 30.60e6, "plain_scan": 12.24e6,
+                        # the shapes taken with three or four lanes
+                        "plain_prop_wide": 30.64e6, "plain_scan_wide": 12.35e6}
 # HBM bytes per launch of the SV MOP-gradient kernel at J=1000, T=1000, K=50, 512
 # filters (FETCH_SIZE x 2 + WRITE_SIZE, copy-calibrated)
 MEASURED_TRAFFIC_MOP = 19.03e9
@@ -49,7 +51,10 @@ MEASURED_TRAFFIC_PANEL = 61.02e9
 # The kernels are VALU- or latency-bound rather than HBM-bound, so each roofline
 # object also carries the VALU issue rate implied by the live kernel time.
 VALU_INSTR_PER_PSTEP = {"plain_prop": 656.0, "plain_scan": 123.0, "seir": 9300.0, "mop_grad": 762.0,
-                        "panel_if2": 13850.0}
+                        "panel_if2": 13850.0,
+                        # the plain kernels in the shapes taken with three or four lanes
+                        # (propagate 256 x 4, scan 256 x 16)
+                        "plain_prop_wide": 595.0, "plain_scan_wide": 67.0}
 # The VALU issue ceiling, measured in this run by the calibration probes
 # (valu_probe): the best of five instruction mixes (integer add/xor with FMAs
 # issues about 1.7 wave64 instructions per CU-cycle on gfx950, above one per
@@ -564,8 +569,13 @@ def main():
     dom = max(kernels, key=lambda k: prof[k][0])
     d = kernels[dom]
     # the whole filter step in the timed lanes: both kernels' algorithmic bytes
-    # and VALU instructions per particle-step over the timed region
+    # and VALU instructions per particle-step (of the launch shapes those lanes
+    # take when the shapes are automatic) over the timed region
     fsteps = T * R * args.steps
+    wide = lanes >= 3 and all(autotune.get(n) == 0 for n in (
+        "plain_scan_rows", "plain_prop_rows", "plain_scan_threads", "plain_prop_threads"))
+    sfx = "_wide" if wide else ""
+    step_instr = VALU_INSTR_PER_PSTEP["plain_prop" + sfx] + VALU_INSTR_PER_PSTEP["plain_scan" + sfx]
     step_bytes = sum(kb.values())
     step_gbps = step_bytes * fsteps / elapsed / 1e9
     roofline = {"bound": "hbm", "achieved": d["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -575,9 +585,11 @@ def main():
                 "valu": valu_rate(VALU_INSTR_PER_PSTEP[dom], J, d["avg_us"] * 1e-6),
                 "filter_step": {"lanes": lanes, "us_per_filter_step": elapsed / fsteps * 1e6,
                                 "bytes_per_filter_step": step_bytes, "achieved": step_gbps,
+                                "traffic_per_filter_step": (MEASURED_TRAFFIC_J1E6["plain_prop" + sfx]
+                                                            + MEASURED_TRAFFIC_J1E6["plain_scan" + sfx])
+                                if J == 1_000_000 else None,
                                 "frac": step_gbps / HBM_PEAK_GBS,
-                                "valu": valu_rate(VALU_INSTR_PER_PSTEP["plain_prop"] + VALU_INSTR_PER_PSTEP["plain_scan"],
-                                                  J * fsteps, elapsed)}}
+                                "valu": valu_rate(step_instr, J * fsteps, elapsed)}}
     log(f"kernels (one lane) {json.dumps(kernels)}")
     log(f"kernels ({lanes} lanes) {json.dumps(kernels_lanes)}")
 
