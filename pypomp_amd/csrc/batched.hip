@@ -490,7 +490,12 @@ static hipError_t launch_batched_algo(const BatchArgs& a, int block, int grid, s
     case MODEL_LG:
       if (a.md.dx == 1 && a.md.dy == 1) return launch_batched_t<LG1, ALGO>(a, block, grid, lds, st);
       if (a.md.dx == 2 && a.md.dy == 2) return launch_batched_t<LG2, ALGO>(a, block, grid, lds, st);
-      return launch_batched_t<LG4, ALGO>(a, block, grid, lds, st);
+      if (a.md.dx <= 4 && a.md.dy <= 4) return launch_batched_t<LG4, ALGO>(a, block, grid, lds, st);
+      // dimensions 5..8: 256-thread workgroups only (the plan fixes them)
+      if (block != 256) return hipErrorInvalidValue;
+      if (a.lds_resident) mp_launch(k_batched<LG8, ALGO, 256, true>, dim3(grid), dim3(256), lds, st, a);
+      else mp_launch(k_batched<LG8, ALGO, 256, false>, dim3(grid), dim3(256), lds, st, a);
+      return hipGetLastError();
     case MODEL_SV: return launch_batched_t<SV, ALGO>(a, block, grid, lds, st);
     case MODEL_SEIR: return launch_batched_t<SEIR, ALGO>(a, block, grid, lds, st);
     case MODEL_SPX: return launch_batched_t<SPX, ALGO>(a, block, grid, lds, st);

@@ -81,7 +81,7 @@ int hip_fail(hipError_t e, const char* where) {
     if (e_ != hipSuccess) return hip_fail(e_, #call);  \
   } while (0)
 
-constexpr int64_t kParamSlot = 512;  // bytes per filter of prepared model parameters
+// kParamSlot (engine.hpp): bytes per filter of prepared model parameters
 
 // Launch-shape parameters (set by the autotuner through mp_tune_set).
 // Defaults from a sweep on LG 2-D, J = 1e6 (2.86e10 particle-steps/s).
@@ -182,8 +182,8 @@ int check_model(const mp_model_desc* m) {
   if (!m) return fail(MP_ERR_ARG, "model descriptor is NULL");
   switch (m->kind) {
     case MP_MODEL_LG:
-      if (m->dx < 1 || m->dx > 4 || m->dy < 1 || m->dy > 4)
-        return fail(MP_ERR_ARG, "LG dimensions must be 1..4 (dx=%d, dy=%d)", m->dx, m->dy);
+      if (m->dx < 1 || m->dx > kLgMaxDim || m->dy < 1 || m->dy > kLgMaxDim)
+        return fail(MP_ERR_ARG, "LG dimensions must be 1..%d (dx=%d, dy=%d)", kLgMaxDim, m->dx, m->dy);
       if (m->S != m->dx) return fail(MP_ERR_ARG, "LG: S must equal dx");
       break;
     case MP_MODEL_SV:
@@ -250,7 +250,8 @@ BatchedPlan plan_batched(int algo, const mp_model_desc* m, const mp_series_desc*
   p.sched_bytes = align256((int64_t)R * p.sched_len * 8);
   p.order_bytes = align256((int64_t)R * 4) + 256;  // filter order + queue counter
   const FilterScratch L = filter_scratch_layout(p.W, J);
-  p.block = batched_block(R, J);
+  // LG of dimension 5..8 has 256-thread workgroups only (a smaller build)
+  p.block = (m->kind == MP_MODEL_LG && (m->dx > 4 || m->dy > 4)) ? 256 : batched_block(R, J);
   const size_t small = batched_small_lds(J, p.block);
   // Keep filters LDS-resident when two workgroups still fit per CU.
   p.lds_resident = (int64_t)small + L.total <= (int64_t)max_lds_bytes() / 2;
@@ -894,7 +895,7 @@ int32_t mp_host_from_est(const mp_model_desc* h_model, float* h_theta, int64_t n
   const ModelDesc md = *reinterpret_cast<const ModelDesc*>(h_model);
   if (md.identity_trans) return MP_OK;
   switch (md.kind) {
-    case MODEL_LG: host_from_est_rows<LG4>(md, h_theta, n); break;
+    case MODEL_LG: host_from_est_rows<LG8>(md, h_theta, n); break;
     case MODEL_SV: host_from_est_rows<SV>(md, h_theta, n); break;
     case MODEL_SEIR: host_from_est_rows<SEIR>(md, h_theta, n); break;
     case MODEL_SPX: host_from_est_rows<SPX>(md, h_theta, n); break;

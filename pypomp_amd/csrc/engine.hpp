@@ -153,7 +153,7 @@ struct PlainState {
 
 // Words of one block's diagnostic sums: ESS, filter mean and prediction mean
 // of up to 7 state words.
-constexpr int kDiagWords = 16;
+constexpr int kDiagWords = 17;  // 1 + 2 S words for S <= 8
 
 // XCDs of an MI355X: the hardware dispatches workgroups over them round-robin
 // by block index, and each has its own L2.
@@ -262,7 +262,10 @@ __host__ __device__ inline HessScratch hess_scratch_layout(int S, int J) {
 }
 
 // Simulation launch (one thread per (row, simulation)); U = 1.
-constexpr int kSimMaxDy = 4;
+constexpr int kSimMaxDy = 8;
+// bytes per filter of prepared model parameters in the plain form (the LG
+// model of dimension 8 takes 1584)
+constexpr int64_t kParamSlot = 2048;
 struct SimArgs {
   ModelDesc md;
   SeriesSet ser;

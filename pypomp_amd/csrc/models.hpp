@@ -290,11 +290,14 @@ struct LGT {
   }
 };
 
-// dx = dy = 1 and dx = dy = 2 get fully constant indexing; other shapes up to
-// 4 use the generic instance with run-time dimensions.
+// dx = dy = 1 and dx = dy = 2 get fully constant indexing; other shapes use
+// a generic instance with run-time dimensions: up to 4, or up to 8 (the
+// prepared parameters then take 1.6 KB, in LDS where the particles share them).
 using LG1 = LGT<1, true>;
 using LG2 = LGT<2, true>;
 using LG4 = LGT<4, false>;
+using LG8 = LGT<8, false>;
+constexpr int kLgMaxDim = 8;
 
 // ---------------------------------------------------------------------------
 // Stochastic volatility with leverage (the MOP workload).  State (v, eta):

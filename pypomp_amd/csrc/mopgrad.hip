@@ -393,6 +393,8 @@ hipError_t launch_mop_grad(const GradArgs& a, int block, int grid, hipStream_t s
     case MODEL_LG:
       if (a.md.dx == 1 && a.md.dy == 1) return GradLauncher<LG1>::run(a, block, grid, lds, st);
       if (a.md.dx == 2 && a.md.dy == 2) return GradLauncher<LG2>::run(a, block, grid, lds, st);
+      // no adjoint above dx = dy = 2 (mop_grad_supported); not instantiated for 8
+      if (a.md.dx > 4 || a.md.dy > 4) return hipErrorNotSupported;
       return GradLauncher<LG4>::run(a, block, grid, lds, st);
     case MODEL_SEIR: return GradLauncher<SEIR>::run(a, block, grid, lds, st);
   }

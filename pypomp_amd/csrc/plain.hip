@@ -631,7 +631,7 @@ __global__ void k_plain_prepare(ModelDesc md, const float* theta, int64_t stride
 template <class M>
 static hipError_t plain_prepare_t(const ModelDesc& md, const float* theta, int64_t stride, int R, int flags,
                                   char* out, int64_t pstride, hipStream_t s) {
-  static_assert(sizeof(typename M::P) <= 512, "prepared parameters exceed their slot");
+  static_assert(sizeof(typename M::P) <= kParamSlot, "prepared parameters exceed their slot");
   mp_launch(k_plain_prepare<M>, dim3((R + 63) / 64), dim3(64), 0, s, md, theta, stride, R, flags, out,
                      pstride);
   return hipGetLastError();
@@ -643,7 +643,8 @@ hipError_t plain_prepare(const ModelDesc& md, const float* theta, int64_t stride
     case MODEL_LG:
       if (md.dx == 1 && md.dy == 1) return plain_prepare_t<LG1>(md, theta, stride, R, flags, out, pstride, s);
       if (md.dx == 2 && md.dy == 2) return plain_prepare_t<LG2>(md, theta, stride, R, flags, out, pstride, s);
-      return plain_prepare_t<LG4>(md, theta, stride, R, flags, out, pstride, s);
+      if (md.dx <= 4 && md.dy <= 4) return plain_prepare_t<LG4>(md, theta, stride, R, flags, out, pstride, s);
+      return plain_prepare_t<LG8>(md, theta, stride, R, flags, out, pstride, s);
     case MODEL_SV: return plain_prepare_t<SV>(md, theta, stride, R, flags, out, pstride, s);
     case MODEL_SEIR: return plain_prepare_t<SEIR>(md, theta, stride, R, flags, out, pstride, s);
     case MODEL_SPX: return plain_prepare_t<SPX>(md, theta, stride, R, flags, out, pstride, s);
@@ -710,7 +711,8 @@ hipError_t plain_kernel(int which, const PlainArgs& a, int ntiles, int threads, 
     case MODEL_LG:
       if (a.md.dx == 1 && a.md.dy == 1) return plain_kernel_t<LG1>(which, a, ntiles, threads, s);
       if (a.md.dx == 2 && a.md.dy == 2) return plain_kernel_t<LG2>(which, a, ntiles, threads, s);
-      return plain_kernel_t<LG4>(which, a, ntiles, threads, s);
+      if (a.md.dx <= 4 && a.md.dy <= 4) return plain_kernel_t<LG4>(which, a, ntiles, threads, s);
+      return plain_kernel_t<LG8>(which, a, ntiles, threads, s);
     case MODEL_SV: return plain_kernel_t<SV>(which, a, ntiles, threads, s);
     case MODEL_SEIR: return plain_kernel_t<SEIR>(which, a, ntiles, threads, s);
     case MODEL_SPX: return plain_kernel_t<SPX>(which, a, ntiles, threads, s);

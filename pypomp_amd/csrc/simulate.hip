@@ -84,7 +84,8 @@ hipError_t launch_simulate(const SimArgs& a, hipStream_t st) {
     case MODEL_LG:
       if (a.md.dx == 1 && a.md.dy == 1) return launch_simulate_t<LG1>(a, st);
       if (a.md.dx == 2 && a.md.dy == 2) return launch_simulate_t<LG2>(a, st);
-      return launch_simulate_t<LG4>(a, st);
+      if (a.md.dx <= 4 && a.md.dy <= 4) return launch_simulate_t<LG4>(a, st);
+      return launch_simulate_t<LG8>(a, st);
     case MODEL_SV: return launch_simulate_t<SV>(a, st);
     case MODEL_SEIR: return launch_simulate_t<SEIR>(a, st);
     case MODEL_SPX: return launch_simulate_t<SPX>(a, st);

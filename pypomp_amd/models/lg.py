@@ -2,7 +2,7 @@
 
 Parameter naming, defaults and transforms follow the reference
 (linear_gaussian.py:53-92 names, :221-280 defaults, :197-218 transforms).  The
-device functor supports dx, dy <= 4.  ``P0`` optionally gives the initial
+device functor supports dx, dy <= 8.  ``P0`` optionally gives the initial
 state its own covariance (Cholesky entries ``P{i}{j}`` appended to theta); the
 reference draws x0 from N(X0, Q).
 """
@@ -26,8 +26,8 @@ class LGModel(NativeModel):
 
     def __init__(self, dx=2, dy=None, separate_init=False):
         dy = dx if dy is None else dy
-        if not (1 <= dx <= 4 and 1 <= dy <= 4):
-            raise ValueError("the native LG model supports dimensions 1..4")
+        if not (1 <= dx <= 8 and 1 <= dy <= 8):
+            raise ValueError("the native LG model supports dimensions 1..8")
         self.dx, self.dy, self.S = int(dx), int(dy), int(dx)
         self.lg_sep_init = int(bool(separate_init))
         self.param_names = (_matrix_names("A", dx, dx) + _matrix_names("C", dy, dx)

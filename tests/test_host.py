@@ -483,3 +483,15 @@ def test_learning_rate_schedules_and_lookup():
     assert base.cosine_decay(0.1, M) == base.cosine_decay(0.1, M) != base
     with pytest.raises(ValueError, match="Cannot apply geometric decay of length 5"):
         base.cosine_decay(0.1, M).geometric_decay(0.5, 5)
+
+
+def test_lg_model_dimensions_up_to_eight():
+    # the native LG model takes dimensions 1..8 (the reference takes any); the
+    # C ABI rejects larger descriptors
+    from pypomp_amd.models import LGModel
+    m = LGModel(8, 5)
+    assert m.d == 64 + 40 + 36 + 15 + 8 and m.S == 8
+    with pytest.raises(ValueError):
+        LGModel(9, 1)
+    with pytest.raises(ValueError):
+        LGModel(2, 9)
