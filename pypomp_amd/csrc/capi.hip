@@ -185,6 +185,13 @@ int check_model(const mp_model_desc* m) {
       if (m->dx < 1 || m->dx > kLgMaxDim || m->dy < 1 || m->dy > kLgMaxDim)
         return fail(MP_ERR_ARG, "LG dimensions must be 1..%d (dx=%d, dy=%d)", kLgMaxDim, m->dx, m->dy);
       if (m->S != m->dx) return fail(MP_ERR_ARG, "LG: S must equal dx");
+      {
+        // A (dx x dx), C (dy x dx), the Q and R triangles, X0 and, with a
+        // separate initial covariance, its triangle (linear_gaussian.py:53-92)
+        const int dx = m->dx, dy = m->dy, tq = dx * (dx + 1) / 2;
+        const int d = dx * dx + dy * dx + tq + dy * (dy + 1) / 2 + dx + (m->lg_sep_init ? tq : 0);
+        if (m->d != d) return fail(MP_ERR_ARG, "LG(%d, %d) has %d parameters, the descriptor says %d", dx, dy, d, m->d);
+      }
       break;
     case MP_MODEL_SV:
       if (m->S != 2 || m->d != 6 || m->dy != 1) return fail(MP_ERR_ARG, "SV model expects S=2, d=6, dy=1");

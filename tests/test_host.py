@@ -495,3 +495,19 @@ def test_lg_model_dimensions_up_to_eight():
         LGModel(9, 1)
     with pytest.raises(ValueError):
         LGModel(2, 9)
+
+
+def test_lg_descriptor_parameter_count_is_checked():
+    # the C ABI rejects an LG descriptor whose d disagrees with its dimensions
+    # (the kernels index theta rows by the dimensions)
+    from pypomp_amd import _native
+    from pypomp_amd.models import LGModel
+    lib = _native.lib()
+    for dx, dy, sep in ((2, 2, False), (6, 5, True), (8, 8, False)):
+        m = LGModel(dx, dy, separate_init=sep)
+        desc = m.desc()
+        th = np.zeros((1, m.d), np.float32)
+        assert lib.mp_host_from_est(desc, th.ctypes.data, 1) == 0
+        desc.d = m.d - 1
+        assert lib.mp_host_from_est(desc, th.ctypes.data, 1) != 0
+        assert "parameters" in lib.mp_last_error().decode()
