@@ -256,7 +256,7 @@ int32_t mp_perfilter(const mp_model_desc* h_model, const mp_series_desc* h_serie
    512 x 2 with one or two lanes, scan 256 x 16 and propagate 256 x 4 with three or four. */
 #define MP_TUNE_PLAIN_SCAN_ROWS 1 /* consecutive particles per scan thread: 1, 2, 4, 8, 16 */
 #define MP_TUNE_PLAIN_PROP_ROWS 2 /* particle rows per propagate block, 1..16 */
-#define MP_TUNE_BATCHED_BLOCK 3   /* batched workgroup size: 0 automatic, 256, 512, 768 */
+#define MP_TUNE_BATCHED_BLOCK 3   /* batched workgroup size: 0 automatic, 256, 512, 768, 1024 (SEIR pfilter and MOP only; 768 elsewhere) */
 #define MP_TUNE_GRAD_BLOCK 4      /* MOP-adjoint workgroup size: 0 automatic, 256, 512 */
 #define MP_TUNE_PLAIN_SCAN_THREADS 5 /* plain scan workgroup size: 256, 512, 1024 */
 #define MP_TUNE_PLAIN_PROP_THREADS 6 /* plain propagate workgroup size: 256, 512, 1024 */

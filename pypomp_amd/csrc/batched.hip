@@ -479,6 +479,16 @@ static hipError_t launch_batched_t(const BatchArgs& a, int block, int grid, size
       if (a.lds_resident) mp_launch(k_batched<M, ALGO, 768, true>, dim3(grid), dim3(768), lds, st, a);
       else mp_launch(k_batched<M, ALGO, 768, false>, dim3(grid), dim3(768), lds, st, a);
       break;
+    case 1024:
+      // 16 waves at 4 per SIMD (128 VGPRs): the models whose particles share
+      // the parameters through LDS tables (batched_block_fits_1024)
+      if constexpr (M::HOIST && ALGO != ALGO_PER) {
+        if (a.lds_resident) mp_launch(k_batched<M, ALGO, 1024, true>, dim3(grid), dim3(1024), lds, st, a);
+        else mp_launch(k_batched<M, ALGO, 1024, false>, dim3(grid), dim3(1024), lds, st, a);
+        break;
+      } else {
+        return hipErrorInvalidValue;
+      }
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

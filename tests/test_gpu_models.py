@@ -45,6 +45,30 @@ def test_sv_mop_matches_oracle(form):
         assert -got[r] == exp, (r, -got[r], exp)
 
 
+@pytest.mark.parametrize("block", [768, 1024, 0])
+def test_seir_workgroup_sizes_keep_the_bits(block):
+    # filters of 2000 particles take 1024-thread workgroups by default (0);
+    # 768 and 1024 give the same bits, and the oracle's
+    import pypomp_amd.functional as F
+    from pypomp_amd import autotune
+    struct, theta, ys = seir_struct(12)
+    keys = tf.split(tf.key(17), 2)
+    saved = autotune.get("batched_block")
+    try:
+        autotune.set("batched_block", block)
+        out = F.pfilter(struct, theta[None], 2000, keys.reshape(1, 2, 2), CLL=True)
+        mop = F.mop(struct, np.stack([oseir.SEIRModel().to_est(theta)] * 2), 2000, 0.9, keys, return_float64=True)
+    finally:
+        autotune.set("batched_block", saved)
+    model = oseir.SEIRModel()
+    ser = fl.make_series(ys, np.arange(1, len(ys) + 1.0), 0.0, nstep=7)
+    for r in range(2):
+        o = fl.pfilter(model, theta, ser, keys[r], 2000, diagnostics=True)
+        assert float(out["logLik64"][0, r]) == o["logLik64"], (block, r)
+        np.testing.assert_array_equal(out["CLL"][0, r], np.array(o["CLL"], dtype=np.float32))
+        assert -mop[r] == fl.mop(model, model.to_est(theta), ser, keys[r], 2000, 0.9), (block, r)
+
+
 def test_seir_pfilter_matches_oracle():
     import pypomp_amd.functional as F
     struct, theta, ys = seir_struct(20)
