@@ -279,7 +279,7 @@ def test_plain_tile_ends_do_not_change_results(settings):
     np.testing.assert_array_equal(out["CLL"], ref["CLL"])
 
 
-@pytest.mark.parametrize("nsets,reps", [(1, 3), (2, 2)])
+@pytest.mark.parametrize("nsets,reps", [(1, 3), (2, 2), (1, 5)])
 def test_plain_lanes_do_not_change_results(nsets, reps):
     # With lanes the plain form runs replicate filters in groups, each on its
     # own stream with its own buffers (a short last group leaves lanes idle):
@@ -297,7 +297,7 @@ def test_plain_lanes_do_not_change_results(nsets, reps):
     one, two = run(1), run(2)
     for k in ("logLik64", "CLL", "ESS", "filter_mean", "prediction_mean"):
         np.testing.assert_array_equal(two[k], one[k], err_msg=k)
-    four = run(4)  # three or four filters: every lane of four busy, or one idle
+    four = run(4)  # three, four or five filters: lanes idle, all busy, or a last group of one
     for k in ("logLik64", "CLL", "ESS", "filter_mean", "prediction_mean"):
         np.testing.assert_array_equal(four[k], one[k], err_msg=k)
     ref = F.pfilter(struct, thetas, J, keys, form="batched", CLL=True)
