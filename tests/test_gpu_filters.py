@@ -305,6 +305,19 @@ def test_plain_lanes_do_not_change_results(nsets, reps):
     np.testing.assert_array_equal(two["CLL"], ref["CLL"])
 
 
+@pytest.mark.parametrize("J", [1, 7, 1000, 4097])
+def test_plain_four_lanes_small_filters_equal_batched(J):
+    # four filters in four lanes take the smaller workgroups (scan 256 x 16,
+    # propagate 256 x 4) at any J, down to one particle
+    import pypomp_amd.functional as F
+    struct, theta, _ = lg2_struct(1)
+    keys = tf.split(tf.key(97 + J), 4).reshape(1, 4, 2)
+    plain = F.pfilter(struct, theta[None], J, keys, form="plain", CLL=True)
+    batched = F.pfilter(struct, theta[None], J, keys, form="batched", CLL=True)
+    np.testing.assert_array_equal(plain["logLik64"], batched["logLik64"])
+    np.testing.assert_array_equal(plain["CLL"], batched["CLL"])
+
+
 @pytest.mark.parametrize("lds_tiles", [1024, 0])
 def test_plain_wide_ancestor_windows_match_batched_and_oracle(lds_tiles):
     # An outlying observation leaves a few hundred particles with non-zero
