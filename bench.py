@@ -532,12 +532,29 @@ def main():
     log(f"timed {args.steps} steps in {elapsed:.3f} s")
     ll = wl.ll.cpu().numpy()
     value = wl.particle_steps * args.steps / elapsed
+    lanes = autotune.get("plain_lanes")
+    # the same workload with the filters strictly one after another (one lane;
+    # every launch carries one filter in both cases)
+    seq = None
+    if lanes > 1:
+        autotune.set("plain_lanes", 1)
+        try:
+            wl.step()
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            for i in range(args.steps):
+                wl.step()
+            torch.cuda.synchronize()
+            el1 = time.perf_counter() - t1
+        finally:
+            autotune.set("plain_lanes", lanes)
+        seq = {"value": wl.particle_steps * args.steps / el1, "ms_per_step": el1 / args.steps * 1e3, "lanes": 1}
+        log(f"one lane: {args.steps} steps in {el1:.3f} s")
 
     # Per-kernel durations from HIP events on each launch's dispatch: one extra
     # step with one lane, where every launch has the GPU to itself (the kernel
     # roofline), and one in the timed lanes, where the filters' launches overlap
     # (their durations then include the time they share the GPU).
-    lanes = autotune.get("plain_lanes")
 
     def profiled_step(nl):
         autotune.set("plain_lanes", nl)
@@ -634,8 +651,11 @@ def main():
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"pfilter LG 2-D, J=1e6, T=200, 64 replicates (plain form, {lanes} at a time)",
+        "config": {"workload": f"pfilter LG 2-D, J=1e6, T=200, 64 replicates (plain form: one filter per "
+                               f"launch, {lanes} in flight on {lanes} streams)",
                    "J": J, "T": T, "replicates": R, "form": "plain", "lanes": lanes},
+        # the same 64 filters strictly one after another
+        "one_filter_at_a_time": seq,
         "roofline": roofline, "cpu_baseline": cpu,
         # Monte Carlo error of the replicates' mean log-likelihood against the exact
         # FP64 Kalman value (an accuracy check, not parity with the reference)
