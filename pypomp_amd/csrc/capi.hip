@@ -257,15 +257,14 @@ BatchedPlan plan_batched(int algo, const mp_model_desc* m, const mp_series_desc*
   p.sched_bytes = align256((int64_t)R * p.sched_len * 8);
   p.order_bytes = align256((int64_t)R * 4) + 256;  // filter order + queue counter
   const FilterScratch L = filter_scratch_layout(p.W, J);
-  // LG of dimension 5..8 has 256-thread workgroups only (a smaller build);
+  // LG of dimension 5..8 has 256-thread workgroups only (a smaller build).
   // 1024 threads only where the kernel fits 128 VGPRs: SEIR with its
   // particle-independent tables (pfilter and MOP, not IF2's per-particle
-  // parameters)
-  // (automatic for the SEIR pfilter in place of 768: 4.43e9 -> 4.76e9
-  // particle-steps/s on 256 filters of 20000)
+  // parameters); automatic for the SEIR pfilter of >= 1536 particles in place
+  // of 768 (4.43e9 -> 4.76e9 particle-steps/s on 256 filters of 20000).
   p.block = (m->kind == MP_MODEL_LG && (m->dx > 4 || m->dy > 4)) ? 256 : batched_block(R, J);
   const bool seir_tab = m->kind == MP_MODEL_SEIR && algo != ALGO_PER;
-  if (!g_batched_block && p.block == 768 && seir_tab && algo == ALGO_PF) p.block = 1024;
+  if (!g_batched_block && p.block == 768 && seir_tab && algo == ALGO_PF && J >= 1536) p.block = 1024;
   if (p.block == 1024 && !seir_tab) p.block = 768;
   const size_t small = batched_small_lds(J, p.block);
   // Keep filters LDS-resident when two workgroups still fit per CU.
