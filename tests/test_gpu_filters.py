@@ -279,8 +279,8 @@ def test_plain_tile_ends_do_not_change_results(settings):
     np.testing.assert_array_equal(out["CLL"], ref["CLL"])
 
 
-@pytest.mark.parametrize("nsets,reps", [(1, 3), (2, 2), (1, 5)])
-def test_plain_lanes_do_not_change_results(nsets, reps):
+@pytest.mark.parametrize("nsets,reps,thresh", [(1, 3, 0.0), (2, 2, 0.0), (1, 5, 0.0), (1, 5, 0.5)])
+def test_plain_lanes_do_not_change_results(nsets, reps, thresh):
     # With lanes the plain form runs replicate filters in groups, each on its
     # own stream with its own buffers (a short last group leaves lanes idle):
     # every output, diagnostics included, keeps the bits of one lane.
@@ -292,7 +292,8 @@ def test_plain_lanes_do_not_change_results(nsets, reps):
 
     def run(lanes):
         return _with_plain_tuning({"plain_lanes": lanes}, lambda: F.pfilter(
-            struct, thetas, J, keys, form="plain", CLL=True, ESS=True, filter_mean=True, prediction_mean=True))
+            struct, thetas, J, keys, thresh=thresh, form="plain", CLL=True, ESS=True, filter_mean=True,
+            prediction_mean=True))
 
     one, two = run(1), run(2)
     for k in ("logLik64", "CLL", "ESS", "filter_mean", "prediction_mean"):
@@ -300,7 +301,7 @@ def test_plain_lanes_do_not_change_results(nsets, reps):
     four = run(4)  # three, four or five filters: lanes idle, all busy, or a last group of one
     for k in ("logLik64", "CLL", "ESS", "filter_mean", "prediction_mean"):
         np.testing.assert_array_equal(four[k], one[k], err_msg=k)
-    ref = F.pfilter(struct, thetas, J, keys, form="batched", CLL=True)
+    ref = F.pfilter(struct, thetas, J, keys, thresh=thresh, form="batched", CLL=True)
     np.testing.assert_array_equal(two["logLik64"], ref["logLik64"])
     np.testing.assert_array_equal(two["CLL"], ref["CLL"])
 
