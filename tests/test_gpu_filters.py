@@ -306,6 +306,48 @@ def test_plain_lanes_do_not_change_results(nsets, reps, thresh):
     np.testing.assert_array_equal(two["CLL"], ref["CLL"])
 
 
+def test_plain_lanes_replay_from_a_hip_graph():
+    # a plain-form call with four lanes, captured on a side stream (the lanes
+    # fork from it and join back by events), replays with the eager bits; eager
+    # calls run between the replays
+    import torch
+    from pypomp_amd import _native
+    from pypomp_amd.functional import _keys_tensor
+    struct, theta, _ = lg2_struct(1)
+    lib = _native.lib()
+    dev = _native.require_device()
+    R, J = 4, 50_000
+    ser, md = struct.device_series(), struct.model.desc()
+    keys = _keys_tensor(tf.split(tf.key(99), R), dev)
+    th = torch.from_numpy(np.repeat(theta[None], R, 0)).to(dev)
+    ll = torch.empty(R, dtype=torch.float64, device=dev)
+    n = int(lib.mp_filter_workspace_size(_native.MP_ALGO_PF, md, ser.desc, R, J, _native.MP_FORM_PLAIN))
+    ws = torch.empty(n, dtype=torch.uint8, device=dev)
+
+    def call():
+        _native.check(lib.mp_pfilter(md, ser.desc, _native.ptr(th), struct.model.d, _native.ptr(keys), None, R, J,
+                                     0.0, 0, _native.MP_FORM_PLAIN, _native.ptr(ll), None, None, None, None,
+                                     _native.ptr(ws), n, _native.stream_ptr()))
+
+    assert lib.mp_tune_get(_native.MP_TUNE_PLAIN_LANES) == 4
+    call()
+    torch.cuda.synchronize()
+    eager = ll.clone()
+    g = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    with torch.cuda.graph(g, stream=side):
+        call()
+    for _ in range(3):
+        ll.fill_(0.0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(ll, eager)
+        call()
+        torch.cuda.synchronize()
+        assert torch.equal(ll, eager)
+
+
 @pytest.mark.parametrize("J", [1, 7, 1000, 4097])
 def test_plain_four_lanes_small_filters_equal_batched(J):
     # four filters in four lanes take the smaller workgroups (scan 256 x 16,
