@@ -13,16 +13,31 @@ constexpr int kRedUsable = 4088;
 __device__ __forceinline__ float nanmax(float a, float b) { return (a > b || a != a) ? a : b; }
 __device__ __forceinline__ float nanmin(float a, float b) { return (a < b || a != a) ? a : b; }
 
-__device__ __forceinline__ float wave_nanmax(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = nanmax(v, __shfl_xor(v, o, WAVE));
-  return v;
+// v from the lane K places lower in the same 16-lane row; the first K lanes
+// of the row keep their own v (DPP row_shr:K, no LDS traffic).
+template <int K>
+__device__ __forceinline__ float row_shr_keep(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x110 + K, 0xf, 0xf, false));
 }
-__device__ __forceinline__ float wave_nanmin(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = nanmin(v, __shfl_xor(v, o, WAVE));
-  return v;
+
+// Maximum (MAX) or minimum of v over the 64 lanes of a wave, NaN if any lane
+// holds NaN (the value nanmax / nanmin give in any order; of two equal zeros
+// either sign may come back).  Row extremes by DPP row shifts, then the four
+// row results by readlanes.  All 64 lanes must be active.
+template <bool MAX>
+__device__ __forceinline__ float wave_nan_extreme(float v) {
+  const bool any_nan = __builtin_amdgcn_ballot_w64(v != v) != 0;
+  auto op = [](float a, float b) { return MAX ? fmaxf(a, b) : fminf(a, b); };
+  v = op(v, row_shr_keep<1>(v));
+  v = op(v, row_shr_keep<2>(v));
+  v = op(v, row_shr_keep<4>(v));
+  v = op(v, row_shr_keep<8>(v));
+  auto row_end = [&](int lane) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane)); };
+  const float r = op(op(row_end(15), row_end(31)), op(row_end(47), row_end(63)));
+  return any_nan ? __builtin_nanf("") : r;
 }
+__device__ __forceinline__ float wave_nanmax(float v) { return wave_nan_extreme<true>(v); }
+__device__ __forceinline__ float wave_nanmin(float v) { return wave_nan_extreme<false>(v); }
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
