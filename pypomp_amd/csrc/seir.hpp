@@ -138,7 +138,11 @@ __device__ __forceinline__ float binominv_t(float u, float n, const B& b) {
 #pragma unroll
     for (int i = 1; i < 5; ++i) {
       float num = fmaxf(ns - (float)(i - 1), 0.f);
-      pmf = pmf * (num / (float)i) * ratio;
+      // num is an integer count: num / 3 in Markstein's three operations equals
+      // the IEEE quotient for every integer 0 .. 2^24 (checked exhaustively);
+      // the divisions by 1, 2 and 4 are exact multiplications
+      const float frac = i == 3 ? div_by(num, 3.0f, 1.0f / 3.0f) : num / (float)i;
+      pmf = pmf * frac * ratio;
       cdf = cdf + pmf;
       bool stalled = (cdf > 1.0f - eps * 10.0f) && (pmf < eps);
       bool fnow = (cdf >= ue) || stalled;
