@@ -50,8 +50,8 @@ MEASURED_TRAFFIC_PANEL = 61.02e9
 # particle-steps per dispatch, rocprofv3 --pmc; profiles/valu_plain_lg2_seir_mopgrad.md).
 # The kernels are VALU- or latency-bound rather than HBM-bound, so each roofline
 # object also carries the VALU issue rate implied by the live kernel time.
-VALU_INSTR_PER_PSTEP = {"plain_prop": 656.0, "plain_scan": 120.0, "seir": 9313.0, "mop_grad": 762.0,
-                        "panel_if2": 13850.0,
+VALU_INSTR_PER_PSTEP = {"plain_prop": 656.0, "plain_scan": 120.0, "seir": 9171.0, "mop_grad": 762.0,
+                        "panel_if2": 13709.0,
                         # the plain kernels in the shapes taken with three or four lanes
                         # (propagate 256 x 4, scan 256 x 16)
                         "plain_prop_wide": 593.0, "plain_scan_wide": 66.0}
