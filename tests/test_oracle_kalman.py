@@ -19,7 +19,7 @@ def random_system(rng, dx, dy):
 
 def test_kalman_equals_dense_joint_gaussian():
     rng = np.random.default_rng(0)
-    for dx, dy in ((1, 1), (2, 2), (3, 2), (2, 1)):
+    for dx, dy in ((1, 1), (2, 2), (3, 2), (2, 1), (6, 5), (8, 8)):
         A, C, Q, R, x0, P0 = random_system(rng, dx, dy)
         ys = rng.normal(size=(6, dy))
         a = kalman.kalman_loglik(ys, A, C, Q, R, x0, P0)
@@ -50,3 +50,28 @@ def test_kalman_fixture_values():
         A, C, LQ, LR, X0, L0 = (np.asarray(m, np.float64) for m in spec.unpack(w[f"lg2_seed{seed}_theta"]))
         v = kalman.kalman_loglik(w[f"lg2_seed{seed}_ys"], A, C, LQ @ LQ.T, LR @ LR.T, X0, L0 @ L0.T)
         assert v == kal[f"lg2_seed{seed}"]
+
+
+def test_oracle_lg_pfilter_of_any_dimension_against_kalman():
+    # the oracle's particle filter on LG models of the generic dimensions (the
+    # reference defaults, data drawn by the oracle's simulator): the mean of
+    # the replicate log-likelihoods lies within Monte Carlo error of the exact
+    # Kalman value
+    from oracle import filters as fl, lg as olg, threefry as tf
+    for dx, dy, seed in ((3, 2, 0), (6, 5, 1)):
+        spec = olg.LGSpec(dx, dy)
+        th = olg.default_theta(dx, dy)
+        theta = np.array([th[n] for n in spec.param_names], np.float32)
+        model = olg.LGModel(spec)
+        T = 10
+        times = np.arange(1, T + 1.0)
+        _, Y = fl.simulate(model, theta, fl.make_series(np.zeros((T, dy), np.float32), times, 0.0, nstep=1),
+                           tf.split(tf.key(700 + seed), 1)[0])
+        ys = np.ascontiguousarray(Y[:, 0, :], dtype=np.float32)
+        A, C, LQ, LR, X0, L0 = (np.asarray(m, np.float64) for m in spec.unpack(theta))
+        exact = kalman.kalman_loglik(ys, A, C, LQ @ LQ.T, LR @ LR.T, X0, L0 @ L0.T)
+        ser = fl.make_series(ys, times, 0.0, nstep=1)
+        keys = tf.split(tf.key(710 + seed), 12)
+        ll = np.array([fl.pfilter(model, theta, ser, k, 1500)["logLik64"] for k in keys])
+        se = ll.std(ddof=1) / np.sqrt(len(ll))
+        assert abs(ll.mean() - exact) < 4 * se + 0.05, (dx, dy, ll.mean(), exact, se)
